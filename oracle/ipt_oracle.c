@@ -1,0 +1,1331 @@
+/*
+ * CPU oracle for the block-IPT eigensolver.  TEST INFRASTRUCTURE ONLY.
+ *
+ * This file restates, in plain C, the arithmetic of the reference solver so
+ * that the HIP library can be checked against it.  Only tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py load it; the product
+ * library never links or calls it.
+ *
+ * Reference files are cited relative to the reference checkout root,
+ * IPT/ = IPT_C_GPU/.
+ *
+ * Pinning (how each function is checked beyond re-typing the formula):
+ *   generators         KAT hash and splitmix64 streams reproduce the
+ *                      reference's published iteration counts / residuals
+ *                      (tests/golden/dense_eps_sweep.json, runtests KATs)
+ *   orc_ipt_dense/csc  reference KAT tables (IPT/results/runtests_summary.csv),
+ *                      published iteration counts of the splitmix64 sweep
+ *                      (IPT/results/gpu_cusolver_vs_ipt_results.csv), fixed
+ *                      point = LAPACK eigenpairs on small inputs
+ *   orc_target_clusters  golden ranges produced by the reference's own
+ *                      host code built from source (oracle/_ref), plus
+ *                      hand-checked small cases
+ *   orc_block_cluster_solve  converged basis spans the LAPACK invariant
+ *                      subspace; width-1 cluster matches plain IPT fixed point
+ *   orc_rayleigh_ritz  LAPACK eigh on the same small problems
+ *   orc_davidson       LAPACK eigenpairs + residual definition; control flow
+ *                      (targets, restart sizes) checked by invariants only:
+ *                      "parity unpinned" for the accept/restart sequence.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_OK 0
+#define ORC_INVALID 1
+#define ORC_ALLOC 2
+
+/* ------------------------------------------------------------------ */
+/* generators                                                          */
+/* ------------------------------------------------------------------ */
+
+/* splitmix64 finaliser, IPT/test/mixed_precision_tf32/benchmark_tf32_cusolver_sweep.cu:434-440 */
+uint64_t orc_splitmix64(uint64_t x)
+{
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+/* counter-keyed uniform in [0,1), benchmark_tf32_cusolver_sweep.cu:442-449 */
+double orc_uniform01(uint64_t seed, long long n, long long row, long long col)
+{
+    uint64_t x = seed + (uint64_t)n * 0xd1b54a32d192ed03ULL;
+    x ^= (uint64_t)(row + 1) * 0x9e3779b97f4a7c15ULL;
+    x ^= (uint64_t)(col + 1) * 0xbf58476d1ce4e5b9ULL;
+    return (double)(orc_splitmix64(x) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* 32-bit hash noise of the dense KAT driver, IPT/test_temp/runtests.cu:46-57 */
+double orc_kat_noise(int row, int col)
+{
+    uint32_t x = (uint32_t)(row + 1) * 747796405u;
+    x ^= (uint32_t)(col + 1) * 2891336453u;
+    x ^= x >> 16;
+    x *= 2246822519u;
+    x ^= x >> 13;
+    x *= 3266489917u;
+    x ^= x >> 16;
+    return ((double)(x & 0x00ffffffu) / 16777216.0) - 0.5;
+}
+
+/* diag(i+1) + 0.01*noise, IPT/test_temp/runtests.cu:59-80 */
+void orc_gen_kat_dense(double *H, int n, int symmetric)
+{
+    for (int c = 0; c < n; ++c) {
+        for (int r = 0; r < n; ++r) {
+            double noise = symmetric
+                               ? 0.5 * (orc_kat_noise(r, c) + orc_kat_noise(c, r))
+                               : orc_kat_noise(r, c);
+            H[(size_t)r + (size_t)c * n] = 1.0e-2 * noise;
+        }
+    }
+    for (int r = 0; r < n; ++r) {
+        H[(size_t)r + (size_t)r * n] += (double)(r + 1);
+    }
+}
+
+/* diag(i+1)+eps*U symmetric sweep matrix,
+ * IPT/test/mixed_precision_tf32/benchmark_tf32_cusolver_sweep.cu:472-503 */
+void orc_gen_eps_dense(double *H, int n, double eps, uint64_t seed)
+{
+    for (int c = 0; c < n; ++c) {
+        for (int r = 0; r <= c; ++r) {
+            double v;
+            if (r == c) {
+                v = (double)(r + 1) + eps * orc_uniform01(seed, n, r, c);
+            } else {
+                v = 0.5 * eps * (orc_uniform01(seed, n, r, c) + orc_uniform01(seed, n, c, r));
+            }
+            H[(size_t)r + (size_t)c * n] = v;
+            H[(size_t)c + (size_t)r * n] = v;
+        }
+    }
+}
+
+/* Standard normal from the splitmix64 streams keyed (i,j) and (j,i), i<j:
+ * Box-Muller z = sqrt(-2 ln(1-u1)) cos(2 pi u2). */
+static double orc_pair_normal(uint64_t seed, long long n, long long i, long long j)
+{
+    long long a = i < j ? i : j;
+    long long b = i < j ? j : i;
+    double u1 = orc_uniform01(seed, n, a, b);
+    double u2 = orc_uniform01(seed, n, b, a);
+    return sqrt(-2.0 * log1p(-u1)) * cos(6.283185307179586 * u2);
+}
+
+static int orc_cmp_double(const void *a, const void *b)
+{
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+/* Sorted diagonal d_i = lo + (hi-lo)*U(seed,n,i,n), ascending. */
+void orc_gen_sorted_diagonal(double *d, int n, uint64_t seed, double lo, double hi)
+{
+    for (int i = 0; i < n; ++i) {
+        d[i] = lo + (hi - lo) * orc_uniform01(seed, n, i, n);
+    }
+    qsort(d, (size_t)n, sizeof(double), orc_cmp_double);
+}
+
+/* H = diag(d) + V, V symmetric, V_ij ~ N(0,1)*vscale/sqrt(n), V_ii = 0. */
+void orc_gen_rand_dense(double *H, int n, uint64_t seed, double lo, double hi,
+                        double vscale)
+{
+    double s = vscale / sqrt((double)n);
+    double *d = (double *)malloc((size_t)n * sizeof(double));
+    orc_gen_sorted_diagonal(d, n, seed, lo, hi);
+    for (int c = 0; c < n; ++c) {
+        for (int r = 0; r < n; ++r) {
+            H[(size_t)r + (size_t)c * n] = (r == c) ? d[r] : s * orc_pair_normal(seed, n, r, c);
+        }
+    }
+    free(d);
+}
+
+/* tridiagonal banded CSC, IPT/test/sparse/runtests_sparse.cu:224-268 */
+int orc_gen_banded_csc(int n, int *col_ptr, int *row_ind, double *values)
+{
+    int cur = 0;
+    for (int c = 0; c < n; ++c) {
+        col_ptr[c] = cur;
+        if (c > 0) {
+            int e = c - 1;
+            row_ind[cur] = c - 1;
+            values[cur] = 1.0e-2 * (0.50 + 0.03 * (double)(e % 7));
+            ++cur;
+        }
+        row_ind[cur] = c;
+        values[cur] = (double)(c + 1);
+        ++cur;
+        if (c + 1 < n) {
+            int e = c;
+            row_ind[cur] = c + 1;
+            values[cur] = 1.0e-2 * (0.50 + 0.03 * (double)(e % 7));
+            ++cur;
+        }
+    }
+    col_ptr[n] = cur;
+    return cur;
+}
+
+/* ------------------------------------------------------------------ */
+/* matrix operand: dense column-major or CSC                           */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int n;
+    const double *dense;  /* column-major n x n, or NULL */
+    const int *col_ptr;   /* CSC when dense == NULL */
+    const int *row_ind;
+    const double *values;
+} OrcMat;
+
+/* Y(:,0:cols) = M * X(:,0:cols); column-major, leading dimension n.
+ * Columns are independent so the loop over them is the parallel one. */
+static void orc_apply(const OrcMat *M, const double *X, int cols, double *Y)
+{
+    int n = M->n;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < cols; ++j) {
+        double *y = Y + (size_t)j * n;
+        const double *x = X + (size_t)j * n;
+        for (int r = 0; r < n; ++r) {
+            y[r] = 0.0;
+        }
+        if (M->dense != NULL) {
+            for (int c = 0; c < n; ++c) {
+                double xc = x[c];
+                const double *hc = M->dense + (size_t)c * n;
+                if (xc == 0.0) {
+                    continue;
+                }
+                for (int r = 0; r < n; ++r) {
+                    y[r] += hc[r] * xc;
+                }
+            }
+        } else {
+            for (int c = 0; c < n; ++c) {
+                double xc = x[c];
+                for (int p = M->col_ptr[c]; p < M->col_ptr[c + 1]; ++p) {
+                    y[M->row_ind[p]] += M->values[p] * xc;
+                }
+            }
+        }
+    }
+}
+
+static double orc_mat_diag(const OrcMat *M, int i)
+{
+    if (M->dense != NULL) {
+        return M->dense[(size_t)i + (size_t)i * M->n];
+    }
+    /* first match in the column, IPT/src/ipt_cuda.cu:1469-1489 */
+    for (int p = M->col_ptr[i]; p < M->col_ptr[i + 1]; ++p) {
+        if (M->row_ind[p] == i) {
+            return M->values[p];
+        }
+    }
+    return 0.0;
+}
+
+/* max_i sum_j |M_ij|, IPT/src/ipt_cuda.cu:2526-2541 */
+static double orc_mat_inf_norm(const OrcMat *M)
+{
+    int n = M->n;
+    double *rs = (double *)calloc((size_t)n, sizeof(double));
+    double norm = 0.0;
+    if (M->dense != NULL) {
+        for (int c = 0; c < n; ++c) {
+            for (int r = 0; r < n; ++r) {
+                rs[r] += fabs(M->dense[(size_t)r + (size_t)c * n]);
+            }
+        }
+    } else {
+        for (int c = 0; c < n; ++c) {
+            for (int p = M->col_ptr[c]; p < M->col_ptr[c + 1]; ++p) {
+                rs[M->row_ind[p]] += fabs(M->values[p]);
+            }
+        }
+    }
+    for (int r = 0; r < n; ++r) {
+        if (rs[r] > norm) {
+            norm = rs[r];
+        }
+    }
+    free(rs);
+    return norm;
+}
+
+static void orc_mat_init(OrcMat *M, const double *dense, const int *cp, const int *ri,
+                         const double *v, int n)
+{
+    M->n = n;
+    M->dense = dense;
+    M->col_ptr = cp;
+    M->row_ind = ri;
+    M->values = v;
+}
+
+double orc_inf_norm(const double *dense, const int *cp, const int *ri, const double *v, int n)
+{
+    OrcMat M;
+    orc_mat_init(&M, dense, cp, ri, v, n);
+    return orc_mat_inf_norm(&M);
+}
+
+/* ------------------------------------------------------------------ */
+/* plain IPT                                                           */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Plain fixed-point IPT for columns 0..k-1.
+ * Loop body: IPT/test_temp/runtests.cu:127-165 (and the kernels
+ * IPT/src/ipt_cuda.cu:1403-1443: c_j = (HZ)_jj - d_j Z_jj,
+ * Z'_rj = (Y_rj - d_r Z_rj - Z_rj c_j) / (d_j - d_r), Z'_jj = 1).
+ * Tolerance stop: IPT/src/ipt_cuda.cu:1663-1688 -- fp = ||F(Z)-Z||_F/||Z||_F,
+ * a non-positive or non-finite ||Z|| is an invalid argument, and the
+ * returned iterate is F(Z) of the last step taken.
+ * Values: lambda_j = (H Z)_jj of the returned iterate (1695-1703).
+ */
+static int orc_ipt_core(const OrcMat *M, int k, const double *Z0, int maxiter,
+                        double tol, int use_tol, double *vectors, double *values,
+                        int *iters_out, double *fp_out)
+{
+    int n = M->n;
+    size_t nk = (size_t)n * (size_t)k;
+    double *x = (double *)calloc(nk, sizeof(double));
+    double *y = (double *)calloc(nk, sizeof(double));
+    double *dg = (double *)calloc((size_t)n, sizeof(double));
+    double *cd = (double *)calloc((size_t)k, sizeof(double));
+    int status = ORC_OK;
+    int done = 0;
+    double fp = NAN;
+
+    if (iters_out) *iters_out = 0;
+    if (fp_out) *fp_out = NAN;
+    if (x == NULL || y == NULL || dg == NULL || cd == NULL) {
+        status = ORC_ALLOC;
+        goto out;
+    }
+    if (Z0 != NULL) {
+        memcpy(x, Z0, nk * sizeof(double));
+    } else {
+        for (int j = 0; j < k; ++j) {
+            x[(size_t)j + (size_t)j * n] = 1.0;
+        }
+    }
+    for (int r = 0; r < n; ++r) {
+        dg[r] = orc_mat_diag(M, r);
+    }
+    for (int it = 0; it < maxiter; ++it) {
+        double *t;
+        orc_apply(M, x, k, y);
+        for (int j = 0; j < k; ++j) {
+            size_t di = (size_t)j + (size_t)j * n;
+            cd[j] = y[di] - dg[j] * x[di];
+        }
+        for (int j = 0; j < k; ++j) {
+            for (int r = 0; r < n; ++r) {
+                size_t idx = (size_t)r + (size_t)j * n;
+                if (r == j) {
+                    y[idx] = 1.0;
+                } else {
+                    double g = 1.0 / (dg[j] - dg[r]);
+                    y[idx] = (y[idx] - dg[r] * x[idx] - x[idx] * cd[j]) * g;
+                }
+            }
+        }
+        done = it + 1;
+        if (use_tol) {
+            double dn = 0.0, xn = 0.0;
+            for (size_t i = 0; i < nk; ++i) {
+                double dlt = y[i] - x[i];
+                dn += dlt * dlt;
+                xn += x[i] * x[i];
+            }
+            dn = sqrt(dn);
+            xn = sqrt(xn);
+            if (xn <= 0.0 || !isfinite(xn)) {
+                status = ORC_INVALID;
+                goto out;
+            }
+            fp = dn / xn;
+            if (fp <= tol) {
+                t = x; x = y; y = t;
+                break;
+            }
+        }
+        t = x; x = y; y = t;
+    }
+    orc_apply(M, x, k, y);
+    for (int j = 0; j < k; ++j) {
+        values[j] = y[(size_t)j + (size_t)j * n];
+    }
+    memcpy(vectors, x, nk * sizeof(double));
+    if (iters_out) *iters_out = done;
+    if (fp_out) *fp_out = fp;
+out:
+    free(x);
+    free(y);
+    free(dg);
+    free(cd);
+    return status;
+}
+
+int orc_ipt_dense(const double *H, int n, int k, const double *Z0, int maxiter,
+                  double tol, int use_tol, double *vectors, double *values,
+                  int *iters, double *fp)
+{
+    OrcMat M;
+    if (H == NULL || n <= 0 || k <= 0 || k > n || maxiter < 0 ||
+        (use_tol && (tol <= 0.0 || maxiter <= 0))) {
+        return ORC_INVALID;
+    }
+    orc_mat_init(&M, H, NULL, NULL, NULL, n);
+    return orc_ipt_core(&M, k, Z0, maxiter, tol, use_tol, vectors, values, iters, fp);
+}
+
+int orc_ipt_csc(const int *cp, const int *ri, const double *v, int n, int k,
+                const double *Z0, int maxiter, double tol, int use_tol,
+                double *vectors, double *values, int *iters, double *fp)
+{
+    OrcMat M;
+    if (cp == NULL || ri == NULL || v == NULL || n <= 0 || k <= 0 || k > n ||
+        maxiter < 0 || (use_tol && (tol <= 0.0 || maxiter <= 0))) {
+        return ORC_INVALID;
+    }
+    orc_mat_init(&M, NULL, cp, ri, v, n);
+    return orc_ipt_core(&M, k, Z0, maxiter, tol, use_tol, vectors, values, iters, fp);
+}
+
+/* ------------------------------------------------------------------ */
+/* degenerate-cluster detection                                        */
+/* ------------------------------------------------------------------ */
+
+/* options vector: [absolute_gap, eps_multiplier, rel_gap_tol,
+ *                  coupling_gap_ratio_tol, eta]   (IPT/src/ipt_preparation.cu:116-146) */
+typedef struct {
+    double absolute_gap, eps_multiplier, rel_gap_tol, ratio_tol, eta;
+    int max_block;
+} OrcDegOpts;
+
+static double orc_lookup(const OrcMat *M, int row, int col)
+{
+    if (M->dense != NULL) {
+        return M->dense[(size_t)row + (size_t)col * M->n];
+    }
+    for (int p = M->col_ptr[col]; p < M->col_ptr[col + 1]; ++p) {
+        if (M->row_ind[p] == row) {
+            return M->values[p];
+        }
+    }
+    return 0.0;
+}
+
+static double orc_gap_floor(double a, double b, double tau_num, const OrcDegOpts *o)
+{
+    /* IPT/src/ipt_preparation.cu:414-424 */
+    double scale = fmax(1.0, fmax(fabs(a), fabs(b)));
+    double tau = o->absolute_gap;
+    tau = fmax(tau, tau_num);
+    tau = fmax(tau, o->rel_gap_tol * scale);
+    return tau;
+}
+
+static int orc_uf_find(int *parent, int x)
+{
+    while (parent[x] != x) {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+typedef struct {
+    double d;
+    int i;
+} OrcKey;
+
+static int orc_cmp_key(const void *a, const void *b)
+{
+    const OrcKey *x = (const OrcKey *)a, *y = (const OrcKey *)b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return (x->i > y->i) - (x->i < y->i);
+}
+
+static int orc_cmp_range(const void *a, const void *b)
+{
+    const int *x = (const int *)a, *y = (const int *)b;
+    if (x[0] != y[0]) return (x[0] > y[0]) - (x[0] < y[0]);
+    return (x[1] > y[1]) - (x[1] < y[1]);
+}
+
+/*
+ * Target-cluster ranges, IPT/src/ipt_preparation.cu:501-645 followed by the
+ * singleton completion of IPT/src/ipt_cuda.cu:4114-4132.
+ * diag must be the (sorted) diagonal of M.  ranges_out holds 2*n ints.
+ * Returns the number of ranges; *too_large is set when a multi-index range
+ * exceeds max_block.
+ */
+static int orc_clusters_core(const OrcMat *M, const double *diag, int k,
+                             const OrcDegOpts *o, int *ranges_out, int *too_large)
+{
+    int n = M->n;
+    double eps = DBL_EPSILON;
+    double maxabs = 0.0, tau_num, ratio_tol;
+    double *off = (double *)calloc((size_t)n, sizeof(double));
+    double *rowmax = (double *)calloc((size_t)n, sizeof(double));
+    OrcKey *keys = (OrcKey *)malloc((size_t)n * sizeof(OrcKey));
+    double *sd = (double *)malloc((size_t)n * sizeof(double));
+    int *parent = (int *)malloc((size_t)n * sizeof(int));
+    int *cmin = (int *)malloc((size_t)n * sizeof(int));
+    int *cmax = (int *)malloc((size_t)n * sizeof(int));
+    int *chas = (int *)calloc((size_t)n, sizeof(int));
+    int *covered = (int *)calloc((size_t)n, sizeof(int));
+    int nr = 0;
+
+    *too_large = 0;
+    for (int i = 0; i < n; ++i) {
+        maxabs = fmax(maxabs, fabs(diag[i]));
+    }
+    /* IPT/src/ipt_preparation.cu:345-351 */
+    tau_num = o->eps_multiplier * eps * fmax(1.0, maxabs);
+    ratio_tol = fmax(o->ratio_tol, eps);
+
+    /* off-diagonal strengths: max over column and row, :353-380 */
+    for (int c = 0; c < n; ++c) {
+        if (M->dense != NULL) {
+            for (int r = 0; r < n; ++r) {
+                double a;
+                if (r == c) continue;
+                a = fabs(M->dense[(size_t)r + (size_t)c * n]);
+                if (a > off[c]) off[c] = a;
+                if (a > rowmax[r]) rowmax[r] = a;
+            }
+        } else {
+            for (int p = M->col_ptr[c]; p < M->col_ptr[c + 1]; ++p) {
+                int r = M->row_ind[p];
+                double a = fabs(M->values[p]);
+                if (r == c) continue;
+                if (a > off[c]) off[c] = a;
+                if (a > rowmax[r]) rowmax[r] = a;
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        off[i] = fmax(off[i], rowmax[i]);
+        keys[i].d = diag[i];
+        keys[i].i = i;
+        parent[i] = i;
+    }
+    qsort(keys, (size_t)n, sizeof(OrcKey), orc_cmp_key);
+    for (int i = 0; i < n; ++i) {
+        sd[i] = keys[i].d;
+    }
+
+    for (int col = 0; col < k; ++col) {
+        double dc = diag[col];
+        /* :426-441 */
+        double tau_c = fmax(orc_gap_floor(dc, dc, tau_num, o), off[col] / ratio_tol);
+        double lo_v = dc - tau_c, hi_v = dc + tau_c;
+        int lo = 0, hi = n;
+        /* lower_bound(lo_v) */
+        {
+            int a = 0, b = n;
+            while (a < b) {
+                int mid = (a + b) / 2;
+                if (sd[mid] < lo_v) a = mid + 1; else b = mid;
+            }
+            lo = a;
+        }
+        /* upper_bound(hi_v) */
+        {
+            int a = 0, b = n;
+            while (a < b) {
+                int mid = (a + b) / 2;
+                if (sd[mid] <= hi_v) a = mid + 1; else b = mid;
+            }
+            hi = a;
+        }
+        for (int pos = lo; pos < hi; ++pos) {
+            int row = keys[pos].i;
+            double dr, coupling, gap, floor_, ratio;
+            int unite;
+            if (row == col) continue;
+            dr = diag[row];
+            /* :396-412 */
+            coupling = fmax(fabs(orc_lookup(M, row, col)), fabs(orc_lookup(M, col, row)));
+            if (!(coupling > 0.0)) {
+                coupling = o->eta * fmin(off[row], off[col]);
+            }
+            /* :443-478 */
+            gap = fabs(dr - dc);
+            floor_ = orc_gap_floor(dr, dc, tau_num, o);
+            ratio = coupling > 0.0 ? coupling / fmax(gap, floor_) : 0.0;
+            unite = (gap <= floor_) || (coupling > 0.0 && ratio >= ratio_tol);
+            if (unite) {
+                int ra = orc_uf_find(parent, row), rb = orc_uf_find(parent, col);
+                if (ra != rb) parent[rb] = ra;
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        cmin[i] = n;
+        cmax[i] = -1;
+    }
+    for (int i = 0; i < n; ++i) {
+        int root = orc_uf_find(parent, i);
+        if (i < cmin[root]) cmin[root] = i;
+        if (i > cmax[root]) cmax[root] = i;
+        if (i < k) chas[root] = 1;
+    }
+    for (int root = 0; root < n; ++root) {
+        if (!chas[root] || cmin[root] >= cmax[root]) continue;
+        ranges_out[2 * nr] = cmin[root];
+        ranges_out[2 * nr + 1] = cmax[root];
+        ++nr;
+    }
+    /* merge overlapping / adjacent ranges, :480-499 */
+    if (nr > 0) {
+        int m = 0;
+        qsort(ranges_out, (size_t)nr, 2 * sizeof(int), orc_cmp_range);
+        for (int i = 1; i < nr; ++i) {
+            if (ranges_out[2 * i] <= ranges_out[2 * m + 1] + 1) {
+                if (ranges_out[2 * i + 1] > ranges_out[2 * m + 1]) {
+                    ranges_out[2 * m + 1] = ranges_out[2 * i + 1];
+                }
+            } else {
+                ++m;
+                ranges_out[2 * m] = ranges_out[2 * i];
+                ranges_out[2 * m + 1] = ranges_out[2 * i + 1];
+            }
+        }
+        nr = m + 1;
+    }
+    for (int i = 0; i < nr; ++i) {
+        if (ranges_out[2 * i + 1] - ranges_out[2 * i] + 1 > o->max_block) {
+            *too_large = 1;
+        }
+        for (int r = ranges_out[2 * i]; r <= ranges_out[2 * i + 1]; ++r) {
+            covered[r] = 1;
+        }
+    }
+    /* singleton targets, IPT/src/ipt_cuda.cu:4114-4132 */
+    for (int t = 0; t < k; ++t) {
+        if (!covered[t]) {
+            ranges_out[2 * nr] = t;
+            ranges_out[2 * nr + 1] = t;
+            ++nr;
+        }
+    }
+    qsort(ranges_out, (size_t)nr, 2 * sizeof(int), orc_cmp_range);
+
+    free(off);
+    free(rowmax);
+    free(keys);
+    free(sd);
+    free(parent);
+    free(cmin);
+    free(cmax);
+    free(chas);
+    free(covered);
+    return nr;
+}
+
+int orc_target_clusters(const double *dense, const int *cp, const int *ri,
+                        const double *v, int n, int k, const double *opts5,
+                        int max_block, int *ranges_out, int *too_large)
+{
+    OrcMat M;
+    OrcDegOpts o;
+    double *diag = (double *)malloc((size_t)n * sizeof(double));
+    int nr;
+    orc_mat_init(&M, dense, cp, ri, v, n);
+    o.absolute_gap = opts5[0];
+    o.eps_multiplier = opts5[1];
+    o.rel_gap_tol = opts5[2];
+    o.ratio_tol = opts5[3];
+    o.eta = opts5[4];
+    o.max_block = max_block;
+    for (int i = 0; i < n; ++i) {
+        diag[i] = orc_mat_diag(&M, i);
+    }
+    nr = orc_clusters_core(&M, diag, k, &o, ranges_out, too_large);
+    free(diag);
+    return nr;
+}
+
+/* ------------------------------------------------------------------ */
+/* block-cluster (Riccati) iteration                                   */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Solve (d_r I - h^T) x = rhs with partial-pivot Gauss-Jordan.  The pivot
+ * rule and the singular threshold 64*DBL_EPSILON follow the GPU kernel
+ * IPT/src/ipt_cuda.cu:2311-2366.  aug is m x (m+1), row-major by equation.
+ */
+static int orc_gauss_jordan(double *aug, int m)
+{
+    for (int pc = 0; pc < m; ++pc) {
+        int pr = pc;
+        double pa = fabs(aug[pc * (m + 1) + pc]);
+        for (int cand = pc + 1; cand < m; ++cand) {
+            double ca = fabs(aug[cand * (m + 1) + pc]);
+            if (ca > pa) {
+                pa = ca;
+                pr = cand;
+            }
+        }
+        if (!isfinite(pa) || pa <= 64.0 * DBL_EPSILON) {
+            return 0;
+        }
+        if (pr != pc) {
+            for (int c = 0; c <= m; ++c) {
+                double t = aug[pc * (m + 1) + c];
+                aug[pc * (m + 1) + c] = aug[pr * (m + 1) + c];
+                aug[pr * (m + 1) + c] = t;
+            }
+        }
+        {
+            double piv = aug[pc * (m + 1) + pc];
+            for (int c = 0; c <= m; ++c) {
+                aug[pc * (m + 1) + c] /= piv;
+            }
+        }
+        for (int e = 0; e < m; ++e) {
+            if (e != pc) {
+                double f = aug[e * (m + 1) + pc];
+                for (int c = pc; c <= m; ++c) {
+                    aug[e * (m + 1) + c] -= f * aug[pc * (m + 1) + c];
+                }
+            }
+        }
+    }
+    return 1;
+}
+
+/*
+ * One cluster [first,last] of width m, basis B (n x m, column-major).
+ * Iteration body: IPT/src/ipt_cuda.cu:2966-3006 with the kernels
+ * 2268-2278 (h = (HB)[first:last, :]) and 2280-2384 (per-row solve, damping,
+ * cluster rows reset to identity).  Stop: ||B'-B||_F / max(1,||B||_F) <= tol
+ * (2999-3003); the returned basis is the last B'.
+ * Returns ORC_OK, or ORC_INVALID on a singular pivot / non-finite norm.
+ */
+static int orc_block_solve_core(const OrcMat *M, const double *diag, int first, int last,
+                                int maxiter, double tol, double damping, double *B,
+                                int *iters_out, double *fp_out)
+{
+    int n = M->n;
+    int m = last - first + 1;
+    size_t nm = (size_t)n * (size_t)m;
+    double *Bn = (double *)calloc(nm, sizeof(double));
+    double *Y = (double *)calloc(nm, sizeof(double));
+    double *h = (double *)calloc((size_t)m * m, sizeof(double));
+    int status = ORC_OK;
+    int done = 0;
+    double fp = NAN;
+
+    memset(B, 0, nm * sizeof(double));
+    for (int c = 0; c < m; ++c) {
+        B[(size_t)(first + c) + (size_t)c * n] = 1.0;
+    }
+    for (int it = 0; it < maxiter; ++it) {
+        int singular = 0;
+        double dn = 0.0, bn = 0.0;
+        orc_apply(M, B, m, Y);
+        for (int c = 0; c < m; ++c) {
+            for (int r = 0; r < m; ++r) {
+                h[r + c * m] = Y[(size_t)(first + r) + (size_t)c * n];
+            }
+        }
+#pragma omp parallel for schedule(static) reduction(| : singular)
+        for (int row = 0; row < n; ++row) {
+            double aug[65 * 64];
+            if (row >= first && row <= last) {
+                for (int c = 0; c < m; ++c) {
+                    Bn[(size_t)row + (size_t)c * n] = (row == first + c) ? 1.0 : 0.0;
+                }
+                continue;
+            }
+            for (int e = 0; e < m; ++e) {
+                for (int u = 0; u < m; ++u) {
+                    aug[e * (m + 1) + u] = (e == u ? diag[row] : 0.0) - h[u + e * m];
+                }
+                aug[e * (m + 1) + m] = diag[row] * B[(size_t)row + (size_t)e * n] -
+                                       Y[(size_t)row + (size_t)e * n];
+            }
+            if (!orc_gauss_jordan(aug, m)) {
+                singular |= 1;
+                continue;
+            }
+            for (int c = 0; c < m; ++c) {
+                double val = aug[c * (m + 1) + m];
+                if (damping < 1.0) {
+                    val = (1.0 - damping) * B[(size_t)row + (size_t)c * n] + damping * val;
+                }
+                if (!isfinite(val)) {
+                    singular |= 1;
+                    val = 0.0;
+                }
+                Bn[(size_t)row + (size_t)c * n] = val;
+            }
+        }
+        for (size_t i = 0; i < nm; ++i) {
+            double dl = Bn[i] - B[i];
+            dn += dl * dl;
+            bn += B[i] * B[i];
+        }
+        dn = sqrt(dn);
+        bn = sqrt(bn);
+        if (singular || !isfinite(dn) || !isfinite(bn) || bn <= 0.0) {
+            status = ORC_INVALID;
+            break;
+        }
+        memcpy(B, Bn, nm * sizeof(double));
+        done = it + 1;
+        fp = dn / fmax(1.0, bn);
+        if (fp <= tol) {
+            break;
+        }
+    }
+    if (iters_out) *iters_out = done;
+    if (fp_out) *fp_out = fp;
+    free(Bn);
+    free(Y);
+    free(h);
+    return status;
+}
+
+int orc_block_cluster_solve(const double *dense, const int *cp, const int *ri,
+                            const double *v, int n, int first, int last, int maxiter,
+                            double tol, double damping, double *B, int *iters,
+                            double *fp)
+{
+    OrcMat M;
+    double *diag;
+    int st;
+    if (first < 0 || last < first || last >= n || last - first + 1 > 64) {
+        return ORC_INVALID;
+    }
+    orc_mat_init(&M, dense, cp, ri, v, n);
+    diag = (double *)malloc((size_t)n * sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        diag[i] = orc_mat_diag(&M, i);
+    }
+    st = orc_block_solve_core(&M, diag, first, last, maxiter, tol, damping, B, iters, fp);
+    free(diag);
+    return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* dense small symmetric eigenproblem (cyclic Jacobi) and Cholesky RR  */
+/* ------------------------------------------------------------------ */
+
+/* Cyclic Jacobi: A (m x m column-major, symmetric) is destroyed; values
+ * ascending, vectors column-major with columns sorted like the values. */
+void orc_sym_eig(double *A, int m, double *w, double *V)
+{
+    for (int i = 0; i < m * m; ++i) V[i] = 0.0;
+    for (int i = 0; i < m; ++i) V[i + i * m] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int c = 0; c < m; ++c) {
+            for (int r = 0; r < m; ++r) {
+                double a = A[r + c * m] * A[r + c * m];
+                tot += a;
+                if (r != c) off += a;
+            }
+        }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < m - 1; ++p) {
+            for (int q = p + 1; q < m; ++q) {
+                double apq = A[p + q * m];
+                double app, aqq, theta, t, c, s;
+                if (apq == 0.0) continue;
+                app = A[p + p * m];
+                aqq = A[q + q * m];
+                theta = (aqq - app) / (2.0 * apq);
+                t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                c = 1.0 / sqrt(t * t + 1.0);
+                s = t * c;
+                for (int i = 0; i < m; ++i) {
+                    double aip = A[i + p * m], aiq = A[i + q * m];
+                    A[i + p * m] = c * aip - s * aiq;
+                    A[i + q * m] = s * aip + c * aiq;
+                }
+                for (int i = 0; i < m; ++i) {
+                    double api = A[p + i * m], aqi = A[q + i * m];
+                    A[p + i * m] = c * api - s * aqi;
+                    A[q + i * m] = s * api + c * aqi;
+                }
+                for (int i = 0; i < m; ++i) {
+                    double vip = V[i + p * m], viq = V[i + q * m];
+                    V[i + p * m] = c * vip - s * viq;
+                    V[i + q * m] = s * vip + c * viq;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < m; ++i) w[i] = A[i + i * m];
+    /* selection sort ascending (stable for ties) */
+    for (int i = 0; i < m; ++i) {
+        int b = i;
+        for (int j = i + 1; j < m; ++j) {
+            if (w[j] < w[b]) b = j;
+        }
+        if (b != i) {
+            double t = w[i];
+            w[i] = w[b];
+            w[b] = t;
+            for (int r = 0; r < m; ++r) {
+                double tv = V[r + i * m];
+                V[r + i * m] = V[r + b * m];
+                V[r + b * m] = tv;
+            }
+        }
+    }
+}
+
+/*
+ * Cholesky Rayleigh-Ritz of basis B (n x bc): Gram G = B^T B, P = B^T H B,
+ * G = L L^T, eig of L^-1 P L^-T (symmetrised), Ritz vectors B L^-T C,
+ * normalised columns, lowest k.  CPU twin IPT/src/ipt_cuda.cu:4388-4513
+ * (Cholesky pivot floor 1e-14 at 4006-4037).  HB may be passed (NULL =>
+ * computed).  Returns ORC_INVALID when the Gram matrix is not positive
+ * definite.
+ */
+static int orc_rr_core(const OrcMat *M, const double *B, const double *HBin, int bc, int k,
+                       double *values, double *vectors)
+{
+    int n = M->n;
+    size_t nb = (size_t)n * bc;
+    double *HB = (double *)malloc(nb * sizeof(double));
+    double *G = (double *)calloc((size_t)bc * bc, sizeof(double));
+    double *P = (double *)calloc((size_t)bc * bc, sizeof(double));
+    double *L = (double *)calloc((size_t)bc * bc, sizeof(double));
+    double *Li = (double *)calloc((size_t)bc * bc, sizeof(double));
+    double *T = (double *)calloc((size_t)bc * bc, sizeof(double));
+    double *w = (double *)calloc((size_t)bc, sizeof(double));
+    double *C = (double *)calloc((size_t)bc * bc, sizeof(double));
+    double *W = (double *)calloc((size_t)bc * k, sizeof(double));
+    int status = ORC_OK;
+
+    if (HBin != NULL) {
+        memcpy(HB, HBin, nb * sizeof(double));
+    } else {
+        orc_apply(M, B, bc, HB);
+    }
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < bc; ++c) {
+        for (int r = 0; r < bc; ++r) {
+            double g = 0.0, p = 0.0;
+            const double *br = B + (size_t)r * n;
+            const double *bcol = B + (size_t)c * n;
+            const double *hc = HB + (size_t)c * n;
+            for (int i = 0; i < n; ++i) {
+                g += br[i] * bcol[i];
+                p += br[i] * hc[i];
+            }
+            G[r + c * bc] = g;
+            P[r + c * bc] = p;
+        }
+    }
+    /* Cholesky lower */
+    for (int c = 0; c < bc; ++c) {
+        for (int r = c; r < bc; ++r) {
+            double s = G[r + c * bc];
+            for (int i = 0; i < c; ++i) s -= L[r + i * bc] * L[c + i * bc];
+            if (r == c) {
+                if (s <= 1.0e-14 || !isfinite(s)) {
+                    status = ORC_INVALID;
+                    goto out;
+                }
+                L[r + c * bc] = sqrt(s);
+            } else {
+                L[r + c * bc] = s / L[c + c * bc];
+            }
+        }
+    }
+    /* L^-1 by forward substitution */
+    for (int c = 0; c < bc; ++c) {
+        for (int r = 0; r < bc; ++r) {
+            double s = (r == c) ? 1.0 : 0.0;
+            if (r < c) {
+                Li[r + c * bc] = 0.0;
+                continue;
+            }
+            for (int i = c; i < r; ++i) s -= L[r + i * bc] * Li[i + c * bc];
+            Li[r + c * bc] = s / L[r + r * bc];
+        }
+    }
+    /* T = Li P Li^T */
+    {
+        double *tmp = (double *)calloc((size_t)bc * bc, sizeof(double));
+        for (int c = 0; c < bc; ++c)
+            for (int r = 0; r < bc; ++r) {
+                double s = 0.0;
+                for (int i = 0; i < bc; ++i) s += Li[r + i * bc] * P[i + c * bc];
+                tmp[r + c * bc] = s;
+            }
+        for (int c = 0; c < bc; ++c)
+            for (int r = 0; r < bc; ++r) {
+                double s = 0.0;
+                for (int i = 0; i < bc; ++i) s += tmp[r + i * bc] * Li[c + i * bc];
+                T[r + c * bc] = s;
+            }
+        free(tmp);
+    }
+    for (int c = 0; c < bc; ++c)
+        for (int r = 0; r < c; ++r) {
+            double s = 0.5 * (T[r + c * bc] + T[c + r * bc]);
+            T[r + c * bc] = s;
+            T[c + r * bc] = s;
+        }
+    orc_sym_eig(T, bc, w, C);
+    /* W = Li^T C(:, :k) */
+    for (int j = 0; j < k; ++j)
+        for (int r = 0; r < bc; ++r) {
+            double s = 0.0;
+            for (int i = 0; i < bc; ++i) s += Li[i + r * bc] * C[i + j * bc];
+            W[r + j * bc] = s;
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < k; ++j) {
+        double nrm = 0.0;
+        double *x = vectors + (size_t)j * n;
+        for (int r = 0; r < n; ++r) {
+            double s = 0.0;
+            for (int i = 0; i < bc; ++i) s += B[(size_t)r + (size_t)i * n] * W[i + j * bc];
+            x[r] = s;
+            nrm += s * s;
+        }
+        if (nrm > 0.0) {
+            double inv = 1.0 / sqrt(nrm);
+            for (int r = 0; r < n; ++r) x[r] *= inv;
+        }
+    }
+    for (int j = 0; j < k; ++j) values[j] = w[j];
+out:
+    free(HB);
+    free(G);
+    free(P);
+    free(L);
+    free(Li);
+    free(T);
+    free(w);
+    free(C);
+    free(W);
+    return status;
+}
+
+int orc_rayleigh_ritz(const double *dense, const int *cp, const int *ri, const double *v,
+                      int n, const double *B, int bc, int k, double *values,
+                      double *vectors)
+{
+    OrcMat M;
+    if (bc < k || k <= 0) return ORC_INVALID;
+    orc_mat_init(&M, dense, cp, ri, v, n);
+    return orc_rr_core(&M, B, NULL, bc, k, values, vectors);
+}
+
+/* Scaled residuals ||Hx - theta x|| / (max(1,normH) max(1,||x||)),
+ * IPT/src/ipt_cuda.cu:2684-2697. */
+static void orc_residuals_core(const OrcMat *M, const double *X, const double *theta, int k,
+                               double normH, double *res)
+{
+    int n = M->n;
+    double *HX = (double *)malloc((size_t)n * k * sizeof(double));
+    orc_apply(M, X, k, HX);
+    for (int j = 0; j < k; ++j) {
+        double rn = 0.0, xn = 0.0;
+        for (int r = 0; r < n; ++r) {
+            double x = X[(size_t)r + (size_t)j * n];
+            double rr = HX[(size_t)r + (size_t)j * n] - theta[j] * x;
+            rn += rr * rr;
+            xn += x * x;
+        }
+        res[j] = sqrt(rn) / (fmax(1.0, normH) * fmax(1.0, sqrt(xn)));
+    }
+    free(HX);
+}
+
+void orc_residuals(const double *dense, const int *cp, const int *ri, const double *v, int n,
+                   const double *X, const double *theta, int k, double normH, double *res)
+{
+    OrcMat M;
+    orc_mat_init(&M, dense, cp, ri, v, n);
+    orc_residuals_core(&M, X, theta, k, normH, res);
+}
+
+/* ------------------------------------------------------------------ */
+/* Davidson refinement                                                 */
+/* ------------------------------------------------------------------ */
+
+/* CGS2 projection + normalisation of one direction against the first
+ * bc columns of B, IPT/src/ipt_cuda.cu:2711-2780.  Returns 1 if accepted. */
+static int orc_project_normalize(double *t, const double *B, int n, int bc, int repeats)
+{
+    double before = 0.0, after = 0.0;
+    double *coef = (double *)malloc((size_t)(bc > 0 ? bc : 1) * sizeof(double));
+    for (int r = 0; r < n; ++r) before += t[r] * t[r];
+    before = sqrt(before);
+    if (!isfinite(before) || before <= DBL_MIN) {
+        free(coef);
+        return 0;
+    }
+    for (int pass = 0; pass < repeats && bc > 0; ++pass) {
+        for (int j = 0; j < bc; ++j) {
+            double s = 0.0;
+            const double *b = B + (size_t)j * n;
+            for (int r = 0; r < n; ++r) s += b[r] * t[r];
+            coef[j] = s;
+        }
+        for (int j = 0; j < bc; ++j) {
+            const double *b = B + (size_t)j * n;
+            for (int r = 0; r < n; ++r) t[r] -= b[r] * coef[j];
+        }
+    }
+    free(coef);
+    for (int r = 0; r < n; ++r) after += t[r] * t[r];
+    after = sqrt(after);
+    if (!isfinite(after) || after <= DBL_MIN) return 0;
+    for (int r = 0; r < n; ++r) t[r] /= after;
+    return 1;
+}
+
+/*
+ * Residual-only Davidson refinement after the block RR,
+ * IPT/src/ipt_cuda.cu:3318-3982 (HEAD variant):
+ *   targets   = pairs with scaled residual > conv_tol (780-796)
+ *   clip      = eps^(2/3) * max(1, |normH|, max|theta|)        (3389-3403)
+ *   correction t_i = r_i / clip(D_i - theta)                   (2449-2465)
+ *   kmax      = min(n, k + max(16, (2k+2)/3)), retain = kmax-k-1 (3333-3336)
+ *   pre-restart when basis > k and basis+|targets| > kmax      (3493-3514)
+ *   trial RR over the extended basis, accept iff finite        (1200-1225)
+ *   restart at >= kmax: k Ritz vectors + retained corrections  (2782-2869)
+ * Inputs: combined basis B (n x bc) and its current Ritz pairs.
+ * Outputs overwrite values/vectors; *steps_done and accepted counts.
+ */
+int orc_davidson(const double *dense, const int *cp, const int *ri, const double *v, int n,
+                 int k, const double *B0, int bc0, double *values, double *vectors,
+                 int steps, double conv_tol, int *accepted_out, int *restarts_out,
+                 int *basis_cols_out, long long *matvecs_out)
+{
+    OrcMat M;
+    double normH, clip, eps23 = pow(DBL_EPSILON, 2.0 / 3.0);
+    int kmax = k + ((2 * k + 2) / 3 > 16 ? (2 * k + 2) / 3 : 16);
+    int retain_limit;
+    int cap;
+    double *basis, *trial, *tvals, *tvecs, *res, *tres, *recent, *dg, *HX, *corr;
+    int bc = bc0, recent_count = 0, accepted_steps = 0, restarts = 0;
+    long long matvecs = 0;
+    int status = ORC_OK;
+
+    orc_mat_init(&M, dense, cp, ri, v, n);
+    if (kmax > n) kmax = n;
+    retain_limit = kmax - k - 1 > 0 ? kmax - k - 1 : 0;
+    cap = (bc0 > kmax ? bc0 : kmax) + k + 1;
+    normH = orc_mat_inf_norm(&M);
+    basis = (double *)calloc((size_t)n * cap, sizeof(double));
+    trial = (double *)calloc((size_t)n * cap, sizeof(double));
+    tvals = (double *)calloc((size_t)k, sizeof(double));
+    tvecs = (double *)calloc((size_t)n * k, sizeof(double));
+    res = (double *)calloc((size_t)k, sizeof(double));
+    tres = (double *)calloc((size_t)k, sizeof(double));
+    recent = (double *)calloc((size_t)n * (k + 1), sizeof(double));
+    dg = (double *)calloc((size_t)n, sizeof(double));
+    HX = (double *)calloc((size_t)n * k, sizeof(double));
+    corr = (double *)calloc((size_t)n, sizeof(double));
+    memcpy(basis, B0, (size_t)n * bc0 * sizeof(double));
+    for (int i = 0; i < n; ++i) dg[i] = orc_mat_diag(&M, i);
+    {
+        double scale = fmax(1.0, fabs(normH));
+        for (int j = 0; j < k; ++j)
+            if (isfinite(values[j])) scale = fmax(scale, fabs(values[j]));
+        clip = eps23 * scale;
+        if (!isfinite(clip) || clip <= 0.0) clip = eps23;
+    }
+    orc_residuals_core(&M, vectors, values, k, normH, res);
+    if (steps < 1) steps = 1;
+    for (int step = 1; step <= steps; ++step) {
+        int ncand = 0, nacc = 0, round_limit, tb;
+        int cand[4096];
+        for (int j = 0; j < k; ++j)
+            if (!isfinite(res[j]) || res[j] > conv_tol) cand[ncand++] = j;
+        if (ncand == 0) break;
+        if (bc > k && bc + ncand > kmax) {
+            /* thick restart, 2782-2869 */
+            int rb = k;
+            int lim = recent_count < retain_limit ? recent_count : retain_limit;
+            memcpy(basis, vectors, (size_t)n * k * sizeof(double));
+            for (int q = 0; q < lim; ++q) {
+                memcpy(corr, recent + (size_t)q * n, (size_t)n * sizeof(double));
+                if (orc_project_normalize(corr, basis, n, rb, 2)) {
+                    memcpy(basis + (size_t)rb * n, corr, (size_t)n * sizeof(double));
+                    ++rb;
+                }
+            }
+            bc = rb;
+            ++restarts;
+        }
+        round_limit = kmax - bc > 0 ? kmax - bc : 0;
+        if (round_limit > k) round_limit = k;
+        memcpy(trial, basis, (size_t)n * bc * sizeof(double));
+        tb = bc;
+        /* residual vectors of current pairs */
+        orc_apply(&M, vectors, k, HX);
+        for (int ci = 0; ci < ncand && nacc < round_limit; ++ci) {
+            int j = cand[ci];
+            double th = values[j];
+            for (int r = 0; r < n; ++r) {
+                double rr = HX[(size_t)r + (size_t)j * n] - th * vectors[(size_t)r + (size_t)j * n];
+                double den = dg[r] - th;
+                if (fabs(den) < clip) den = den < 0.0 ? -clip : clip;
+                corr[r] = rr / den;
+            }
+            if (!orc_project_normalize(corr, trial, n, tb, 2)) continue;
+            memcpy(trial + (size_t)tb * n, corr, (size_t)n * sizeof(double));
+            ++tb;
+            ++nacc;
+        }
+        if (nacc == 0) break;
+        matvecs += tb;
+        if (orc_rr_core(&M, trial, NULL, tb, k, tvals, tvecs) != ORC_OK) {
+            status = ORC_INVALID;
+            break;
+        }
+        orc_residuals_core(&M, tvecs, tvals, k, normH, tres);
+        {
+            int finite = 1;
+            for (int j = 0; j < k; ++j)
+                if (!isfinite(tres[j])) finite = 0;
+            if (!finite) break;
+        }
+        memcpy(basis, trial, (size_t)n * tb * sizeof(double));
+        recent_count = tb - bc;
+        memcpy(recent, trial + (size_t)bc * n, (size_t)n * recent_count * sizeof(double));
+        bc = tb;
+        memcpy(values, tvals, (size_t)k * sizeof(double));
+        memcpy(vectors, tvecs, (size_t)n * k * sizeof(double));
+        memcpy(res, tres, (size_t)k * sizeof(double));
+        ++accepted_steps;
+        if (bc >= kmax) {
+            int rb = k;
+            int lim = recent_count < retain_limit ? recent_count : retain_limit;
+            memcpy(basis, vectors, (size_t)n * k * sizeof(double));
+            for (int q = 0; q < lim; ++q) {
+                memcpy(corr, recent + (size_t)q * n, (size_t)n * sizeof(double));
+                if (orc_project_normalize(corr, basis, n, rb, 2)) {
+                    memcpy(basis + (size_t)rb * n, corr, (size_t)n * sizeof(double));
+                    ++rb;
+                }
+            }
+            bc = rb;
+            ++restarts;
+        }
+    }
+    if (accepted_out) *accepted_out = accepted_steps;
+    if (restarts_out) *restarts_out = restarts;
+    if (basis_cols_out) *basis_cols_out = bc;
+    if (matvecs_out) *matvecs_out = matvecs;
+    free(basis);
+    free(trial);
+    free(tvals);
+    free(tvecs);
+    free(res);
+    free(tres);
+    free(recent);
+    free(dg);
+    free(HX);
+    free(corr);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* full block-cluster pipeline: clusters -> per-cluster solve -> RR     */
+/* (-> optional Davidson).  Matrix must already have a sorted diagonal. */
+/* IPT/src/ipt_cuda.cu:4712-5016                                        */
+/* ------------------------------------------------------------------ */
+int orc_block_pipeline(const double *dense, const int *cp, const int *ri, const double *v,
+                       int n, int k, const double *opts5, int max_block, int maxiter,
+                       double tol, double damping, int davidson_steps, double conv_tol,
+                       double *values, double *vectors, int *basis_cols_out,
+                       int *max_iters_out, long long *matvecs_out)
+{
+    OrcMat M;
+    OrcDegOpts o;
+    int *ranges = (int *)malloc((size_t)4 * n * sizeof(int));
+    double *diag = (double *)malloc((size_t)n * sizeof(double));
+    int nr, too_large = 0, bc = 0, off = 0, maxit = 0, status = ORC_OK;
+    long long matvecs = 0;
+    double *B = NULL;
+
+    orc_mat_init(&M, dense, cp, ri, v, n);
+    o.absolute_gap = opts5[0];
+    o.eps_multiplier = opts5[1];
+    o.rel_gap_tol = opts5[2];
+    o.ratio_tol = opts5[3];
+    o.eta = opts5[4];
+    o.max_block = max_block;
+    for (int i = 0; i < n; ++i) diag[i] = orc_mat_diag(&M, i);
+    nr = orc_clusters_core(&M, diag, k, &o, ranges, &too_large);
+    if (too_large) {
+        status = ORC_INVALID;
+        goto out;
+    }
+    for (int i = 0; i < nr; ++i) bc += ranges[2 * i + 1] - ranges[2 * i] + 1;
+    if (bc < k) {
+        status = ORC_INVALID;
+        goto out;
+    }
+    B = (double *)calloc((size_t)n * bc, sizeof(double));
+    for (int i = 0; i < nr; ++i) {
+        int a = ranges[2 * i], b = ranges[2 * i + 1], it = 0;
+        double fp;
+        status = orc_block_solve_core(&M, diag, a, b, maxiter, tol, damping,
+                                      B + (size_t)off * n, &it, &fp);
+        if (status != ORC_OK) goto out;
+        if (it > maxit) maxit = it;
+        matvecs += (long long)it * (b - a + 1);
+        off += b - a + 1;
+    }
+    status = orc_rr_core(&M, B, NULL, bc, k, values, vectors);
+    matvecs += bc;
+    if (status != ORC_OK) goto out;
+    if (davidson_steps > 0) {
+        int acc, rs, bco;
+        long long mv = 0;
+        status = orc_davidson(dense, cp, ri, v, n, k, B, bc, values, vectors, davidson_steps,
+                              conv_tol, &acc, &rs, &bco, &mv);
+        matvecs += mv;
+    }
+out:
+    if (basis_cols_out) *basis_cols_out = bc;
+    if (max_iters_out) *max_iters_out = maxit;
+    if (matvecs_out) *matvecs_out = matvecs;
+    free(ranges);
+    free(diag);
+    free(B);
+    return status;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,131 @@
+"""Write the golden fixtures under tests/golden/ from data files the reference
+checkout holds (result tables and the H2O CSC fixture), and cluster ranges
+from the reference's own host code built from source in oracle/_ref.
+
+Run in the container where /root/reference exists:
+    python tests/golden/make_golden.py
+Only numbers are written (JSON); no reference source text is copied.
+"""
+import csv
+import json
+import os
+import re
+import sys
+
+REF = "/root/reference/IPT_C_GPU"
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def _write(name, obj):
+    path = os.path.join(HERE, name)
+    with open(path, "w") as f:
+        json.dump(obj, f, indent=1, sort_keys=True)
+    print("wrote", path)
+
+
+def runtests_kat():
+    """Dense KAT table: IPT/results/runtests_summary.csv (cases of
+    IPT/test_temp/runtests.cu:304-313, 20 iterations, tolerance 5e-9)."""
+    rows = []
+    with open(os.path.join(REF, "results/runtests_summary.csv")) as f:
+        for r in csv.DictReader(f):
+            rows.append({
+                "case": r["case"], "n": int(r["n"]), "k": int(r["k"]),
+                "symmetric": int(r["symmetric"]),
+                "vector_max_abs_diff": float(r["vector_max_abs_diff"]),
+                "value_max_abs_diff": float(r["value_max_abs_diff"]),
+                "max_relative_residual": float(r["max_relative_residual"]),
+                "status": r["status"],
+            })
+    _write("dense_kat.json", {"iterations": 20, "diff_tol": 5e-9, "residual_tol": 5e-9,
+                              "source": "IPT/results/runtests_summary.csv", "cases": rows})
+
+
+def sparse_kat():
+    """Sparse KAT table: IPT/results/sparse/sparse_runtests_summary.csv."""
+    rows = []
+    with open(os.path.join(REF, "results/sparse/sparse_runtests_summary.csv")) as f:
+        for r in csv.DictReader(f):
+            rows.append({k: (r[k] if k in ("case", "status") else float(r[k])) for k in r})
+    _write("sparse_kat.json", {"source": "IPT/results/sparse/sparse_runtests_summary.csv",
+                               "cases": rows})
+
+
+def dense_eps_sweep():
+    """splitmix64 sweep: IPT/results/gpu_cusolver_vs_ipt_results.csv with the
+    configuration of IPT/results/gpu_cusolver_vs_ipt_summary.txt (seed
+    20260603+N, eps 0.01, tol 1e-12, k = N)."""
+    rows = []
+    with open(os.path.join(REF, "results/gpu_cusolver_vs_ipt_results.csv")) as f:
+        for r in csv.DictReader(f):
+            rows.append({"n": int(r["N"]), "epsilon": float(r["epsilon"]),
+                         "ipt_iterations": int(r["ipt_iterations"]),
+                         "ipt_fixed_point_residual": float(r["ipt_fixed_point_residual"]),
+                         "residual_ipt_frobenius": float(r["residual_ipt"])})
+    _write("dense_eps_sweep.json", {"seed_base": 20260603, "tol": 1e-12, "maxiter": 1000,
+                                    "source": "IPT/results/gpu_cusolver_vs_ipt_results.csv",
+                                    "rows": rows})
+
+
+def h2o_csc():
+    """10x10 H2O FCI matrix W (IPT/test/sparse/h20_w_csc.h), numbers only."""
+    text = open(os.path.join(REF, "test/sparse/h20_w_csc.h")).read()
+
+    def arr(name, conv):
+        m = re.search(name + r"\[\d+\]\s*=\s*\{([^}]*)\}", text)
+        return [conv(t) for t in m.group(1).replace("\n", " ").split(",") if t.strip()]
+
+    n = int(re.search(r"H20_W_CSC_N\s*=\s*(\d+)", text).group(1))
+    obj = {"n": n, "col_ptr": arr("H20_W_CSC_COL_PTR", int),
+           "row_ind": arr("H20_W_CSC_ROW_IND", int),
+           "values": arr("H20_W_CSC_VALUES", float),
+           "k": 1, "iterations": 80, "diff_tol": 1e-8, "residual_tol": 1e-10,
+           "source": "IPT/test/sparse/h20_w_csc.h"}
+    assert len(obj["values"]) == obj["col_ptr"][-1]
+    _write("h2o_w_csc.json", obj)
+
+
+def cluster_ranges():
+    """Cluster ranges from the reference's ipt_target_degenerate_subspaces
+    compiled from source (oracle/_ref) on seeded inputs of this repo's
+    generators."""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    from oracle import oracle as orc
+
+    cases = []
+    specs = [
+        ("rand_dense_n512_k16_s0", 512, 16, 0, 0.0, 10.0, 0.05),
+        ("rand_dense_n1024_k32_s3", 1024, 32, 3, 0.0, 50.0, 0.1),
+        ("rand_dense_n2048_k16_s0", 2048, 16, 0, 0.0, 10.0, 0.05),
+        ("rand_dense_n2048_k64_s1", 2048, 64, 1, 0.0, 6.25, 0.1),
+        ("eps_dense_n256_k256", 256, 256, 20260603 + 256, None, None, 0.05),
+    ]
+    for name, n, k, seed, lo, hi, vs in specs:
+        if lo is None:
+            H = orc.eps_dense(n, vs, seed)
+        else:
+            H = orc.rand_dense(n, seed, lo, hi, vs)
+        cp = np.zeros(n + 1, dtype=np.int32)
+        nz = H != 0.0
+        cp[1:] = np.cumsum(nz.sum(axis=0))
+        ri = np.concatenate([np.nonzero(nz[:, c])[0] for c in range(n)]).astype(np.int32)
+        v = np.concatenate([H[nz[:, c], c] for c in range(n)])
+        got = orc.ref_target_clusters(cp, ri, v, k)
+        if got is None:
+            raise SystemExit("oracle/_ref not built: run make -C oracle ref")
+        ranges, too_large = got
+        cases.append({"name": name, "kind": "eps" if lo is None else "rand", "n": n, "k": k,
+                      "seed": seed, "lo": lo, "hi": hi, "vscale": vs,
+                      "ranges": ranges, "too_large": too_large})
+    _write("cluster_ranges.json", {"source": "IPT/src/ipt_preparation.cu:501-645 via oracle/_ref",
+                                   "cases": cases})
+
+
+if __name__ == "__main__":
+    runtests_kat()
+    sparse_kat()
+    dense_eps_sweep()
+    h2o_csc()
+    cluster_ranges()
