@@ -1,0 +1,238 @@
+"""Python mirror of the reference's C entry points (IPT/src/ipt_cuda.cu:262-308).
+
+Each function takes numpy arrays (column-major where the C ABI says so),
+calls libipt_hip.so, and returns numpy results plus the IPTCudaResult
+fields.  Device-memory entry points take raw device pointers; the small
+`hip` helper below (ctypes on the HIP runtime the library already loaded)
+allocates and copies device memory for callers without another framework.
+"""
+import ctypes
+
+import numpy as np
+
+from .abi import IptError, IPTCudaResult, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def result_to_dict(r):
+    out = {}
+    for name, _ in IPTCudaResult._fields_:
+        if name in ("vectors", "values", "davidson_history", "davidson_selection_history",
+                    "davidson_block_history"):
+            continue
+        v = getattr(r, name)
+        out[name] = v.decode(errors="replace") if isinstance(v, bytes) else v
+    return out
+
+
+def _finish(status, r, where):
+    if status != 0:
+        raise IptError(status, where)
+    n, k = r.n, r.k
+    vec = np.ctypeslib.as_array(r.vectors, shape=(n * k,)).copy().reshape((n, k), order="F")
+    val = np.ctypeslib.as_array(r.values, shape=(k,)).copy()
+    info = result_to_dict(r)
+    lib().ipt_cuda_free_result(ctypes.byref(r))
+    return vec, val, info
+
+
+# ------------------------------------------------------------------ dense host
+
+def ipt_cuda(H, k, iterations):
+    H = _f64(H)
+    r = IPTCudaResult()
+    st = lib().ipt_cuda(_d(H), H.shape[0], int(k), int(iterations), ctypes.byref(r))
+    return _finish(st, r, "ipt_cuda")
+
+
+def ipt_cuda_tol(H, k, tol, maxiter):
+    H = _f64(H)
+    r = IPTCudaResult()
+    st = lib().ipt_cuda_tol(_d(H), H.shape[0], int(k), float(tol), int(maxiter), ctypes.byref(r))
+    return _finish(st, r, "ipt_cuda_tol")
+
+
+def ipt_cuda_with_initial(H, Z0, k, iterations):
+    H, Z0 = _f64(H), _f64(Z0)
+    r = IPTCudaResult()
+    st = lib().ipt_cuda_with_initial(_d(H), _d(Z0), H.shape[0], int(k), int(iterations), ctypes.byref(r))
+    return _finish(st, r, "ipt_cuda_with_initial")
+
+
+def ipt_cuda_with_initial_tol(H, Z0, k, tol, maxiter):
+    H, Z0 = _f64(H), _f64(Z0)
+    r = IPTCudaResult()
+    st = lib().ipt_cuda_with_initial_tol(_d(H), _d(Z0), H.shape[0], int(k), float(tol), int(maxiter),
+                                         ctypes.byref(r))
+    return _finish(st, r, "ipt_cuda_with_initial_tol")
+
+
+# ------------------------------------------------------------------ sparse host
+
+def ipt_cuda_sparse_csc(col_ptr, row_ind, values, k, iterations):
+    cp, ri, v = _i32(col_ptr), _i32(row_ind), np.ascontiguousarray(values, dtype=np.float64)
+    n = len(cp) - 1
+    r = IPTCudaResult()
+    st = lib().ipt_cuda_sparse_csc(_i(cp), _i(ri), _d(v), n, int(k), int(cp[-1]), int(iterations),
+                                   ctypes.byref(r))
+    return _finish(st, r, "ipt_cuda_sparse_csc")
+
+
+def ipt_cuda_sparse_csc_tol(col_ptr, row_ind, values, k, tol, maxiter):
+    cp, ri, v = _i32(col_ptr), _i32(row_ind), np.ascontiguousarray(values, dtype=np.float64)
+    n = len(cp) - 1
+    r = IPTCudaResult()
+    st = lib().ipt_cuda_sparse_csc_tol(_i(cp), _i(ri), _d(v), n, int(k), int(cp[-1]), float(tol),
+                                       int(maxiter), ctypes.byref(r))
+    return _finish(st, r, "ipt_cuda_sparse_csc_tol")
+
+
+def ipt_cuda_sparse_csc_with_initial_tol(col_ptr, row_ind, values, Z0, k, tol, maxiter):
+    cp, ri, v = _i32(col_ptr), _i32(row_ind), np.ascontiguousarray(values, dtype=np.float64)
+    Z0 = _f64(Z0)
+    n = len(cp) - 1
+    r = IPTCudaResult()
+    st = lib().ipt_cuda_sparse_csc_with_initial_tol(_i(cp), _i(ri), _d(v), _d(Z0), n, int(k), int(cp[-1]),
+                                                    float(tol), int(maxiter), ctypes.byref(r))
+    return _finish(st, r, "ipt_cuda_sparse_csc_with_initial_tol")
+
+
+# ------------------------------------------------------------------ HIP plumbing
+
+class _Hip:
+    """Minimal ctypes view of the HIP runtime already loaded by the library."""
+
+    def __init__(self):
+        lib()  # make sure the runtime the library uses is the one we bind
+        h = ctypes.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipFree.argtypes = [ctypes.c_void_p]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        h.hipDeviceSynchronize.argtypes = []
+        h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        h.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        h.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+        h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        h.hipGetErrorString.restype = ctypes.c_char_p
+        self.h = h
+
+    def check(self, e, what):
+        if e != 0:
+            raise RuntimeError("%s: HIP error %d %s" % (what, e, self.h.hipGetErrorString(e).decode()))
+
+    def device_count(self):
+        c = ctypes.c_int(0)
+        e = self.h.hipGetDeviceCount(ctypes.byref(c))
+        return c.value if e == 0 else 0
+
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        self.check(self.h.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(int(nbytes), 1))), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        if p:
+            self.h.hipFree(ctypes.c_void_p(p))
+
+    def to_device(self, arr):
+        arr = np.asarray(arr)
+        p = self.malloc(arr.nbytes)
+        self.check(self.h.hipMemcpy(ctypes.c_void_p(p), arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, 1),
+                   "hipMemcpy H2D")
+        return p
+
+    def to_host(self, p, shape, dtype=np.float64, order="F"):
+        out = np.empty(shape, dtype=dtype, order=order)
+        self.check(self.h.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(p), out.nbytes, 2),
+                   "hipMemcpy D2H")
+        return out
+
+    def synchronize(self):
+        self.check(self.h.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        _hip = _Hip()
+    return _hip
+
+
+# ------------------------------------------------------------------ device
+
+def ipt_cuda_device_tol(d_H, n, k, tol, maxiter, handle=None):
+    """Device entry point; returns (d_vectors, d_values, iterations, fp)
+    as raw pointers (free with free_device_result)."""
+    dv, dl = ctypes.c_void_p(), ctypes.c_void_p()
+    it = ctypes.c_int(0)
+    fp = ctypes.c_double(0.0)
+    st = lib().ipt_cuda_device_tol(ctypes.c_void_p(d_H), int(n), int(k), float(tol), int(maxiter),
+                                   handle, ctypes.byref(dv), ctypes.byref(dl), ctypes.byref(it),
+                                   ctypes.byref(fp))
+    if st != 0:
+        raise IptError(st, "ipt_cuda_device_tol")
+    return dv.value, dl.value, it.value, fp.value
+
+
+def ipt_cuda_device(d_H, n, k, iterations, handle=None):
+    dv, dl = ctypes.c_void_p(), ctypes.c_void_p()
+    st = lib().ipt_cuda_device(ctypes.c_void_p(d_H), int(n), int(k), int(iterations), handle,
+                               ctypes.byref(dv), ctypes.byref(dl))
+    if st != 0:
+        raise IptError(st, "ipt_cuda_device")
+    return dv.value, dl.value
+
+
+def ipt_cuda_sparse_csc_device_tol(d_cp, d_ri, d_v, n, k, nnz, tol, maxiter, handle=None):
+    dv, dl = ctypes.c_void_p(), ctypes.c_void_p()
+    it = ctypes.c_int(0)
+    fp = ctypes.c_double(0.0)
+    st = lib().ipt_cuda_sparse_csc_device_tol(ctypes.c_void_p(d_cp), ctypes.c_void_p(d_ri),
+                                              ctypes.c_void_p(d_v), int(n), int(k), int(nnz), float(tol),
+                                              int(maxiter), handle, ctypes.byref(dv), ctypes.byref(dl),
+                                              ctypes.byref(it), ctypes.byref(fp))
+    if st != 0:
+        raise IptError(st, "ipt_cuda_sparse_csc_device_tol")
+    return dv.value, dl.value, it.value, fp.value
+
+
+def free_device_result(d_vectors, d_values):
+    lib().ipt_cuda_free_device_result(ctypes.c_void_p(d_vectors), ctypes.c_void_p(d_values))
+
+
+def create_handle(stream=None):
+    h = ctypes.c_void_p()
+    st = lib().ipt_hip_create_handle(ctypes.byref(h), ctypes.c_void_p(stream) if stream else None)
+    if st != 0:
+        raise IptError(st, "ipt_hip_create_handle")
+    return h.value
+
+
+def destroy_handle(h):
+    return lib().ipt_hip_destroy_handle(ctypes.c_void_p(h))

@@ -1,0 +1,875 @@
+// Block-cluster IPT -> FP64 Rayleigh-Ritz -> Davidson refinement.
+//
+// Reference pipeline: IPT/src/ipt_cuda.cu:4712-5016 (sparse CSC, selected by
+// IPT_BLOCK_CLUSTER=1) with ipt_block_cluster_solve_basis_gpu (2871-3037),
+// ipt_block_cluster_rayleigh_ritz_gpu (3039-3316) and
+// ipt_block_davidson_enrich_gpu (3318-3982).
+//
+// MI355X design differences (results are the same up to rounding):
+//  * all clusters iterate together: one H*B product per iteration over the
+//    concatenated cluster bases (the reference solves clusters one after
+//    another); a cluster that reaches its tolerance is frozen on the device
+//    and its columns are carried through unchanged, so every cluster stops
+//    at exactly the iteration the sequential solve would;
+//  * Rayleigh-Ritz keeps H*B from the last product instead of recomputing
+//    it; Davidson keeps H*V for the whole search space and multiplies only
+//    the new correction block by H;
+//  * Davidson corrections are orthogonalised as a block (two classical
+//    Gram-Schmidt passes against the basis, then a Cholesky QR of the block
+//    with rejection of numerically dependent directions) instead of one
+//    vector at a time.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cstring>
+#include <functional>
+
+#include "ipt_block.h"
+#include "ipt_prep.h"
+
+namespace ipt {
+
+namespace {
+
+double seconds_since(const std::chrono::steady_clock::time_point &t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Y(:, 0:nc) = H X(:, 0:nc), both column-major with leading dimension n.
+using ApplyFn = std::function<int(const double *X, double *Y, int nc)>;
+
+struct BlockParams {
+    int maxiter = 100;
+    double tol = 1e-12;
+    double damping = 1.0;
+    int davidson = 0;
+    int davidson_steps = 1;
+    int davidson_extra = 0;
+    double conv_tol = 1e-13;
+};
+
+// Rayleigh-Ritz over V (n x cols) with HV = H V: generalized problem
+// (V^T H V) c = theta (V^T V) c (Cholesky-based, rocsolver dsygvd as the
+// reference's cusolverDnDsygvd); X = V C(:, :k), HX = HV C(:, :k), columns
+// of X normalised (HX scaled alike).  theta gets all `cols` values.
+int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int cols, int k, double *theta,
+                  double *X, double *HX)
+{
+    double *G = nullptr;
+    IPT_STCHK(context_scratch(ctx, SLOT_RR, (size_t)(2 * cols * cols + cols + 64) * sizeof(double), (void **)&G));
+    double *P = G + (size_t)cols * cols;
+    double *E = P + (size_t)cols * cols;
+    int *info = reinterpret_cast<int *>(E + cols);
+    IPT_STCHK(gemm_tn_tall(ctx, V, n, V, n, G, cols, n, cols, cols));
+    IPT_STCHK(gemm_tn_tall(ctx, V, n, HV, n, P, cols, n, cols, cols));
+    void *blas = nullptr;
+    IPT_STCHK(context_blas(ctx, &blas));
+    rocblas_status rs = rocsolver_dsygvd((rocblas_handle)blas, rocblas_eform_ax, rocblas_evect_original,
+                                         rocblas_fill_lower, cols, P, cols, G, cols, theta, E, info);
+    if (rs != rocblas_status_success) {
+        fprintf(stderr, "IPT Rayleigh-Ritz: rocsolver_dsygvd status %d\n", (int)rs);
+        return IPT_CUDA_CUSOLVER_ERROR;
+    }
+    int hinfo = 0;
+    IPT_HIPCHK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (hinfo != 0) {
+        fprintf(stderr, "IPT Rayleigh-Ritz failed: info=%d\n", hinfo);
+        return IPT_CUDA_CUSOLVER_ERROR;
+    }
+    IPT_STCHK(gemm_nn_small(ctx, V, n, P, cols, X, n, n, cols, k, 1.0, 0.0));
+    IPT_STCHK(gemm_nn_small(ctx, HV, n, P, cols, HX, n, n, cols, k, 1.0, 0.0));
+    double *sq = E;  // reuse (cols >= k)
+    IPT_STCHK(column_sqnorms(ctx, X, n, n, k, sq));
+    IPT_STCHK(scale_columns_inv_sqrt(ctx, X, HX, n, n, k, sq));
+    return IPT_CUDA_SUCCESS;
+}
+
+int read_host(Context *ctx, void *dst, const void *src, size_t bytes)
+{
+    IPT_HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
+
+// Project the block T (n x t) against the first `cols` columns of V (two
+// classical Gram-Schmidt passes) and orthonormalise it with a Cholesky QR;
+// directions whose norm after projection falls below 1e-10 of their norm
+// before are dropped.  Returns the kept count in *kept; kept columns are
+// compacted to the front of T.  IPT/src/ipt_cuda.cu:2711-2780 (per vector).
+int project_orthonormalize(Context *ctx, const double *V, int n, int cols, double *T, int t, int *kept,
+                           double *work /* n x t */)
+{
+    *kept = 0;
+    if (t <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    double *C = nullptr;
+    const size_t cwords = (size_t)std::max(cols, t) * (size_t)t;
+    IPT_STCHK(context_scratch(ctx, SLOT_ORTHO, (cwords + (size_t)t) * sizeof(double), (void **)&C));
+    double *sq = C + cwords;
+    std::vector<double> before((size_t)t);
+    IPT_STCHK(column_sqnorms(ctx, T, n, n, t, sq));
+    IPT_STCHK(read_host(ctx, before.data(), sq, (size_t)t * sizeof(double)));
+    if (cols > 0) {
+        for (int pass = 0; pass < 2; ++pass) {
+            IPT_STCHK(gemm_tn_tall(ctx, V, n, T, n, C, cols, n, cols, t));
+            IPT_STCHK(gemm_nn_small(ctx, V, n, C, cols, T, n, n, cols, t, -1.0, 1.0));
+        }
+    }
+    // Gram of the projected block, Cholesky with rejection on the host
+    IPT_STCHK(gemm_tn_tall(ctx, T, n, T, n, C, t, n, t, t));
+    std::vector<double> g((size_t)t * t);
+    IPT_STCHK(read_host(ctx, g.data(), C, g.size() * sizeof(double)));
+    // sequential Gram-Schmidt in coefficient space: L (t x kept) such that
+    // Q = T W with W = L^-T restricted to kept columns.
+    std::vector<int> keep;
+    std::vector<double> L((size_t)t * t, 0.0);  // column-major, rows = original index
+    for (int j = 0; j < t; ++j) {
+        // r_jj^2 = g_jj - sum_kept l_jk^2 ; l_ik = (g_ik - sum ...)/l_kk
+        std::vector<double> col((size_t)t, 0.0);
+        for (int i = 0; i < t; ++i) {
+            col[(size_t)i] = g[(size_t)i + (size_t)j * t];
+        }
+        for (size_t q = 0; q < keep.size(); ++q) {
+            const double ljq = L[(size_t)j + q * t];
+            for (int i = 0; i < t; ++i) {
+                col[(size_t)i] -= L[(size_t)i + q * t] * ljq;
+            }
+        }
+        const double d2 = col[(size_t)j];
+        const double b = before[(size_t)j];
+        if (!(d2 > 0.0) || !std::isfinite(d2) || !std::isfinite(b) || b <= DBL_MIN ||
+            std::sqrt(d2) <= 1e-10 * std::sqrt(b)) {
+            continue;
+        }
+        const double djj = std::sqrt(d2);
+        const size_t q = keep.size();
+        for (int i = 0; i < t; ++i) {
+            L[(size_t)i + q * t] = col[(size_t)i] / djj;
+        }
+        keep.push_back(j);
+    }
+    const int kc = (int)keep.size();
+    *kept = kc;
+    if (kc == 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    // Lk = rows `keep` of L (kc x kc lower triangular); W (t x kc) = S Lk^-T
+    // where S selects the kept columns of T.
+    std::vector<double> Lk((size_t)kc * kc, 0.0);
+    for (int a = 0; a < kc; ++a) {
+        for (int b = 0; b < kc; ++b) {
+            Lk[(size_t)a + (size_t)b * kc] = L[(size_t)keep[(size_t)a] + (size_t)b * t];
+        }
+    }
+    // inverse of lower triangular Lk
+    std::vector<double> Li((size_t)kc * kc, 0.0);
+    for (int c = 0; c < kc; ++c) {
+        for (int r = c; r < kc; ++r) {
+            double s = (r == c) ? 1.0 : 0.0;
+            for (int i = c; i < r; ++i) {
+                s -= Lk[(size_t)r + (size_t)i * kc] * Li[(size_t)i + (size_t)c * kc];
+            }
+            Li[(size_t)r + (size_t)c * kc] = s / Lk[(size_t)r + (size_t)r * kc];
+        }
+    }
+    std::vector<double> W((size_t)t * kc, 0.0);
+    for (int a = 0; a < kc; ++a) {
+        for (int b = 0; b < kc; ++b) {
+            // W(keep[a], b) = (Li^T)(a, b) = Li(b, a)
+            W[(size_t)keep[(size_t)a] + (size_t)b * t] = Li[(size_t)b + (size_t)a * kc];
+        }
+    }
+    IPT_HIPCHK(hipMemcpyAsync(C, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    IPT_STCHK(gemm_nn_small(ctx, T, n, C, t, work, n, n, t, kc, 1.0, 0.0));
+    IPT_HIPCHK(hipMemcpyAsync(T, work, (size_t)n * kc * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
+
+struct DavidsonStats {
+    int attempted = 0;
+    int accepted_steps = 0;
+    int rejected_steps = 0;
+    int restarts = 0;
+    double residual_before = NAN;
+    double residual_after = NAN;
+    int target_index = -1;
+    double clip = NAN;
+    long long matvecs = 0;
+    int basis_cols = 0;
+};
+
+double max_of(const std::vector<double> &v, int *idx)
+{
+    double m = NAN;
+    *idx = -1;
+    for (size_t i = 0; i < v.size(); ++i) {
+        if (!std::isfinite(m) || v[i] > m) {
+            m = v[i];
+            *idx = (int)i;
+        }
+    }
+    return m;
+}
+
+// Residual-only Davidson refinement, IPT/src/ipt_cuda.cu:3318-3982 (HEAD):
+// corrections for every pair above conv_tol, thick restart to the k Ritz
+// vectors plus the last correction block, trial accepted when finite.
+// V/HV: n x cap buffers holding the search space (first *cols columns).
+int davidson(Context *ctx, const ApplyFn &apply, const double *d, int n, int k, double normH, double *V,
+             double *HV, int *cols_io, int cap, double *theta, double *X, double *HX, const BlockParams &p,
+             DavidsonStats *st)
+{
+    const int steps = std::max(1, std::max(0, p.davidson_steps) + std::max(0, p.davidson_extra));
+    int kmax = std::min(n, k + std::max(16, (2 * k + 2) / 3));
+    if (kmax > cap) {
+        kmax = cap;
+    }
+    const int retain = std::max(0, kmax - k - 1);
+    int cols = *cols_io;
+    st->attempted = 1;
+
+    DevBuf<double> R, T, HT, work, tX, tHX, ttheta, res, recent, hrecent;
+    DevBuf<int> idx;
+    IPT_STCHK(R.alloc((size_t)n * k));
+    IPT_STCHK(T.alloc((size_t)n * k));
+    IPT_STCHK(HT.alloc((size_t)n * k));
+    IPT_STCHK(work.alloc((size_t)n * std::max(k, retain + 1)));
+    IPT_STCHK(tX.alloc((size_t)n * k));
+    IPT_STCHK(tHX.alloc((size_t)n * k));
+    IPT_STCHK(ttheta.alloc((size_t)cap));
+    IPT_STCHK(res.alloc((size_t)k));
+    IPT_STCHK(recent.alloc((size_t)n * k));
+    IPT_STCHK(hrecent.alloc((size_t)n * k));
+    IPT_STCHK(idx.alloc((size_t)k));
+    int recent_count = 0;
+
+    std::vector<double> htheta((size_t)k), hres((size_t)k);
+    IPT_STCHK(read_host(ctx, htheta.data(), theta, (size_t)k * sizeof(double)));
+    {
+        double scale = std::max(1.0, std::fabs(normH));
+        for (double th : htheta) {
+            if (std::isfinite(th)) {
+                scale = std::max(scale, std::fabs(th));
+            }
+        }
+        st->clip = std::pow(DBL_EPSILON, 2.0 / 3.0) * scale;
+        if (!std::isfinite(st->clip) || st->clip <= 0.0) {
+            st->clip = std::pow(DBL_EPSILON, 2.0 / 3.0);
+        }
+    }
+    IPT_STCHK(residual_norms(ctx, X, HX, n, n, k, theta, normH, res.p, R.p));
+    IPT_STCHK(read_host(ctx, hres.data(), res.p, (size_t)k * sizeof(double)));
+    st->residual_before = max_of(hres, &st->target_index);
+
+    auto restart = [&]() -> int {
+        // basis = k Ritz vectors + up to `retain` previous corrections
+        IPT_HIPCHK(hipMemcpyAsync(V, X, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(HV, HX, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        int nc = k;
+        const int lim = std::min(retain, recent_count);
+        if (lim > 0) {
+            // project recent against X, orthonormalise; H*recent follows the
+            // same linear combination: recompute H on the kept block.
+            IPT_HIPCHK(hipMemcpyAsync(T.p, recent.p, (size_t)n * lim * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            int kept = 0;
+            IPT_STCHK(project_orthonormalize(ctx, V, n, nc, T.p, lim, &kept, work.p));
+            if (kept > 0) {
+                IPT_STCHK(apply(T.p, HT.p, kept));
+                st->matvecs += kept;
+                IPT_HIPCHK(hipMemcpyAsync(V + (size_t)nc * n, T.p, (size_t)n * kept * sizeof(double),
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+                IPT_HIPCHK(hipMemcpyAsync(HV + (size_t)nc * n, HT.p, (size_t)n * kept * sizeof(double),
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+                nc += kept;
+            }
+        }
+        cols = nc;
+        ++st->restarts;
+        return IPT_CUDA_SUCCESS;
+    };
+
+    for (int step = 1; step <= steps; ++step) {
+        std::vector<int> cand;
+        for (int j = 0; j < k; ++j) {
+            if (!std::isfinite(hres[(size_t)j]) || hres[(size_t)j] > p.conv_tol) {
+                cand.push_back(j);
+            }
+        }
+        if (cand.empty()) {
+            break;
+        }
+        if (cols > k && cols + (int)cand.size() > kmax) {
+            IPT_STCHK(restart());
+        }
+        const int round_limit = std::min(k, std::max(0, kmax - cols));
+        if ((int)cand.size() > round_limit) {
+            cand.resize((size_t)round_limit);
+        }
+        if (cand.empty()) {
+            ++st->rejected_steps;
+            break;
+        }
+        const int nt = (int)cand.size();
+        IPT_HIPCHK(hipMemcpyAsync(idx.p, cand.data(), (size_t)nt * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        IPT_STCHK(diag_correction(ctx, R.p, n, d, theta, idx.p, nt, st->clip, T.p, n, n));
+        int kept = 0;
+        IPT_STCHK(project_orthonormalize(ctx, V, n, cols, T.p, nt, &kept, work.p));
+        if (kept == 0) {
+            ++st->rejected_steps;
+            break;
+        }
+        IPT_STCHK(apply(T.p, HT.p, kept));
+        st->matvecs += kept;
+        IPT_HIPCHK(hipMemcpyAsync(V + (size_t)cols * n, T.p, (size_t)n * kept * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(HV + (size_t)cols * n, HT.p, (size_t)n * kept * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+        const int tcols = cols + kept;
+        int rst = rayleigh_ritz(ctx, V, HV, n, tcols, k, ttheta.p, tX.p, tHX.p);
+        if (rst != IPT_CUDA_SUCCESS) {
+            return rst;
+        }
+        std::vector<double> tres((size_t)k);
+        IPT_STCHK(residual_norms(ctx, tX.p, tHX.p, n, n, k, ttheta.p, normH, res.p, R.p));
+        IPT_STCHK(read_host(ctx, tres.data(), res.p, (size_t)k * sizeof(double)));
+        bool finite = true;
+        for (double r : tres) {
+            if (!std::isfinite(r)) {
+                finite = false;
+            }
+        }
+        if (!finite) {
+            // rejected trial: residual vectors R must be those of the
+            // current pairs again
+            IPT_STCHK(residual_norms(ctx, X, HX, n, n, k, theta, normH, res.p, R.p));
+            ++st->rejected_steps;
+            break;
+        }
+        cols = tcols;
+        IPT_HIPCHK(hipMemcpyAsync(X, tX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(HX, tHX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(theta, ttheta.p, (size_t)tcols * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(recent.p, T.p, (size_t)n * kept * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        recent_count = kept;
+        hres = tres;
+        ++st->accepted_steps;
+        if (cols >= kmax) {
+            IPT_STCHK(restart());
+        }
+    }
+    int mi = -1;
+    st->residual_after = max_of(hres, &mi);
+    st->basis_cols = cols;
+    *cols_io = cols;
+    return IPT_CUDA_SUCCESS;
+}
+
+}  // namespace
+
+struct BlockResult {
+    DevBuf<double> theta;  // >= k
+    DevBuf<double> X;      // n x k (sorted basis)
+    DevBuf<double> HX;     // n x k
+    int iterations = 0;
+    double fp_max = NAN;
+    int basis_cols = 0;
+    long long matvecs = 0;
+    double iteration_time = 0.0;
+    double rr_time = 0.0;
+    double davidson_time = 0.0;
+    DavidsonStats dav;
+};
+
+// Block-cluster IPT on a prepared (sorted-diagonal) operator.
+int block_pipeline(Context *ctx, const ApplyFn &apply, const double *d, int n, int k,
+                   const std::vector<std::pair<int, int>> &ranges, double normH, const BlockParams &p,
+                   BlockResult *out)
+{
+    BlockPlan plan;
+    IPT_STCHK(block_plan_create(ctx, ranges, n, &plan));
+    const int bc = plan.basis_cols;
+    if (bc < k) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int cap = bc;
+    if (p.davidson) {
+        const int kmax = std::min(n, k + std::max(16, (2 * k + 2) / 3));
+        cap = std::max(bc, kmax) + k + 1;
+        cap = std::min(cap, std::max(bc, n));
+    }
+    DevBuf<double> Ba, Bb, Y;
+    IPT_STCHK(Ba.alloc((size_t)n * cap));
+    IPT_STCHK(Bb.alloc((size_t)n * bc));
+    IPT_STCHK(Y.alloc((size_t)n * cap));
+    IPT_STCHK(block_identity(ctx, plan, Ba.p, n, n));
+    double *cur = Ba.p, *nxt = Bb.p;
+    auto t0 = std::chrono::steady_clock::now();
+    int done = 0;
+    for (int it = 0; it < p.maxiter; ++it) {
+        IPT_STCHK(apply(cur, Y.p, bc));
+        out->matvecs += bc;
+        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, p.tol));
+        std::swap(cur, nxt);
+        done = it + 1;
+        int flags[3];
+        IPT_STCHK(read_host(ctx, flags, plan.flags.p, 3 * sizeof(int)));
+        if (flags[1] != 0) {
+            fprintf(stderr,
+                    "IPT block-cluster: block solve failed (%s) at iteration %d; identity fallback is disabled\n",
+                    (flags[1] & 2) ? "singular" : "non-finite norm", it + 1);
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        if (flags[2] == 0) {
+            break;
+        }
+    }
+    (void)done;
+    // per-cluster iterations and residuals
+    {
+        std::vector<int> its((size_t)plan.nclusters);
+        std::vector<double> fps((size_t)plan.nclusters);
+        IPT_STCHK(read_host(ctx, its.data(), plan.iters.p, its.size() * sizeof(int)));
+        IPT_STCHK(read_host(ctx, fps.data(), plan.fp.p, fps.size() * sizeof(double)));
+        for (int c = 0; c < plan.nclusters; ++c) {
+            out->iterations = std::max(out->iterations, its[(size_t)c]);
+            if (std::isfinite(fps[(size_t)c]) && (!std::isfinite(out->fp_max) || fps[(size_t)c] > out->fp_max)) {
+                out->fp_max = fps[(size_t)c];
+            }
+        }
+    }
+    out->iteration_time = seconds_since(t0);
+    // the basis lives in `cur`; Davidson needs it in the big buffer Ba
+    if (cur != Ba.p) {
+        IPT_HIPCHK(hipMemcpyAsync(Ba.p, cur, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    Bb.release();
+    auto t1 = std::chrono::steady_clock::now();
+    IPT_STCHK(apply(Ba.p, Y.p, bc));
+    out->matvecs += bc;
+    IPT_STCHK(out->theta.alloc((size_t)cap));
+    IPT_STCHK(out->X.alloc((size_t)n * k));
+    IPT_STCHK(out->HX.alloc((size_t)n * k));
+    IPT_STCHK(rayleigh_ritz(ctx, Ba.p, Y.p, n, bc, k, out->theta.p, out->X.p, out->HX.p));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    out->rr_time = seconds_since(t1);
+    out->basis_cols = bc;
+    if (p.davidson) {
+        auto t2 = std::chrono::steady_clock::now();
+        int cols = bc;
+        IPT_STCHK(davidson(ctx, apply, d, n, k, normH, Ba.p, Y.p, &cols, cap, out->theta.p, out->X.p, out->HX.p, p,
+                           &out->dav));
+        out->matvecs += out->dav.matvecs;
+        out->basis_cols = cols;
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        out->davidson_time = seconds_since(t2);
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
+namespace {
+
+BlockParams params_from_env(int maxiter, double tol, int use_tol)
+{
+    // IPT/src/ipt_cuda.cu:4672-4680, 3330-3340, 4915
+    BlockParams p;
+    p.maxiter = env_int("IPT_BLOCK_CLUSTER_MAXITER", maxiter > 0 ? maxiter : 100);
+    if (p.maxiter <= 0) {
+        p.maxiter = 100;
+    }
+    p.tol = env_double("IPT_BLOCK_CLUSTER_TOL", use_tol ? tol : 1.0e-12);
+    p.damping = env_double("IPT_BLOCK_CLUSTER_DAMPING", 1.0);
+    p.davidson = env_flag("IPT_DAVIDSON_ENRICH");
+    p.davidson_steps = env_int("IPT_DAVIDSON_STEPS", 1);
+    p.davidson_extra = env_int("IPT_DAVIDSON_EXTRA_STEPS", 0);
+    p.conv_tol = env_double("IPT_DAVIDSON_CONVERGED_TOL", 1.0e-13);
+    return p;
+}
+
+void fill_result(IPTCudaResult *r, const BlockResult &b, int n, int k)
+{
+    r->n = n;
+    r->k = k;
+    r->iterations = b.iterations;
+    r->basis_cols = b.basis_cols;
+    r->matvecs = b.matvecs;
+    r->fixed_point_residual = b.fp_max;
+    r->iteration_time_sec = b.iteration_time;
+    r->rayleigh_ritz_time_sec = b.rr_time;
+    r->davidson_time_sec = b.davidson_time;
+    r->davidson_attempted = b.dav.attempted;
+    r->davidson_accepted = b.dav.accepted_steps > 0;
+    r->davidson_target_index = b.dav.target_index;
+    r->davidson_residual_before = b.dav.residual_before;
+    r->davidson_residual_after = b.dav.residual_after;
+    r->davidson_denom_clip = b.dav.clip;
+    r->davidson_restart_count = b.dav.restarts;
+    r->accepted_steps = b.dav.accepted_steps;
+    r->rejected_steps = b.dav.rejected_steps;
+}
+
+void result_init_block(IPTCudaResult *r)
+{
+    // IPT/src/ipt_cuda.cu:4745-4771
+    memset(r, 0, sizeof(*r));
+    r->fixed_point_residual = NAN;
+    r->basis_orthogonality_frobenius_error = NAN;
+    r->basis_orthogonality_max_abs_error = NAN;
+    r->ritz_vectors_orthogonality_frobenius_error = NAN;
+    r->ritz_vectors_orthogonality_max_abs_error = NAN;
+    r->davidson_target_index = -1;
+    r->davidson_residual_before = NAN;
+    r->davidson_residual_after = NAN;
+    r->davidson_denom_clip = NAN;
+    r->adaptive_coupling_tau = NAN;
+    r->adaptive_target_block_start = -1;
+    r->adaptive_target_block_end = -1;
+    r->best_so_far_enabled = 1;
+    r->best_so_far_step = -1;
+    snprintf(r->best_so_far_source, sizeof(r->best_so_far_source), "none");
+    r->best_so_far_max_residual = NAN;
+    r->best_so_far_max_residual_index = -1;
+    r->pair28_best_residual = NAN;
+    r->pair29_best_residual = NAN;
+    r->gap28_29 = NAN;
+    r->relative_gap28_29 = NAN;
+    r->cluster_residual_fro = NAN;
+    r->cluster_residual_max = NAN;
+}
+
+// Dense matrix on the device: GPU-side preparation (diagonal, symmetry,
+// off-diagonal strengths, infinity norm), cluster detection on the host
+// from the few coupling values it needs, permutation to sorted order when
+// the diagonal is not ascending.
+struct DensePrep {
+    std::vector<int> perm;              // perm[sorted] = original
+    bool permuted = false;
+    DevBuf<double> Hs;                  // permuted copy when needed
+    const double *H = nullptr;          // matrix used by the solver
+    DevBuf<double> d;                   // sorted diagonal (device)
+    DevBuf<int> dperm;
+    std::vector<double> hd;             // sorted diagonal (host)
+    double normH = 0.0;
+    std::vector<std::pair<int, int>> ranges;
+};
+
+struct WindowCoupling : CouplingSource {
+    int nn;
+    const std::vector<double> *dg;
+    const std::vector<double> *off;
+    std::vector<std::pair<long, double>> lut;  // key row*n+col -> value
+    int n() const override { return nn; }
+    double diag(int i) const override { return (*dg)[(size_t)i]; }
+    double value(int row, int col) const override
+    {
+        const long key = (long)row * nn + col;
+        auto it = std::lower_bound(lut.begin(), lut.end(), key,
+                                   [](const std::pair<long, double> &e, long kk) { return e.first < kk; });
+        if (it != lut.end() && it->first == key) {
+            return it->second;
+        }
+        return 0.0;
+    }
+    void offdiag_strengths(std::vector<double> *s) const override { *s = *off; }
+};
+
+int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr)
+{
+    IPT_STCHK(pr->d.alloc((size_t)n));
+    DevBuf<double> colsum, colmax;
+    DevBuf<int> asym;
+    IPT_STCHK(colsum.alloc((size_t)n));
+    IPT_STCHK(colmax.alloc((size_t)n));
+    IPT_STCHK(asym.alloc(1));
+    IPT_STCHK(dense_diagonal(ctx, dH, n, n, pr->d.p));
+    IPT_STCHK(dense_scan(ctx, dH, n, n, colsum.p, colmax.p, asym.p));
+    std::vector<double> hd((size_t)n), hsum((size_t)n), hmax((size_t)n);
+    int hasym = 0;
+    IPT_STCHK(read_host(ctx, hd.data(), pr->d.p, (size_t)n * sizeof(double)));
+    IPT_STCHK(read_host(ctx, hsum.data(), colsum.p, (size_t)n * sizeof(double)));
+    IPT_STCHK(read_host(ctx, hmax.data(), colmax.p, (size_t)n * sizeof(double)));
+    IPT_STCHK(read_host(ctx, &hasym, asym.p, sizeof(int)));
+    if (hasym) {
+        fprintf(stderr, "IPT block-cluster path currently requires a real symmetric matrix\n");
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    // symmetric: row sums == column sums
+    pr->normH = 0.0;
+    for (double s : hsum) {
+        pr->normH = std::max(pr->normH, s);
+    }
+    std::vector<int> inv;
+    stable_sort_permutation(hd, &pr->perm, &inv);
+    pr->permuted = false;
+    for (int i = 0; i < n; ++i) {
+        if (pr->perm[(size_t)i] != i) {
+            pr->permuted = true;
+            break;
+        }
+    }
+    IPT_STCHK(pr->dperm.alloc((size_t)n));
+    IPT_HIPCHK(hipMemcpyAsync(pr->dperm.p, pr->perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
+    std::vector<double> sd((size_t)n), soff((size_t)n);
+    for (int s = 0; s < n; ++s) {
+        sd[(size_t)s] = hd[(size_t)pr->perm[(size_t)s]];
+        soff[(size_t)s] = hmax[(size_t)pr->perm[(size_t)s]];
+    }
+    if (pr->permuted) {
+        IPT_STCHK(pr->Hs.alloc((size_t)n * n));
+        IPT_STCHK(dense_permute(ctx, dH, n, pr->Hs.p, n, n, pr->dperm.p));
+        IPT_HIPCHK(hipMemcpyAsync(pr->d.p, sd.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        pr->H = pr->Hs.p;
+    } else {
+        pr->H = dH;
+    }
+    pr->hd = sd;
+    // candidate (row, target) pairs of the detection windows
+    const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
+    const double eps = DBL_EPSILON;
+    double maxabs = 0.0;
+    for (double x : sd) {
+        maxabs = std::max(maxabs, std::fabs(x));
+    }
+    const double tau_num = o.eps_multiplier * eps * std::max(1.0, maxabs);
+    const double ratio_tol = std::max(o.ratio_tol, eps);
+    std::vector<int> rc;
+    for (int col = 0; col < k; ++col) {
+        const double dc = sd[(size_t)col];
+        const double scale = std::max(1.0, std::fabs(dc));
+        const double fl = std::max(std::max(o.absolute_gap, tau_num), o.rel_gap_tol * scale);
+        const double tau_c = std::max(fl, soff[(size_t)col] / ratio_tol);
+        const int lo = (int)(std::lower_bound(sd.begin(), sd.end(), dc - tau_c) - sd.begin());
+        const int hi = (int)(std::upper_bound(sd.begin(), sd.end(), dc + tau_c) - sd.begin());
+        for (int r = lo; r < hi; ++r) {
+            if (r != col) {
+                rc.push_back(r);
+                rc.push_back(col);
+            }
+        }
+    }
+    WindowCoupling wc;
+    wc.nn = n;
+    wc.dg = &sd;
+    wc.off = &soff;
+    const int npairs = (int)(rc.size() / 2);
+    if (npairs > 0) {
+        DevBuf<int> drc;
+        DevBuf<double> dv;
+        IPT_STCHK(drc.alloc(rc.size()));
+        IPT_STCHK(dv.alloc(rc.size()));
+        IPT_HIPCHK(hipMemcpyAsync(drc.p, rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        IPT_STCHK(gather_pairs(ctx, pr->H, n, drc.p, npairs, dv.p));
+        std::vector<double> hv(rc.size());
+        IPT_STCHK(read_host(ctx, hv.data(), dv.p, hv.size() * sizeof(double)));
+        for (int i = 0; i < npairs; ++i) {
+            const int r = rc[(size_t)2 * i], c = rc[(size_t)2 * i + 1];
+            wc.lut.emplace_back((long)r * n + c, hv[(size_t)2 * i]);
+            wc.lut.emplace_back((long)c * n + r, hv[(size_t)2 * i + 1]);
+        }
+        std::sort(wc.lut.begin(), wc.lut.end());
+    }
+    bool too_large = false;
+    pr->ranges = target_clusters(wc, sd, k, o, &too_large);
+    if (too_large) {
+        int worst = 0;
+        for (const auto &r : pr->ranges) {
+            worst = std::max(worst, r.second - r.first + 1);
+        }
+        fprintf(stderr, "IPT block-cluster invalid argument at block detection: block_size=%d max_block_size=%d\n",
+                worst, o.max_block);
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
+}  // namespace
+
+// Dense block-cluster solve on a device matrix; outputs in the caller's
+// row order.  Used by the dense host entry points and the device entry
+// points when IPT_BLOCK_CLUSTER=1.
+int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter, double tol, int use_tol,
+                       double **d_vec_out, double **d_val_out, IPTCudaResult *info)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    DensePrep pr;
+    IPT_STCHK(dense_prepare(ctx, dH, n, k, &pr));
+    const double prep = seconds_since(t0);
+    const BlockParams p = params_from_env(maxiter, tol, use_tol);
+    const double *H = pr.H;
+    ApplyFn apply = [&](const double *X, double *Y, int nc) { return gemm_f64(ctx, H, n, X, n, Y, n, n, nc); };
+    BlockResult br;
+    IPT_STCHK(block_pipeline(ctx, apply, pr.d.p, n, k, pr.ranges, pr.normH, p, &br));
+    DevBuf<double> vec, val;
+    IPT_STCHK(vec.alloc((size_t)n * k));
+    IPT_STCHK(val.alloc((size_t)k));
+    if (pr.permuted) {
+        IPT_STCHK(unpermute_rows(ctx, br.X.p, vec.p, n, n, k, pr.dperm.p));
+    } else {
+        IPT_HIPCHK(hipMemcpyAsync(vec.p, br.X.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    IPT_HIPCHK(hipMemcpyAsync(val.p, br.theta.p, (size_t)k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (info != nullptr) {
+        fill_result(info, br, n, k);
+        info->preparation_time_sec = prep;
+        info->solve_time_sec = seconds_since(t0);
+    }
+    *d_vec_out = vec.take();
+    *d_val_out = val.take();
+    return IPT_CUDA_SUCCESS;
+}
+
+int block_dense_host_entry(const double *H, int n, int k, int maxiter, double tol, int use_tol,
+                           IPTCudaResult *result)
+{
+    result_init_block(result);
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(nullptr, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    auto run = [&]() -> int {
+        const size_t nn = (size_t)n * n, nk = (size_t)n * k;
+        auto t0 = std::chrono::steady_clock::now();
+        DevBuf<double> dH;
+        IPT_STCHK(dH.alloc(nn));
+        IPT_HIPCHK(hipMemcpy(dH.p, H, nn * sizeof(double), hipMemcpyHostToDevice));
+        result->transfer_setup_time_sec = seconds_since(t0);
+        double *dv = nullptr, *dl = nullptr;
+        IPT_STCHK(block_dense_device(ctx, dH.p, n, k, maxiter, tol, use_tol, &dv, &dl, result));
+        result->vectors = (double *)calloc(nk, sizeof(double));
+        result->values = (double *)calloc((size_t)k, sizeof(double));
+        hipError_t e1 = hipSuccess, e2 = hipSuccess;
+        if (result->vectors != nullptr && result->values != nullptr) {
+            e1 = hipMemcpy(result->vectors, dv, nk * sizeof(double), hipMemcpyDeviceToHost);
+            e2 = hipMemcpy(result->values, dl, (size_t)k * sizeof(double), hipMemcpyDeviceToHost);
+        }
+        ipt_cuda_free_device_result(dv, dl);
+        if (result->vectors == nullptr || result->values == nullptr) {
+            return IPT_CUDA_ALLOCATION_FAILED;
+        }
+        IPT_HIPCHK(e1);
+        IPT_HIPCHK(e2);
+        return IPT_CUDA_SUCCESS;
+    };
+    st = run();
+    if (st != IPT_CUDA_SUCCESS) {
+        ipt_cuda_free_result(result);
+    }
+    return st;
+}
+
+int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double *values, int n, int k, int nnz,
+                            int maxiter, double tol, int use_tol, IPTCudaResult *result)
+{
+    result_init_block(result);
+    if (!csc_valid(col_ptr, row_ind, n, nnz)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(nullptr, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    auto run = [&]() -> int {
+        auto t0 = std::chrono::steady_clock::now();
+        // IPT/src/ipt_cuda.cu:4551-4683
+        std::vector<double> d0;
+        csc_diagonal_host(col_ptr, row_ind, values, n, &d0);
+        std::vector<int> perm, inv;
+        stable_sort_permutation(d0, &perm, &inv);
+        HostCsc a;
+        permute_csc(col_ptr, row_ind, values, n, inv, &a);
+        std::vector<double> d;
+        csc_diagonal_host(a.col_ptr.data(), a.row_ind.data(), a.values.data(), n, &d);
+        if (!csc_is_symmetric(a)) {
+            fprintf(stderr, "IPT block-cluster path currently requires a real Hermitian CSC matrix\n");
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
+        bool too_large = false;
+        CscCoupling cc(&a);
+        std::vector<std::pair<int, int>> ranges = target_clusters(cc, d, k, o, &too_large);
+        if (too_large) {
+            int worst = 0;
+            for (const auto &r : ranges) {
+                worst = std::max(worst, r.second - r.first + 1);
+            }
+            fprintf(stderr,
+                    "IPT block-cluster invalid argument at block detection: block_size=%d max_block_size=%d\n",
+                    worst, o.max_block);
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        const double normH = csc_inf_norm(col_ptr, row_ind, values, n);
+        result->preparation_time_sec = seconds_since(t0);
+        auto t1 = std::chrono::steady_clock::now();
+        // symmetric: CSC arrays are the CSR arrays of the same matrix
+        DevBuf<int> rp, ci, dperm;
+        DevBuf<double> vv, dd, xr, yr;
+        const int pn = a.col_ptr[(size_t)n];
+        IPT_STCHK(rp.alloc((size_t)n + 1));
+        IPT_STCHK(ci.alloc((size_t)std::max(pn, 1)));
+        IPT_STCHK(vv.alloc((size_t)std::max(pn, 1)));
+        IPT_STCHK(dd.alloc((size_t)n));
+        IPT_STCHK(dperm.alloc((size_t)n));
+        IPT_HIPCHK(hipMemcpy(rp.p, a.col_ptr.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice));
+        if (pn > 0) {
+            IPT_HIPCHK(hipMemcpy(ci.p, a.row_ind.data(), (size_t)pn * sizeof(int), hipMemcpyHostToDevice));
+            IPT_HIPCHK(hipMemcpy(vv.p, a.values.data(), (size_t)pn * sizeof(double), hipMemcpyHostToDevice));
+        }
+        IPT_HIPCHK(hipMemcpy(dd.p, d.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+        IPT_HIPCHK(hipMemcpy(dperm.p, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+        result->transfer_setup_time_sec = seconds_since(t1);
+        auto t2 = std::chrono::steady_clock::now();
+        const BlockParams p = params_from_env(maxiter, tol, use_tol);
+        int maxcols = 0;
+        for (const auto &r : ranges) {
+            maxcols += r.second - r.first + 1;
+        }
+        maxcols = std::max(maxcols, std::min(n, k + std::max(16, (2 * k + 2) / 3)) + k + 1);
+        IPT_STCHK(xr.alloc((size_t)n * std::min(maxcols, 128)));
+        IPT_STCHK(yr.alloc((size_t)n * std::min(maxcols, 128)));
+        ApplyFn apply = [&](const double *X, double *Y, int nc) -> int {
+            for (int c0 = 0; c0 < nc; c0 += 128) {
+                const int w = std::min(128, nc - c0);
+                IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));
+                IPT_STCHK(spmm_csr_rowmajor(ctx, rp.p, ci.p, vv.p, n, xr.p, w, yr.p, w, w));
+                IPT_STCHK(rowmajor_to_colmajor(ctx, yr.p, w, Y + (size_t)c0 * n, n, n, w));
+            }
+            return IPT_CUDA_SUCCESS;
+        };
+        BlockResult br;
+        IPT_STCHK(block_pipeline(ctx, apply, dd.p, n, k, ranges, normH, p, &br));
+        DevBuf<double> vec;
+        IPT_STCHK(vec.alloc((size_t)n * k));
+        IPT_STCHK(unpermute_rows(ctx, br.X.p, vec.p, n, n, k, dperm.p));
+        result->vectors = (double *)calloc((size_t)n * k, sizeof(double));
+        result->values = (double *)calloc((size_t)k, sizeof(double));
+        if (result->vectors == nullptr || result->values == nullptr) {
+            return IPT_CUDA_ALLOCATION_FAILED;
+        }
+        IPT_HIPCHK(hipMemcpy(result->vectors, vec.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost));
+        IPT_HIPCHK(hipMemcpy(result->values, br.theta.p, (size_t)k * sizeof(double), hipMemcpyDeviceToHost));
+        fill_result(result, br, n, k);
+        result->solve_time_sec = seconds_since(t2);
+        return IPT_CUDA_SUCCESS;
+    };
+    st = run();
+    if (st != IPT_CUDA_SUCCESS) {
+        ipt_cuda_free_result(result);
+    }
+    return st;
+}
+
+}  // namespace ipt

@@ -1,0 +1,79 @@
+// Block-cluster IPT and FP64 Rayleigh-Ritz: shared device-side plan and
+// kernel launchers.
+#pragma once
+
+#include "ipt_internal.h"
+
+namespace ipt {
+
+struct BlockCluster {
+    int first;    // first matrix row of the cluster (sorted order)
+    int last;     // last matrix row
+    int off;      // first basis column
+    int m;        // width = last - first + 1
+    int nblocks;  // row blocks of its update launch (partials per cluster)
+    int pad;
+};
+
+// Device-resident description of all clusters, grouped by lane class
+// P = next_pow2(m) in {1,...,64}.
+struct BlockPlan {
+    int nclusters = 0;
+    int basis_cols = 0;
+    int class_count[7] = {0, 0, 0, 0, 0, 0, 0};
+    int class_offset[7] = {0, 0, 0, 0, 0, 0, 0};
+    std::vector<BlockCluster> host;
+    DevBuf<BlockCluster> clusters;
+    DevBuf<int> class_lists;  // cluster ids, class-major
+    DevBuf<int> active;       // 1 while the cluster iterates
+    DevBuf<int> iters;        // iterations done per cluster
+    DevBuf<double> fp;        // last fixed-point residual per cluster
+    DevBuf<int> flags;        // [0] singular, [1] status bits, [2] active count
+    int partial_stride = 0;   // max nblocks over clusters
+};
+
+// Build the plan for ranges [first,last] (sorted by first) on n rows.
+int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &ranges, int n, BlockPlan *plan);
+// Reset iteration state (all clusters active, counters zero).
+int block_plan_reset(Context *ctx, BlockPlan *plan);
+// B = identity blocks of every cluster (column-major, ld).
+int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int n);
+// One batched update B -> Bn from Y = H B; then per-cluster norms, freeze
+// converged clusters (fp <= tol), count iterations, and set flags.
+int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
+                 const double *d, int n, double damping, double tol);
+
+// C (m x nc, ldc) = A^T B for A (nrows x m, lda), B (nrows x nc, ldb), all
+// column-major FP64; fixed-order two-stage reduction.
+int gemm_tn_tall(Context *ctx, const double *A, long lda, const double *B, long ldb, double *C, long ldc,
+                 long nrows, int m, int nc);
+// C (nrows x nc, ldc) = A (nrows x m, lda) * W (m x nc, ldw); alpha/beta form
+// C = alpha*A*W + beta*C.
+int gemm_nn_small(Context *ctx, const double *A, long lda, const double *W, long ldw, double *C, long ldc,
+                  long nrows, int m, int nc, double alpha, double beta);
+// sq[j] = sum_r X[r + j*ld]^2
+int column_sqnorms(Context *ctx, const double *X, long ld, long nrows, int nc, double *sq);
+// X[:, j] *= 1/sqrt(sq[j]) (and the same for HX when non-NULL)
+int scale_columns_inv_sqrt(Context *ctx, double *X, double *HX, long ld, long nrows, int nc,
+                           const double *sq);
+// res[j] = ||HX_j - theta_j X_j|| / (max(1,normH) max(1,||X_j||)); also
+// writes R = HX - X theta when R != NULL.
+int residual_norms(Context *ctx, const double *X, const double *HX, long ld, long nrows, int nc,
+                   const double *theta, double normH, double *res, double *R);
+// out[perm[s] + j*ld] = in[s + j*ld]
+int unpermute_rows(Context *ctx, const double *in, double *out, long ld, long nrows, int nc, const int *perm);
+
+}  // namespace ipt
+
+namespace ipt {
+// Dense preparation scan: column abs sums (colsum), column max |offdiag|
+// (colmax, as doubles), symmetry flag asym (device int).
+int dense_scan(Context *ctx, const double *H, long ld, int n, double *colsum, double *colmax, int *asym);
+// out = P H P^T with perm[sorted] = original.
+int dense_permute(Context *ctx, const double *H, long ldh, double *out, long ld, int n, const int *perm);
+// vals[2i] = H(r_i, c_i), vals[2i+1] = H(c_i, r_i) for pairs rc[2i], rc[2i+1].
+int gather_pairs(Context *ctx, const double *H, long ld, const int *d_rc, int count, double *d_vals);
+// T(:, q) = R(:, idx[q]) / clip(d - theta[idx[q]])
+int diag_correction(Context *ctx, const double *R, long ldr, const double *d, const double *theta,
+                    const int *d_idx, int nt, double clip, double *T, long ldt, long nrows);
+}  // namespace ipt

@@ -1,0 +1,808 @@
+// Batched block-cluster (Riccati) IPT update and the FP64 Rayleigh-Ritz
+// building blocks (tall-skinny A^T B, tall x small products, column norms,
+// residual norms, unpermute).
+//
+// Block update (IPT/src/ipt_cuda.cu:2280-2384): for cluster rows
+// [first,last] of width m with basis columns [off, off+m) and h =
+// (HB)[first:last, off:off+m], every row r outside the cluster solves
+//     (d_r I - h^T) x = d_r B_r - (HB)_r,   B'_r = x  (optional damping)
+// and cluster rows are reset to the identity.  The reference gives each
+// row a thread block with the pivot search done by one thread; here a row
+// is owned by a group of P = next_pow2(m) lanes of one wavefront, lane t
+// holding equation t in registers; partial pivoting is implicit (the pivot
+// lane is picked by a shuffle arg-max among lanes not yet used) so no row
+// swap moves data.  A pivot with |a| <= 64*DBL_EPSILON or a non-finite
+// result flags the solve as singular, as in the reference.
+#include <cfloat>
+
+#include "ipt_block.h"
+
+namespace ipt {
+
+static inline int blocks_for(long total, int threads, int cap)
+{
+    long b = (total + threads - 1) / threads;
+    if (b < 1) {
+        b = 1;
+    }
+    return (int)(b > cap ? cap : b);
+}
+
+static inline int lane_class(int m)
+{
+    int c = 0;
+    while ((1 << c) < m) {
+        ++c;
+    }
+    return c;
+}
+
+// ------------------------------------------------------------------ block update
+
+template <int P>
+__global__ __launch_bounds__(256) void k_block_update(
+    const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
+    const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
+    const int *__restrict__ class_list, const int *__restrict__ active, double damping,
+    double *__restrict__ partials, int pstride, int *__restrict__ flags)
+{
+    constexpr int GROUPS = 256 / P;
+    const int cid = class_list[blockIdx.y];
+    const BlockCluster cl = clusters[cid];
+    const int m = cl.m;
+    const int t = threadIdx.x % P;
+    const int grp = threadIdx.x / P;
+    const long row = (long)blockIdx.x * GROUPS + grp;
+    const bool live = active[cid] != 0;
+    double sdn = 0.0, sbn = 0.0;
+
+    if (row < n) {
+        const double *Bc = B + (long)cl.off * ld;
+        const double *Yc = Y + (long)cl.off * ld;
+        double *Bnc = Bn + (long)cl.off * ld;
+        if (!live) {
+            if (t < m) {
+                Bnc[row + (long)t * ld] = Bc[row + (long)t * ld];
+            }
+        } else if (row >= cl.first && row <= cl.last) {
+            if (t < m) {
+                const double nv = (row == cl.first + t) ? 1.0 : 0.0;
+                const double ov = Bc[row + (long)t * ld];
+                Bnc[row + (long)t * ld] = nv;
+                sdn += (nv - ov) * (nv - ov);
+                sbn += ov * ov;
+            }
+        } else {
+            const double dr = d[row];
+            double a[P + 1];
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                a[u] = 0.0;
+                if (t < m && u < m) {
+                    // h(u, t) = Y[first+u, off+t]
+                    a[u] = (u == t ? dr : 0.0) - Yc[cl.first + u + (long)t * ld];
+                }
+            }
+            a[P] = (t < m) ? dr * Bc[row + (long)t * ld] - Yc[row + (long)t * ld] : 0.0;
+            bool used = false;
+            int mypc = -1;
+            bool bad = false;
+#pragma unroll
+            for (int pc = 0; pc < P; ++pc) {
+                if (pc < m && !bad) {
+                    double cand = (t < m && !used) ? fabs(a[pc]) : -1.0;
+                    int cidx = t;
+#pragma unroll
+                    for (int o = 1; o < P; o <<= 1) {
+                        const double oc = __shfl_xor(cand, o, P);
+                        const int oi = __shfl_xor(cidx, o, P);
+                        if (oc > cand || (oc == cand && oi < cidx)) {
+                            cand = oc;
+                            cidx = oi;
+                        }
+                    }
+                    if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
+                        bad = true;
+                    } else {
+                        const double inv = 1.0 / __shfl(a[pc], cidx, P);
+                        const double f = a[pc];
+#pragma unroll
+                        for (int c = pc; c <= P; ++c) {
+                            const double pr = __shfl(a[c], cidx, P) * inv;
+                            if (t == cidx) {
+                                a[c] = pr;
+                            } else {
+                                a[c] -= f * pr;
+                            }
+                        }
+                        if (t == cidx) {
+                            used = true;
+                            mypc = pc;
+                        }
+                    }
+                }
+            }
+            if (bad) {
+                if (t == 0) {
+                    atomicOr(&flags[0], 1);
+                }
+            } else if (mypc >= 0) {
+                double v = a[P];
+                const double ov = Bc[row + (long)mypc * ld];
+                if (damping < 1.0) {
+                    v = (1.0 - damping) * ov + damping * v;
+                }
+                if (!isfinite(v)) {
+                    atomicOr(&flags[0], 1);
+                    v = 0.0;
+                }
+                Bnc[row + (long)mypc * ld] = v;
+                sdn += (v - ov) * (v - ov);
+                sbn += ov * ov;
+            }
+        }
+    }
+    __shared__ double sa[4], sb[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        sdn += __shfl_down(sdn, o, 64);
+        sbn += __shfl_down(sbn, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = sdn;
+        sb[threadIdx.x >> 6] = sbn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long slot = (long)cid * pstride + blockIdx.x;
+        partials[2 * slot] = sa[0] + sa[1] + sa[2] + sa[3];
+        partials[2 * slot + 1] = sb[0] + sb[1] + sb[2] + sb[3];
+    }
+}
+
+// Per-cluster fixed-point residual fp = ||B'-B||_F / max(1, ||B||_F)
+// (IPT/src/ipt_cuda.cu:2984-3003); clusters reaching fp <= tol are frozen.
+// flags[0]: singular (reset here after reading), flags[1]: status bits
+// (1 non-finite norm, 2 singular), flags[2]: clusters still active.
+// Single block.
+__global__ void k_block_finalize(const double *__restrict__ partials, int pstride,
+                                 const BlockCluster *__restrict__ clusters, int nclusters,
+                                 int *__restrict__ active, double *__restrict__ fp_out,
+                                 int *__restrict__ iters, double tol, int *__restrict__ flags)
+{
+    __shared__ int s_sing;
+    __shared__ int s_live;
+    if (threadIdx.x == 0) {
+        s_sing = flags[0];
+        s_live = 0;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < nclusters; c += blockDim.x) {
+        if (!active[c]) {
+            continue;
+        }
+        const int nb = clusters[c].nblocks;
+        double dn = 0.0, bn = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            dn += partials[2 * ((long)c * pstride + b)];
+            bn += partials[2 * ((long)c * pstride + b) + 1];
+        }
+        dn = sqrt(dn);
+        bn = sqrt(bn);
+        if (!isfinite(dn) || !isfinite(bn) || bn <= 0.0) {
+            atomicOr(&flags[1], 1);
+        }
+        const double fp = dn / fmax(1.0, bn);
+        fp_out[c] = fp;
+        iters[c] += 1;
+        if (fp <= tol) {
+            active[c] = 0;
+        } else {
+            atomicAdd(&s_live, 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_sing) {
+            atomicOr(&flags[1], 2);
+        }
+        flags[0] = 0;
+        flags[2] = s_live;
+    }
+}
+
+__global__ void k_block_reset(int *__restrict__ active, int *__restrict__ iters, double *__restrict__ fp,
+                              int nclusters, int *__restrict__ flags)
+{
+    for (int c = threadIdx.x; c < nclusters; c += blockDim.x) {
+        active[c] = 1;
+        iters[c] = 0;
+        fp[c] = NAN;
+    }
+    if (threadIdx.x == 0) {
+        flags[0] = 0;
+        flags[1] = 0;
+        flags[2] = nclusters;
+    }
+}
+
+__global__ void k_block_identity(double *__restrict__ B, long ld, int n, const BlockCluster *__restrict__ clusters,
+                                 int nclusters, int basis_cols)
+{
+    const long total = (long)n * basis_cols;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long r = idx % n;
+        const int col = (int)(idx / n);
+        // cluster owning column col (clusters are few; linear scan)
+        int c = 0;
+        while (c + 1 < nclusters && clusters[c + 1].off <= col) {
+            ++c;
+        }
+        const BlockCluster cl = clusters[c];
+        B[r + (long)col * ld] = (r == cl.first + (col - cl.off)) ? 1.0 : 0.0;
+    }
+}
+
+int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &ranges, int n, BlockPlan *plan)
+{
+    plan->nclusters = (int)ranges.size();
+    plan->host.clear();
+    std::vector<std::vector<int>> lists(7);
+    int off = 0;
+    plan->partial_stride = 1;
+    for (int c = 0; c < plan->nclusters; ++c) {
+        BlockCluster cl;
+        cl.first = ranges[(size_t)c].first;
+        cl.last = ranges[(size_t)c].second;
+        cl.m = cl.last - cl.first + 1;
+        cl.off = off;
+        const int cls = lane_class(cl.m);
+        if (cls > 6) {
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        const int rows = 256 >> cls;
+        cl.nblocks = (n + rows - 1) / rows;
+        cl.pad = 0;
+        plan->partial_stride = std::max(plan->partial_stride, cl.nblocks);
+        off += cl.m;
+        lists[(size_t)cls].push_back(c);
+        plan->host.push_back(cl);
+    }
+    plan->basis_cols = off;
+    std::vector<int> flat;
+    for (int cls = 0; cls < 7; ++cls) {
+        plan->class_offset[cls] = (int)flat.size();
+        plan->class_count[cls] = (int)lists[(size_t)cls].size();
+        flat.insert(flat.end(), lists[(size_t)cls].begin(), lists[(size_t)cls].end());
+    }
+    const int nc = std::max(plan->nclusters, 1);
+    IPT_STCHK(plan->clusters.alloc((size_t)nc));
+    IPT_STCHK(plan->class_lists.alloc(std::max<size_t>(flat.size(), 1)));
+    IPT_STCHK(plan->active.alloc((size_t)nc));
+    IPT_STCHK(plan->iters.alloc((size_t)nc));
+    IPT_STCHK(plan->fp.alloc((size_t)nc));
+    IPT_STCHK(plan->flags.alloc(4));
+    if (plan->nclusters > 0) {
+        IPT_HIPCHK(hipMemcpyAsync(plan->clusters.p, plan->host.data(), plan->host.size() * sizeof(BlockCluster),
+                                  hipMemcpyHostToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(plan->class_lists.p, flat.data(), flat.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    }
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return block_plan_reset(ctx, plan);
+}
+
+int block_plan_reset(Context *ctx, BlockPlan *plan)
+{
+    hipLaunchKernelGGL(k_block_reset, dim3(1), dim3(256), 0, ctx->stream, plan->active.p, plan->iters.p,
+                       plan->fp.p, plan->nclusters, plan->flags.p);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int n)
+{
+    hipLaunchKernelGGL(k_block_identity, dim3(blocks_for((long)n * plan.basis_cols, 256, 16384)), dim3(256), 0,
+                       ctx->stream, B, ld, n, plan.clusters.p, plan.nclusters, plan.basis_cols);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
+                 const double *d, int n, double damping, double tol)
+{
+    double *partials = nullptr;
+    IPT_STCHK(context_scratch(ctx, SLOT_REDUCE,
+                              (size_t)std::max(plan.nclusters, 1) * plan.partial_stride * 2 * sizeof(double),
+                              (void **)&partials));
+    for (int cls = 0; cls < 7; ++cls) {
+        const int cnt = plan.class_count[cls];
+        if (cnt == 0) {
+            continue;
+        }
+        const int rows = 256 >> cls;
+        dim3 grid((n + rows - 1) / rows, cnt);
+        const int *list = plan.class_lists.p + plan.class_offset[cls];
+        const int ps = plan.partial_stride;
+#define IPT_BLK(PP)                                                                                           \
+    hipLaunchKernelGGL(k_block_update<PP>, grid, dim3(256), 0, ctx->stream, B, Y, Bn, ld, d, n,              \
+                       plan.clusters.p, list, plan.active.p, damping, partials, ps, plan.flags.p)
+        switch (cls) {
+        case 0: IPT_BLK(1); break;
+        case 1: IPT_BLK(2); break;
+        case 2: IPT_BLK(4); break;
+        case 3: IPT_BLK(8); break;
+        case 4: IPT_BLK(16); break;
+        case 5: IPT_BLK(32); break;
+        default: IPT_BLK(64); break;
+        }
+#undef IPT_BLK
+        IPT_HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_block_finalize, dim3(1), dim3(256), 0, ctx->stream, partials, plan.partial_stride,
+                       plan.clusters.p, plan.nclusters, plan.active.p, plan.fp.p, plan.iters.p, tol,
+                       plan.flags.p);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// ------------------------------------------------------------------ A^T B (tall)
+
+// C_partial[z] (m x nc) = A[rows of chunk z]^T B[rows of chunk z].  Tile of
+// 32 x 32 outputs per block (16x16 threads, 2x2 each), rows staged through
+// LDS 64 at a time.
+__global__ __launch_bounds__(256) void k_gemm_tn_partial(const double *__restrict__ A, long lda,
+                                                         const double *__restrict__ B, long ldb,
+                                                         double *__restrict__ P, long nrows, int m, int nc,
+                                                         long chunk)
+{
+    __shared__ double as[64][33];
+    __shared__ double bs[64][33];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const long r0 = (long)blockIdx.z * chunk;
+    long r1 = r0 + chunk;
+    if (r1 > nrows) {
+        r1 = nrows;
+    }
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (long rb = r0; rb < r1; rb += 64) {
+        for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+            const int rr = e & 63, cc = e >> 6;
+            const long r = rb + rr;
+            as[rr][cc] = (r < r1 && i0 + cc < m) ? A[r + (long)(i0 + cc) * lda] : 0.0;
+            bs[rr][cc] = (r < r1 && j0 + cc < nc) ? B[r + (long)(j0 + cc) * ldb] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int rr = 0; rr < 64; ++rr) {
+            const double a0 = as[rr][ty], a1 = as[rr][ty + 16];
+            const double b0 = bs[rr][tx], b1 = bs[rr][tx + 16];
+            acc[0][0] += a0 * b0;
+            acc[0][1] += a0 * b1;
+            acc[1][0] += a1 * b0;
+            acc[1][1] += a1 * b1;
+        }
+        __syncthreads();
+    }
+    double *Pz = P + (long)blockIdx.z * m * nc;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
+            if (i < m && j < nc) {
+                Pz[i + (long)j * m] = acc[a][b];
+            }
+        }
+    }
+}
+
+__global__ void k_sum_partials(const double *__restrict__ P, int splits, int m, int nc, double *__restrict__ C,
+                               long ldc)
+{
+    const long total = (long)m * nc;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int z = 0; z < splits; ++z) {
+            s += P[(long)z * total + idx];
+        }
+        const long i = idx % m, j = idx / m;
+        C[i + j * ldc] = s;
+    }
+}
+
+int gemm_tn_tall(Context *ctx, const double *A, long lda, const double *B, long ldb, double *C, long ldc,
+                 long nrows, int m, int nc)
+{
+    if (m <= 0 || nc <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    const int gx = (m + 31) / 32, gy = (nc + 31) / 32;
+    long splits = 2048 / (gx * gy);
+    if (splits < 1) {
+        splits = 1;
+    }
+    long chunk = (nrows + splits - 1) / splits;
+    if (chunk < 256) {
+        chunk = 256;
+    }
+    chunk = (chunk + 63) / 64 * 64;
+    splits = (nrows + chunk - 1) / chunk;
+    if (splits < 1) {
+        splits = 1;
+    }
+    double *P = nullptr;
+    IPT_STCHK(context_scratch(ctx, SLOT_SMALL, (size_t)splits * m * nc * sizeof(double), (void **)&P));
+    hipLaunchKernelGGL(k_gemm_tn_partial, dim3(gx, gy, (unsigned)splits), dim3(256), 0, ctx->stream, A, lda, B,
+                       ldb, P, nrows, m, nc, chunk);
+    IPT_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials, dim3(blocks_for((long)m * nc, 256, 1024)), dim3(256), 0, ctx->stream, P,
+                       (int)splits, m, nc, C, ldc);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// ------------------------------------------------------------------ A W (tall x small)
+
+// 256 threads = 64 rows x 4 column groups; W staged in LDS 64 rows at a
+// time; each thread accumulates up to 16 output columns of one row.
+__global__ __launch_bounds__(256) void k_gemm_nn_small(const double *__restrict__ A, long lda,
+                                                       const double *__restrict__ W, long ldw,
+                                                       double *__restrict__ C, long ldc, long nrows, int m,
+                                                       int nc, double alpha, double beta)
+{
+    __shared__ double ws[64][65];
+    const int lr = threadIdx.x & 63;
+    const int cg = threadIdx.x >> 6;  // 0..3
+    const long r = (long)blockIdx.x * 64 + lr;
+    const int c0 = blockIdx.y * 64;
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        acc[q] = 0.0;
+    }
+    for (int i0 = 0; i0 < m; i0 += 64) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int ii = e & 63, jj = e >> 6;
+            ws[ii][jj] = (i0 + ii < m && c0 + jj < nc) ? W[(i0 + ii) + (long)(c0 + jj) * ldw] : 0.0;
+        }
+        __syncthreads();
+        const int ilim = (m - i0) < 64 ? (m - i0) : 64;
+        if (r < nrows) {
+            for (int ii = 0; ii < ilim; ++ii) {
+                const double a = A[r + (long)(i0 + ii) * lda];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    acc[q] += a * ws[ii][cg + 4 * q];
+                }
+            }
+        }
+    }
+    if (r < nrows) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = c0 + cg + 4 * q;
+            if (c < nc) {
+                double *dst = C + r + (long)c * ldc;
+                *dst = (beta == 0.0) ? alpha * acc[q] : alpha * acc[q] + beta * (*dst);
+            }
+        }
+    }
+}
+
+int gemm_nn_small(Context *ctx, const double *A, long lda, const double *W, long ldw, double *C, long ldc,
+                  long nrows, int m, int nc, double alpha, double beta)
+{
+    if (nrows <= 0 || nc <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    dim3 grid((unsigned)((nrows + 63) / 64), (unsigned)((nc + 63) / 64));
+    hipLaunchKernelGGL(k_gemm_nn_small, grid, dim3(256), 0, ctx->stream, A, lda, W, ldw, C, ldc, nrows, m, nc,
+                       alpha, beta);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// ------------------------------------------------------------------ norms
+
+// one block per column, fixed-order reduction
+__global__ void k_col_sqnorm(const double *__restrict__ X, long ld, long nrows, double *__restrict__ sq)
+{
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (long r = threadIdx.x; r < nrows; r += blockDim.x) {
+        const double x = X[r + (long)j * ld];
+        s += x * x;
+    }
+    __shared__ double sh[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_down(s, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sq[j] = sh[0] + sh[1] + sh[2] + sh[3];
+    }
+}
+
+int column_sqnorms(Context *ctx, const double *X, long ld, long nrows, int nc, double *sq)
+{
+    if (nc <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    hipLaunchKernelGGL(k_col_sqnorm, dim3(nc), dim3(256), 0, ctx->stream, X, ld, nrows, sq);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+__global__ void k_scale_cols(double *__restrict__ X, double *__restrict__ HX, long ld, long nrows, int nc,
+                             const double *__restrict__ sq)
+{
+    const long total = nrows * nc;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long r = idx % nrows, j = idx / nrows;
+        const double s = 1.0 / sqrt(sq[j]);
+        X[r + j * ld] *= s;
+        if (HX != nullptr) {
+            HX[r + j * ld] *= s;
+        }
+    }
+}
+
+int scale_columns_inv_sqrt(Context *ctx, double *X, double *HX, long ld, long nrows, int nc, const double *sq)
+{
+    hipLaunchKernelGGL(k_scale_cols, dim3(blocks_for(nrows * nc, 256, 16384)), dim3(256), 0, ctx->stream, X, HX,
+                       ld, nrows, nc, sq);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// IPT/src/ipt_cuda.cu:2399-2447, 2684-2697: R = HX - X theta and the scaled
+// residual norm per column.  One block per column.
+__global__ void k_residual_norms(const double *__restrict__ X, const double *__restrict__ HX, long ld, long nrows,
+                                 const double *__restrict__ theta, double scale, double *__restrict__ res,
+                                 double *__restrict__ R)
+{
+    const int j = blockIdx.x;
+    const double th = theta[j];
+    double rs = 0.0, xs = 0.0;
+    for (long r = threadIdx.x; r < nrows; r += blockDim.x) {
+        const double x = X[r + (long)j * ld];
+        const double rr = HX[r + (long)j * ld] - th * x;
+        if (R != nullptr) {
+            R[r + (long)j * ld] = rr;
+        }
+        rs += rr * rr;
+        xs += x * x;
+    }
+    __shared__ double sa[4], sb[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        rs += __shfl_down(rs, o, 64);
+        xs += __shfl_down(xs, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = rs;
+        sb[threadIdx.x >> 6] = xs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double rn = sqrt(sa[0] + sa[1] + sa[2] + sa[3]);
+        const double xn = sqrt(sb[0] + sb[1] + sb[2] + sb[3]);
+        res[j] = rn / (scale * fmax(1.0, xn));
+    }
+}
+
+int residual_norms(Context *ctx, const double *X, const double *HX, long ld, long nrows, int nc,
+                   const double *theta, double normH, double *res, double *R)
+{
+    if (nc <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    hipLaunchKernelGGL(k_residual_norms, dim3(nc), dim3(256), 0, ctx->stream, X, HX, ld, nrows, theta,
+                       std::max(1.0, normH), res, R);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+__global__ void k_unpermute(const double *__restrict__ in, double *__restrict__ out, long ld, long nrows, int nc,
+                            const int *__restrict__ perm)
+{
+    const long total = nrows * nc;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long s = idx % nrows, j = idx / nrows;
+        out[(long)perm[s] + j * ld] = in[s + j * ld];
+    }
+}
+
+int unpermute_rows(Context *ctx, const double *in, double *out, long ld, long nrows, int nc, const int *perm)
+{
+    hipLaunchKernelGGL(k_unpermute, dim3(blocks_for(nrows * nc, 256, 16384)), dim3(256), 0, ctx->stream, in, out,
+                       ld, nrows, nc, perm);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+}  // namespace ipt
+
+namespace ipt {
+
+// ------------------------------------------------------------------ dense prep
+
+// One pass over tile pairs (bi <= bj) of a dense column-major H:
+//  * symmetry: |H_rc - H_cr| <= 1e-12 max(1,|H_rc|,|H_cr|) (IPT/src/ipt_preparation.cu:860-883)
+//  * per-column partial abs sums over each 64-row block (for ||H||_inf of
+//    a symmetric matrix) written to colsum_part[rowblock * n + col]
+//  * per-column max |offdiag| via atomicMax on the IEEE bits (order free)
+__global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H, long ld, int n, int nb,
+                                                    double *__restrict__ colsum_part,
+                                                    unsigned long long *__restrict__ colmax_bits,
+                                                    int *__restrict__ asym)
+{
+    __shared__ double tt[64][65];
+    // decode blockIdx.x -> (bi, bj), bi <= bj, row-major over the upper triangle
+    long idx = blockIdx.x;
+    int bi = 0;
+    while (idx >= nb - bi) {
+        idx -= nb - bi;
+        ++bi;
+    }
+    const int bj = bi + (int)idx;
+    const int r = threadIdx.x & 63;
+    const int cg = threadIdx.x >> 6;
+    const long R0 = (long)bi * 64, C0 = (long)bj * 64;
+    int bad = 0;
+    // T' = H[bj rows, bi cols] -> LDS, with its column stats
+    for (int q = 0; q < 16; ++q) {
+        const int c = cg * 16 + q;
+        const long gr = C0 + r, gc = R0 + c;
+        const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+        tt[r][c] = v;
+        if (bi != bj) {
+            double s = fabs(v);
+            double mx = (gr != gc) ? fabs(v) : 0.0;
+            for (int o = 32; o > 0; o >>= 1) {
+                s += __shfl_down(s, o, 64);
+                mx = fmax(mx, __shfl_down(mx, o, 64));
+            }
+            if (r == 0 && gc < n) {
+                colsum_part[(long)bj * n + gc] = s;
+                atomicMax(&colmax_bits[gc], (unsigned long long)__double_as_longlong(mx));
+            }
+        }
+    }
+    __syncthreads();
+    // T = H[bi rows, bj cols]: compare with T'^T, column stats
+    for (int q = 0; q < 16; ++q) {
+        const int c = cg * 16 + q;
+        const long gr = R0 + r, gc = C0 + c;
+        const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+        const double w = tt[c][r];
+        const double scale = fmax(1.0, fmax(fabs(v), fabs(w)));
+        if (fabs(v - w) > 1e-12 * scale) {
+            bad = 1;
+        }
+        double s = fabs(v);
+        double mx = (gr != gc) ? fabs(v) : 0.0;
+        for (int o = 32; o > 0; o >>= 1) {
+            s += __shfl_down(s, o, 64);
+            mx = fmax(mx, __shfl_down(mx, o, 64));
+        }
+        if (r == 0 && gc < n) {
+            colsum_part[(long)bi * n + gc] = s;
+            atomicMax(&colmax_bits[gc], (unsigned long long)__double_as_longlong(mx));
+        }
+    }
+    if (bad) {
+        atomicOr(asym, 1);
+    }
+}
+
+__global__ void k_colsum_reduce(const double *__restrict__ part, int nb, int n, double *__restrict__ colsum)
+{
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            s += part[(long)b * n + c];
+        }
+        colsum[c] = s;
+    }
+}
+
+int dense_scan(Context *ctx, const double *H, long ld, int n, double *colsum, double *colmax, int *asym)
+{
+    const int nb = (n + 63) / 64;
+    double *part = nullptr;
+    IPT_STCHK(context_scratch(ctx, SLOT_SOLVER, (size_t)nb * n * sizeof(double), (void **)&part));
+    IPT_HIPCHK(hipMemsetAsync(colmax, 0, (size_t)n * sizeof(double), ctx->stream));
+    IPT_HIPCHK(hipMemsetAsync(asym, 0, sizeof(int), ctx->stream));
+    const long pairs = (long)nb * (nb + 1) / 2;
+    hipLaunchKernelGGL(k_dense_scan, dim3((unsigned)pairs), dim3(256), 0, ctx->stream, H, ld, n, nb, part,
+                       (unsigned long long *)colmax, asym);
+    IPT_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_colsum_reduce, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, ctx->stream, part, nb, n,
+                       colsum);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// out[r + c*ld] = H[perm[r] + perm[c]*ldh]  (P H P^T)
+__global__ void k_dense_permute(const double *__restrict__ H, long ldh, double *__restrict__ out, long ld, int n,
+                                const int *__restrict__ perm)
+{
+    const long total = (long)n * n;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long r = idx % n, c = idx / n;
+        out[r + c * ld] = H[(long)perm[r] + (long)perm[c] * ldh];
+    }
+}
+
+int dense_permute(Context *ctx, const double *H, long ldh, double *out, long ld, int n, const int *perm)
+{
+    hipLaunchKernelGGL(k_dense_permute, dim3(blocks_for((long)n * n, 256, 65536)), dim3(256), 0, ctx->stream, H,
+                       ldh, out, ld, n, perm);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// vals[2i] = H[rows[i] + cols[i]*ld], vals[2i+1] = H[cols[i] + rows[i]*ld]
+__global__ void k_gather_pairs(const double *__restrict__ H, long ld, const int *__restrict__ rc, int count,
+                               double *__restrict__ vals)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const long r = rc[2 * i], c = rc[2 * i + 1];
+        vals[2 * i] = H[r + c * ld];
+        vals[2 * i + 1] = H[c + r * ld];
+    }
+}
+
+int gather_pairs(Context *ctx, const double *H, long ld, const int *d_rc, int count, double *d_vals)
+{
+    if (count <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    hipLaunchKernelGGL(k_gather_pairs, dim3(blocks_for(count, 256, 1024)), dim3(256), 0, ctx->stream, H, ld, d_rc,
+                       count, d_vals);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// Davidson correction t_i = r_i / clip(D_i - theta), IPT/src/ipt_cuda.cu:2449-2465,
+// for several target pairs at once: T(:, q) from R(:, idx[q]).
+__global__ void k_diag_correction(const double *__restrict__ R, long ldr, const double *__restrict__ d,
+                                  const double *__restrict__ theta, const int *__restrict__ idx, int nt,
+                                  double clip, double *__restrict__ T, long ldt, long nrows)
+{
+    const long total = nrows * nt;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const long r = e % nrows;
+        const int q = (int)(e / nrows);
+        const int j = idx[q];
+        double den = d[r] - theta[j];
+        if (fabs(den) < clip) {
+            den = den < 0.0 ? -clip : clip;
+        }
+        T[r + (long)q * ldt] = R[r + (long)j * ldr] / den;
+    }
+}
+
+int diag_correction(Context *ctx, const double *R, long ldr, const double *d, const double *theta,
+                    const int *d_idx, int nt, double clip, double *T, long ldt, long nrows)
+{
+    if (nt <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    hipLaunchKernelGGL(k_diag_correction, dim3(blocks_for(nrows * nt, 256, 16384)), dim3(256), 0, ctx->stream, R,
+                       ldr, d, theta, d_idx, nt, clip, T, ldt, nrows);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+}  // namespace ipt

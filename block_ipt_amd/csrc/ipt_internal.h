@@ -1,0 +1,149 @@
+// Internal declarations shared by the translation units of libipt_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/ipt_hip.h"
+
+#define IPT_HIPCHK(call)                                                      \
+    do {                                                                      \
+        hipError_t ipt_e_ = (call);                                           \
+        if (ipt_e_ != hipSuccess) {                                           \
+            fprintf(stderr, "HIP error at %s:%d: %s\n", __FILE__, __LINE__,   \
+                    hipGetErrorString(ipt_e_));                               \
+            return IPT_CUDA_CUDA_ERROR;                                       \
+        }                                                                     \
+    } while (0)
+
+#define IPT_STCHK(call)                                                       \
+    do {                                                                      \
+        int ipt_s_ = (call);                                                  \
+        if (ipt_s_ != IPT_CUDA_SUCCESS) {                                     \
+            return ipt_s_;                                                    \
+        }                                                                     \
+    } while (0)
+
+namespace ipt {
+
+// ------------------------------------------------------------------ context
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    void *blas = nullptr;        // rocblas_handle, created on first use
+    double *host_scalars = nullptr;  // pinned, 4096 doubles
+    int *host_flags = nullptr;       // pinned, 4096 ints
+    void *slots[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t slot_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Reusable device scratch owned by the context (grown on demand, freed with
+// the context).  Slots are private to one library routine each:
+enum {
+    SLOT_GEMM_SPLITK = 0,
+    SLOT_REDUCE = 1,
+    SLOT_SMALL = 2,
+    SLOT_SOLVER = 3,
+    SLOT_RR = 4,
+    SLOT_ORTHO = 5,
+    SLOT_LOWP = 6,
+    SLOT_MISC = 7
+};
+int context_scratch(Context *ctx, int slot, size_t bytes, void **out);
+
+// NULL handle -> library context of the current device.
+Context *context_from_handle(void *handle, int *status);
+int context_blas(Context *ctx, void **blas_out);
+
+// Device buffer owned by the caller's scope.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t n)
+    {
+        release();
+        if (n == 0) {
+            return IPT_CUDA_SUCCESS;
+        }
+        if (hipMalloc((void **)&p, n * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            fprintf(stderr, "IPT: device allocation of %zu bytes failed\n", n * sizeof(T));
+            return IPT_CUDA_ALLOCATION_FAILED;
+        }
+        count = n;
+        return IPT_CUDA_SUCCESS;
+    }
+    void release()
+    {
+        if (p != nullptr) {
+            (void)hipFree(p);
+        }
+        p = nullptr;
+        count = 0;
+    }
+    T *take()
+    {
+        T *q = p;
+        p = nullptr;
+        count = 0;
+        return q;
+    }
+};
+
+// ------------------------------------------------------------------ options
+
+double env_double(const char *name, double default_value);
+int env_int(const char *name, int default_value);
+int env_flag(const char *name);
+int env_flag_default(const char *name, int default_value);
+
+// ------------------------------------------------------------------ kernels
+// All launchers enqueue on ctx->stream and return an IPT status.
+
+// Y = H * Z; H column-major n x n (ldh), Z column-major n x ncols (ldz),
+// Y column-major (ldy).  FP64 MFMA.
+int gemm_f64(Context *ctx, const double *H, long ldh, const double *Z, long ldz, double *Y,
+             long ldy, int n, int ncols);
+
+// d[i] = H[i + i*ldh]
+int dense_diagonal(Context *ctx, const double *H, long ldh, int n, double *d);
+// CSC diagonal (first match per column), IPT/src/ipt_cuda.cu:1469-1489.
+int csc_diagonal(Context *ctx, const int *col_ptr, const int *row_ind, const double *values, int n,
+                 double *d);
+
+// Z = I(:, 0:k) in column-major layout (ld).
+int set_identity(Context *ctx, double *Z, long ld, int n, int k);
+
+// Plain IPT update (one iteration after Y = H Z) with the squared Frobenius
+// norms of F(Z)-Z and Z written to d_norms[0], d_norms[1] (device).
+// Layout: element (r,j) at r*rs + j*cs.  Znew must not alias Y or Z.
+int plain_update(Context *ctx, const double *Y, const double *Z, double *Znew, const double *d,
+                 int n, int k, long rs, long cs, double *d_norms);
+
+// values[j] = Y[j*rs + j*cs]
+int gather_diagonal(Context *ctx, const double *Y, int k, long rs, long cs, double *values);
+
+// CSR SpMM: Y(n x k, row-major ld ldy) = A(CSR) * X(n x k, row-major ld ldx).
+int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,
+                      int n, const double *X, long ldx, double *Y, long ldy, int k);
+
+// Layout copies between column-major (ld) and row-major (ld) n x k.
+int colmajor_to_rowmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k);
+int rowmajor_to_colmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k);
+
+}  // namespace ipt

@@ -1,0 +1,305 @@
+// Elementwise / reduction kernels of the plain IPT path and the CSR SpMM.
+#include "ipt_internal.h"
+
+namespace ipt {
+
+static inline int blocks_for(long total, int threads, int cap)
+{
+    long b = (total + threads - 1) / threads;
+    if (b < 1) {
+        b = 1;
+    }
+    return (int)(b > cap ? cap : b);
+}
+
+// ------------------------------------------------------------------ basics
+
+__global__ void k_dense_diag(const double *__restrict__ H, long ldh, int n, double *__restrict__ d)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        d[i] = H[i + i * ldh];
+    }
+}
+
+int dense_diagonal(Context *ctx, const double *H, long ldh, int n, double *d)
+{
+    hipLaunchKernelGGL(k_dense_diag, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, ctx->stream, H,
+                       ldh, n, d);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// first stored entry with row == col, IPT/src/ipt_cuda.cu:1469-1489
+__global__ void k_csc_diag(const int *__restrict__ cp, const int *__restrict__ ri,
+                           const double *__restrict__ v, int n, double *__restrict__ d)
+{
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        double x = 0.0;
+        for (int p = cp[c]; p < cp[c + 1]; ++p) {
+            if (ri[p] == c) {
+                x = v[p];
+                break;
+            }
+        }
+        d[c] = x;
+    }
+}
+
+int csc_diagonal(Context *ctx, const int *col_ptr, const int *row_ind, const double *values, int n,
+                 double *d)
+{
+    hipLaunchKernelGGL(k_csc_diag, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, ctx->stream,
+                       col_ptr, row_ind, values, n, d);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+__global__ void k_identity(double *__restrict__ Z, long ld, int n, int k)
+{
+    const long total = (long)n * k;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long r = idx % n;
+        const long c = idx / n;
+        Z[r + c * ld] = (r == c) ? 1.0 : 0.0;
+    }
+}
+
+int set_identity(Context *ctx, double *Z, long ld, int n, int k)
+{
+    hipLaunchKernelGGL(k_identity, dim3(blocks_for((long)n * k, 256, 16384)), dim3(256), 0,
+                       ctx->stream, Z, ld, n, k);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// ------------------------------------------------------------------ plain update
+
+// Block-wide sum of two doubles; result valid in thread 0.
+__device__ inline void block_sum2(double &a, double &b)
+{
+    __shared__ double sa[16], sb[16];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        a = 0.0;
+        b = 0.0;
+        for (int i = 0; i < nw; ++i) {
+            a += sa[i];
+            b += sb[i];
+        }
+    }
+}
+
+// One plain IPT step, IPT/src/ipt_cuda.cu:1403-1443 fused with 1457-1467:
+//   c_j   = (HZ)_jj - d_j Z_jj
+//   Z'_rj = (Y_rj - d_r Z_rj - Z_rj c_j) * G_rj,  G_rj = 1/(d_j - d_r)
+//           (NaN when the gap is zero or not finite, 1369-1396 with min_gap 0)
+//   Z'_jj = 1
+// and per-block partial sums of |Z'-Z|^2, |Z|^2.
+__global__ __launch_bounds__(256) void k_plain_update(const double *Y, const double *__restrict__ Z,
+                                                      double *Znew, const double *__restrict__ d,
+                                                      int n, int k, long rs, long cs, int rowmajor,
+                                                      double *__restrict__ partials)
+{
+    const long total = (long)n * k;
+    double sdn = 0.0, szn = 0.0;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        long r, j;
+        if (rowmajor) {
+            r = idx / k;
+            j = idx % k;
+        } else {
+            r = idx % n;
+            j = idx / n;
+        }
+        const long off = r * rs + j * cs;
+        const long dj_off = j * rs + j * cs;
+        const double z = Z[off];
+        const double dj = d[j];
+        double nz;
+        if (r == j) {
+            nz = 1.0;
+        } else {
+            const double cj = Y[dj_off] - dj * Z[dj_off];
+            const double gap = dj - d[r];
+            const double g = (isfinite(gap) && gap != 0.0) ? 1.0 / gap : (double)NAN;
+            nz = (Y[off] - d[r] * z - z * cj) * g;
+        }
+        const double dl = nz - z;
+        sdn += dl * dl;
+        szn += z * z;
+        // Y(j,j) is read by every thread of column j: write after the grid
+        // finished reading would need a barrier, so the caller never aliases
+        // Znew with Y.
+        Znew[off] = nz;
+    }
+    block_sum2(sdn, szn);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sdn;
+        partials[2 * blockIdx.x + 1] = szn;
+    }
+}
+
+__global__ void k_sum_pairs(const double *__restrict__ partials, int count, double *__restrict__ out)
+{
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+int plain_update(Context *ctx, const double *Y, const double *Z, double *Znew, const double *d,
+                 int n, int k, long rs, long cs, double *d_norms)
+{
+    const int blocks = blocks_for((long)n * k, 256, 2048);
+    double *partials = nullptr;
+    IPT_STCHK(context_scratch(ctx, SLOT_REDUCE, (size_t)blocks * 2 * sizeof(double), (void **)&partials));
+    const int rowmajor = (cs == 1 && k > 1) ? 1 : 0;
+    hipLaunchKernelGGL(k_plain_update, dim3(blocks), dim3(256), 0, ctx->stream, Y, Z, Znew, d, n, k,
+                       rs, cs, rowmajor, partials);
+    IPT_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, ctx->stream, partials, blocks, d_norms);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+__global__ void k_gather_diag(const double *__restrict__ Y, int k, long rs, long cs, double *__restrict__ v)
+{
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < k; j += gridDim.x * blockDim.x) {
+        v[j] = Y[(long)j * rs + (long)j * cs];
+    }
+}
+
+int gather_diagonal(Context *ctx, const double *Y, int k, long rs, long cs, double *values)
+{
+    hipLaunchKernelGGL(k_gather_diag, dim3(blocks_for(k, 256, 64)), dim3(256), 0, ctx->stream, Y, k,
+                       rs, cs, values);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// ------------------------------------------------------------------ SpMM
+
+// Wavefront-per-row CSR SpMM with row-major X / Y.  The 64 lanes are split
+// into G = 64/S groups of S lanes (S = next power of two >= k, capped at
+// 64); group g walks nonzeros g, g+G, ... of the row and lane s of the
+// group accumulates column s (and s+S, ...) -- every nonzero gathers one
+// contiguous row of X.  Groups are combined with cross-lane shuffles.
+template <int S>
+__global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ rp,
+                                                       const int *__restrict__ ci,
+                                                       const double *__restrict__ vals, int n,
+                                                       const double *__restrict__ X, long ldx,
+                                                       double *__restrict__ Y, long ldy, int k)
+{
+    constexpr int G = 64 / S;
+    const int lane = threadIdx.x & 63;
+    const int g = lane / S;
+    const int s = lane % S;
+    const long wave_global = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long waves = ((long)gridDim.x * blockDim.x) >> 6;
+    for (long row = wave_global; row < n; row += waves) {
+        const int pb = rp[row], pe = rp[row + 1];
+        for (int c0 = 0; c0 < k; c0 += S) {
+            const int col = c0 + s;
+            double acc = 0.0;
+            for (int p = pb + g; p < pe; p += G) {
+                const int c = ci[p];
+                const double a = vals[p];
+                if (col < k) {
+                    acc += a * X[(long)c * ldx + col];
+                }
+            }
+#pragma unroll
+            for (int off = S; off < 64; off <<= 1) {
+                acc += __shfl_xor(acc, off, 64);
+            }
+            if (g == 0 && col < k) {
+                Y[row * ldy + col] = acc;
+            }
+        }
+    }
+}
+
+int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,
+                      int n, const double *X, long ldx, double *Y, long ldy, int k)
+{
+    const int blocks = blocks_for((long)n * 64, 256, 65536);
+    dim3 g(blocks), b(256);
+    if (k <= 1) {
+        hipLaunchKernelGGL(k_spmm_rowmajor<1>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+    } else if (k <= 2) {
+        hipLaunchKernelGGL(k_spmm_rowmajor<2>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+    } else if (k <= 4) {
+        hipLaunchKernelGGL(k_spmm_rowmajor<4>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+    } else if (k <= 8) {
+        hipLaunchKernelGGL(k_spmm_rowmajor<8>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+    } else if (k <= 16) {
+        hipLaunchKernelGGL(k_spmm_rowmajor<16>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+    } else if (k <= 32) {
+        hipLaunchKernelGGL(k_spmm_rowmajor<32>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+    } else {
+        hipLaunchKernelGGL(k_spmm_rowmajor<64>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+    }
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// ------------------------------------------------------------------ layouts
+
+__global__ void k_col_to_row(const double *__restrict__ A, long lda, double *__restrict__ B, long ldb,
+                             int n, int k)
+{
+    const long total = (long)n * k;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long r = idx / k, c = idx % k;
+        B[r * ldb + c] = A[r + c * lda];
+    }
+}
+
+__global__ void k_row_to_col(const double *__restrict__ A, long lda, double *__restrict__ B, long ldb,
+                             int n, int k)
+{
+    const long total = (long)n * k;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const long r = idx % n, c = idx / n;
+        B[r + c * ldb] = A[r * lda + c];
+    }
+}
+
+int colmajor_to_rowmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k)
+{
+    hipLaunchKernelGGL(k_col_to_row, dim3(blocks_for((long)n * k, 256, 16384)), dim3(256), 0,
+                       ctx->stream, A, lda, B, ldb, n, k);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+int rowmajor_to_colmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k)
+{
+    hipLaunchKernelGGL(k_row_to_col, dim3(blocks_for((long)n * k, 256, 16384)), dim3(256), 0,
+                       ctx->stream, A, lda, B, ldb, n, k);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+}  // namespace ipt

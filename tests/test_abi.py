@@ -1,0 +1,108 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the
+public headers declare, the ctypes struct layout equals the C layout, and
+argument validation returns the reference's status codes without touching
+a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+import block_ipt_amd
+from block_ipt_amd import abi, api
+
+
+def _declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(",
+                             text, flags=re.M):
+            if not m.group(1).startswith("ipt"):
+                continue
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    l = abi.lib()
+    names = _declared_functions()
+    assert "ipt_cuda_status_string" in names and "ipt_cuda_sparse_csc_device_tol" in names
+    missing = [n for n in sorted(names) if not hasattr(l, n)]
+    assert not missing, missing
+    # the Python binding covers exactly the declared surface
+    assert set(abi.SIGNATURES) == names
+
+
+def test_result_struct_layout_matches_c(tmp_path):
+    src = tmp_path / "probe.c"
+    src.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "ipt_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(IPTCudaResult),
+         offsetof(IPTCudaResult, matvecs), offsetof(IPTCudaResult, davidson_history),
+         offsetof(IPTCudaResult, best_so_far_source), offsetof(IPTCudaResult, cluster_soft_locked_count),
+         sizeof(IPTDavidsonHistoryEntry), sizeof(IPTDavidsonBlockHistoryEntry));
+  return 0; }
+""")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    R = abi.IPTCudaResult
+    want = [ctypes.sizeof(R), R.matvecs.offset, R.davidson_history.offset, R.best_so_far_source.offset,
+            R.cluster_soft_locked_count.offset, ctypes.sizeof(abi.IPTDavidsonHistoryEntry),
+            ctypes.sizeof(abi.IPTDavidsonBlockHistoryEntry)]
+    assert got == want
+
+
+def test_status_strings():
+    l = abi.lib()
+    assert l.ipt_cuda_status_string(0) == b"success"
+    assert l.ipt_cuda_status_string(1) == b"invalid argument"
+    assert l.ipt_cuda_status_string(2) == b"allocation failed"
+    assert l.ipt_cuda_status_string(99) == b"unknown error"
+
+
+def test_free_result_resets_fields():
+    r = abi.IPTCudaResult()
+    r.n = 5
+    abi.lib().ipt_cuda_free_result(ctypes.byref(r))
+    assert r.n == 0 and not r.vectors and np.isnan(r.fixed_point_residual)
+    assert r.davidson_target_index == -1 and r.pair28_lock_state == b"unknown"
+
+
+@pytest.mark.parametrize("n,k,tol,maxiter", [(0, 1, 1e-12, 10), (4, 0, 1e-12, 10), (4, 5, 1e-12, 10),
+                                             (4, 2, 0.0, 10), (4, 2, 1e-12, 0), (4, 2, -1.0, 10)])
+def test_invalid_arguments_rejected_before_gpu(n, k, tol, maxiter):
+    H = np.eye(max(n, 1))
+    with pytest.raises(abi.IptError) as e:
+        api.ipt_cuda_tol(H[:max(n, 1), :max(n, 1)] if n else H, k, tol, maxiter) if n else \
+            _raw_tol(n, k, tol, maxiter)
+    assert e.value.status == 1
+
+
+def _raw_tol(n, k, tol, maxiter):
+    r = abi.IPTCudaResult()
+    H = np.eye(2)
+    st = abi.lib().ipt_cuda_tol(H.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, k, tol, maxiter,
+                                ctypes.byref(r))
+    raise abi.IptError(st, "ipt_cuda_tol")
+
+
+def test_null_pointers_rejected():
+    r = abi.IPTCudaResult()
+    assert abi.lib().ipt_cuda(None, 4, 2, 5, ctypes.byref(r)) == 1
+    assert abi.lib().ipt_cuda_sparse_csc(None, None, None, 4, 2, 0, 5, ctypes.byref(r)) == 1
+    dv, dl = ctypes.c_void_p(), ctypes.c_void_p()
+    assert abi.lib().ipt_cuda_device(None, 4, 2, 5, None, ctypes.byref(dv), ctypes.byref(dl)) == 1
+    assert dv.value is None and dl.value is None

@@ -1,0 +1,125 @@
+"""GPU parity of the block-cluster -> Rayleigh-Ritz (-> Davidson) path
+against the CPU oracle and LAPACK."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+from block_ipt_amd import api  # noqa: E402
+from block_ipt_amd.abi import IptError  # noqa: E402
+
+
+@pytest.fixture
+def block_env(monkeypatch):
+    monkeypatch.setenv("IPT_BLOCK_CLUSTER", "1")
+    for v in ("IPT_DAVIDSON_ENRICH", "IPT_DAVIDSON_STEPS", "IPT_BLOCK_CLUSTER_MAXITER", "IPT_BLOCK_CLUSTER_TOL",
+              "IPT_COUPLING_GAP_RATIO_TOL", "IPT_MAX_LIFT_BLOCK_SIZE"):
+        monkeypatch.delenv(v, raising=False)
+    return monkeypatch
+
+
+def _dense_to_csc(H):
+    n = H.shape[0]
+    nz = H != 0.0
+    cp = np.concatenate([[0], np.cumsum(nz.sum(0))]).astype(np.int32)
+    ri = np.concatenate([np.nonzero(nz[:, c])[0] for c in range(n)]).astype(np.int32)
+    v = np.concatenate([H[nz[:, c], c] for c in range(n)])
+    return cp, ri, v
+
+
+def _check_eigs(H, vec, val, k, rel=1e-10, angle=1e-8, res_tol=1e-10):
+    w, V = sl.eigh(H, subset_by_index=[0, k - 1])
+    assert np.all(np.abs(val - w) <= rel * np.maximum(1, np.abs(w))), np.abs(val - w).max()
+    assert sl.subspace_angles(vec, V).max() <= angle
+    res = orc.residuals(vec, val, dense=H)
+    assert res.max() <= res_tol
+
+
+@pytest.mark.parametrize("n,k,seed,hi,vs", [(512, 16, 0, 10.0, 0.05), (700, 24, 2, 20.0, 0.08),
+                                            (2048, 16, 0, 10.0, 0.05)])
+def test_dense_block_matches_oracle_and_lapack(block_env, n, k, seed, hi, vs):
+    H = orc.rand_dense(n, seed, 0.0, hi, vs)
+    gv, gl, info = api.ipt_cuda_tol(H, k, 1e-12, 100)
+    _check_eigs(H, gv, gl, k)
+    ov, ovec, oinfo = orc.block_pipeline(k, dense=H, maxiter=100, tol=1e-12)
+    assert np.abs(gl - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
+    assert info["basis_cols"] == oinfo["basis_cols"]
+    assert info["iterations"] == oinfo["max_iterations"]
+
+
+def test_sparse_block_matches_oracle(block_env):
+    H = orc.rand_dense(600, 9, 0.0, 10.0, 0.06)
+    H[np.abs(H) < 2e-3] = 0.0
+    cp, ri, v = _dense_to_csc(H)
+    gv, gl, info = api.ipt_cuda_sparse_csc_tol(cp, ri, v, 20, 1e-12, 100)
+    _check_eigs(H, gv, gl, 20)
+    ov, ovec, oinfo = orc.block_pipeline(20, csc=(cp, ri, v), maxiter=100, tol=1e-12)
+    assert np.abs(gl - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
+    assert info["iterations"] == oinfo["max_iterations"]
+
+
+def test_sparse_block_unsorted_input(block_env):
+    H = orc.rand_dense(400, 4, 0.0, 8.0, 0.05)
+    p = np.random.default_rng(2).permutation(400)
+    Hp = H[np.ix_(p, p)]
+    gv, gl, info = api.ipt_cuda_sparse_csc_tol(*_dense_to_csc(Hp), 12, 1e-12, 100)
+    _check_eigs(Hp, gv, gl, 12)
+
+
+def test_dense_block_unsorted_input(block_env):
+    H = orc.rand_dense(500, 6, 0.0, 10.0, 0.05)
+    p = np.random.default_rng(5).permutation(500)
+    Hp = np.asfortranarray(H[np.ix_(p, p)])
+    gv, gl, info = api.ipt_cuda_tol(Hp, 10, 1e-12, 100)
+    _check_eigs(Hp, gv, gl, 10)
+
+
+def test_davidson_refinement(block_env):
+    """Few block iterations, then Davidson to the converged tolerance."""
+    block_env.setenv("IPT_BLOCK_CLUSTER_MAXITER", "2")
+    block_env.setenv("IPT_DAVIDSON_ENRICH", "1")
+    block_env.setenv("IPT_DAVIDSON_STEPS", "40")
+    block_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "1e-12")
+    n, k = 800, 16
+    H = orc.rand_dense(n, 3, 0.0, 10.0, 0.05)
+    gv, gl, info = api.ipt_cuda_tol(H, k, 1e-12, 100)
+    assert info["davidson_attempted"] == 1 and info["accepted_steps"] >= 1
+    assert info["davidson_residual_after"] <= 1e-12
+    _check_eigs(H, gv, gl, k, res_tol=1e-12)
+    ov, ovec, _ = orc.block_pipeline(k, dense=H, maxiter=2, tol=1e-12, davidson_steps=40, conv_tol=1e-12)
+    assert np.abs(gl - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
+
+
+def test_device_entry_block_path(block_env):
+    hip = api.hip()
+    n, k = 600, 20
+    H = orc.rand_dense(n, 8, 0.0, 12.0, 0.06)
+    dH = hip.to_device(np.asfortranarray(H))
+    try:
+        dv, dl, it, fp = api.ipt_cuda_device_tol(dH, n, k, 1e-12, 100)
+        vec, val = hip.to_host(dv, (n, k)), hip.to_host(dl, (k,))
+        api.free_device_result(dv, dl)
+    finally:
+        hip.free(dH)
+    _check_eigs(H, vec, val, k)
+    assert fp <= 1e-12
+
+
+def test_too_large_cluster_rejected(block_env):
+    H = orc.rand_dense(2048, 1, 0.0, 6.25, 0.1)
+    with pytest.raises(IptError) as e:
+        api.ipt_cuda_tol(H, 64, 1e-12, 100)
+    assert e.value.status == 1
+
+
+def test_nonsymmetric_rejected(block_env):
+    H = orc.kat_dense(64, 0)
+    with pytest.raises(IptError) as e:
+        api.ipt_cuda_tol(H, 4, 1e-12, 100)
+    assert e.value.status == 1
+    with pytest.raises(IptError) as e:
+        api.ipt_cuda_sparse_csc_tol(*_dense_to_csc(H), 4, 1e-12, 100)
+    assert e.value.status == 1
