@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double *__restrict__ H, 
         __syncthreads();
     }
 
-    // D layout of v_mfma_f64_16x16x4f64: lane l, element i -> row 4*(l/16)+i, col l%16.
+    // D layout of v_mfma_f64_16x16x4f64: lane l, element i -> row 4*i + l/16, col l%16.
     double *Yp = Y + (long)blockIdx.z * split_stride;
 #pragma unroll
     for (int cb = 0; cb < NB; ++cb) {
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double *__restrict__ H, 
         if (c < ncols) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const long r = rowbase + 4 * kq + i;
+                const long r = rowbase + 4 * i + kq;
                 if (r < n) {
                     Yp[r + (long)c * ldy] = acc[cb][i];
                 }

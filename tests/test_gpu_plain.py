@@ -41,8 +41,11 @@ def test_dense_eps_sweep_iterations(n):
     gv, gl, info = api.ipt_cuda_tol(H, n, g["tol"], g["maxiter"])
     assert info["iterations"] == row["ipt_iterations"]
     assert 0.2 < info["fixed_point_residual"] / row["ipt_fixed_point_residual"] < 5.0
+    # ||HV - V Lambda||_F evaluated on the host carries its own rounding
+    # noise of order eps*||H||_F, which dominates at n = 2048
     fro = np.linalg.norm(H @ gv - gv * gl)
-    assert 0.2 < fro / row["residual_ipt_frobenius"] < 5.0
+    noise = 2.0 * np.finfo(float).eps * np.linalg.norm(H)
+    assert 0.2 * row["residual_ipt_frobenius"] < fro < max(5.0 * row["residual_ipt_frobenius"], noise)
     if n <= 512:
         ov, ol, oit, ofp = orc.ipt_dense(H, n, tol=g["tol"], maxiter=g["maxiter"])
         assert oit == info["iterations"]
