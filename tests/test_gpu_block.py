@@ -41,8 +41,8 @@ def _check_eigs(H, vec, val, k, rel=1e-10, angle=1e-8, res_tol=1e-10):
     rabs = np.linalg.norm(H @ vec - vec * val, axis=0)
     for j in range(k):
         gap = np.min(np.abs(np.delete(w, j) - w[j]))
-        cosang = min(1.0, abs(V[:, j] @ vec[:, j]) / np.linalg.norm(vec[:, j]))
-        ang = np.arccos(cosang) if cosang < 1 - 1e-20 else 0.0
+        x = vec[:, j] / np.linalg.norm(vec[:, j])
+        ang = np.linalg.norm(x - V[:, j] * (V[:, j] @ x))  # sin of the angle
         assert ang <= max(angle, 10 * rabs[j] / gap), (j, ang, gap)
 
 
