@@ -161,6 +161,15 @@ SIGNATURES = {
                                                   _c.POINTER(_vp), _ip, _dp]),
     "ipt_hip_create_handle": (_c.c_int, [_c.POINTER(_vp), _vp]),
     "ipt_hip_destroy_handle": (_c.c_int, [_vp]),
+    # include/ipt_hip_ext.h
+    "ipt_hip_generate_dense": (_c.c_int, [_vp, _c.c_int, _c.c_ulonglong, _c.c_double, _c.c_double,
+                                          _c.c_double, _vp]),
+    "ipt_hip_lowp_product": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _vp, _vp]),
+    "ipt_hip_profile_enable": (None, [_c.c_int]),
+    "ipt_hip_profile_reset": (None, []),
+    "ipt_hip_profile_get": (_c.c_int, [_c.c_char_p, _dp, _c.POINTER(_c.c_longlong)]),
+    "ipt_hip_last_solve_stats": (_c.c_int, [_vp, _ip, _ip, _c.POINTER(_c.c_longlong),
+                                            _c.POINTER(_c.c_longlong), _ip]),
 }
 
 _lib = None

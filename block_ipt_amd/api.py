@@ -236,3 +236,53 @@ def create_handle(stream=None):
 
 def destroy_handle(h):
     return lib().ipt_hip_destroy_handle(ctypes.c_void_p(h))
+
+
+# ------------------------------------------------------------------ extensions
+
+LOWP_MODES = {"fp16": 0, "bf16": 1, "bf16x3": 2}
+
+
+def generate_dense(d_H, n, seed, lo, hi, vscale, handle=None):
+    """Fill device buffer d_H (n*n doubles) with the synthetic dense matrix
+    described in include/ipt_hip_ext.h."""
+    st = lib().ipt_hip_generate_dense(ctypes.c_void_p(d_H), int(n), int(seed), float(lo), float(hi),
+                                      float(vscale), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_generate_dense")
+
+
+def lowp_product(d_H, n, mode, d_B, d_Ba, d_Ya, d_diag, nc, d_Y, handle=None):
+    st = lib().ipt_hip_lowp_product(ctypes.c_void_p(d_H), int(n), LOWP_MODES.get(mode, mode),
+                                    ctypes.c_void_p(d_B), ctypes.c_void_p(d_Ba), ctypes.c_void_p(d_Ya),
+                                    ctypes.c_void_p(d_diag), int(nc), ctypes.c_void_p(d_Y), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_lowp_product")
+
+
+def profile_enable(on=True):
+    lib().ipt_hip_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    lib().ipt_hip_profile_reset()
+
+
+def profile_get(kernel):
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_longlong(0)
+    st = lib().ipt_hip_profile_get(kernel.encode(), ctypes.byref(ms), ctypes.byref(n))
+    if st != 0:
+        raise IptError(st, "ipt_hip_profile_get")
+    return ms.value, n.value
+
+
+def last_solve_stats(handle=None):
+    b, s_, dav = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    f64, lp = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    st = lib().ipt_hip_last_solve_stats(handle, ctypes.byref(b), ctypes.byref(s_), ctypes.byref(f64),
+                                        ctypes.byref(lp), ctypes.byref(dav))
+    if st != 0:
+        raise IptError(st, "ipt_hip_last_solve_stats")
+    return {"basis_cols": b.value, "steps": s_.value, "fp64_products": f64.value, "lowp_products": lp.value,
+            "davidson_steps": dav.value}

@@ -20,6 +20,9 @@ SOURCES = [
     "ipt_kernels.hip",
     "ipt_gemm_f64.hip",
     "ipt_block_kernels.hip",
+    "ipt_gemm_lp.hip",
+    "ipt_gen.hip",
+    "ipt_profile.cpp",
 ]
 
 # Packed-FP32 VALU ops are disabled for all device code.

@@ -28,6 +28,7 @@
 #include <functional>
 
 #include "ipt_block.h"
+#include "ipt_lowp.h"
 #include "ipt_prep.h"
 
 namespace ipt {
@@ -50,6 +51,9 @@ struct BlockParams {
     int davidson_steps = 1;
     int davidson_extra = 0;
     double conv_tol = 1e-13;
+    int lowp_mode = -1;     // -1: every product in FP64
+    int anchor_every = 20;  // low-precision steps between exact FP64 steps
+    int verbose = 0;
 };
 
 // Rayleigh-Ritz over V (n x cols) with HV = H V: generalized problem
@@ -387,13 +391,20 @@ struct BlockResult {
     double iteration_time = 0.0;
     double rr_time = 0.0;
     double davidson_time = 0.0;
+    long long fp64_products = 0;  // H * block products in FP64
+    long long lowp_products = 0;  // low-precision correction products
     DavidsonStats dav;
 };
 
-// Block-cluster IPT on a prepared (sorted-diagonal) operator.
-int block_pipeline(Context *ctx, const ApplyFn &apply, const double *d, int n, int k,
-                   const std::vector<std::pair<int, int>> &ranges, double normH, const BlockParams &p,
-                   BlockResult *out)
+// Block-cluster IPT on a prepared (sorted-diagonal) operator.  With a
+// low-precision operator the iteration alternates `anchor_every` steps whose
+// product is the anchored correction (see ipt_gemm_lp.hip) with one exact
+// FP64 step; clusters are frozen only on exact steps, so the stopping rule
+// is the FP64 one.  `identity_product` (optional) fills H * identity blocks
+// without a product (dense: gathered columns of H).
+int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_product, const LowpOperator *lowp,
+                   const double *d, int n, int k, const std::vector<std::pair<int, int>> &ranges, double normH,
+                   const BlockParams &p, BlockResult *out)
 {
     BlockPlan plan;
     IPT_STCHK(block_plan_create(ctx, ranges, n, &plan));
@@ -407,20 +418,48 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const double *d, int n, i
         cap = std::max(bc, kmax) + k + 1;
         cap = std::min(cap, std::max(bc, n));
     }
-    DevBuf<double> Ba, Bb, Y;
+    const bool mixed = lowp != nullptr;
+    DevBuf<double> Ba, Bb, Y, Banc, Yanc;
     IPT_STCHK(Ba.alloc((size_t)n * cap));
     IPT_STCHK(Bb.alloc((size_t)n * bc));
     IPT_STCHK(Y.alloc((size_t)n * cap));
     IPT_STCHK(block_identity(ctx, plan, Ba.p, n, n));
+    if (mixed) {
+        IPT_STCHK(Banc.alloc((size_t)n * bc));
+        IPT_STCHK(Yanc.alloc((size_t)n * bc));
+        IPT_HIPCHK(hipMemcpyAsync(Banc.p, Ba.p, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        if (identity_product) {
+            IPT_STCHK(identity_product(Banc.p, Yanc.p, bc));
+        } else {
+            IPT_STCHK(apply(Banc.p, Yanc.p, bc));
+            out->fp64_products += 1;
+        }
+    }
     double *cur = Ba.p, *nxt = Bb.p;
     auto t0 = std::chrono::steady_clock::now();
-    int done = 0;
     for (int it = 0; it < p.maxiter; ++it) {
-        IPT_STCHK(apply(cur, Y.p, bc));
-        out->matvecs += bc;
-        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, p.tol));
+        const bool exact = !mixed || (it % (p.anchor_every + 1) == p.anchor_every);
+        if (exact) {
+            IPT_STCHK(apply(cur, Y.p, bc));
+            out->fp64_products += 1;
+            out->matvecs += bc;
+        } else {
+            IPT_STCHK(lowp_apply_delta(ctx, *lowp, cur, Banc.p, Yanc.p, n, d, bc, Y.p));
+            out->lowp_products += 1;
+        }
+        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, exact ? p.tol : -1.0));
+        if (mixed && exact) {
+            // the exact pair (B, H B) becomes the anchor of the next steps
+            IPT_HIPCHK(hipMemcpyAsync(Banc.p, cur, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            IPT_HIPCHK(hipMemcpyAsync(Yanc.p, Y.p, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+        }
         std::swap(cur, nxt);
-        done = it + 1;
+        if (!exact) {
+            continue;
+        }
         int flags[3];
         IPT_STCHK(read_host(ctx, flags, plan.flags.p, 3 * sizeof(int)));
         if (flags[1] != 0) {
@@ -433,7 +472,6 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const double *d, int n, i
             break;
         }
     }
-    (void)done;
     // per-cluster iterations and residuals
     {
         std::vector<int> its((size_t)plan.nclusters);
@@ -456,6 +494,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const double *d, int n, i
     auto t1 = std::chrono::steady_clock::now();
     IPT_STCHK(apply(Ba.p, Y.p, bc));
     out->matvecs += bc;
+    out->fp64_products += 1;
     IPT_STCHK(out->theta.alloc((size_t)cap));
     IPT_STCHK(out->X.alloc((size_t)n * k));
     IPT_STCHK(out->HX.alloc((size_t)n * k));
@@ -492,6 +531,22 @@ BlockParams params_from_env(int maxiter, double tol, int use_tol)
     p.davidson_steps = env_int("IPT_DAVIDSON_STEPS", 1);
     p.davidson_extra = env_int("IPT_DAVIDSON_EXTRA_STEPS", 0);
     p.conv_tol = env_double("IPT_DAVIDSON_CONVERGED_TOL", 1.0e-13);
+    // additions: precision of the correction products and anchor interval
+    const char *prec = getenv("IPT_PRECISION");
+    if (prec != nullptr) {
+        if (!strcmp(prec, "fp16")) {
+            p.lowp_mode = LOWP_FP16;
+        } else if (!strcmp(prec, "bf16")) {
+            p.lowp_mode = LOWP_BF16;
+        } else if (!strcmp(prec, "bf16x3") || !strcmp(prec, "tf32")) {
+            p.lowp_mode = LOWP_BF16X3;
+        }
+    }
+    p.anchor_every = env_int("IPT_MIXED_ANCHOR_EVERY", 20);
+    if (p.anchor_every <= 0) {
+        p.anchor_every = 20;
+    }
+    p.verbose = env_flag("IPT_HIP_VERBOSE");
     return p;
 }
 
@@ -559,6 +614,7 @@ struct DensePrep {
     DevBuf<int> dperm;
     std::vector<double> hd;             // sorted diagonal (host)
     double normH = 0.0;
+    double max_offdiag = 0.0;
     std::vector<std::pair<int, int>> ranges;
 };
 
@@ -634,6 +690,10 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr)
         pr->H = dH;
     }
     pr->hd = sd;
+    pr->max_offdiag = 0.0;
+    for (double x : soff) {
+        pr->max_offdiag = std::max(pr->max_offdiag, x);
+    }
     // candidate (row, target) pairs of the detection windows
     const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
     const double eps = DBL_EPSILON;
@@ -708,8 +768,38 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     const BlockParams p = params_from_env(maxiter, tol, use_tol);
     const double *H = pr.H;
     ApplyFn apply = [&](const double *X, double *Y, int nc) { return gemm_f64(ctx, H, n, X, n, Y, n, n, nc); };
+    // H * (identity block of cluster [first,last]) = columns first..last of H
+    const std::vector<std::pair<int, int>> &rg = pr.ranges;
+    ApplyFn identity_product = [&](const double *, double *Y, int) -> int {
+        int off = 0;
+        for (const auto &r : rg) {
+            const int m = r.second - r.first + 1;
+            IPT_HIPCHK(hipMemcpyAsync(Y + (size_t)off * n, H + (size_t)r.first * n, (size_t)n * m * sizeof(double),
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+            off += m;
+        }
+        return IPT_CUDA_SUCCESS;
+    };
+    LowpOperator lowp;
+    const LowpOperator *lp = nullptr;
+    if (p.lowp_mode >= 0) {
+        IPT_STCHK(lowp_operator_create(ctx, H, n, n, p.lowp_mode, pr.max_offdiag, &lowp));
+        lp = &lowp;
+    }
     BlockResult br;
-    IPT_STCHK(block_pipeline(ctx, apply, pr.d.p, n, k, pr.ranges, pr.normH, p, &br));
+    IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp, pr.d.p, n, k, pr.ranges, pr.normH, p, &br));
+    ctx->last_basis_cols = br.basis_cols;
+    ctx->last_steps = br.iterations;
+    ctx->last_fp64_products = br.fp64_products;
+    ctx->last_lowp_products = br.lowp_products;
+    ctx->last_davidson_steps = br.dav.accepted_steps;
+    if (p.verbose) {
+        fprintf(stderr,
+                "IPT block-cluster dense: n=%d k=%d clusters=%zu basis_cols=%d precision=%d steps=%d "
+                "fp64_products=%lld lowp_products=%lld davidson_steps=%d max_fp=%.3e\n",
+                n, k, pr.ranges.size(), br.basis_cols, p.lowp_mode, br.iterations, br.fp64_products,
+                br.lowp_products, br.dav.accepted_steps, br.fp_max);
+    }
     DevBuf<double> vec, val;
     IPT_STCHK(vec.alloc((size_t)n * k));
     IPT_STCHK(val.alloc((size_t)k));
@@ -850,7 +940,12 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
             return IPT_CUDA_SUCCESS;
         };
         BlockResult br;
-        IPT_STCHK(block_pipeline(ctx, apply, dd.p, n, k, ranges, normH, p, &br));
+        IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), nullptr, dd.p, n, k, ranges, normH, p, &br));
+        ctx->last_basis_cols = br.basis_cols;
+        ctx->last_steps = br.iterations;
+        ctx->last_fp64_products = br.fp64_products;
+        ctx->last_lowp_products = br.lowp_products;
+        ctx->last_davidson_steps = br.dav.accepted_steps;
         DevBuf<double> vec;
         IPT_STCHK(vec.alloc((size_t)n * k));
         IPT_STCHK(unpermute_rows(ctx, br.X.p, vec.p, n, n, k, dperm.p));
@@ -873,3 +968,21 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
 }
 
 }  // namespace ipt
+
+using namespace ipt;
+
+extern "C" int ipt_hip_last_solve_stats(void *handle, int *basis_cols, int *steps, long long *fp64_products,
+                                        long long *lowp_products, int *davidson_steps)
+{
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    if (basis_cols) *basis_cols = ctx->last_basis_cols;
+    if (steps) *steps = ctx->last_steps;
+    if (fp64_products) *fp64_products = ctx->last_fp64_products;
+    if (lowp_products) *lowp_products = ctx->last_lowp_products;
+    if (davidson_steps) *davidson_steps = ctx->last_davidson_steps;
+    return IPT_CUDA_SUCCESS;
+}

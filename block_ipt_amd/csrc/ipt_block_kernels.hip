@@ -315,6 +315,7 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
     IPT_STCHK(context_scratch(ctx, SLOT_REDUCE,
                               (size_t)std::max(plan.nclusters, 1) * plan.partial_stride * 2 * sizeof(double),
                               (void **)&partials));
+    const ProfileToken prof = profile_begin(ctx, PROF_BLOCK_UPDATE);
     for (int cls = 0; cls < 7; ++cls) {
         const int cnt = plan.class_count[cls];
         if (cnt == 0) {
@@ -339,6 +340,7 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
 #undef IPT_BLK
         IPT_HIPCHK(hipGetLastError());
     }
+    profile_end(ctx, prof);
     hipLaunchKernelGGL(k_block_finalize, dim3(1), dim3(256), 0, ctx->stream, partials, plan.partial_stride,
                        plan.clusters.p, plan.nclusters, plan.active.p, plan.fp.p, plan.iters.p, tol,
                        plan.flags.p);

@@ -178,6 +178,7 @@ static int gemm_f64_pass(Context *ctx, const double *H, long ldh, const double *
         out_ld = n;
         stride = (long)n * ncols;
     }
+    const ProfileToken prof = profile_begin(ctx, PROF_GEMM_F64);
     switch (NB) {
     case 1: launch_gemm<1>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
     case 2: launch_gemm<2>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
@@ -188,6 +189,7 @@ static int gemm_f64_pass(Context *ctx, const double *H, long ldh, const double *
     case 7: launch_gemm<7>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
     default: launch_gemm<8>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
     }
+    profile_end(ctx, prof);
     IPT_HIPCHK(hipGetLastError());
     if (splits > 1) {
         const long total = (long)n * ncols;

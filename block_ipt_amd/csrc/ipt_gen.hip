@@ -1,0 +1,79 @@
+// Synthetic dense workload generator (device side).  The recipe is stated
+// in include/ipt_hip_ext.h; the splitmix64 counter stream follows
+// IPT/test/mixed_precision_tf32/benchmark_tf32_cusolver_sweep.cu:434-449.
+#include <algorithm>
+
+#include "ipt_internal.h"
+#include "../../include/ipt_hip_ext.h"
+
+namespace ipt {
+
+__host__ __device__ inline unsigned long long splitmix64(unsigned long long x)
+{
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+__host__ __device__ inline double uniform01(unsigned long long seed, long long n, long long row, long long col)
+{
+    unsigned long long x = seed + (unsigned long long)n * 0xd1b54a32d192ed03ULL;
+    x ^= (unsigned long long)(row + 1) * 0x9e3779b97f4a7c15ULL;
+    x ^= (unsigned long long)(col + 1) * 0xbf58476d1ce4e5b9ULL;
+    return (double)(splitmix64(x) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_gen_dense(double *__restrict__ H, int n, unsigned long long seed, double s,
+                            const double *__restrict__ d)
+{
+    const long total = (long)n * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const long r = t % n, c = t / n;
+        double v;
+        if (r == c) {
+            v = d[r];
+        } else {
+            const long a = r < c ? r : c, b = r < c ? c : r;
+            const double u1 = uniform01(seed, n, a, b);
+            const double u2 = uniform01(seed, n, b, a);
+            v = s * sqrt(-2.0 * log1p(-u1)) * cos(6.283185307179586 * u2);
+        }
+        H[t] = v;
+    }
+}
+
+}  // namespace ipt
+
+using namespace ipt;
+
+extern "C" int ipt_hip_generate_dense(double *d_H, int n, unsigned long long seed, double lo, double hi,
+                                      double vscale, void *handle)
+{
+    if (d_H == nullptr || n <= 0) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    std::vector<double> d((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        d[(size_t)i] = lo + (hi - lo) * uniform01(seed, n, i, n);
+    }
+    std::sort(d.begin(), d.end());
+    DevBuf<double> dd;
+    IPT_STCHK(dd.alloc((size_t)n));
+    IPT_HIPCHK(hipMemcpyAsync(dd.p, d.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const long total = (long)n * n;
+    long blocks = (total + 255) / 256;
+    if (blocks > 65536) {
+        blocks = 65536;
+    }
+    hipLaunchKernelGGL(k_gen_dense, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d_H, n, seed,
+                       vscale / std::sqrt((double)n), dd.p);
+    IPT_HIPCHK(hipGetLastError());
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}

@@ -43,6 +43,12 @@ struct Context {
     double *host_scalars = nullptr;  // pinned, 4096 doubles
     int *host_flags = nullptr;       // pinned, 4096 ints
     void *slots[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // statistics of the last block-cluster solve
+    int last_basis_cols = 0;
+    int last_steps = 0;
+    long long last_fp64_products = 0;
+    long long last_lowp_products = 0;
+    int last_davidson_steps = 0;
     size_t slot_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
@@ -104,6 +110,18 @@ struct DevBuf {
         return q;
     }
 };
+
+// ------------------------------------------------------------------ profiling
+
+enum { PROF_GEMM_F64 = 0, PROF_LOWP_GEMM = 1, PROF_BLOCK_UPDATE = 2, PROF_SPMM = 3, PROF_COUNT = 4 };
+struct ProfileToken {
+    int id = -1;
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+bool profile_on();
+ProfileToken profile_begin(Context *ctx, int id);
+void profile_end(Context *ctx, const ProfileToken &t);
 
 // ------------------------------------------------------------------ options
 

@@ -243,6 +243,7 @@ int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, cons
 {
     const int blocks = blocks_for((long)n * 64, 256, 65536);
     dim3 g(blocks), b(256);
+    const ProfileToken prof = profile_begin(ctx, PROF_SPMM);
     if (k <= 1) {
         hipLaunchKernelGGL(k_spmm_rowmajor<1>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
     } else if (k <= 2) {
@@ -258,6 +259,7 @@ int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, cons
     } else {
         hipLaunchKernelGGL(k_spmm_rowmajor<64>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
     }
+    profile_end(ctx, prof);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

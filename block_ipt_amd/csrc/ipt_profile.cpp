@@ -1,0 +1,108 @@
+// Kernel timing instrumentation: HIP events recorded around the launches
+// of the main kernels on the stream they run on; summed on request.
+#include <cstring>
+#include <mutex>
+
+#include "ipt_internal.h"
+#include "../../include/ipt_hip_ext.h"
+
+namespace ipt {
+
+namespace {
+const char *kNames[PROF_COUNT] = {"gemm_f64", "lowp_gemm", "block_update", "spmm"};
+std::mutex g_mu;
+bool g_on = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev[PROF_COUNT];
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t take_event()
+{
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    return e;
+}
+}  // namespace
+
+bool profile_on() { return g_on; }
+
+ProfileToken profile_begin(Context *ctx, int id)
+{
+    ProfileToken t;
+    t.id = -1;
+    if (!g_on || id < 0 || id >= PROF_COUNT) {
+        return t;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    t.start = take_event();
+    t.stop = take_event();
+    if (t.start == nullptr || t.stop == nullptr) {
+        return t;
+    }
+    (void)hipEventRecord(t.start, ctx->stream);
+    t.id = id;
+    return t;
+}
+
+void profile_end(Context *ctx, const ProfileToken &t)
+{
+    if (t.id < 0) {
+        return;
+    }
+    (void)hipEventRecord(t.stop, ctx->stream);
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_ev[t.id].emplace_back(t.start, t.stop);
+}
+
+}  // namespace ipt
+
+using namespace ipt;
+
+extern "C" void ipt_hip_profile_enable(int on) { g_on = on != 0; }
+
+extern "C" void ipt_hip_profile_reset(void)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    for (int i = 0; i < PROF_COUNT; ++i) {
+        for (auto &p : g_ev[i]) {
+            (void)hipEventSynchronize(p.second);
+            g_pool.push_back(p.first);
+            g_pool.push_back(p.second);
+        }
+        g_ev[i].clear();
+    }
+}
+
+extern "C" int ipt_hip_profile_get(const char *kernel, double *total_ms, long long *launches)
+{
+    if (kernel == nullptr || total_ms == nullptr || launches == nullptr) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int id = -1;
+    for (int i = 0; i < PROF_COUNT; ++i) {
+        if (!strcmp(kernel, kNames[i])) {
+            id = i;
+        }
+    }
+    if (id < 0) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    double sum = 0.0;
+    for (auto &p : g_ev[id]) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.second) != hipSuccess || hipEventElapsedTime(&ms, p.first, p.second) != hipSuccess) {
+            return IPT_CUDA_CUDA_ERROR;
+        }
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (long long)g_ev[id].size();
+    return IPT_CUDA_SUCCESS;
+}

@@ -195,6 +195,34 @@ typedef struct {
 static void orc_apply(const OrcMat *M, const double *X, int cols, double *Y)
 {
     int n = M->n;
+    if (M->dense != NULL) {
+        /* rows are independent: threads own row chunks and stream the
+         * column-major matrix once per column of X */
+        const int chunk = 512;
+        const int nchunks = (n + chunk - 1) / chunk;
+#pragma omp parallel for schedule(static)
+        for (int b = 0; b < nchunks; ++b) {
+            const int r0 = b * chunk, r1 = r0 + chunk < n ? r0 + chunk : n;
+            for (int j = 0; j < cols; ++j) {
+                double *y = Y + (size_t)j * n;
+                const double *x = X + (size_t)j * n;
+                for (int r = r0; r < r1; ++r) {
+                    y[r] = 0.0;
+                }
+                for (int c = 0; c < n; ++c) {
+                    const double xc = x[c];
+                    const double *hc = M->dense + (size_t)c * n;
+                    if (xc == 0.0) {
+                        continue;
+                    }
+                    for (int r = r0; r < r1; ++r) {
+                        y[r] += hc[r] * xc;
+                    }
+                }
+            }
+        }
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < cols; ++j) {
         double *y = Y + (size_t)j * n;
@@ -202,18 +230,7 @@ static void orc_apply(const OrcMat *M, const double *X, int cols, double *Y)
         for (int r = 0; r < n; ++r) {
             y[r] = 0.0;
         }
-        if (M->dense != NULL) {
-            for (int c = 0; c < n; ++c) {
-                double xc = x[c];
-                const double *hc = M->dense + (size_t)c * n;
-                if (xc == 0.0) {
-                    continue;
-                }
-                for (int r = 0; r < n; ++r) {
-                    y[r] += hc[r] * xc;
-                }
-            }
-        } else {
+        {
             for (int c = 0; c < n; ++c) {
                 double xc = x[c];
                 for (int p = M->col_ptr[c]; p < M->col_ptr[c + 1]; ++p) {
