@@ -108,11 +108,14 @@ __global__ __launch_bounds__(256) void k_block_update(
                         const double f = a[pc];
 #pragma unroll
                         for (int c = pc; c <= P; ++c) {
-                            const double pr = __shfl(a[c], cidx, P) * inv;
-                            if (t == cidx) {
-                                a[c] = pr;
-                            } else {
-                                a[c] -= f * pr;
+                            // columns m..P-1 are zero padding (m is uniform per block)
+                            if (c < m || c == P) {
+                                const double pr = __shfl(a[c], cidx, P) * inv;
+                                if (t == cidx) {
+                                    a[c] = pr;
+                                } else {
+                                    a[c] -= f * pr;
+                                }
                             }
                         }
                         if (t == cidx) {
@@ -161,33 +164,37 @@ __global__ __launch_bounds__(256) void k_block_update(
 
 // Per-cluster fixed-point residual fp = ||B'-B||_F / max(1, ||B||_F)
 // (IPT/src/ipt_cuda.cu:2984-3003); clusters reaching fp <= tol are frozen.
-// flags[0]: singular (reset here after reading), flags[1]: status bits
-// (1 non-finite norm, 2 singular), flags[2]: clusters still active.
-// Single block.
-__global__ void k_block_finalize(const double *__restrict__ partials, int pstride,
-                                 const BlockCluster *__restrict__ clusters, int nclusters,
-                                 int *__restrict__ active, double *__restrict__ fp_out,
-                                 int *__restrict__ iters, double tol, int *__restrict__ flags)
+// One block per cluster, fixed-order reduction of the update's partials.
+// flags[1] |= 1 on a non-finite norm.
+__global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict__ partials, int pstride,
+                                                        const BlockCluster *__restrict__ clusters,
+                                                        int *__restrict__ active, double *__restrict__ fp_out,
+                                                        int *__restrict__ iters, double tol,
+                                                        int *__restrict__ flags)
 {
-    __shared__ int s_sing;
-    __shared__ int s_live;
-    if (threadIdx.x == 0) {
-        s_sing = flags[0];
-        s_live = 0;
+    const int c = blockIdx.x;
+    if (!active[c]) {
+        return;
+    }
+    const int nb = clusters[c].nblocks;
+    double dn = 0.0, bn = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        dn += partials[2 * ((long)c * pstride + b)];
+        bn += partials[2 * ((long)c * pstride + b) + 1];
+    }
+    __shared__ double sa[4], sb[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        dn += __shfl_down(dn, o, 64);
+        bn += __shfl_down(bn, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = dn;
+        sb[threadIdx.x >> 6] = bn;
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < nclusters; c += blockDim.x) {
-        if (!active[c]) {
-            continue;
-        }
-        const int nb = clusters[c].nblocks;
-        double dn = 0.0, bn = 0.0;
-        for (int b = 0; b < nb; ++b) {
-            dn += partials[2 * ((long)c * pstride + b)];
-            bn += partials[2 * ((long)c * pstride + b) + 1];
-        }
-        dn = sqrt(dn);
-        bn = sqrt(bn);
+    if (threadIdx.x == 0) {
+        dn = sqrt(sa[0] + sa[1] + sa[2] + sa[3]);
+        bn = sqrt(sb[0] + sb[1] + sb[2] + sb[3]);
         if (!isfinite(dn) || !isfinite(bn) || bn <= 0.0) {
             atomicOr(&flags[1], 1);
         }
@@ -196,18 +203,23 @@ __global__ void k_block_finalize(const double *__restrict__ partials, int pstrid
         iters[c] += 1;
         if (fp <= tol) {
             active[c] = 0;
-        } else {
-            atomicAdd(&s_live, 1);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_sing) {
-            atomicOr(&flags[1], 2);
-        }
-        flags[0] = 0;
-        flags[2] = s_live;
+}
+
+// flags[1] |= 2 when a solve was singular (flags[0], then cleared);
+// flags[2] = clusters still active.  Single thread.
+__global__ void k_block_flags(const int *__restrict__ active, int nclusters, int *__restrict__ flags)
+{
+    if (flags[0]) {
+        flags[1] |= 2;
     }
+    flags[0] = 0;
+    int live = 0;
+    for (int c = 0; c < nclusters; ++c) {
+        live += active[c] != 0;
+    }
+    flags[2] = live;
 }
 
 __global__ void k_block_reset(int *__restrict__ active, int *__restrict__ iters, double *__restrict__ fp,
@@ -341,9 +353,11 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
         IPT_HIPCHK(hipGetLastError());
     }
     profile_end(ctx, prof);
-    hipLaunchKernelGGL(k_block_finalize, dim3(1), dim3(256), 0, ctx->stream, partials, plan.partial_stride,
-                       plan.clusters.p, plan.nclusters, plan.active.p, plan.fp.p, plan.iters.p, tol,
+    hipLaunchKernelGGL(k_block_finalize, dim3(plan.nclusters), dim3(256), 0, ctx->stream, partials,
+                       plan.partial_stride, plan.clusters.p, plan.active.p, plan.fp.p, plan.iters.p, tol,
                        plan.flags.p);
+    IPT_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_block_flags, dim3(1), dim3(1), 0, ctx->stream, plan.active.p, plan.nclusters, plan.flags.p);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

@@ -144,9 +144,10 @@ int lowp_operator_create(Context *ctx, const double *H, long ld, int n, int mode
 
 // ------------------------------------------------------------------ delta
 
-// colmax[c] = max_r |B - Ba|  (IEEE bits, order free)
-__global__ void k_delta_colmax(const double *__restrict__ B, const double *__restrict__ Ba, long ld, int n, int nc,
-                               unsigned long long *__restrict__ colmax)
+// colmax[c] = max_r |B - Ba|: blockIdx.y = column, blockIdx.x = row
+// quarter; one atomicMax on the IEEE bits per block (order free).
+__global__ __launch_bounds__(256) void k_delta_colmax(const double *__restrict__ B, const double *__restrict__ Ba,
+                                                      long ld, int n, int nc, unsigned long long *__restrict__ colmax)
 {
     const int c = blockIdx.y;
     double m = 0.0;
@@ -156,8 +157,16 @@ __global__ void k_delta_colmax(const double *__restrict__ B, const double *__res
     for (int o = 32; o > 0; o >>= 1) {
         m = fmax(m, __shfl_down(m, o, 64));
     }
-    if ((threadIdx.x & 63) == 0 && m > 0.0) {
-        atomicMax(&colmax[c], (unsigned long long)__double_as_longlong(m));
+    __shared__ double sm[4];
+    if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+        if (m > 0.0) {
+            atomicMax(&colmax[c], (unsigned long long)__double_as_longlong(m));
+        }
     }
 }
 
@@ -198,9 +207,12 @@ __global__ void k_delta_convert(const double *__restrict__ B, const double *__re
 
 // ------------------------------------------------------------------ GEMM
 
-// Partial products P[z] (FP32, column-major n x nc) over k blocks of split
-// z.  Workgroup = 4 wavefronts x 2 row blocks = 128 rows; each wavefront
-// reuses every B fragment for its two row blocks.
+// Partial products P[z] (FP32, column-major n x nc) over the k blocks of
+// split z.  A workgroup (4 wavefronts) owns two 16-row blocks; its waves
+// take interleaved k blocks (wave w: kb0+w, kb0+w+4, ...) so the four
+// consecutive 1 KiB tiles in flight per row block are adjacent in HBM, and
+// the next k block of a wave is loaded before the MFMAs of the current one.
+// The four partial accumulators are summed through LDS at the end.
 template <int NB, int MODE>
 __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi, const uint4 *__restrict__ Elo,
                                                    const uint4 *__restrict__ Zhi, const uint4 *__restrict__ Zlo,
@@ -209,13 +221,15 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int rb0 = (blockIdx.x * 4 + wave) * 2;
+    const int rb0 = blockIdx.x * 2;
     const int kb0 = blockIdx.z * kb_per_split;
     int kb1 = kb0 + kb_per_split;
     if (kb1 > nkb) {
         kb1 = nkb;
     }
     const bool ok0 = rb0 < nrb, ok1 = rb0 + 1 < nrb;
+    const int li = lane & 15, lk = lane >> 4;
+    const uint4 zero = {0u, 0u, 0u, 0u};
     f4_t acc[2][NB];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -224,42 +238,84 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
             acc[a][c] = (f4_t){0.f, 0.f, 0.f, 0.f};
         }
     }
-    const int li = lane & 15, lk = lane >> 4;
-    const uint4 zero = {0u, 0u, 0u, 0u};
-    for (int kb = kb0; kb < kb1; ++kb) {
-        const uint4 a0 = ok0 ? Ehi[((long)rb0 * nkb + kb) * 64 + lane] : zero;
-        const uint4 a1 = ok1 ? Ehi[((long)(rb0 + 1) * nkb + kb) * 64 + lane] : zero;
-        uint4 a0l = zero, a1l = zero;
-        if (MODE == LOWP_BF16X3) {
-            a0l = ok0 ? Elo[((long)rb0 * nkb + kb) * 64 + lane] : zero;
-            a1l = ok1 ? Elo[((long)(rb0 + 1) * nkb + kb) * 64 + lane] : zero;
-        }
+    const long arow0 = (long)rb0 * nkb * 64 + lane;
+    const long arow1 = arow0 + (long)nkb * 64;
+    auto loadA = [&](const uint4 *E, int kb, uint4 &x0, uint4 &x1) {
+        x0 = ok0 ? E[arow0 + (long)kb * 64] : zero;
+        x1 = ok1 ? E[arow1 + (long)kb * 64] : zero;
+    };
+    auto loadB = [&](const uint4 *Z, int kb, uint4 *b) {
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) {
             const int col = cb * 16 + li;
-            const long zoff = ((long)col * np + (long)kb * 32 + 8 * lk) >> 3;
-            const uint4 b = col < nc ? Zhi[zoff] : zero;
+            b[cb] = col < nc ? Z[((long)col * np + (long)kb * 32 + 8 * lk) >> 3] : zero;
+        }
+    };
+    int kb = kb0 + wave;
+    uint4 a0, a1, a0l = zero, a1l = zero, bh[NB], bl[NB];
+    if (kb < kb1) {
+        loadA(Ehi, kb, a0, a1);
+        loadB(Zhi, kb, bh);
+        if (MODE == LOWP_BF16X3) {
+            loadA(Elo, kb, a0l, a1l);
+            loadB(Zlo, kb, bl);
+        }
+    }
+    for (; kb < kb1; kb += 4) {
+        uint4 c0 = a0, c1 = a1, c0l = a0l, c1l = a1l, ch[NB], cl[NB];
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) {
+            ch[cb] = bh[cb];
+            if (MODE == LOWP_BF16X3) {
+                cl[cb] = bl[cb];
+            }
+        }
+        const int kn = kb + 4;
+        if (kn < kb1) {
+            loadA(Ehi, kn, a0, a1);
+            loadB(Zhi, kn, bh);
+            if (MODE == LOWP_BF16X3) {
+                loadA(Elo, kn, a0l, a1l);
+                loadB(Zlo, kn, bl);
+            }
+        }
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) {
             if (MODE == LOWP_FP16) {
-                const h8_t av0 = __builtin_bit_cast(h8_t, a0), av1 = __builtin_bit_cast(h8_t, a1);
-                const h8_t bv = __builtin_bit_cast(h8_t, b);
-                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av0, bv, acc[0][cb], 0, 0, 0);
-                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av1, bv, acc[1][cb], 0, 0, 0);
+                const h8_t bv = __builtin_bit_cast(h8_t, ch[cb]);
+                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c0), bv, acc[0][cb], 0, 0, 0);
+                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c1), bv, acc[1][cb], 0, 0, 0);
             } else {
-                const b8_t av0 = __builtin_bit_cast(b8_t, a0), av1 = __builtin_bit_cast(b8_t, a1);
-                const b8_t bv = __builtin_bit_cast(b8_t, b);
-                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av0, bv, acc[0][cb], 0, 0, 0);
-                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av1, bv, acc[1][cb], 0, 0, 0);
+                const b8_t bv = __builtin_bit_cast(b8_t, ch[cb]);
+                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), bv, acc[0][cb], 0, 0, 0);
+                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), bv, acc[1][cb], 0, 0, 0);
                 if (MODE == LOWP_BF16X3) {
-                    const uint4 bl = col < nc ? Zlo[zoff] : zero;
-                    const b8_t blv = __builtin_bit_cast(b8_t, bl);
-                    const b8_t al0 = __builtin_bit_cast(b8_t, a0l), al1 = __builtin_bit_cast(b8_t, a1l);
-                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av0, blv, acc[0][cb], 0, 0, 0);
-                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av1, blv, acc[1][cb], 0, 0, 0);
-                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al0, bv, acc[0][cb], 0, 0, 0);
-                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al1, bv, acc[1][cb], 0, 0, 0);
+                    const b8_t blv = __builtin_bit_cast(b8_t, cl[cb]);
+                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), blv, acc[0][cb], 0, 0, 0);
+                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), blv, acc[1][cb], 0, 0, 0);
+                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0l), bv, acc[0][cb], 0, 0, 0);
+                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1l), bv, acc[1][cb], 0, 0, 0);
                 }
             }
         }
+    }
+    // reduce the four waves' accumulators through LDS (waves 1..3 -> 0)
+    __shared__ float red[3][2 * NB * 4][64];
+    if (wave > 0) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    red[wave - 1][(a * NB + cb) * 4 + v][lane] = acc[a][cb][v];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (wave != 0) {
+        return;
     }
     // D layout (f32 16x16x32): lane l, element v -> row 4*(l/16)+v, col l%16
     float *Pz = P + (long)blockIdx.z * n * nc;
@@ -269,14 +325,13 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) {
             const int c = cb * 16 + li;
-            if (c >= nc) {
-                continue;
-            }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
+                const int slot = (a * NB + cb) * 4 + v;
+                const float sum = acc[a][cb][v] + red[0][slot][lane] + red[1][slot][lane] + red[2][slot][lane];
                 const long r = (long)rb * 16 + 4 * lk + v;
-                if (r < n) {
-                    Pz[r + (long)c * n] = acc[a][cb][v];
+                if (c < nc && r < n) {
+                    Pz[r + (long)c * n] = sum;
                 }
             }
         }
@@ -328,7 +383,7 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
     for (int c0 = 0; c0 < nc; c0 += 128) {
         const int w = std::min(128, nc - c0);
         // scratch: colmax (w u64) | colscale (w) | Z hi/lo (np*w halves each) | partials
-        const int wgx = (op.nrb + 7) / 8;
+        const int wgx = (op.nrb + 1) / 2;
         int splits = std::max(1, 1024 / std::max(wgx, 1));
         splits = std::min(splits, op.nkb);
         const int kbs = (op.nkb + splits - 1) / splits;
@@ -345,7 +400,7 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         const double *Bc = B + (size_t)c0 * ld;
         const double *Bac = Ba + (size_t)c0 * ld;
         IPT_HIPCHK(hipMemsetAsync(colmax, 0, (size_t)w * 8, ctx->stream));
-        hipLaunchKernelGGL(k_delta_colmax, dim3(std::min(64, (n + 255) / 256), w), dim3(256), 0, ctx->stream, Bc, Bac,
+        hipLaunchKernelGGL(k_delta_colmax, dim3(std::min(8, (n + 255) / 256), w), dim3(256), 0, ctx->stream, Bc, Bac,
                            ld, n, w, colmax);
         IPT_HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_delta_convert, dim3(blocks_for((long)op.np * w, 256, 16384)), dim3(256), 0, ctx->stream,
