@@ -32,6 +32,19 @@ namespace ipt {
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+
+// streamed once: non-temporal 16-byte load
+__device__ inline uint4 load_nt(const uint4 *p)
+{
+    const u4_t v = __builtin_nontemporal_load(reinterpret_cast<const u4_t *>(p));
+    uint4 r;
+    r.x = v.x;
+    r.y = v.y;
+    r.z = v.z;
+    r.w = v.w;
+    return r;
+}
 
 static inline int blocks_for(long total, int threads, int cap)
 {
@@ -208,11 +221,11 @@ __global__ void k_delta_convert(const double *__restrict__ B, const double *__re
 // ------------------------------------------------------------------ GEMM
 
 // Partial products P[z] (FP32, column-major n x nc) over the k blocks of
-// split z.  A workgroup (4 wavefronts) owns two 16-row blocks; its waves
-// take interleaved k blocks (wave w: kb0+w, kb0+w+4, ...) so the four
-// consecutive 1 KiB tiles in flight per row block are adjacent in HBM, and
-// the next k block of a wave is loaded before the MFMAs of the current one.
-// The four partial accumulators are summed through LDS at the end.
+// split z.  A workgroup (4 wavefronts) owns 8 row blocks (128 rows); all
+// waves walk the same k blocks so each B fragment is fetched from L2 once
+// per workgroup and re-served from L1 to the other waves, and each
+// fragment feeds two MFMAs per wave.  E tiles are streamed with
+// non-temporal loads (no reuse, keep L2 for B) two k blocks ahead of use.
 template <int NB, int MODE>
 __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi, const uint4 *__restrict__ Elo,
                                                    const uint4 *__restrict__ Zhi, const uint4 *__restrict__ Zlo,
@@ -221,7 +234,7 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int rb0 = blockIdx.x * 2;
+    const int rb0 = (blockIdx.x * 4 + wave) * 2;
     const int kb0 = blockIdx.z * kb_per_split;
     int kb1 = kb0 + kb_per_split;
     if (kb1 > nkb) {
@@ -241,81 +254,74 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
     const long arow0 = (long)rb0 * nkb * 64 + lane;
     const long arow1 = arow0 + (long)nkb * 64;
     auto loadA = [&](const uint4 *E, int kb, uint4 &x0, uint4 &x1) {
-        x0 = ok0 ? E[arow0 + (long)kb * 64] : zero;
-        x1 = ok1 ? E[arow1 + (long)kb * 64] : zero;
+        x0 = zero;
+        x1 = zero;
+        if (kb < kb1) {
+            if (ok0) {
+                x0 = load_nt(E + arow0 + (long)kb * 64);
+            }
+            if (ok1) {
+                x1 = load_nt(E + arow1 + (long)kb * 64);
+            }
+        }
     };
     auto loadB = [&](const uint4 *Z, int kb, uint4 *b) {
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) {
             const int col = cb * 16 + li;
-            b[cb] = col < nc ? Z[((long)col * np + (long)kb * 32 + 8 * lk) >> 3] : zero;
+            b[cb] = (col < nc && kb < kb1) ? Z[((long)col * np + (long)kb * 32 + 8 * lk) >> 3] : zero;
         }
     };
-    int kb = kb0 + wave;
-    uint4 a0, a1, a0l = zero, a1l = zero, bh[NB], bl[NB];
-    if (kb < kb1) {
-        loadA(Ehi, kb, a0, a1);
-        loadB(Zhi, kb, bh);
+    // E tiles DEPTH k blocks ahead in a register ring; B fragments are
+    // fetched at use (L1/L2 hits: every wave of the workgroup reads them)
+    constexpr int DEPTH = 4;
+    uint4 a0[DEPTH], a1[DEPTH], a0l[DEPTH], a1l[DEPTH];
+#pragma unroll
+    for (int u = 0; u < DEPTH; ++u) {
+        loadA(Ehi, kb0 + u, a0[u], a1[u]);
         if (MODE == LOWP_BF16X3) {
-            loadA(Elo, kb, a0l, a1l);
-            loadB(Zlo, kb, bl);
+            loadA(Elo, kb0 + u, a0l[u], a1l[u]);
         }
     }
-    for (; kb < kb1; kb += 4) {
-        uint4 c0 = a0, c1 = a1, c0l = a0l, c1l = a1l, ch[NB], cl[NB];
+    for (int kb = kb0; kb < kb1; kb += DEPTH) {
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb) {
-            ch[cb] = bh[cb];
-            if (MODE == LOWP_BF16X3) {
-                cl[cb] = bl[cb];
-            }
-        }
-        const int kn = kb + 4;
-        if (kn < kb1) {
-            loadA(Ehi, kn, a0, a1);
-            loadB(Zhi, kn, bh);
-            if (MODE == LOWP_BF16X3) {
-                loadA(Elo, kn, a0l, a1l);
-                loadB(Zlo, kn, bl);
-            }
-        }
-#pragma unroll
-        for (int cb = 0; cb < NB; ++cb) {
-            if (MODE == LOWP_FP16) {
-                const h8_t bv = __builtin_bit_cast(h8_t, ch[cb]);
-                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c0), bv, acc[0][cb], 0, 0, 0);
-                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c1), bv, acc[1][cb], 0, 0, 0);
-            } else {
-                const b8_t bv = __builtin_bit_cast(b8_t, ch[cb]);
-                acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), bv, acc[0][cb], 0, 0, 0);
-                acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), bv, acc[1][cb], 0, 0, 0);
+        for (int u = 0; u < DEPTH; ++u) {
+            const int kc = kb + u;
+            if (kc < kb1) {
+                uint4 bh[NB], bl[NB];
+                loadB(Zhi, kc, bh);
                 if (MODE == LOWP_BF16X3) {
-                    const b8_t blv = __builtin_bit_cast(b8_t, cl[cb]);
-                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), blv, acc[0][cb], 0, 0, 0);
-                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), blv, acc[1][cb], 0, 0, 0);
-                    acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0l), bv, acc[0][cb], 0, 0, 0);
-                    acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1l), bv, acc[1][cb], 0, 0, 0);
+                    loadB(Zlo, kc, bl);
+                }
+                const uint4 c0 = a0[u], c1 = a1[u];
+                uint4 c0l = zero, c1l = zero;
+                if (MODE == LOWP_BF16X3) {
+                    c0l = a0l[u];
+                    c1l = a1l[u];
+                    loadA(Elo, kc + DEPTH, a0l[u], a1l[u]);
+                }
+                loadA(Ehi, kc + DEPTH, a0[u], a1[u]);
+#pragma unroll
+                for (int cb = 0; cb < NB; ++cb) {
+                    if (MODE == LOWP_FP16) {
+                        const h8_t bv = __builtin_bit_cast(h8_t, bh[cb]);
+                        acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c0), bv, acc[0][cb], 0, 0, 0);
+                        acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c1), bv, acc[1][cb], 0, 0, 0);
+                    } else {
+                        const b8_t bv = __builtin_bit_cast(b8_t, bh[cb]);
+                        acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), bv, acc[0][cb], 0, 0, 0);
+                        acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), bv, acc[1][cb], 0, 0, 0);
+                        if (MODE == LOWP_BF16X3) {
+                            const b8_t blv = __builtin_bit_cast(b8_t, bl[cb]);
+                            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), blv, acc[0][cb], 0, 0, 0);
+                            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), blv, acc[1][cb], 0, 0, 0);
+                            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0l), bv, acc[0][cb], 0, 0, 0);
+                            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1l), bv, acc[1][cb], 0, 0, 0);
+                        }
+                    }
                 }
             }
         }
-    }
-    // reduce the four waves' accumulators through LDS (waves 1..3 -> 0)
-    __shared__ float red[3][2 * NB * 4][64];
-    if (wave > 0) {
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-#pragma unroll
-            for (int cb = 0; cb < NB; ++cb) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    red[wave - 1][(a * NB + cb) * 4 + v][lane] = acc[a][cb][v];
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (wave != 0) {
-        return;
     }
     // D layout (f32 16x16x32): lane l, element v -> row 4*(l/16)+v, col l%16
     float *Pz = P + (long)blockIdx.z * n * nc;
@@ -325,13 +331,14 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) {
             const int c = cb * 16 + li;
+            if (c >= nc) {
+                continue;
+            }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int slot = (a * NB + cb) * 4 + v;
-                const float sum = acc[a][cb][v] + red[0][slot][lane] + red[1][slot][lane] + red[2][slot][lane];
                 const long r = (long)rb * 16 + 4 * lk + v;
-                if (c < nc && r < n) {
-                    Pz[r + (long)c * n] = sum;
+                if (r < n) {
+                    Pz[r + (long)c * n] = acc[a][cb][v];
                 }
             }
         }
@@ -383,7 +390,7 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
     for (int c0 = 0; c0 < nc; c0 += 128) {
         const int w = std::min(128, nc - c0);
         // scratch: colmax (w u64) | colscale (w) | Z hi/lo (np*w halves each) | partials
-        const int wgx = (op.nrb + 1) / 2;
+        const int wgx = (op.nrb + 7) / 8;
         int splits = std::max(1, 1024 / std::max(wgx, 1));
         splits = std::min(splits, op.nkb);
         const int kbs = (op.nkb + splits - 1) / splits;
