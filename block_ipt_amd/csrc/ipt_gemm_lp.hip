@@ -25,6 +25,7 @@
 
 #include "ipt_block.h"
 #include "ipt_lowp.h"
+
 #include "../../include/ipt_hip_ext.h"
 
 namespace ipt {
@@ -226,7 +227,7 @@ __global__ void k_delta_convert(const double *__restrict__ B, const double *__re
 // per workgroup and re-served from L1 to the other waves, and each
 // fragment feeds two MFMAs per wave.  E tiles are streamed with
 // non-temporal loads (no reuse, keep L2 for B) two k blocks ahead of use.
-template <int NB, int MODE>
+template <int NB, int MODE, int DEPTH>
 __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi, const uint4 *__restrict__ Elo,
                                                    const uint4 *__restrict__ Zhi, const uint4 *__restrict__ Zlo,
                                                    float *__restrict__ P, int n, int nc, int nkb, int nrb, int np,
@@ -272,15 +273,17 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
             b[cb] = (col < nc && kb < kb1) ? Z[((long)col * np + (long)kb * 32 + 8 * lk) >> 3] : zero;
         }
     };
-    // E tiles DEPTH k blocks ahead in a register ring; B fragments are
-    // fetched at use (L1/L2 hits: every wave of the workgroup reads them)
-    constexpr int DEPTH = 4;
-    uint4 a0[DEPTH], a1[DEPTH], a0l[DEPTH], a1l[DEPTH];
+    // E tiles and B fragments of k block kb+DEPTH are issued together while
+    // kb is consumed: vector loads complete in issue order, so everything a
+    // step needs was issued DEPTH steps earlier than anything it does not.
+    uint4 a0[DEPTH], a1[DEPTH], a0l[DEPTH], a1l[DEPTH], bh[DEPTH][NB], bl[DEPTH][NB];
 #pragma unroll
     for (int u = 0; u < DEPTH; ++u) {
         loadA(Ehi, kb0 + u, a0[u], a1[u]);
+        loadB(Zhi, kb0 + u, bh[u]);
         if (MODE == LOWP_BF16X3) {
             loadA(Elo, kb0 + u, a0l[u], a1l[u]);
+            loadB(Zlo, kb0 + u, bl[u]);
         }
     }
     for (int kb = kb0; kb < kb1; kb += DEPTH) {
@@ -288,37 +291,31 @@ __global__ __launch_bounds__(256) void k_lowp_gemm(const uint4 *__restrict__ Ehi
         for (int u = 0; u < DEPTH; ++u) {
             const int kc = kb + u;
             if (kc < kb1) {
-                uint4 bh[NB], bl[NB];
-                loadB(Zhi, kc, bh);
-                if (MODE == LOWP_BF16X3) {
-                    loadB(Zlo, kc, bl);
-                }
-                const uint4 c0 = a0[u], c1 = a1[u];
-                uint4 c0l = zero, c1l = zero;
-                if (MODE == LOWP_BF16X3) {
-                    c0l = a0l[u];
-                    c1l = a1l[u];
-                    loadA(Elo, kc + DEPTH, a0l[u], a1l[u]);
-                }
-                loadA(Ehi, kc + DEPTH, a0[u], a1[u]);
+                // consume slot u, then refill it with k block kc + DEPTH
 #pragma unroll
                 for (int cb = 0; cb < NB; ++cb) {
                     if (MODE == LOWP_FP16) {
-                        const h8_t bv = __builtin_bit_cast(h8_t, bh[cb]);
-                        acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c0), bv, acc[0][cb], 0, 0, 0);
-                        acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, c1), bv, acc[1][cb], 0, 0, 0);
+                        const h8_t bv = __builtin_bit_cast(h8_t, bh[u][cb]);
+                        acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, a0[u]), bv, acc[0][cb], 0, 0, 0);
+                        acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, a1[u]), bv, acc[1][cb], 0, 0, 0);
                     } else {
-                        const b8_t bv = __builtin_bit_cast(b8_t, bh[cb]);
-                        acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), bv, acc[0][cb], 0, 0, 0);
-                        acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), bv, acc[1][cb], 0, 0, 0);
+                        const b8_t bv = __builtin_bit_cast(b8_t, bh[u][cb]);
+                        acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, a0[u]), bv, acc[0][cb], 0, 0, 0);
+                        acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, a1[u]), bv, acc[1][cb], 0, 0, 0);
                         if (MODE == LOWP_BF16X3) {
-                            const b8_t blv = __builtin_bit_cast(b8_t, bl[cb]);
-                            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0), blv, acc[0][cb], 0, 0, 0);
-                            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1), blv, acc[1][cb], 0, 0, 0);
-                            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c0l), bv, acc[0][cb], 0, 0, 0);
-                            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, c1l), bv, acc[1][cb], 0, 0, 0);
+                            const b8_t blv = __builtin_bit_cast(b8_t, bl[u][cb]);
+                            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, a0[u]), blv, acc[0][cb], 0, 0, 0);
+                            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, a1[u]), blv, acc[1][cb], 0, 0, 0);
+                            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, a0l[u]), bv, acc[0][cb], 0, 0, 0);
+                            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8_t, a1l[u]), bv, acc[1][cb], 0, 0, 0);
                         }
                     }
+                }
+                loadA(Ehi, kc + DEPTH, a0[u], a1[u]);
+                loadB(Zhi, kc + DEPTH, bh[u]);
+                if (MODE == LOWP_BF16X3) {
+                    loadA(Elo, kc + DEPTH, a0l[u], a1l[u]);
+                    loadB(Zlo, kc + DEPTH, bl[u]);
                 }
             }
         }
@@ -363,12 +360,12 @@ __global__ void k_lowp_epilogue(const float *__restrict__ P, int splits, int n, 
     }
 }
 
-template <int MODE>
+template <int MODE, int DEP>
 static void launch_lowp(int NB, dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
                         const uint4 *Zlo, float *P, int n, int nc, int nkb, int nrb, int np, int kbs)
 {
 #define IPT_LP(NBV)                                                                                              \
-    hipLaunchKernelGGL((k_lowp_gemm<NBV, MODE>), grid, dim3(256), 0, st, Ehi, Elo, Zhi, Zlo, P, n, nc, nkb, nrb, np, \
+    hipLaunchKernelGGL((k_lowp_gemm<NBV, MODE, DEP>), grid, dim3(256), 0, st, Ehi, Elo, Zhi, Zlo, P, n, nc, nkb, nrb, np, \
                        kbs)
     switch (NB) {
     case 1: IPT_LP(1); break;
@@ -420,12 +417,24 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         const uint4 *Zh = reinterpret_cast<const uint4 *>(zhi);
         const uint4 *Zl = reinterpret_cast<const uint4 *>(zlo);
         const ProfileToken prof = profile_begin(ctx, PROF_LOWP_GEMM);
-        if (op.mode == LOWP_FP16) {
-            launch_lowp<LOWP_FP16>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
-        } else if (op.mode == LOWP_BF16) {
-            launch_lowp<LOWP_BF16>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+        // prefetch depth (k blocks in flight per wavefront); tuning knob
+        static const int depth = env_int("IPT_LOWP_DEPTH", 1);
+        if (op.mode == LOWP_BF16X3) {
+            launch_lowp<LOWP_BF16X3, 1>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+        } else if (op.mode == LOWP_FP16) {
+            if (depth >= 4) {
+                launch_lowp<LOWP_FP16, 4>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+            } else if (depth >= 2) {
+                launch_lowp<LOWP_FP16, 2>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+            } else {
+                launch_lowp<LOWP_FP16, 1>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+            }
         } else {
-            launch_lowp<LOWP_BF16X3>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+            if (depth >= 2) {
+                launch_lowp<LOWP_BF16, 2>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+            } else {
+                launch_lowp<LOWP_BF16, 1>(NB, grid, ctx->stream, Ehi, Elo, Zh, Zl, P, n, w, op.nkb, op.nrb, op.np, kbs);
+            }
         }
         profile_end(ctx, prof);
         IPT_HIPCHK(hipGetLastError());
