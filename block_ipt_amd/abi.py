@@ -129,13 +129,39 @@ class IPTCudaResult(ctypes.Structure):
     ]
 
 
+def _two_phase_struct(name, lo):
+    """Result structs of include/ipt_hip_mixed.h (reference
+    ipt_mixed_precision_cuda.cu:19-59): ints, doubles, device pointers."""
+    return type(name, (_c.Structure,), {"_fields_": [
+        ("n", _c.c_int), ("k", _c.c_int), ("%s_iterations" % lo, _c.c_int), ("fp64_iterations", _c.c_int),
+        ("%s_time_sec" % lo, _c.c_double), ("fp64_time_sec", _c.c_double), ("total_time_sec", _c.c_double),
+        ("%s_fixed_point_residual" % lo, _c.c_double), ("fp64_fixed_point_residual", _c.c_double),
+        ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]})
+
+
+IPTMixedCudaDeviceResult = _two_phase_struct("IPTMixedCudaDeviceResult", "fp16")
+IPTMixedFP32CudaDeviceResult = _two_phase_struct("IPTMixedFP32CudaDeviceResult", "fp32")
+IPTMixedTF32CudaDeviceResult = _two_phase_struct("IPTMixedTF32CudaDeviceResult", "tf32")
+
+
+class IPTRayleighMixedCudaDeviceResult(_c.Structure):
+    """Reference ipt_rayleigh_mixed_cuda.cu:19-34."""
+    _fields_ = [
+        ("n", _c.c_int), ("k", _c.c_int), ("tf32_iterations", _c.c_int), ("fp64_iterations", _c.c_int),
+        ("tf32_time_sec", _c.c_double), ("orthogonalize_time_sec", _c.c_double),
+        ("rayleigh_transform_time_sec", _c.c_double), ("fp64_time_sec", _c.c_double),
+        ("backtransform_time_sec", _c.c_double), ("total_time_sec", _c.c_double),
+        ("tf32_fixed_point_residual", _c.c_double), ("fp64_fixed_point_residual", _c.c_double),
+        ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]
+
+
 _dp = _c.POINTER(_c.c_double)
 _ip = _c.POINTER(_c.c_int)
 _dpp = _c.POINTER(_dp)
 _vp = _c.c_void_p
 _R = _c.POINTER(IPTCudaResult)
 
-# name -> (restype, argtypes); every symbol declared in include/ipt_hip.h
+# name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     "ipt_cuda_status_string": (_c.c_char_p, [_c.c_int]),
     "ipt_cuda_free_result": (None, [_R]),
@@ -170,6 +196,23 @@ SIGNATURES = {
     "ipt_hip_profile_get": (_c.c_int, [_c.c_char_p, _dp, _c.POINTER(_c.c_longlong)]),
     "ipt_hip_last_solve_stats": (_c.c_int, [_vp, _ip, _ip, _c.POINTER(_c.c_longlong),
                                             _c.POINTER(_c.c_longlong), _ip]),
+    # include/ipt_hip_mixed.h
+    "ipt_mixed_precision_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
+                                                       _c.c_double, _c.c_int, _vp,
+                                                       _c.POINTER(IPTMixedCudaDeviceResult)]),
+    "ipt_mixed_precision_cuda_free_device_result": (None, [_c.POINTER(IPTMixedCudaDeviceResult)]),
+    "ipt_mixed_precision_fp32_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
+                                                            _c.c_double, _c.c_int, _vp,
+                                                            _c.POINTER(IPTMixedFP32CudaDeviceResult)]),
+    "ipt_mixed_precision_fp32_cuda_free_device_result": (None, [_c.POINTER(IPTMixedFP32CudaDeviceResult)]),
+    "ipt_mixed_precision_tf32_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
+                                                            _c.c_double, _c.c_int, _vp,
+                                                            _c.POINTER(IPTMixedTF32CudaDeviceResult)]),
+    "ipt_mixed_precision_tf32_cuda_free_device_result": (None, [_c.POINTER(IPTMixedTF32CudaDeviceResult)]),
+    "ipt_rayleigh_mixed_tf32_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
+                                                           _c.c_double, _c.c_int, _vp,
+                                                           _c.POINTER(IPTRayleighMixedCudaDeviceResult)]),
+    "ipt_rayleigh_mixed_cuda_free_device_result": (None, [_c.POINTER(IPTRayleighMixedCudaDeviceResult)]),
 }
 
 _lib = None

@@ -10,6 +10,7 @@ import ctypes
 
 import numpy as np
 
+from . import abi
 from .abi import IptError, IPTCudaResult, lib
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -238,9 +239,40 @@ def destroy_handle(h):
     return lib().ipt_hip_destroy_handle(ctypes.c_void_p(h))
 
 
+_MIXED = {
+    "fp16": ("ipt_mixed_precision_cuda_device_tol", "ipt_mixed_precision_cuda_free_device_result",
+             abi.IPTMixedCudaDeviceResult),
+    "fp32": ("ipt_mixed_precision_fp32_cuda_device_tol", "ipt_mixed_precision_fp32_cuda_free_device_result",
+             abi.IPTMixedFP32CudaDeviceResult),
+    "tf32": ("ipt_mixed_precision_tf32_cuda_device_tol", "ipt_mixed_precision_tf32_cuda_free_device_result",
+             abi.IPTMixedTF32CudaDeviceResult),
+    "rayleigh_tf32": ("ipt_rayleigh_mixed_tf32_cuda_device_tol", "ipt_rayleigh_mixed_cuda_free_device_result",
+                      abi.IPTRayleighMixedCudaDeviceResult),
+}
+
+
+def mixed_device_tol(kind, d_H, n, k, lo_tol, lo_maxiter, hi_tol, hi_maxiter, handle=None):
+    """Two-phase mixed-precision entry (include/ipt_hip_mixed.h); kind is
+    fp16 | fp32 | tf32 | rayleigh_tf32.  Returns (vectors, values, info)
+    copied to the host; the device result is freed."""
+    name, free_name, R = _MIXED[kind]
+    r = R()
+    st = getattr(lib(), name)(ctypes.c_void_p(d_H), int(n), int(k), float(lo_tol), int(lo_maxiter),
+                              float(hi_tol), int(hi_maxiter), handle, ctypes.byref(r))
+    if st != 0:
+        raise IptError(st, name)
+    info = {f[0]: getattr(r, f[0]) for f in R._fields_ if not f[0].startswith("d_")}
+    try:
+        vec = hip().to_host(r.d_vectors, (n, k))
+        val = hip().to_host(r.d_values, (k,))
+    finally:
+        getattr(lib(), free_name)(ctypes.byref(r))
+    return vec, val, info
+
+
 # ------------------------------------------------------------------ extensions
 
-LOWP_MODES = {"fp16": 0, "bf16": 1, "bf16x3": 2}
+LOWP_MODES = {"fp16": 0, "bf16": 1, "bf16x3": 2, "fp32": 3}
 
 
 def generate_dense(d_H, n, seed, lo, hi, vscale, handle=None):

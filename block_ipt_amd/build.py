@@ -16,6 +16,7 @@ SOURCES = [
     "ipt_context.cpp",
     "ipt_prep.cpp",
     "ipt_plain.cpp",
+    "ipt_mixed.cpp",
     "ipt_block.cpp",
     "ipt_kernels.hip",
     "ipt_gemm_f64.hip",

@@ -540,6 +540,8 @@ BlockParams params_from_env(int maxiter, double tol, int use_tol)
             p.lowp_mode = LOWP_BF16;
         } else if (!strcmp(prec, "bf16x3") || !strcmp(prec, "tf32")) {
             p.lowp_mode = LOWP_BF16X3;
+        } else if (!strcmp(prec, "fp32")) {
+            p.lowp_mode = LOWP_FP32;
         }
     }
     p.anchor_every = env_int("IPT_MIXED_ANCHOR_EVERY", 20);

@@ -61,11 +61,12 @@ __device__ inline float bf16_bits_to_float(unsigned short b) { return __uint_as_
 
 // ------------------------------------------------------------------ pack E
 
-// One thread per (tile, lane): 8 elements.  mode 0: fp16 of scale*E;
-// mode 1: bf16 of E; mode 2: bf16 hi + bf16 lo of E.
+// One thread per (tile, lane): 8 elements.  FP16: fp16 of scale*E;
+// BF16: bf16 of E; BF16X3: bf16 hi + bf16 lo of E; FP32: E in FP32 with
+// k = 4j + l/16 (plane 0: j < 4, plane 1: j >= 4); the 16-bit modes hold
+// k = 8(l/16) + j.
 __global__ __launch_bounds__(256) void k_pack_E(const double *__restrict__ H, long ld, int n, int nkb, int nrb,
-                                                int mode, float scale, unsigned short *__restrict__ hi,
-                                                unsigned short *__restrict__ lo)
+                                                int mode, float scale, uint4 *__restrict__ hi, uint4 *__restrict__ lo)
 {
     const long total = (long)nrb * nkb * 64;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
@@ -73,22 +74,35 @@ __global__ __launch_bounds__(256) void k_pack_E(const double *__restrict__ H, lo
         const int l = (int)(t & 63);
         const long rb = tile / nkb, kb = tile % nkb;
         const long r = rb * 16 + (l & 15);
-        const long k0 = kb * 32 + 8 * (l >> 4);
+        double e[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const long k = mode == LOWP_FP32 ? kb * 32 + 4 * j + (l >> 4) : kb * 32 + 8 * (l >> 4) + j;
+            e[j] = (r < n && k < n && r != k) ? H[r + k * ld] : 0.0;
+        }
+        if (mode == LOWP_FP32) {
+            uint4 p0, p1;
+            p0.x = __float_as_uint((float)e[0]);
+            p0.y = __float_as_uint((float)e[1]);
+            p0.z = __float_as_uint((float)e[2]);
+            p0.w = __float_as_uint((float)e[3]);
+            p1.x = __float_as_uint((float)e[4]);
+            p1.y = __float_as_uint((float)e[5]);
+            p1.z = __float_as_uint((float)e[6]);
+            p1.w = __float_as_uint((float)e[7]);
+            hi[t] = p0;
+            lo[t] = p1;
+            continue;
+        }
         unsigned short vh[8], vl[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const long k = k0 + j;
-            double e = 0.0;
-            if (r < n && k < n && r != k) {
-                e = H[r + k * ld];
-            }
-            if (mode == 0) {
-                vh[j] = to_half_bits((float)(e * (double)scale));
+            if (mode == LOWP_FP16) {
+                vh[j] = to_half_bits((float)(e[j] * (double)scale));
                 vl[j] = 0;
             } else {
-                const float f = (float)e;
-                vh[j] = to_bf16_bits(f);
-                vl[j] = to_bf16_bits((float)(e - (double)bf16_bits_to_float(vh[j])));
+                vh[j] = to_bf16_bits((float)e[j]);
+                vl[j] = to_bf16_bits((float)(e[j] - (double)bf16_bits_to_float(vh[j])));
             }
         }
         uint4 ph, pl;
@@ -96,13 +110,13 @@ __global__ __launch_bounds__(256) void k_pack_E(const double *__restrict__ H, lo
         ph.y = vh[2] | ((unsigned)vh[3] << 16);
         ph.z = vh[4] | ((unsigned)vh[5] << 16);
         ph.w = vh[6] | ((unsigned)vh[7] << 16);
-        reinterpret_cast<uint4 *>(hi)[t] = ph;
-        if (mode == 2) {
+        hi[t] = ph;
+        if (mode == LOWP_BF16X3) {
             pl.x = vl[0] | ((unsigned)vl[1] << 16);
             pl.y = vl[2] | ((unsigned)vl[3] << 16);
             pl.z = vl[4] | ((unsigned)vl[5] << 16);
             pl.w = vl[6] | ((unsigned)vl[7] << 16);
-            reinterpret_cast<uint4 *>(lo)[t] = pl;
+            lo[t] = pl;
         }
     }
 }
@@ -132,27 +146,29 @@ int lowp_operator_create(Context *ctx, const double *H, long ld, int n, int mode
     }
     const size_t elems = (size_t)op->nrb * op->nkb * 512;
     IPT_STCHK(op->hi.alloc(elems));
-    if (mode == LOWP_BF16X3) {
+    if (lowp_two_planes(mode)) {
         IPT_STCHK(op->lo.alloc(elems));
     }
     const long total = (long)op->nrb * op->nkb * 64;
     hipLaunchKernelGGL(k_pack_E, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, ctx->stream, H, ld, n, op->nkb,
-                       op->nrb, mode, op->scale, op->hi.p, op->lo.p);
+                       op->nrb, mode, op->scale, reinterpret_cast<uint4 *>(op->hi.p),
+                       reinterpret_cast<uint4 *>(op->lo.p));
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }
 
 // ------------------------------------------------------------------ delta
 
-// colmax[c] = max_r |B - Ba|: blockIdx.y = column, blockIdx.x = row
-// quarter; one atomicMax on the IEEE bits per block (order free).
+// colmax[c] = max_r |B - Ba| (Ba NULL: zero): blockIdx.y = column,
+// blockIdx.x = row slice; one atomicMax on the IEEE bits per block (order
+// free: the maximum does not depend on it).
 __global__ __launch_bounds__(256) void k_delta_colmax(const double *__restrict__ B, const double *__restrict__ Ba,
                                                       long ld, int n, int nc, unsigned long long *__restrict__ colmax)
 {
     const int c = blockIdx.y;
     double m = 0.0;
     for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
-        m = fmax(m, fabs(B[r + (long)c * ld] - Ba[r + (long)c * ld]));
+        m = fmax(m, fabs(B[r + (long)c * ld] - (Ba != nullptr ? Ba[r + (long)c * ld] : 0.0)));
     }
     for (int o = 32; o > 0; o >>= 1) {
         m = fmax(m, __shfl_down(m, o, 64));
@@ -172,9 +188,10 @@ __global__ __launch_bounds__(256) void k_delta_colmax(const double *__restrict__
 
 // s_c = 2^ceil(log2 max) (1 when the column did not move); Z = (B-Ba)/s_c
 // written fragment-major: for k block kb and column block cb the 64 lanes'
-// MFMA B fragments (k = 32kb + 8(l/16) + j, column 16cb + l%16) are one
-// contiguous KiB, so a workgroup stages a run of k blocks with plain
-// coalesced 16-byte loads.  Columns >= nc and rows >= n are zero.
+// MFMA B fragments (column 16cb + l%16; k as in k_pack_E) are one
+// contiguous KiB per plane, so a wavefront loads its fragments with plain
+// coalesced 16-byte loads.  Columns >= nc and rows >= n are zero.  Ba may be
+// NULL (zero anchor).
 __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict__ B, const double *__restrict__ Ba,
                                                        long ld, int n, int nc, int nkb, int nb, int mode,
                                                        const unsigned long long *__restrict__ colmax,
@@ -187,7 +204,6 @@ __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict_
         const int cb = (int)((t >> 6) % nb);
         const long kb = (t >> 6) / nb;
         const int c = cb * 16 + (l & 15);
-        const long k0 = kb * 32 + 8 * (l >> 4);
         double s = 1.0;
         if (c < nc) {
             const double mx = __longlong_as_double((long long)colmax[c]);
@@ -200,20 +216,39 @@ __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict_
                 colscale[c] = s;  // k block 0, first lane group: once per column
             }
         }
+        double x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const long k = mode == LOWP_FP32 ? kb * 32 + 4 * j + (l >> 4) : kb * 32 + 8 * (l >> 4) + j;
+            x[j] = 0.0;
+            if (c < nc && k < n) {
+                const long o = k + (long)c * ld;
+                x[j] = (B[o] - (Ba != nullptr ? Ba[o] : 0.0)) / s;
+            }
+        }
+        if (mode == LOWP_FP32) {
+            uint4 p0, p1;
+            p0.x = __float_as_uint((float)x[0]);
+            p0.y = __float_as_uint((float)x[1]);
+            p0.z = __float_as_uint((float)x[2]);
+            p0.w = __float_as_uint((float)x[3]);
+            p1.x = __float_as_uint((float)x[4]);
+            p1.y = __float_as_uint((float)x[5]);
+            p1.z = __float_as_uint((float)x[6]);
+            p1.w = __float_as_uint((float)x[7]);
+            zhi[t] = p0;
+            zlo[t] = p1;
+            continue;
+        }
         unsigned short vh[8], vl[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const long k = k0 + j;
-            double x = 0.0;
-            if (c < nc && k < n) {
-                x = (B[k + (long)c * ld] - Ba[k + (long)c * ld]) / s;
-            }
             if (mode == LOWP_FP16) {
-                vh[j] = to_half_bits((float)x);
+                vh[j] = to_half_bits((float)x[j]);
                 vl[j] = 0;
             } else {
-                vh[j] = to_bf16_bits((float)x);
-                vl[j] = to_bf16_bits((float)(x - (double)bf16_bits_to_float(vh[j])));
+                vh[j] = to_bf16_bits((float)x[j]);
+                vl[j] = to_bf16_bits((float)(x[j] - (double)bf16_bits_to_float(vh[j])));
             }
         }
         uint4 ph;
@@ -234,7 +269,8 @@ __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict_
 }
 
 // Y = Ya + d (B - Ba) + colscale/scale * sum_z P[z]; one thread per P
-// lane entry (4 rows of one column), splits summed in order.
+// lane entry (4 rows of one column), splits summed in order.  Ya and Ba
+// may be NULL (zero).
 __global__ __launch_bounds__(256) void k_lowp_epilogue(const f4_t *__restrict__ P, int splits, int n, int nc, int nrb,
                                                        int nb, const double *__restrict__ Ya,
                                                        const double *__restrict__ B, const double *__restrict__ Ba,
@@ -267,40 +303,27 @@ __global__ __launch_bounds__(256) void k_lowp_epilogue(const f4_t *__restrict__ 
             const long r = r0 + v;
             if (r < n) {
                 const long o = r + (long)c * ld;
-                Y[o] = Ya[o] + d[r] * (B[o] - Ba[o]) + sv[v] * f;
+                const double ba = Ba != nullptr ? Ba[o] : 0.0;
+                Y[o] = (Ya != nullptr ? Ya[o] : 0.0) + d[r] * (B[o] - ba) + sv[v] * f;
             }
         }
     }
 }
 
-// k blocks per wavefront: B fragments held in registers (KW x NB x 4
-// VGPRs per plane; at most 40 uint4 per wavefront)
-template <int MODE>
-static int lowp_kw(int nb)
+// k blocks per wavefront: the B fragments it holds in registers take
+// KW x NB x 4 VGPRs per plane; chosen so no variant spills (256 VGPRs at two
+// wavefronts per SIMD)
+constexpr int lowp_kw(int mode, int nb)
 {
-    const int cap = MODE == LOWP_BF16X3 ? 20 : 40;
-    int kw = 8;
-    while (kw > 1 && kw * nb > cap) {
-        kw >>= 1;
-    }
-    return kw;
+    return lowp_two_planes(mode) ? (nb == 1 ? 8 : nb <= 5 ? 4 : 2) : (nb <= 5 ? 8 : 4);
 }
 
 template <int MODE, int NB>
 static void launch_lowp_nb(dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
                            const uint4 *Zlo, float *P, int nkb, int nrb, int rbw)
 {
-    const int kw = lowp_kw<MODE>(NB);
-    if (kw >= 8) {
-        hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, (NB * 8 <= 40 ? 8 : 1)>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo,
-                           P, nkb, nrb, rbw);
-    } else if (kw == 4) {
-        hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, 4>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw);
-    } else if (kw == 2) {
-        hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, 2>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw);
-    } else {
-        hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, 1>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw);
-    }
+    hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, lowp_kw(MODE, NB)>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, P, nkb,
+                       nrb, rbw);
 }
 
 template <int MODE>
@@ -326,7 +349,7 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
     for (int c0 = 0; c0 < nc; c0 += 128) {
         const int w = std::min(128, nc - c0);
         const int NB = (w + 15) / 16;
-        const int kw = op.mode == LOWP_BF16X3 ? lowp_kw<LOWP_BF16X3>(NB) : lowp_kw<LOWP_FP16>(NB);
+        const int kw = lowp_kw(op.mode, NB);
         // one split = the k blocks of 8 wavefronts; about one workgroup of
         // 8 wavefronts per CU over 256 CUs
         const int kspan = 8 * kw;
@@ -348,7 +371,9 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         uint4 *zlo = zhi + zvec;
         float *P = reinterpret_cast<float *>(base + poff);
         const double *Bc = B + (size_t)c0 * ld;
-        const double *Bac = Ba + (size_t)c0 * ld;
+        // a NULL anchor stays NULL in every column chunk (zero anchor)
+        const double *Bac = Ba != nullptr ? Ba + (size_t)c0 * ld : nullptr;
+        const double *Yac = Ya != nullptr ? Ya + (size_t)c0 * ld : nullptr;
         IPT_HIPCHK(hipMemsetAsync(colmax, 0, (size_t)w * 8, ctx->stream));
         hipLaunchKernelGGL(k_delta_colmax, dim3(std::min(8, (n + 255) / 256), w), dim3(256), 0, ctx->stream, Bc, Bac,
                            ld, n, w, colmax);
@@ -362,6 +387,8 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         const ProfileToken prof = profile_begin(ctx, PROF_LOWP_GEMM);
         if (op.mode == LOWP_BF16X3) {
             launch_lowp<LOWP_BF16X3>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, P, op.nkb, op.nrb, rbw);
+        } else if (op.mode == LOWP_FP32) {
+            launch_lowp<LOWP_FP32>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, P, op.nkb, op.nrb, rbw);
         } else if (op.mode == LOWP_FP16) {
             launch_lowp<LOWP_FP16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, P, op.nkb, op.nrb, rbw);
         } else {
@@ -371,7 +398,7 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         IPT_HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_lowp_epilogue, dim3(blocks_for((long)op.nrb * NB * 64, 256, 16384)), dim3(256), 0,
                            ctx->stream, reinterpret_cast<const f4_t *>(P), splits, n, w, op.nrb, NB,
-                           Ya + (size_t)c0 * ld, Bc, Bac, ld, d, colscale, 1.0 / (double)op.scale,
+                           Yac, Bc, Bac, ld, d, colscale, 1.0 / (double)op.scale,
                            Y + (size_t)c0 * ld);
         IPT_HIPCHK(hipGetLastError());
     }
@@ -386,7 +413,7 @@ extern "C" int ipt_hip_lowp_product(const double *d_H, int n, int mode, const do
                                     const double *d_Ya, const double *d_diag, int nc, double *d_Y, void *handle)
 {
     if (d_H == nullptr || d_B == nullptr || d_Ba == nullptr || d_Ya == nullptr || d_diag == nullptr ||
-        d_Y == nullptr || n <= 0 || nc <= 0 || mode < 0 || mode > 2) {
+        d_Y == nullptr || n <= 0 || nc <= 0 || mode < LOWP_FP16 || mode > LOWP_FP32) {
         return IPT_CUDA_INVALID_ARGUMENT;
     }
     int st = IPT_CUDA_SUCCESS;

@@ -29,8 +29,8 @@ __device__ inline uint4 load_nt(const uint4 *p)
 
 // ------------------------------------------------------------------ GEMM
 
-// Partial products of split z (k blocks [64z, 64z + 64) for FP16/BF16,
-// [32z, 32z + 32) for BF16x3) for every row block.
+// Partial products of split z (k blocks [8 KW z, 8 KW (z + 1)), KW = 8 or
+// 4 k blocks per wavefront) for every row block.
 //
 // The B fragments of a split are small (KW k blocks x NB column blocks x
 // 1 KiB per wavefront) and are held in registers for the whole kernel:
@@ -56,7 +56,8 @@ __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi
 {
     constexpr int W = 8;  // wavefronts per workgroup
     constexpr bool X3 = MODE == LOWP_BF16X3;
-    constexpr int NZ = X3 ? 2 : 1;
+    constexpr bool TWO = lowp_two_planes(MODE);
+    constexpr int NZ = TWO ? 2 : 1;
     __shared__ f4_t red[2][W][NB][64];
 
     const int lane = threadIdx.x & 63;
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi
         for (int cb = 0; cb < NB; ++cb) {
             const long i = ((long)(kbw + kk) * NB + cb) * 64 + lane;
             bz[0][kk][cb] = Zhi[i];
-            if (X3) {
+            if (TWO) {
                 bz[NZ - 1][kk][cb] = Zlo[i];
             }
         }
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi
 #pragma unroll
         for (int kk = 0; kk < KW; ++kk) {
             eh[slot][kk] = load_nt(ph + kk * 64);
-            if (X3) {
+            if (TWO) {
                 el[slot][kk] = load_nt(pl + kk * 64);
             }
         }
@@ -106,7 +107,18 @@ __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi
         for (int kk = 0; kk < KW; ++kk) {
 #pragma unroll
             for (int cb = 0; cb < NB; ++cb) {
-                if (MODE == LOWP_FP16) {
+                if (MODE == LOWP_FP32) {
+                    // lane l holds k = 4j + l/16 of the k block, j = 0..7
+                    const uint4 a0 = eh[slot][kk], a1 = el[slot][kk];
+                    const uint4 b0 = bz[0][kk][cb], b1 = bz[NZ - 1][kk][cb];
+                    const unsigned int av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                    const unsigned int bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(av[j]), __uint_as_float(bv[j]),
+                                                                       acc[cb], 0, 0, 0);
+                    }
+                } else if (MODE == LOWP_FP16) {
                     acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8_t, eh[slot][kk]),
                                                                      __builtin_bit_cast(h8_t, bz[0][kk][cb]), acc[cb],
                                                                      0, 0, 0);

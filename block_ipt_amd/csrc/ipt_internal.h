@@ -160,6 +160,10 @@ int gather_diagonal(Context *ctx, const double *Y, int k, long rs, long cs, doub
 int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,
                       int n, const double *X, long ldx, double *Y, long ldy, int k);
 
+// A <- (A + A^T)/2 for a square column-major n x n (ld) matrix
+// (IPT/src/mixed_precision/ipt_rayleigh_mixed_cuda.cu:84-106).
+int symmetrize(Context *ctx, double *A, long ld, int n);
+
 // Layout copies between column-major (ld) and row-major (ld) n x k.
 int colmajor_to_rowmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k);
 int rowmajor_to_colmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k);

@@ -73,6 +73,27 @@ int set_identity(Context *ctx, double *Z, long ld, int n, int k)
     return IPT_CUDA_SUCCESS;
 }
 
+// Upper and lower triangle replaced by their average (one thread per pair).
+__global__ void k_symmetrize(double *A, long ld, int n)
+{
+    const long total = (long)n * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const long r = t % n, c = t / n;
+        if (r < c) {
+            const double avg = 0.5 * (A[r + c * ld] + A[c + r * ld]);
+            A[r + c * ld] = avg;
+            A[c + r * ld] = avg;
+        }
+    }
+}
+
+int symmetrize(Context *ctx, double *A, long ld, int n)
+{
+    hipLaunchKernelGGL(k_symmetrize, dim3(blocks_for((long)n * n, 256, 4096)), dim3(256), 0, ctx->stream, A, ld, n);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
 // ------------------------------------------------------------------ plain update
 
 // Block-wide sum of two doubles; result valid in thread 0.

@@ -5,7 +5,14 @@
 
 namespace ipt {
 
-enum { LOWP_FP16 = 0, LOWP_BF16 = 1, LOWP_BF16X3 = 2 };
+// FP16 / BF16: one 16-bit plane of E and of the operand.  BF16X3: hi and lo
+// BF16 planes (three products).  FP32: E and the operand in FP32, the two
+// planes holding the first and second four of a lane's eight k values
+// (MFMA 16x16x4 FP32).
+enum { LOWP_FP16 = 0, LOWP_BF16 = 1, LOWP_BF16X3 = 2, LOWP_FP32 = 3 };
+
+// Modes whose E and operand occupy two 16-byte planes per lane and tile.
+inline constexpr bool lowp_two_planes(int mode) { return mode == LOWP_BF16X3 || mode == LOWP_FP32; }
 
 // E = H - diag(H) packed in MFMA A-operand order (16 x 32 tiles).
 struct LowpOperator {
@@ -16,7 +23,7 @@ struct LowpOperator {
     int np = 0;       // padded k extent (nkb * 32)
     float scale = 1.f;  // E was multiplied by `scale` before rounding (FP16)
     DevBuf<unsigned short> hi;
-    DevBuf<unsigned short> lo;  // BF16x3 only
+    DevBuf<unsigned short> lo;  // second plane (BF16x3, FP32)
 };
 
 // Pack E from a dense column-major FP64 H; max_abs_offdiag sizes the FP16 scale.
@@ -24,7 +31,8 @@ int lowp_operator_create(Context *ctx, const double *H, long ld, int n, int mode
                          LowpOperator *op);
 
 // Y(:, 0:nc) = Ya + diag(d)(B - Ba) + E (B - Ba), the low-precision part in
-// FP32-accumulated MFMA; all column-major with leading dimension ld.
+// FP32-accumulated MFMA; all column-major with leading dimension ld.  Ba and
+// Ya may be NULL (zero anchor: Y = diag(d) B + E B).
 int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, const double *Ba, const double *Ya,
                      long ld, const double *d, int nc, double *Y);
 

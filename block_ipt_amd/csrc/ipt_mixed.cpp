@@ -1,0 +1,289 @@
+// Two-phase mixed-precision entry points (low-precision plain IPT, then
+// FP64 plain IPT restarted from its iterate) and the Rayleigh-mixed entry
+// point (low-precision IPT, QR, FP64 IPT on the k x k Rayleigh quotient).
+//
+// Reference: IPT/src/mixed_precision/ipt_mixed_precision_cuda.cu:413-1225
+// and ipt_rayleigh_mixed_cuda.cu:141-400.  The reference keeps the
+// low-precision iterate in FP16/FP32 storage and multiplies it by an
+// FP16/FP32 copy of E = H - diag(H) with cublasGemmEx.  Here the product
+// E Z runs on MFMA in the requested input precision (FP32 accumulation)
+// through the same operator as the block path's correction products
+// (lowp_apply_delta with a zero anchor), and the iterate and the update stay
+// FP64; the FP64 phase is the plain loop of ipt_plain.cpp.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+
+#include "../../include/ipt_hip_mixed.h"
+#include "ipt_block.h"
+#include "ipt_lowp.h"
+#include "ipt_plain.h"
+
+namespace ipt {
+
+namespace {
+
+double now_seconds()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool mixed_args_ok(const double *dH, int n, int k, double lo_tol, int lo_maxiter, double hi_tol, int hi_maxiter)
+{
+    return !(dH == nullptr || n <= 0 || k <= 0 || k > n || !(lo_tol > 0.0) || !(hi_tol > 0.0) ||
+             lo_maxiter <= 0 || hi_maxiter <= 0);
+}
+
+// max |H_rc|, r != c, from the device scan (sizes the FP16 scale).
+int offdiag_absmax(Context *ctx, const double *dH, int n, double *out)
+{
+    DevBuf<double> colsum, colmax;
+    DevBuf<int> asym;
+    IPT_STCHK(colsum.alloc((size_t)n));
+    IPT_STCHK(colmax.alloc((size_t)n));
+    IPT_STCHK(asym.alloc(1));
+    IPT_STCHK(dense_scan(ctx, dH, n, n, colsum.p, colmax.p, asym.p));
+    std::vector<double> h((size_t)n);
+    IPT_HIPCHK(hipMemcpyAsync(h.data(), colmax.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    double m = 0.0;
+    for (double x : h) {
+        m = std::max(m, x);
+    }
+    *out = m;
+    return IPT_CUDA_SUCCESS;
+}
+
+// Low-precision phase (ipt_mixed_precision_cuda.cu:413-800): plain IPT from
+// the identity with H Z = diag(d) Z + E Z, E Z in `mode`, to `tol`.  On
+// success *d_init owns the returned iterate (device, n x k column-major).
+int lowp_phase(Context *ctx, const double *dH, int n, int k, int mode, double tol, int maxiter,
+               DevBuf<double> *d_init, int *iters, double *fp)
+{
+    double mx = 0.0;
+    IPT_STCHK(offdiag_absmax(ctx, dH, n, &mx));
+    LowpOperator op;
+    IPT_STCHK(lowp_operator_create(ctx, dH, n, n, mode, mx, &op));
+    const size_t nk = (size_t)n * (size_t)k;
+    DevBuf<double> a, b, y, dg;
+    IPT_STCHK(a.alloc(nk));
+    IPT_STCHK(b.alloc(nk));
+    IPT_STCHK(y.alloc(nk));
+    IPT_STCHK(dg.alloc((size_t)n));
+    IPT_STCHK(set_identity(ctx, a.p, n, n, k));
+    IPT_STCHK(dense_diagonal(ctx, dH, n, n, dg.p));
+    auto apply = [&](const double *X, double *Y) {
+        return lowp_apply_delta(ctx, op, X, nullptr, nullptr, n, dg.p, k, Y);
+    };
+    double *x = nullptr;
+    IPT_STCHK(plain_loop(ctx, apply, n, k, 1, n, dg.p, a.p, b.p, y.p, maxiter, tol, 1, &x, iters, fp));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    d_init->release();
+    d_init->p = (x == a.p) ? a.take() : b.take();
+    d_init->count = nk;
+    return IPT_CUDA_SUCCESS;
+}
+
+struct TwoPhase {
+    int lo_iters = 0, hi_iters = 0;
+    double lo_time = NAN, hi_time = NAN, total = NAN, lo_fp = NAN, hi_fp = NAN;
+    double *vec = nullptr, *val = nullptr;
+};
+
+int two_phase(const double *dH, int n, int k, int mode, double lo_tol, int lo_maxiter, double hi_tol,
+              int hi_maxiter, void *handle, TwoPhase *out)
+{
+    if (!mixed_args_ok(dH, n, k, lo_tol, lo_maxiter, hi_tol, hi_maxiter)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    const double t0 = now_seconds();
+    DevBuf<double> init;
+    IPT_STCHK(lowp_phase(ctx, dH, n, k, mode, lo_tol, lo_maxiter, &init, &out->lo_iters, &out->lo_fp));
+    const double t1 = now_seconds();
+    IPT_STCHK(dense_plain_device(ctx, dH, n, k, init.p, hi_maxiter, hi_tol, 1, &out->vec, &out->val,
+                                 &out->hi_iters, &out->hi_fp));
+    const double t2 = now_seconds();
+    out->lo_time = t1 - t0;
+    out->hi_time = t2 - t1;
+    out->total = t2 - t0;
+    return IPT_CUDA_SUCCESS;
+}
+
+// Zero counts and pointers, NaN times and residuals (the reference's reset,
+// ipt_mixed_precision_cuda.cu:94-155, ipt_rayleigh_mixed_cuda.cu:50-70).
+// All four result structs are four ints, then doubles, then the two device
+// pointers.
+template <typename R>
+void reset_two_phase(R *r)
+{
+    static_assert(offsetof(R, d_vectors) % sizeof(double) == 0 && 4 * sizeof(int) % sizeof(double) == 0,
+                  "result layout: ints, doubles, pointers");
+    memset(r, 0, sizeof(*r));
+    double *first = reinterpret_cast<double *>(reinterpret_cast<char *>(r) + 4 * sizeof(int));
+    double *end = reinterpret_cast<double *>(&r->d_vectors);
+    for (double *q = first; q < end; ++q) {
+        *q = NAN;
+    }
+}
+
+// Householder QR of Q in place (Q <- first k columns of the orthogonal
+// factor), rocSOLVER geqrf + orgqr as the reference's cuSOLVER pair
+// (ipt_rayleigh_mixed_cuda.cu:141-254).
+int orthogonalize_qr(Context *ctx, double *Q, int n, int k)
+{
+    void *blas = nullptr;
+    IPT_STCHK(context_blas(ctx, &blas));
+    DevBuf<double> tau;
+    IPT_STCHK(tau.alloc((size_t)k));
+    if (rocsolver_dgeqrf((rocblas_handle)blas, n, k, Q, n, tau.p) != rocblas_status_success) {
+        return IPT_CUDA_CUSOLVER_ERROR;
+    }
+    if (rocsolver_dorgqr((rocblas_handle)blas, n, k, k, Q, n, tau.p) != rocblas_status_success) {
+        return IPT_CUDA_CUSOLVER_ERROR;
+    }
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
+
+}  // namespace
+}  // namespace ipt
+
+using namespace ipt;
+
+#define IPT_TWO_PHASE_ENTRY(NAME, FREE, TYPE, MODE, LO_IT, LO_TIME, LO_FP)                                        \
+    extern "C" void FREE(TYPE *result)                                                                            \
+    {                                                                                                             \
+        if (result == nullptr) {                                                                                  \
+            return;                                                                                               \
+        }                                                                                                         \
+        (void)hipFree(result->d_vectors);                                                                         \
+        (void)hipFree(result->d_values);                                                                          \
+        reset_two_phase(result);                                                                                  \
+    }                                                                                                             \
+    extern "C" int NAME(const double *dH, int n, int k, double lo_tol, int lo_maxiter, double hi_tol,             \
+                        int hi_maxiter, void *handle, TYPE *result)                                               \
+    {                                                                                                             \
+        if (result == nullptr) {                                                                                  \
+            return IPT_CUDA_INVALID_ARGUMENT;                                                                     \
+        }                                                                                                         \
+        reset_two_phase(result);                                                                                  \
+        TwoPhase tp;                                                                                              \
+        const int st = two_phase(dH, n, k, MODE, lo_tol, lo_maxiter, hi_tol, hi_maxiter, handle, &tp);            \
+        if (st != IPT_CUDA_SUCCESS) {                                                                             \
+            (void)hipFree(tp.vec);                                                                                \
+            (void)hipFree(tp.val);                                                                                \
+            return st;                                                                                            \
+        }                                                                                                         \
+        result->n = n;                                                                                            \
+        result->k = k;                                                                                            \
+        result->LO_IT = tp.lo_iters;                                                                              \
+        result->fp64_iterations = tp.hi_iters;                                                                    \
+        result->LO_TIME = tp.lo_time;                                                                             \
+        result->fp64_time_sec = tp.hi_time;                                                                       \
+        result->total_time_sec = tp.total;                                                                        \
+        result->LO_FP = tp.lo_fp;                                                                                 \
+        result->fp64_fixed_point_residual = tp.hi_fp;                                                             \
+        result->d_vectors = tp.vec;                                                                               \
+        result->d_values = tp.val;                                                                                \
+        return IPT_CUDA_SUCCESS;                                                                                  \
+    }
+
+IPT_TWO_PHASE_ENTRY(ipt_mixed_precision_cuda_device_tol, ipt_mixed_precision_cuda_free_device_result,
+                    IPTMixedCudaDeviceResult, LOWP_FP16, fp16_iterations, fp16_time_sec, fp16_fixed_point_residual)
+IPT_TWO_PHASE_ENTRY(ipt_mixed_precision_fp32_cuda_device_tol, ipt_mixed_precision_fp32_cuda_free_device_result,
+                    IPTMixedFP32CudaDeviceResult, LOWP_FP32, fp32_iterations, fp32_time_sec,
+                    fp32_fixed_point_residual)
+IPT_TWO_PHASE_ENTRY(ipt_mixed_precision_tf32_cuda_device_tol, ipt_mixed_precision_tf32_cuda_free_device_result,
+                    IPTMixedTF32CudaDeviceResult, LOWP_BF16X3, tf32_iterations, tf32_time_sec,
+                    tf32_fixed_point_residual)
+
+#undef IPT_TWO_PHASE_ENTRY
+
+extern "C" void ipt_rayleigh_mixed_cuda_free_device_result(IPTRayleighMixedCudaDeviceResult *result)
+{
+    if (result == nullptr) {
+        return;
+    }
+    (void)hipFree(result->d_vectors);
+    (void)hipFree(result->d_values);
+    reset_two_phase(result);
+}
+
+extern "C" int ipt_rayleigh_mixed_tf32_cuda_device_tol(const double *dH, int n, int k, double tf32_tol,
+                                                       int tf32_maxiter, double fp64_tol, int fp64_maxiter,
+                                                       void *handle, IPTRayleighMixedCudaDeviceResult *result)
+{
+    if (result == nullptr) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    reset_two_phase(result);
+    if (!mixed_args_ok(dH, n, k, tf32_tol, tf32_maxiter, fp64_tol, fp64_maxiter)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    double t = now_seconds();
+    DevBuf<double> Q;
+    IPT_STCHK(lowp_phase(ctx, dH, n, k, LOWP_BF16X3, tf32_tol, tf32_maxiter, &Q, &result->tf32_iterations,
+                         &result->tf32_fixed_point_residual));
+    result->tf32_time_sec = now_seconds() - t;
+
+    t = now_seconds();
+    IPT_STCHK(orthogonalize_qr(ctx, Q.p, n, k));
+    result->orthogonalize_time_sec = now_seconds() - t;
+
+    // B = Q^T (H Q), symmetrised (ipt_rayleigh_mixed_cuda.cu:319-335)
+    t = now_seconds();
+    const size_t nk = (size_t)n * (size_t)k;
+    DevBuf<double> HQ, B;
+    IPT_STCHK(HQ.alloc(nk));
+    IPT_STCHK(B.alloc((size_t)k * k));
+    IPT_STCHK(gemm_f64(ctx, dH, n, Q.p, n, HQ.p, n, n, k));
+    IPT_STCHK(gemm_tn_tall(ctx, Q.p, n, HQ.p, n, B.p, k, n, k, k));
+    IPT_STCHK(symmetrize(ctx, B.p, k, k));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    result->rayleigh_transform_time_sec = now_seconds() - t;
+
+    // plain FP64 IPT on the k x k Rayleigh quotient (ipt_rayleigh_mixed_cuda.cu:345-352)
+    t = now_seconds();
+    double *dz = nullptr, *dval = nullptr;
+    st = dense_plain_device(ctx, B.p, k, k, nullptr, fp64_maxiter, fp64_tol, 1, &dz, &dval,
+                            &result->fp64_iterations, &result->fp64_fixed_point_residual);
+    if (st != IPT_CUDA_SUCCESS) {
+        return st;
+    }
+    DevBuf<double> Z, vals;
+    Z.p = dz;
+    Z.count = (size_t)k * k;
+    vals.p = dval;
+    vals.count = (size_t)k;
+    result->fp64_time_sec = now_seconds() - t;
+
+    // vectors = Q Z (ipt_rayleigh_mixed_cuda.cu:355-362)
+    t = now_seconds();
+    DevBuf<double> V;
+    IPT_STCHK(V.alloc(nk));
+    IPT_STCHK(gemm_nn_small(ctx, Q.p, n, Z.p, k, V.p, n, n, k, k, 1.0, 0.0));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    result->backtransform_time_sec = now_seconds() - t;
+    result->total_time_sec = result->tf32_time_sec + result->orthogonalize_time_sec +
+                             result->rayleigh_transform_time_sec + result->fp64_time_sec +
+                             result->backtransform_time_sec;
+    result->n = n;
+    result->k = k;
+    result->d_vectors = V.take();
+    result->d_values = vals.take();
+    return IPT_CUDA_SUCCESS;
+}

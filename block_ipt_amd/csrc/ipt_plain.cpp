@@ -10,6 +10,7 @@
 #include <functional>
 
 #include "ipt_internal.h"
+#include "ipt_plain.h"
 #include "ipt_prep.h"
 
 namespace ipt {
@@ -28,6 +29,8 @@ double seconds_since(const std::chrono::steady_clock::time_point &t0)
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+
+}  // namespace
 
 // Generic fixed-point loop.  apply(X, Y) computes Y = H X in the layout
 // (rs, cs); X/Y/Xn are n x k buffers in that layout.  On return *x_final
@@ -107,6 +110,8 @@ int dense_plain_device(Context *ctx, const double *dH, int n, int k, const doubl
     *d_val_out = vals.take();
     return IPT_CUDA_SUCCESS;
 }
+
+namespace {
 
 // Sparse plain IPT with a device CSR copy of H; iterate kept row-major.
 int sparse_plain_device_csr(Context *ctx, const int *d_rp, const int *d_ci, const double *d_v,

@@ -106,3 +106,46 @@ def test_null_pointers_rejected():
     dv, dl = ctypes.c_void_p(), ctypes.c_void_p()
     assert abi.lib().ipt_cuda_device(None, 4, 2, 5, None, ctypes.byref(dv), ctypes.byref(dl)) == 1
     assert dv.value is None and dl.value is None
+
+
+def test_mixed_result_struct_layout_matches_c(tmp_path):
+    src = tmp_path / "probe_mixed.c"
+    src.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "ipt_hip_mixed.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n",
+         sizeof(IPTMixedCudaDeviceResult), offsetof(IPTMixedCudaDeviceResult, fp16_time_sec),
+         offsetof(IPTMixedCudaDeviceResult, d_vectors), sizeof(IPTMixedFP32CudaDeviceResult),
+         sizeof(IPTMixedTF32CudaDeviceResult), sizeof(IPTRayleighMixedCudaDeviceResult),
+         offsetof(IPTRayleighMixedCudaDeviceResult, fp64_fixed_point_residual),
+         offsetof(IPTRayleighMixedCudaDeviceResult, d_values));
+  return 0; }
+""")
+    exe = tmp_path / "probe_mixed"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    M, R = abi.IPTMixedCudaDeviceResult, abi.IPTRayleighMixedCudaDeviceResult
+    want = [ctypes.sizeof(M), M.fp16_time_sec.offset, M.d_vectors.offset,
+            ctypes.sizeof(abi.IPTMixedFP32CudaDeviceResult), ctypes.sizeof(abi.IPTMixedTF32CudaDeviceResult),
+            ctypes.sizeof(R), R.fp64_fixed_point_residual.offset, R.d_values.offset]
+    assert got == want
+
+
+def test_mixed_free_resets_fields():
+    r = abi.IPTMixedTF32CudaDeviceResult()
+    r.n = 7
+    r.tf32_iterations = 3
+    abi.lib().ipt_mixed_precision_tf32_cuda_free_device_result(ctypes.byref(r))
+    assert r.n == 0 and r.tf32_iterations == 0 and not r.d_vectors
+    assert np.isnan(r.tf32_time_sec) and np.isnan(r.fp64_fixed_point_residual)
+
+
+def test_mixed_invalid_arguments_rejected_before_gpu():
+    r = abi.IPTMixedCudaDeviceResult()
+    st = abi.lib().ipt_mixed_precision_cuda_device_tol(None, 4, 2, 1e-3, 10, 1e-12, 10, None, ctypes.byref(r))
+    assert st == 1 and not r.d_vectors
+    st = abi.lib().ipt_mixed_precision_cuda_device_tol(ctypes.c_void_p(8), 4, 2, 0.0, 10, 1e-12, 10, None,
+                                                       ctypes.byref(r))
+    assert st == 1

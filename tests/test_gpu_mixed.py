@@ -12,12 +12,13 @@ pytestmark = pytest.mark.gpu
 from block_ipt_amd import api  # noqa: E402
 
 
-@pytest.mark.parametrize("mode,tol", [("fp16", 3e-3), ("bf16", 2e-2), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("mode,tol", [("fp16", 3e-3), ("bf16", 2e-2), ("bf16x3", 1e-4), ("fp32", 1e-5)])
 @pytest.mark.parametrize("n,nc", [(300, 7), (517, 40), (1024, 130)])
 def test_lowp_product_error_bound(mode, tol, n, nc):
     """Y = Ya + D(B-Ba) + E(B-Ba): the low-precision term's error relative
     to ||E (B-Ba)|| stays at the input precision (unit roundoff 2^-11 FP16,
-    2^-8 BF16, ~2^-16 BF16x3, times a small accumulation factor)."""
+    2^-8 BF16, ~2^-16 BF16x3, 2^-24 FP32, times a small accumulation
+    factor)."""
     hip = api.hip()
     H = orc.rand_dense(n, 1, 0.0, 10.0, 0.1)
     rng = np.random.default_rng(n + nc)
@@ -64,7 +65,7 @@ def mixed_env(monkeypatch):
     return monkeypatch
 
 
-@pytest.mark.parametrize("mode", ["fp16", "bf16", "bf16x3"])
+@pytest.mark.parametrize("mode", ["fp16", "bf16", "bf16x3", "fp32"])
 def test_mixed_block_solve_matches_fp64(mixed_env, mode):
     """C1 recipe (N=2048, k=16, d ~ U[0,10], V scale 0.05/sqrt(N))."""
     n, k = 2048, 16
@@ -80,3 +81,77 @@ def test_mixed_block_solve_matches_fp64(mixed_env, mode):
     assert res.max() <= 1e-10
     # anchored steps converge like the FP64 iteration (a broken product would not)
     assert im["iterations"] <= 2 * i64["iterations"] + 12
+
+
+# ------------------------------------------------------------------ two-phase entry points
+
+def _plain_fp64(dH, n, k, tol, maxiter):
+    hip = api.hip()
+    dv, dl, it, fp = api.ipt_cuda_device_tol(dH, n, k, tol, maxiter)
+    try:
+        return hip.to_host(dv, (n, k)), hip.to_host(dl, (k,)), it
+    finally:
+        api.free_device_result(dv, dl)
+
+
+@pytest.fixture
+def plain_env(monkeypatch):
+    for v in ("IPT_BLOCK_CLUSTER", "IPT_PRECISION"):
+        monkeypatch.delenv(v, raising=False)
+    return monkeypatch
+
+
+@pytest.mark.parametrize("kind,lo_tol", [("fp16", 1e-3), ("fp32", 1e-6), ("tf32", 1e-6)])
+@pytest.mark.parametrize("n,k", [(512, 512), (1000, 37)])
+def test_two_phase_matches_plain_fp64(plain_env, kind, lo_tol, n, k):
+    """ipt_mixed_precision{,_fp32,_tf32}_cuda_device_tol: low-precision IPT to
+    lo_tol, FP64 IPT from there to 1e-12.  The result is the FP64 fixed point
+    (same as the plain FP64 solve to 1e-10) and the FP64 phase needs fewer
+    iterations than a cold FP64 start (the splitmix64 sweep matrix,
+    eps 0.01, as in the reference's mixed benchmarks)."""
+    hip = api.hip()
+    H = orc.eps_dense(n, 0.01, 20260603 + n)
+    dH = hip.to_device(np.asfortranarray(H))
+    try:
+        v64, l64, it64 = _plain_fp64(dH, n, k, 1e-12, 100)
+        vm, lm, info = api.mixed_device_tol(kind, dH, n, k, lo_tol, 100, 1e-12, 100)
+    finally:
+        hip.free(dH)
+    lo = {"fp16": "fp16", "fp32": "fp32", "tf32": "tf32"}[kind]
+    assert info["n"] == n and info["k"] == k
+    assert info["%s_fixed_point_residual" % lo] <= lo_tol
+    assert info["fp64_fixed_point_residual"] <= 1e-12
+    assert info["fp64_iterations"] < it64
+    assert np.abs(lm - l64).max() <= 1e-10 * max(1.0, np.abs(l64).max())
+    assert np.abs(vm - v64).max() <= 1e-9
+    assert info["total_time_sec"] >= info["fp64_time_sec"] > 0.0
+
+
+def test_rayleigh_mixed_tf32(plain_env):
+    """ipt_rayleigh_mixed_tf32_cuda_device_tol: BF16x3 IPT -> QR -> FP64 IPT
+    on Q^T H Q -> Q Z.  Ritz values of the low-precision subspace are
+    accurate to the square of its error; vectors to the error itself."""
+    hip = api.hip()
+    n, k = 1024, 24
+    H = orc.eps_dense(n, 0.01, 20260603 + n)
+    dH = hip.to_device(np.asfortranarray(H))
+    try:
+        v64, l64, _ = _plain_fp64(dH, n, k, 1e-12, 100)
+        vr, lr, info = api.mixed_device_tol("rayleigh_tf32", dH, n, k, 1e-6, 100, 1e-12, 100)
+    finally:
+        hip.free(dH)
+    assert info["tf32_fixed_point_residual"] <= 1e-6 and info["fp64_fixed_point_residual"] <= 1e-12
+    order = np.argsort(lr)
+    assert np.abs(np.sort(lr) - np.sort(l64)).max() <= 1e-9 * max(1.0, np.abs(l64).max())
+    # orthonormal Ritz vectors, residual at the subspace's accuracy
+    G = vr.T @ vr
+    assert np.abs(G - np.eye(k)).max() <= 1e-10
+    R = H @ vr - vr * lr
+    assert np.linalg.norm(R, axis=0).max() <= 1e-5
+    assert order.size == k
+
+
+def test_two_phase_invalid_arguments():
+    with pytest.raises(api.IptError) as e:
+        api.mixed_device_tol("fp16", 1, 4, 8, 1e-3, 10, 1e-12, 10)  # k > n
+    assert e.value.status == 1
