@@ -272,7 +272,7 @@ def mixed_device_tol(kind, d_H, n, k, lo_tol, lo_maxiter, hi_tol, hi_maxiter, ha
 
 # ------------------------------------------------------------------ extensions
 
-LOWP_MODES = {"fp16": 0, "bf16": 1, "bf16x3": 2, "fp32": 3}
+LOWP_MODES = {"fp16": 0, "bf16": 1, "bf16x3": 2, "fp32": 3, "fp8": 4}
 
 
 def generate_dense(d_H, n, seed, lo, hi, vscale, handle=None):

@@ -439,7 +439,9 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     double *cur = Ba.p, *nxt = Bb.p;
     auto t0 = std::chrono::steady_clock::now();
     for (int it = 0; it < p.maxiter; ++it) {
-        const bool exact = !mixed || (it % (p.anchor_every + 1) == p.anchor_every);
+        // every (anchor_every+1)-th step is exact, and so is the last one, so
+        // that the returned basis always comes out of an exact step
+        const bool exact = !mixed || (it % (p.anchor_every + 1) == p.anchor_every) || it == p.maxiter - 1;
         if (exact) {
             IPT_STCHK(apply(cur, Y.p, bc));
             out->fp64_products += 1;
@@ -542,6 +544,8 @@ BlockParams params_from_env(int maxiter, double tol, int use_tol)
             p.lowp_mode = LOWP_BF16X3;
         } else if (!strcmp(prec, "fp32")) {
             p.lowp_mode = LOWP_FP32;
+        } else if (!strcmp(prec, "fp8")) {
+            p.lowp_mode = LOWP_FP8;
         }
     }
     p.anchor_every = env_int("IPT_MIXED_ANCHOR_EVERY", 20);

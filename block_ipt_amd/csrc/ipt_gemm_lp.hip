@@ -59,6 +59,44 @@ __device__ inline unsigned short to_bf16_bits(float x)
 
 __device__ inline float bf16_bits_to_float(unsigned short b) { return __uint_as_float(((unsigned int)b) << 16); }
 
+// float -> OCP FP8 E4M3 bits (1-4-3, bias 7, max 448, no infinity), round to
+// nearest even, saturating at 448; x finite.
+__device__ inline unsigned int to_e4m3(float x)
+{
+    const unsigned int sign = (__float_as_uint(x) >> 24) & 0x80u;
+    const float a = fminf(fabsf(x), 448.0f);
+    unsigned int code;
+    if (a < 0.015625f) {
+        // below 2^-6: subnormal grid of 2^-9 (a rounded multiple of 8 is the
+        // smallest normal, whose code is also 8)
+        code = (unsigned int)__builtin_rintf(a * 512.0f);
+    } else {
+        const unsigned int ua = __float_as_uint(a);
+        unsigned int e = ((ua >> 23) & 0xFFu) - 127u + 7u;
+        unsigned int r = (ua >> 20) & 7u;
+        const unsigned int rem = ua & 0xFFFFFu;
+        if (rem > 0x80000u || (rem == 0x80000u && (r & 1u))) {
+            ++r;
+        }
+        if (r == 8u) {
+            r = 0u;
+            ++e;
+        }
+        code = (e << 3) | r;
+        if (code > 0x7Eu) {
+            code = 0x7Eu;
+        }
+    }
+    return sign | code;
+}
+
+// eight floats -> 8 E4M3 bytes (element j in byte j)
+__device__ inline void pack_e4m3(const float *v, unsigned int &lo, unsigned int &hi)
+{
+    lo = to_e4m3(v[0]) | (to_e4m3(v[1]) << 8) | (to_e4m3(v[2]) << 16) | (to_e4m3(v[3]) << 24);
+    hi = to_e4m3(v[4]) | (to_e4m3(v[5]) << 8) | (to_e4m3(v[6]) << 16) | (to_e4m3(v[7]) << 24);
+}
+
 // ------------------------------------------------------------------ pack E
 
 // One thread per (tile, lane): 8 elements.  FP16: fp16 of scale*E;
@@ -121,6 +159,67 @@ __global__ __launch_bounds__(256) void k_pack_E(const double *__restrict__ H, lo
     }
 }
 
+// FP8 tile scales: one block per 128 x 128 tile (blockIdx.x = k tile,
+// blockIdx.y = row tile) of E = offdiag(H); scale = max|E_tile| / 448, or 1
+// for an all-zero tile.
+__global__ __launch_bounds__(256) void k_tile_scale(const double *__restrict__ H, long ld, int n, int nkt,
+                                                    float *__restrict__ tscale)
+{
+    const long r0 = (long)blockIdx.y * 128, c0 = (long)blockIdx.x * 128;
+    double m = 0.0;
+    for (int e = threadIdx.x; e < 128 * 128; e += 256) {
+        const long r = r0 + (e & 127), c = c0 + (e >> 7);
+        if (r < n && c < n && r != c) {
+            m = fmax(m, fabs(H[r + c * ld]));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m = fmax(m, __shfl_down(m, o, 64));
+    }
+    __shared__ double sm[4];
+    if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+        tscale[(long)blockIdx.y * nkt + blockIdx.x] = m > 0.0 ? (float)(m / 448.0) : 1.0f;
+    }
+}
+
+// FP8 E: one thread per (row block, k-block pair, lane); bytes 0-7 the
+// even k block, 8-15 the odd one, each k = 32 kb + 8(l/16) + j.
+__global__ __launch_bounds__(256) void k_pack_E_fp8(const double *__restrict__ H, long ld, int n, int nkb, int nrb,
+                                                    const float *__restrict__ tscale, uint4 *__restrict__ out)
+{
+    const int nkp = nkb / 2;
+    const long total = (long)nrb * nkp * 64;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const int l = (int)(t & 63);
+        const long rb = (t >> 6) / nkp, kp = (t >> 6) % nkp;
+        const long r = rb * 16 + (l & 15);
+        unsigned int w[4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const long kb = 2 * kp + h;
+            const float inv = 1.0f / tscale[(rb >> 3) * (nkb >> 2) + (kb >> 2)];
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const long k = kb * 32 + 8 * (l >> 4) + j;
+                v[j] = (r < n && k < n && r != k) ? (float)H[r + k * ld] * inv : 0.0f;
+            }
+            pack_e4m3(v, w[2 * h], w[2 * h + 1]);
+        }
+        uint4 q;
+        q.x = w[0];
+        q.y = w[1];
+        q.z = w[2];
+        q.w = w[3];
+        out[t] = q;
+    }
+}
+
 int lowp_operator_create(Context *ctx, const double *H, long ld, int n, int mode, double max_abs_offdiag,
                          LowpOperator *op)
 {
@@ -143,6 +242,18 @@ int lowp_operator_create(Context *ctx, const double *H, long ld, int n, int mode
             e = -100;
         }
         op->scale = (float)std::ldexp(1.0, e);
+    }
+    if (mode == LOWP_FP8) {
+        const int nkt = op->nkb / 4, nrt = (op->nrb + 7) / 8;
+        IPT_STCHK(op->tscale.alloc((size_t)nrt * nkt));
+        IPT_STCHK(op->hi.alloc((size_t)op->nrb * op->nkb * 256));  // one byte per element
+        hipLaunchKernelGGL(k_tile_scale, dim3(nkt, nrt), dim3(256), 0, ctx->stream, H, ld, n, nkt, op->tscale.p);
+        IPT_HIPCHK(hipGetLastError());
+        const long total = (long)op->nrb * (op->nkb / 2) * 64;
+        hipLaunchKernelGGL(k_pack_E_fp8, dim3(blocks_for(total, 256, 65536)), dim3(256), 0, ctx->stream, H, ld, n,
+                           op->nkb, op->nrb, op->tscale.p, reinterpret_cast<uint4 *>(op->hi.p));
+        IPT_HIPCHK(hipGetLastError());
+        return IPT_CUDA_SUCCESS;
     }
     const size_t elems = (size_t)op->nrb * op->nkb * 512;
     IPT_STCHK(op->hi.alloc(elems));
@@ -198,11 +309,11 @@ __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict_
                                                        double *__restrict__ colscale, uint4 *__restrict__ zhi,
                                                        uint4 *__restrict__ zlo)
 {
-    const long total = (long)nkb * nb * 64;
+    const long total = (long)(mode == LOWP_FP8 ? nkb / 2 : nkb) * nb * 64;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
         const int l = (int)(t & 63);
         const int cb = (int)((t >> 6) % nb);
-        const long kb = (t >> 6) / nb;
+        const long kb = (t >> 6) / nb;  // FP8: k-block pair
         const int c = cb * 16 + (l & 15);
         double s = 1.0;
         if (c < nc) {
@@ -210,11 +321,37 @@ __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict_
             if (mx > 0.0) {
                 int e = 0;
                 (void)frexp(mx, &e);  // mx = f * 2^e, f in [0.5,1)
-                s = ldexp(1.0, e);
+                // FP8: scaled column maximum in [128, 256) (E4M3 max 448)
+                s = ldexp(1.0, mode == LOWP_FP8 ? e - 8 : e);
             }
             if (t < 64 * nb && (l >> 4) == 0) {
                 colscale[c] = s;  // k block 0, first lane group: once per column
             }
+        }
+        if (mode == LOWP_FP8) {
+            unsigned int w[4];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const long k = (2 * kb + h) * 32 + 8 * (l >> 4) + j;
+                    double x = 0.0;
+                    if (c < nc && k < n) {
+                        const long o = k + (long)c * ld;
+                        x = (B[o] - (Ba != nullptr ? Ba[o] : 0.0)) / s;
+                    }
+                    v[j] = (float)x;
+                }
+                pack_e4m3(v, w[2 * h], w[2 * h + 1]);
+            }
+            uint4 q;
+            q.x = w[0];
+            q.y = w[1];
+            q.z = w[2];
+            q.w = w[3];
+            zhi[t] = q;
+            continue;
         }
         double x[8];
 #pragma unroll
@@ -315,30 +452,31 @@ __global__ __launch_bounds__(256) void k_lowp_epilogue(const f4_t *__restrict__ 
 // wavefronts per SIMD)
 constexpr int lowp_kw(int mode, int nb)
 {
-    return lowp_two_planes(mode) ? (nb == 1 ? 8 : nb <= 5 ? 4 : 2) : (nb <= 5 ? 8 : 4);
+    return mode == LOWP_FP8 ? (nb <= 5 ? 8 : 4)
+                            : lowp_two_planes(mode) ? (nb == 1 ? 8 : nb <= 5 ? 4 : 2) : (nb <= 5 ? 8 : 4);
 }
 
 template <int MODE, int NB>
 static void launch_lowp_nb(dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
-                           const uint4 *Zlo, float *P, int nkb, int nrb, int rbw)
+                           const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw)
 {
-    hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, lowp_kw(MODE, NB)>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, P, nkb,
-                       nrb, rbw);
+    hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, lowp_kw(MODE, NB)>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, ts, P,
+                       nkb, nrb, rbw);
 }
 
 template <int MODE>
 static void launch_lowp(int NB, dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
-                        const uint4 *Zlo, float *P, int nkb, int nrb, int rbw)
+                        const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw)
 {
     switch (NB) {
-    case 1: launch_lowp_nb<MODE, 1>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
-    case 2: launch_lowp_nb<MODE, 2>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
-    case 3: launch_lowp_nb<MODE, 3>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
-    case 4: launch_lowp_nb<MODE, 4>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
-    case 5: launch_lowp_nb<MODE, 5>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
-    case 6: launch_lowp_nb<MODE, 6>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
-    case 7: launch_lowp_nb<MODE, 7>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
-    default: launch_lowp_nb<MODE, 8>(grid, st, Ehi, Elo, Zhi, Zlo, P, nkb, nrb, rbw); break;
+    case 1: launch_lowp_nb<MODE, 1>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    case 2: launch_lowp_nb<MODE, 2>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    case 3: launch_lowp_nb<MODE, 3>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    case 4: launch_lowp_nb<MODE, 4>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    case 5: launch_lowp_nb<MODE, 5>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    case 6: launch_lowp_nb<MODE, 6>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    case 7: launch_lowp_nb<MODE, 7>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    default: launch_lowp_nb<MODE, 8>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
     }
 }
 
@@ -378,21 +516,25 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         hipLaunchKernelGGL(k_delta_colmax, dim3(std::min(8, (n + 255) / 256), w), dim3(256), 0, ctx->stream, Bc, Bac,
                            ld, n, w, colmax);
         IPT_HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_delta_convert, dim3(blocks_for((long)zvec, 256, 16384)), dim3(256), 0, ctx->stream, Bc,
+        const size_t zconv = op.mode == LOWP_FP8 ? zvec / 2 : zvec;  // FP8: one entry per k-block pair
+        hipLaunchKernelGGL(k_delta_convert, dim3(blocks_for((long)zconv, 256, 16384)), dim3(256), 0, ctx->stream, Bc,
                            Bac, ld, n, w, op.nkb, NB, op.mode, colmax, colscale, zhi, zlo);
         IPT_HIPCHK(hipGetLastError());
         dim3 grid(gx, splits);
         const uint4 *Ehi = reinterpret_cast<const uint4 *>(op.hi.p);
         const uint4 *Elo = reinterpret_cast<const uint4 *>(op.lo.p);
         const ProfileToken prof = profile_begin(ctx, PROF_LOWP_GEMM);
+        const float *ts = op.tscale.p;
         if (op.mode == LOWP_BF16X3) {
-            launch_lowp<LOWP_BF16X3>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, P, op.nkb, op.nrb, rbw);
+            launch_lowp<LOWP_BF16X3>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
         } else if (op.mode == LOWP_FP32) {
-            launch_lowp<LOWP_FP32>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, P, op.nkb, op.nrb, rbw);
+            launch_lowp<LOWP_FP32>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
+        } else if (op.mode == LOWP_FP8) {
+            launch_lowp<LOWP_FP8>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
         } else if (op.mode == LOWP_FP16) {
-            launch_lowp<LOWP_FP16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, P, op.nkb, op.nrb, rbw);
+            launch_lowp<LOWP_FP16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
         } else {
-            launch_lowp<LOWP_BF16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, P, op.nkb, op.nrb, rbw);
+            launch_lowp<LOWP_BF16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
         }
         profile_end(ctx, prof);
         IPT_HIPCHK(hipGetLastError());
@@ -413,7 +555,7 @@ extern "C" int ipt_hip_lowp_product(const double *d_H, int n, int mode, const do
                                     const double *d_Ya, const double *d_diag, int nc, double *d_Y, void *handle)
 {
     if (d_H == nullptr || d_B == nullptr || d_Ba == nullptr || d_Ya == nullptr || d_diag == nullptr ||
-        d_Y == nullptr || n <= 0 || nc <= 0 || mode < LOWP_FP16 || mode > LOWP_FP32) {
+        d_Y == nullptr || n <= 0 || nc <= 0 || mode < LOWP_FP16 || mode > LOWP_FP8) {
         return IPT_CUDA_INVALID_ARGUMENT;
     }
     int st = IPT_CUDA_SUCCESS;

@@ -45,19 +45,28 @@ __device__ inline uint4 load_nt(const uint4 *p)
 // wavefront in flight across the MFMAs and the barrier, and the result is
 // deterministic.
 //
+// tscale: FP8 only (else unused).
 // P layout: tile (z, rb, cb) of 1 KiB at ((z * nrb + rb) * NB + cb) * 256
 // floats, lane l, element v = MFMA D layout (row 4(l/16) + v, column l%16).
 // Row blocks past nrb are handled by clamped loads whose results are never
 // stored; nkb is a multiple of 64, so every split is full.
+//
+// FP8: a 16-byte lane entry holds two k blocks (bytes 0-7 the even one), so
+// a wavefront's KW k blocks are KW/2 loads; the partial over each 128-wide k
+// tile (4 k blocks) is scaled by that E tile's scale before it is added.
 template <int NB, int MODE, int KW>
 __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi, const uint4 *__restrict__ Elo,
                                                    const uint4 *__restrict__ Zhi, const uint4 *__restrict__ Zlo,
-                                                   float *__restrict__ P, int nkb, int nrb, int rb_per_wg)
+                                                   const float *__restrict__ tscale, float *__restrict__ P, int nkb,
+                                                   int nrb, int rb_per_wg)
 {
     constexpr int W = 8;  // wavefronts per workgroup
     constexpr bool X3 = MODE == LOWP_BF16X3;
+    constexpr bool F8 = MODE == LOWP_FP8;
     constexpr bool TWO = lowp_two_planes(MODE);
     constexpr int NZ = TWO ? 2 : 1;
+    constexpr int LK = F8 ? KW / 2 : KW;  // 16-byte lane entries per row block and wavefront
+    static_assert(!F8 || KW % 4 == 0, "FP8 wavefronts own whole 128-wide k tiles");
     __shared__ f4_t red[2][W][NB][64];
 
     const int lane = threadIdx.x & 63;
@@ -66,28 +75,30 @@ __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi
     const int kbw = (z * W + wave) * KW;
     const int rbA = blockIdx.x * rb_per_wg;
     const int rbB = rbA + rb_per_wg < nrb ? rbA + rb_per_wg : nrb;
+    const int nke = F8 ? nkb / 2 : nkb;  // lane entries per row block
+    const int ke0 = F8 ? kbw / 2 : kbw;  // first entry of this wavefront
 
-    uint4 bz[NZ][KW][NB];
+    uint4 bz[NZ][LK][NB];
 #pragma unroll
-    for (int kk = 0; kk < KW; ++kk) {
+    for (int kk = 0; kk < LK; ++kk) {
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) {
-            const long i = ((long)(kbw + kk) * NB + cb) * 64 + lane;
+            const long i = ((long)(ke0 + kk) * NB + cb) * 64 + lane;
             bz[0][kk][cb] = Zhi[i];
             if (TWO) {
                 bz[NZ - 1][kk][cb] = Zlo[i];
             }
         }
     }
-    // tile (rb, kbw + kk) of this lane: rowbase(rb) + kk * 64
-    const long kofs = (long)kbw * 64 + lane;
-    uint4 eh[2][KW], el[2][KW];
+    // entry (rb, ke0 + kk) of this lane: rowbase(rb) + kk * 64
+    const long kofs = (long)ke0 * 64 + lane;
+    uint4 eh[2][LK], el[2][LK];
     auto loadE = [&](int rb, int slot) {
-        const long o = (long)(rb < rbB ? rb : rbB - 1) * nkb * 64 + kofs;
+        const long o = (long)(rb < rbB ? rb : rbB - 1) * nke * 64 + kofs;
         const uint4 *ph = Ehi + o;
         const uint4 *pl = Elo + o;
 #pragma unroll
-        for (int kk = 0; kk < KW; ++kk) {
+        for (int kk = 0; kk < LK; ++kk) {
             eh[slot][kk] = load_nt(ph + kk * 64);
             if (TWO) {
                 el[slot][kk] = load_nt(pl + kk * 64);
@@ -103,8 +114,39 @@ __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi
         for (int cb = 0; cb < NB; ++cb) {
             acc[cb] = (f4_t){0.f, 0.f, 0.f, 0.f};
         }
+        if constexpr (F8) {
+            const int rbs = rb < rbB ? rb : rbB - 1;
+            const float *ts = tscale + (long)(rbs >> 3) * (nkb >> 2) + (kbw >> 2);
 #pragma unroll
-        for (int kk = 0; kk < KW; ++kk) {
+            for (int g = 0; g < KW / 4; ++g) {
+                f4_t part[NB];
+#pragma unroll
+                for (int cb = 0; cb < NB; ++cb) {
+                    part[cb] = (f4_t){0.f, 0.f, 0.f, 0.f};
+                }
+#pragma unroll
+                for (int kk = 2 * g; kk < 2 * g + 2; ++kk) {
+                    const uint4 a = eh[slot][kk];
+                    const long a0 = (long)(((unsigned long)a.y << 32) | a.x);
+                    const long a1 = (long)(((unsigned long)a.w << 32) | a.z);
+#pragma unroll
+                    for (int cb = 0; cb < NB; ++cb) {
+                        const uint4 b = bz[0][kk][cb];
+                        const long b0 = (long)(((unsigned long)b.y << 32) | b.x);
+                        const long b1 = (long)(((unsigned long)b.w << 32) | b.z);
+                        part[cb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a0, b0, part[cb], 0, 0, 0);
+                        part[cb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a1, b1, part[cb], 0, 0, 0);
+                    }
+                }
+                const float sc = ts[g];
+#pragma unroll
+                for (int cb = 0; cb < NB; ++cb) {
+                    acc[cb] += sc * part[cb];
+                }
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < (F8 ? 0 : KW); ++kk) {
 #pragma unroll
             for (int cb = 0; cb < NB; ++cb) {
                 if (MODE == LOWP_FP32) {

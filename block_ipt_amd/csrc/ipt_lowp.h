@@ -8,8 +8,10 @@ namespace ipt {
 // FP16 / BF16: one 16-bit plane of E and of the operand.  BF16X3: hi and lo
 // BF16 planes (three products).  FP32: E and the operand in FP32, the two
 // planes holding the first and second four of a lane's eight k values
-// (MFMA 16x16x4 FP32).
-enum { LOWP_FP16 = 0, LOWP_BF16 = 1, LOWP_BF16X3 = 2, LOWP_FP32 = 3 };
+// (MFMA 16x16x4 FP32).  FP8: OCP E4M3 with one scale per 128 x 128 tile of E
+// (max|E_tile| / 448) and a power-of-two scale per operand column; a lane's
+// 16 bytes hold two consecutive k blocks (MFMA 16x16x32 FP8).
+enum { LOWP_FP16 = 0, LOWP_BF16 = 1, LOWP_BF16X3 = 2, LOWP_FP32 = 3, LOWP_FP8 = 4 };
 
 // Modes whose E and operand occupy two 16-byte planes per lane and tile.
 inline constexpr bool lowp_two_planes(int mode) { return mode == LOWP_BF16X3 || mode == LOWP_FP32; }
@@ -24,6 +26,7 @@ struct LowpOperator {
     float scale = 1.f;  // E was multiplied by `scale` before rounding (FP16)
     DevBuf<unsigned short> hi;
     DevBuf<unsigned short> lo;  // second plane (BF16x3, FP32)
+    DevBuf<float> tscale;       // FP8: scale of tile (row tile rt, k tile kt) at rt * (nkb/4) + kt
 };
 
 // Pack E from a dense column-major FP64 H; max_abs_offdiag sizes the FP16 scale.

@@ -22,8 +22,9 @@ int ipt_hip_generate_dense(double *d_H, int n, unsigned long long seed, double l
                            void *handle);
 
 /* Y = Ya + diag(d)(B - Ba) + E (B - Ba), E = H - diag(H) rounded to
- * precision `mode` (0 fp16, 1 bf16, 2 bf16x3), FP32-accumulated MFMA.  All
- * device, column-major n x nc (leading dimension n); H n x n. */
+ * precision `mode` (0 fp16, 1 bf16, 2 bf16x3, 3 fp32, 4 fp8 E4M3 with one
+ * scale per 128 x 128 tile of E), FP32-accumulated MFMA.  All device,
+ * column-major n x nc (leading dimension n); H n x n. */
 int ipt_hip_lowp_product(const double *d_H, int n, int mode, const double *d_B, const double *d_Ba,
                          const double *d_Ya, const double *d_diag, int nc, double *d_Y, void *handle);
 

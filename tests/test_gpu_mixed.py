@@ -12,13 +12,15 @@ pytestmark = pytest.mark.gpu
 from block_ipt_amd import api  # noqa: E402
 
 
-@pytest.mark.parametrize("mode,tol", [("fp16", 3e-3), ("bf16", 2e-2), ("bf16x3", 1e-4), ("fp32", 1e-5)])
+@pytest.mark.parametrize("mode,tol", [("fp16", 3e-3), ("bf16", 2e-2), ("bf16x3", 1e-4), ("fp32", 1e-5),
+                                      ("fp8", 0.1)])
 @pytest.mark.parametrize("n,nc", [(300, 7), (517, 40), (1024, 130)])
 def test_lowp_product_error_bound(mode, tol, n, nc):
     """Y = Ya + D(B-Ba) + E(B-Ba): the low-precision term's error relative
     to ||E (B-Ba)|| stays at the input precision (unit roundoff 2^-11 FP16,
-    2^-8 BF16, ~2^-16 BF16x3, 2^-24 FP32, times a small accumulation
-    factor)."""
+    2^-8 BF16, ~2^-16 BF16x3, 2^-24 FP32, 2^-4 FP8 E4M3 on both operands,
+    times a small accumulation factor).  A wrong FP8 encoding (another
+    exponent bias, the FNUZ variant) is off by a factor of two and fails."""
     hip = api.hip()
     H = orc.rand_dense(n, 1, 0.0, 10.0, 0.1)
     rng = np.random.default_rng(n + nc)
@@ -81,6 +83,26 @@ def test_mixed_block_solve_matches_fp64(mixed_env, mode):
     assert res.max() <= 1e-10
     # anchored steps converge like the FP64 iteration (a broken product would not)
     assert im["iterations"] <= 2 * i64["iterations"] + 12
+
+
+def test_fp8_block_solve_with_refinement(mixed_env):
+    """FP8 E4M3 correction products (per-tile scaled E) anchored by FP64
+    products, then the FP64 Rayleigh-Ritz + Davidson refinement: the same 16
+    pairs as LAPACK to 1e-10 with residual <= 1e-10.  (Without refinement a
+    slowly converging cluster stalls at the FP8 noise floor, about 1e-9.)"""
+    n, k = 2048, 16
+    H = orc.rand_dense(n, 0, 0.0, 10.0, 0.05)
+    mixed_env.setenv("IPT_PRECISION", "fp8")
+    mixed_env.setenv("IPT_MIXED_ANCHOR_EVERY", "5")
+    mixed_env.setenv("IPT_DAVIDSON_ENRICH", "1")
+    mixed_env.setenv("IPT_DAVIDSON_STEPS", "10")
+    mixed_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "1e-11")
+    vm, lm, im = api.ipt_cuda_tol(H, k, 1e-12, 400)
+    w = sl.eigh(H, eigvals_only=True, subset_by_index=[0, k - 1])
+    assert np.abs(lm - w).max() <= 1e-10 * max(1.0, np.abs(w).max())
+    res = orc.residuals(vm, lm, dense=H)
+    assert res.max() <= 1e-10
+    assert api.last_solve_stats()["lowp_products"] > 0
 
 
 # ------------------------------------------------------------------ two-phase entry points
