@@ -3,6 +3,7 @@
     python -m block_ipt_amd.build
 """
 import os
+import signal
 import subprocess
 import sys
 
@@ -46,7 +47,9 @@ def build(verbose=False, jobs=8):
         cmd = [hipcc, "-x", "hip"] + HIP_FLAGS + ["-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd))
-        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        # own session, so a timed-out compile can be ended with its children
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            start_new_session=True)))
         if len(procs) >= jobs:
             _drain(procs)
     _drain(procs)
@@ -63,7 +66,14 @@ def build(verbose=False, jobs=8):
 def _drain(procs):
     failed = []
     for src, p in procs:
-        out, _ = p.communicate()
+        try:
+            # a runaway optimisation must fail the build, not hang it
+            out, _ = p.communicate(timeout=900)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            out, _ = p.communicate()
+            failed.append((src, "hipcc timed out after 900 s\n" + out.decode(errors="replace")))
+            continue
         text = out.decode(errors="replace")
         if p.returncode != 0:
             failed.append((src, text))
