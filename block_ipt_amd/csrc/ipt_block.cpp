@@ -29,6 +29,7 @@
 
 #include "ipt_block.h"
 #include "ipt_lowp.h"
+#include "ipt_small_eig.h"
 #include "ipt_prep.h"
 
 namespace ipt {
@@ -56,10 +57,14 @@ struct BlockParams {
     int verbose = 0;
 };
 
+// Projected problems up to this order are solved on the host.
+constexpr int kHostEigMax = 128;
+
 // Rayleigh-Ritz over V (n x cols) with HV = H V: generalized problem
-// (V^T H V) c = theta (V^T V) c (Cholesky-based, rocsolver dsygvd as the
-// reference's cusolverDnDsygvd); X = V C(:, :k), HX = HV C(:, :k), columns
-// of X normalised (HX scaled alike).  theta gets all `cols` values.
+// (V^T H V) c = theta (V^T V) c, Cholesky-based like the reference's
+// cusolverDnDsygvd (host Householder + QL for cols <= kHostEigMax,
+// rocsolver dsygvd above); X = V C(:, :k), HX = HV C(:, :k), columns of X
+// normalised (HX scaled alike).  theta (device) gets all `cols` values.
 int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int cols, int k, double *theta,
                   double *X, double *HX)
 {
@@ -70,20 +75,40 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
     int *info = reinterpret_cast<int *>(E + cols);
     IPT_STCHK(gemm_tn_tall(ctx, V, n, V, n, G, cols, n, cols, cols));
     IPT_STCHK(gemm_tn_tall(ctx, V, n, HV, n, P, cols, n, cols, cols));
-    void *blas = nullptr;
-    IPT_STCHK(context_blas(ctx, &blas));
-    rocblas_status rs = rocsolver_dsygvd((rocblas_handle)blas, rocblas_eform_ax, rocblas_evect_original,
-                                         rocblas_fill_lower, cols, P, cols, G, cols, theta, E, info);
-    if (rs != rocblas_status_success) {
-        fprintf(stderr, "IPT Rayleigh-Ritz: rocsolver_dsygvd status %d\n", (int)rs);
-        return IPT_CUDA_CUSOLVER_ERROR;
-    }
-    int hinfo = 0;
-    IPT_HIPCHK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (hinfo != 0) {
-        fprintf(stderr, "IPT Rayleigh-Ritz failed: info=%d\n", hinfo);
-        return IPT_CUDA_CUSOLVER_ERROR;
+    if (cols <= kHostEigMax) {
+        // small projected problem: solved on the host (one round trip of
+        // 2 cols^2 doubles) instead of a few hundred tiny solver launches
+        std::vector<double> hg((size_t)cols * cols), hp((size_t)cols * cols), hc((size_t)cols * cols),
+            ht((size_t)cols);
+        IPT_HIPCHK(hipMemcpyAsync(hg.data(), G, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int st = sym_generalized_eig(cols, hg.data(), hp.data(), ht.data(), hc.data());
+        if (st != 0) {
+            fprintf(stderr, "IPT Rayleigh-Ritz failed: %s\n",
+                    st == 1 ? "Gram matrix not positive definite" : "eigenvalue iteration did not converge");
+            return IPT_CUDA_CUSOLVER_ERROR;
+        }
+        IPT_HIPCHK(hipMemcpyAsync(P, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(theta, ht.data(), ht.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        // the host vectors must stay alive until the copies have run
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    } else {
+        void *blas = nullptr;
+        IPT_STCHK(context_blas(ctx, &blas));
+        rocblas_status rs = rocsolver_dsygvd((rocblas_handle)blas, rocblas_eform_ax, rocblas_evect_original,
+                                             rocblas_fill_lower, cols, P, cols, G, cols, theta, E, info);
+        if (rs != rocblas_status_success) {
+            fprintf(stderr, "IPT Rayleigh-Ritz: rocsolver_dsygvd status %d\n", (int)rs);
+            return IPT_CUDA_CUSOLVER_ERROR;
+        }
+        int hinfo = 0;
+        IPT_HIPCHK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (hinfo != 0) {
+            fprintf(stderr, "IPT Rayleigh-Ritz failed: info=%d\n", hinfo);
+            return IPT_CUDA_CUSOLVER_ERROR;
+        }
     }
     IPT_STCHK(gemm_nn_small(ctx, V, n, P, cols, X, n, n, cols, k, 1.0, 0.0));
     IPT_STCHK(gemm_nn_small(ctx, HV, n, P, cols, HX, n, n, cols, k, 1.0, 0.0));
