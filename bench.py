@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "bf16x3"])
     ap.add_argument("--anchor-every", type=int, default=20)
-    ap.add_argument("--block-tol", type=float, default=1e-9)
+    ap.add_argument("--block-tol", type=float, default=1e-8)
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed solves of the plain FP64 path (0: skip)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=2)

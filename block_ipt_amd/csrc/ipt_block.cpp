@@ -463,10 +463,18 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     }
     double *cur = Ba.p, *nxt = Bb.p;
     auto t0 = std::chrono::steady_clock::now();
+    // Exact steps are scheduled: at most anchor_every low-precision steps
+    // apart, earlier when the per-cluster convergence rate measured between
+    // the last two exact steps predicts that every active cluster reaches
+    // tol sooner (only exact steps may freeze a cluster, so steps after
+    // convergence would otherwise be wasted).  The last step is always
+    // exact, so the returned basis comes out of an exact step.
+    int next_exact = p.anchor_every;
+    int t_prev = -1;
+    std::vector<double> fp_prev((size_t)plan.nclusters, NAN), fp_now((size_t)plan.nclusters);
+    std::vector<int> act((size_t)plan.nclusters);
     for (int it = 0; it < p.maxiter; ++it) {
-        // every (anchor_every+1)-th step is exact, and so is the last one, so
-        // that the returned basis always comes out of an exact step
-        const bool exact = !mixed || (it % (p.anchor_every + 1) == p.anchor_every) || it == p.maxiter - 1;
+        const bool exact = !mixed || it == next_exact || it == p.maxiter - 1;
         if (exact) {
             IPT_STCHK(apply(cur, Y.p, bc));
             out->fp64_products += 1;
@@ -488,7 +496,39 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             continue;
         }
         int flags[3];
+        if (mixed) {
+            IPT_HIPCHK(hipMemcpyAsync(act.data(), plan.active.p, act.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+            IPT_HIPCHK(hipMemcpyAsync(fp_now.data(), plan.fp.p, fp_now.size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        }
         IPT_STCHK(read_host(ctx, flags, plan.flags.p, 3 * sizeof(int)));
+        if (mixed) {
+            int steps = p.anchor_every + 1;
+            if (t_prev >= 0) {
+                double need = 0.0;
+                bool known = true;
+                for (int c = 0; c < plan.nclusters; ++c) {
+                    if (!act[(size_t)c]) {
+                        continue;
+                    }
+                    const double f1 = fp_now[(size_t)c], f0 = fp_prev[(size_t)c];
+                    if (!(f1 > 0.0) || !(f0 > f1) || !std::isfinite(f0)) {
+                        known = false;
+                        break;
+                    }
+                    const double rate = std::log(f1 / f0) / (double)(it - t_prev);  // < 0
+                    need = std::max(need, std::log(p.tol / f1) / rate);
+                }
+                if (known) {
+                    // 10% margin: a late exact step costs less than an extra one
+                    steps = std::min(steps, std::max(2, (int)std::ceil(1.1 * need) + 1));
+                }
+            }
+            fp_prev = fp_now;
+            t_prev = it;
+            next_exact = it + steps;
+        }
         if (flags[1] != 0) {
             fprintf(stderr,
                     "IPT block-cluster: block solve failed (%s) at iteration %d; identity fallback is disabled\n",
