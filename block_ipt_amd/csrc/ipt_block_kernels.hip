@@ -39,20 +39,20 @@ static inline int lane_class(int m)
 
 // ------------------------------------------------------------------ block update
 
+// Rows [lblk * 256/P, (lblk + 1) * 256/P) of cluster cid.
 template <int P>
-__global__ __launch_bounds__(256) void k_block_update(
-    const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
-    const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
-    const int *__restrict__ class_list, const int *__restrict__ active, double damping,
-    double *__restrict__ partials, int pstride, int *__restrict__ flags)
+__device__ __forceinline__ void block_update_rows(
+    int cid, int lblk, const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn,
+    long ld, const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
+    const int *__restrict__ active, double damping, double *__restrict__ partials, int pstride,
+    int *__restrict__ flags)
 {
     constexpr int GROUPS = 256 / P;
-    const int cid = class_list[blockIdx.y];
     const BlockCluster cl = clusters[cid];
     const int m = cl.m;
     const int t = threadIdx.x % P;
     const int grp = threadIdx.x / P;
-    const long row = (long)blockIdx.x * GROUPS + grp;
+    const long row = (long)lblk * GROUPS + grp;
     const bool live = active[cid] != 0;
     double sdn = 0.0, sbn = 0.0;
 
@@ -156,9 +156,50 @@ __global__ __launch_bounds__(256) void k_block_update(
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const long slot = (long)cid * pstride + blockIdx.x;
+        const long slot = (long)cid * pstride + lblk;
         partials[2 * slot] = sa[0] + sa[1] + sa[2] + sa[3];
         partials[2 * slot + 1] = sb[0] + sb[1] + sb[2] + sb[3];
+    }
+}
+
+// One class: blockIdx.y = cluster of the class list, blockIdx.x = row block.
+template <int P>
+__global__ __launch_bounds__(256) void k_block_update(
+    const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
+    const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
+    const int *__restrict__ class_list, const int *__restrict__ active, double damping,
+    double *__restrict__ partials, int pstride, int *__restrict__ flags)
+{
+    block_update_rows<P>(class_list[blockIdx.y], blockIdx.x, B, Y, Bn, ld, d, n, clusters, active, damping,
+                         partials, pstride, flags);
+}
+
+// All clusters of widths up to 32 in one launch (their row blocks run
+// concurrently instead of one class after another): block b updates row
+// block blk_local[b] of cluster blk_cluster[b].
+__global__ __launch_bounds__(256) void k_block_update_fused(
+    const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
+    const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
+    const int *__restrict__ blk_cluster, const int *__restrict__ blk_local, const int *__restrict__ active,
+    double damping, double *__restrict__ partials, int pstride, int *__restrict__ flags)
+{
+    const int cid = blk_cluster[blockIdx.x];
+    const int lblk = blk_local[blockIdx.x];
+    const int m = clusters[cid].m;
+    if (m <= 1) {
+        block_update_rows<1>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
+    } else if (m <= 2) {
+        block_update_rows<2>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
+    } else if (m <= 4) {
+        block_update_rows<4>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
+    } else if (m <= 8) {
+        block_update_rows<8>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
+    } else if (m <= 16) {
+        block_update_rows<16>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride,
+                              flags);
+    } else {
+        block_update_rows<32>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride,
+                              flags);
     }
 }
 
@@ -294,6 +335,28 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
     IPT_STCHK(plan->iters.alloc((size_t)nc));
     IPT_STCHK(plan->fp.alloc((size_t)nc));
     IPT_STCHK(plan->flags.alloc(4));
+    // fused launch table: row blocks of every cluster of width <= 32, widest
+    // clusters first (their blocks take longest)
+    {
+        std::vector<int> bc, bl;
+        for (int cls = 5; cls >= 0; --cls) {
+            for (int c : lists[(size_t)cls]) {
+                for (int b = 0; b < plan->host[(size_t)c].nblocks; ++b) {
+                    bc.push_back(c);
+                    bl.push_back(b);
+                }
+            }
+        }
+        plan->fused_blocks = (int)bc.size();
+        IPT_STCHK(plan->blk_cluster.alloc(std::max<size_t>(bc.size(), 1)));
+        IPT_STCHK(plan->blk_local.alloc(std::max<size_t>(bl.size(), 1)));
+        if (!bc.empty()) {
+            IPT_HIPCHK(hipMemcpyAsync(plan->blk_cluster.p, bc.data(), bc.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, ctx->stream));
+            IPT_HIPCHK(hipMemcpyAsync(plan->blk_local.p, bl.data(), bl.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
     if (plan->nclusters > 0) {
         IPT_HIPCHK(hipMemcpyAsync(plan->clusters.p, plan->host.data(), plan->host.size() * sizeof(BlockCluster),
                                   hipMemcpyHostToDevice, ctx->stream));
@@ -328,28 +391,20 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
                               (size_t)std::max(plan.nclusters, 1) * plan.partial_stride * 2 * sizeof(double),
                               (void **)&partials));
     const ProfileToken prof = profile_begin(ctx, PROF_BLOCK_UPDATE);
-    for (int cls = 0; cls < 7; ++cls) {
-        const int cnt = plan.class_count[cls];
-        if (cnt == 0) {
-            continue;
-        }
-        const int rows = 256 >> cls;
-        dim3 grid((n + rows - 1) / rows, cnt);
-        const int *list = plan.class_lists.p + plan.class_offset[cls];
-        const int ps = plan.partial_stride;
-#define IPT_BLK(PP)                                                                                           \
-    hipLaunchKernelGGL(k_block_update<PP>, grid, dim3(256), 0, ctx->stream, B, Y, Bn, ld, d, n,              \
-                       plan.clusters.p, list, plan.active.p, damping, partials, ps, plan.flags.p)
-        switch (cls) {
-        case 0: IPT_BLK(1); break;
-        case 1: IPT_BLK(2); break;
-        case 2: IPT_BLK(4); break;
-        case 3: IPT_BLK(8); break;
-        case 4: IPT_BLK(16); break;
-        case 5: IPT_BLK(32); break;
-        default: IPT_BLK(64); break;
-        }
-#undef IPT_BLK
+    if (plan.fused_blocks > 0) {
+        hipLaunchKernelGGL(k_block_update_fused, dim3(plan.fused_blocks), dim3(256), 0, ctx->stream, B, Y, Bn, ld,
+                           d, n, plan.clusters.p, plan.blk_cluster.p, plan.blk_local.p, plan.active.p, damping,
+                           partials, plan.partial_stride, plan.flags.p);
+        IPT_HIPCHK(hipGetLastError());
+    }
+    if (plan.class_count[6] > 0) {
+        // widths 33..64 keep their own launch (their registers would lower
+        // the occupancy of every fused block)
+        const int rows = 256 >> 6;
+        dim3 grid((n + rows - 1) / rows, plan.class_count[6]);
+        hipLaunchKernelGGL(k_block_update<64>, grid, dim3(256), 0, ctx->stream, B, Y, Bn, ld, d, n,
+                           plan.clusters.p, plan.class_lists.p + plan.class_offset[6], plan.active.p, damping,
+                           partials, plan.partial_stride, plan.flags.p);
         IPT_HIPCHK(hipGetLastError());
     }
     profile_end(ctx, prof);

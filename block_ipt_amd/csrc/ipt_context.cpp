@@ -4,11 +4,138 @@
 
 #include <climits>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 
 #include "ipt_internal.h"
 
 namespace ipt {
+
+// ------------------------------------------------------------------ allocation cache
+
+namespace {
+std::mutex g_pool_mutex;
+thread_local hipStream_t t_pool_stream = nullptr;
+struct PoolBlock {
+    hipStream_t stream;
+    size_t bytes;
+};
+std::multimap<std::pair<hipStream_t, size_t>, void *> g_pool_free;
+std::unordered_map<void *, PoolBlock> g_pool_live;
+size_t g_pool_cached = 0;
+constexpr size_t kPoolCap = (size_t)32 << 30;  // cached bytes kept at most
+
+size_t pool_round(size_t b)
+{
+    const size_t g = b <= ((size_t)1 << 20) ? 256 : ((size_t)2 << 20);
+    return (b + g - 1) / g * g;
+}
+
+void pool_trim_all()
+{
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        for (auto &e : g_pool_free) {
+            drop.push_back(e.second);
+        }
+        g_pool_free.clear();
+        g_pool_cached = 0;
+    }
+    if (!drop.empty()) {
+        (void)hipDeviceSynchronize();
+        for (void *q : drop) {
+            (void)hipFree(q);
+        }
+    }
+}
+}  // namespace
+
+void pool_set_stream(hipStream_t s)
+{
+    t_pool_stream = s;
+}
+
+int pool_alloc(void **p, size_t bytes)
+{
+    const size_t rb = pool_round(bytes);
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        auto it = g_pool_free.find({t_pool_stream, rb});
+        if (it != g_pool_free.end()) {
+            *p = it->second;
+            g_pool_free.erase(it);
+            g_pool_cached -= rb;
+            g_pool_live[*p] = {t_pool_stream, rb};
+            return IPT_CUDA_SUCCESS;
+        }
+    }
+    void *q = nullptr;
+    if (hipMalloc(&q, rb) != hipSuccess) {
+        (void)hipGetLastError();
+        pool_trim_all();  // cached blocks of other sizes may be what is missing
+        if (hipMalloc(&q, rb) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            return IPT_CUDA_ALLOCATION_FAILED;
+        }
+    }
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    g_pool_live[q] = {t_pool_stream, rb};
+    *p = q;
+    return IPT_CUDA_SUCCESS;
+}
+
+void pool_free(void *p)
+{
+    if (p == nullptr) {
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        auto it = g_pool_live.find(p);
+        if (it != g_pool_live.end()) {
+            const PoolBlock b = it->second;
+            g_pool_live.erase(it);
+            if (g_pool_cached + b.bytes <= kPoolCap) {
+                g_pool_free.insert({{b.stream, b.bytes}, p});
+                g_pool_cached += b.bytes;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);  // not cached (foreign, detached, or over the cap)
+}
+
+void pool_detach(void *p)
+{
+    if (p == nullptr) {
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_pool_mutex);
+    g_pool_live.erase(p);
+}
+
+void pool_release_stream(hipStream_t s)
+{
+    std::vector<void *> drop;
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        for (auto it = g_pool_free.begin(); it != g_pool_free.end();) {
+            if (it->first.first == s) {
+                drop.push_back(it->second);
+                g_pool_cached -= it->first.second;
+                it = g_pool_free.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    for (void *q : drop) {
+        (void)hipFree(q);
+    }
+}
 
 namespace {
 std::mutex g_ctx_mutex;
@@ -37,6 +164,7 @@ void context_destroy(Context *ctx)
     }
     if (ctx->stream != nullptr) {
         (void)hipStreamSynchronize(ctx->stream);
+        pool_release_stream(ctx->stream);
     }
     for (int i = 0; i < 8; ++i) {
         if (ctx->slots[i] != nullptr) {
@@ -63,7 +191,9 @@ Context *context_from_handle(void *handle, int *status)
 {
     *status = IPT_CUDA_SUCCESS;
     if (handle != nullptr) {
-        return static_cast<Context *>(handle);
+        Context *c = static_cast<Context *>(handle);
+        pool_set_stream(c->stream);
+        return c;
     }
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
@@ -81,6 +211,7 @@ Context *context_from_handle(void *handle, int *status)
         }
         g_default_ctx[dev] = ctx;
     }
+    pool_set_stream(g_default_ctx[dev]->stream);
     return g_default_ctx[dev];
 }
 

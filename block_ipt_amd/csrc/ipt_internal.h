@@ -71,6 +71,18 @@ Context *context_from_handle(void *handle, int *status);
 int context_blas(Context *ctx, void **blas_out);
 
 // Device buffer owned by the caller's scope.
+// Device memory of library routines comes from a cache keyed by the
+// library stream it is used on (set by context_from_handle for the calling
+// thread): a block released by one call is handed to a later call on the
+// same stream, whose work is ordered after every use of the block, so
+// repeated solves make no hipMalloc/hipFree calls (hipFree synchronises the
+// device).  Blocks handed to callers are detached and owned by them.
+int pool_alloc(void **p, size_t bytes);
+void pool_free(void *p);
+void pool_detach(void *p);
+void pool_set_stream(hipStream_t s);
+void pool_release_stream(hipStream_t s);
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -85,26 +97,28 @@ struct DevBuf {
         if (n == 0) {
             return IPT_CUDA_SUCCESS;
         }
-        if (hipMalloc((void **)&p, n * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
+        void *q = nullptr;
+        if (pool_alloc(&q, n * sizeof(T)) != IPT_CUDA_SUCCESS) {
             fprintf(stderr, "IPT: device allocation of %zu bytes failed\n", n * sizeof(T));
             return IPT_CUDA_ALLOCATION_FAILED;
         }
+        p = static_cast<T *>(q);
         count = n;
         return IPT_CUDA_SUCCESS;
     }
     void release()
     {
         if (p != nullptr) {
-            (void)hipFree(p);
+            pool_free(p);
         }
         p = nullptr;
         count = 0;
     }
+    // hand the allocation to the caller (it is then released with hipFree)
     T *take()
     {
         T *q = p;
+        pool_detach(q);
         p = nullptr;
         count = 0;
         return q;
