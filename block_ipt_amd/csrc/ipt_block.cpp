@@ -709,7 +709,11 @@ struct WindowCoupling : CouplingSource {
     void offdiag_strengths(std::vector<double> *s) const override { *s = *off; }
 };
 
-int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr)
+// `early` (may be empty) is called as soon as the working matrix and its
+// largest off-diagonal magnitude are known, before the host-side cluster
+// detection, so device work it enqueues overlaps that host work.
+int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
+                  const std::function<int(const DensePrep &)> &early)
 {
     IPT_STCHK(pr->d.alloc((size_t)n));
     DevBuf<double> colsum, colmax;
@@ -721,10 +725,10 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr)
     IPT_STCHK(dense_scan(ctx, dH, n, n, colsum.p, colmax.p, asym.p));
     std::vector<double> hd((size_t)n), hsum((size_t)n), hmax((size_t)n);
     int hasym = 0;
-    IPT_STCHK(read_host(ctx, hd.data(), pr->d.p, (size_t)n * sizeof(double)));
-    IPT_STCHK(read_host(ctx, hsum.data(), colsum.p, (size_t)n * sizeof(double)));
-    IPT_STCHK(read_host(ctx, hmax.data(), colmax.p, (size_t)n * sizeof(double)));
-    IPT_STCHK(read_host(ctx, &hasym, asym.p, sizeof(int)));
+    IPT_HIPCHK(hipMemcpyAsync(hd.data(), pr->d.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipMemcpyAsync(hsum.data(), colsum.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipMemcpyAsync(hmax.data(), colmax.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_STCHK(read_host(ctx, &hasym, asym.p, sizeof(int)));  // one synchronisation for all four
     if (hasym) {
         fprintf(stderr, "IPT block-cluster path currently requires a real symmetric matrix\n");
         return IPT_CUDA_INVALID_ARGUMENT;
@@ -764,6 +768,9 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr)
     pr->max_offdiag = 0.0;
     for (double x : soff) {
         pr->max_offdiag = std::max(pr->max_offdiag, x);
+    }
+    if (early) {
+        IPT_STCHK(early(*pr));
     }
     // candidate (row, target) pairs of the detection windows
     const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
@@ -833,10 +840,20 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
                        double **d_vec_out, double **d_val_out, IPTCudaResult *info)
 {
     auto t0 = std::chrono::steady_clock::now();
-    DensePrep pr;
-    IPT_STCHK(dense_prepare(ctx, dH, n, k, &pr));
-    const double prep = seconds_since(t0);
     const BlockParams p = params_from_env(maxiter, tol, use_tol);
+    DensePrep pr;
+    LowpOperator lowp;
+    const LowpOperator *lp = nullptr;
+    // the low-precision copy of E is packed while the host detects clusters
+    auto pack = [&](const DensePrep &q) -> int {
+        if (p.lowp_mode >= 0) {
+            IPT_STCHK(lowp_operator_create(ctx, q.H, n, n, p.lowp_mode, q.max_offdiag, &lowp));
+            lp = &lowp;
+        }
+        return IPT_CUDA_SUCCESS;
+    };
+    IPT_STCHK(dense_prepare(ctx, dH, n, k, &pr, pack));
+    const double prep = seconds_since(t0);
     const double *H = pr.H;
     ApplyFn apply = [&](const double *X, double *Y, int nc) { return gemm_f64(ctx, H, n, X, n, Y, n, n, nc); };
     // H * (identity block of cluster [first,last]) = columns first..last of H
@@ -851,12 +868,6 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
         }
         return IPT_CUDA_SUCCESS;
     };
-    LowpOperator lowp;
-    const LowpOperator *lp = nullptr;
-    if (p.lowp_mode >= 0) {
-        IPT_STCHK(lowp_operator_create(ctx, H, n, n, p.lowp_mode, pr.max_offdiag, &lowp));
-        lp = &lowp;
-    }
     BlockResult br;
     IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp, pr.d.p, n, k, pr.ranges, pr.normH, p, &br));
     ctx->last_basis_cols = br.basis_cols;
