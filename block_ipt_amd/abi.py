@@ -196,6 +196,11 @@ SIGNATURES = {
     "ipt_hip_profile_get": (_c.c_int, [_c.c_char_p, _dp, _c.POINTER(_c.c_longlong)]),
     "ipt_hip_last_solve_stats": (_c.c_int, [_vp, _ip, _ip, _c.POINTER(_c.c_longlong),
                                             _c.POINTER(_c.c_longlong), _ip]),
+    "ipt_hip_last_davidson_stats": (_c.c_int, [_vp, _ip, _ip, _ip, _ip, _c.POINTER(_c.c_longlong), _dp]),
+    "ipt_hip_mlp_features": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int, _vp, _vp, _vp]),
+    "ipt_hip_mlp_infer": (_c.c_int, [_vp, _vp, _vp, _c.c_longlong, _vp, _vp]),
+    "ipt_hip_mlp_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_longlong, _vp, _vp, _vp]),
+    "ipt_hip_mlp_adam_step": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_float, _c.c_float, _c.c_int, _vp]),
     # include/ipt_hip_mixed.h
     "ipt_mixed_precision_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
                                                        _c.c_double, _c.c_int, _vp,

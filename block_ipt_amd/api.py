@@ -318,3 +318,70 @@ def last_solve_stats(handle=None):
         raise IptError(st, "ipt_hip_last_solve_stats")
     return {"basis_cols": b.value, "steps": s_.value, "fp64_products": f64.value, "lowp_products": lp.value,
             "davidson_steps": dav.value}
+
+
+def last_davidson_stats(handle=None):
+    acc, rej, ma, mr = (ctypes.c_int(0) for _ in range(4))
+    mv, res = ctypes.c_longlong(0), ctypes.c_double(0.0)
+    st = lib().ipt_hip_last_davidson_stats(handle, ctypes.byref(acc), ctypes.byref(rej), ctypes.byref(ma),
+                                           ctypes.byref(mr), ctypes.byref(mv), ctypes.byref(res))
+    if st != 0:
+        raise IptError(st, "ipt_hip_last_davidson_stats")
+    return {"accepted": acc.value, "rejected": rej.value, "mlp_accepted": ma.value, "mlp_rejected": mr.value,
+            "matvecs": mv.value, "residual_after": res.value}
+
+
+# ------------------------------------------------------------------ learned preconditioner
+
+MLP_NPARAM = 4481
+
+
+def _vpp(p):
+    return ctypes.c_void_p(p) if p else None
+
+
+def mlp_infer(d_params, d_feat, d_qf, count, d_out, handle=None):
+    st = lib().ipt_hip_mlp_infer(_vpp(d_params), _vpp(d_feat), _vpp(d_qf), int(count), _vpp(d_out), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_mlp_infer")
+
+
+def mlp_gradient(d_params, d_feat, d_q, d_t, d_w, count, d_grad, d_loss, handle=None):
+    st = lib().ipt_hip_mlp_gradient(_vpp(d_params), _vpp(d_feat), _vpp(d_q), _vpp(d_t), _vpp(d_w), int(count),
+                                    _vpp(d_grad), _vpp(d_loss), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_mlp_gradient")
+
+
+def mlp_adam_step(d_params, d_m, d_v, d_grad, gscale, lr, step, handle=None):
+    st = lib().ipt_hip_mlp_adam_step(_vpp(d_params), _vpp(d_m), _vpp(d_v), _vpp(d_grad), float(gscale), float(lr),
+                                     int(step), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_mlp_adam_step")
+
+
+def mlp_features(d_R, ldr, d_diag, d_rowabs, d_theta, d_idx, n, nt, d_feat, d_qf, handle=None):
+    st = lib().ipt_hip_mlp_features(_vpp(d_R), int(ldr), _vpp(d_diag), _vpp(d_rowabs), _vpp(d_theta), _vpp(d_idx),
+                                    int(n), int(nt), _vpp(d_feat), _vpp(d_qf), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_mlp_features")
+
+
+def save_mlp_model(path, params):
+    """Model file read by the library (IPT_DAVIDSON_MLP_MODEL): the 8-byte
+    tag IPTMLP01, the parameter count (int32), then FP32 parameters."""
+    p = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+    if p.size != MLP_NPARAM:
+        raise ValueError("expected %d parameters" % MLP_NPARAM)
+    with open(path, "wb") as f:
+        f.write(b"IPTMLP01")
+        np.array([MLP_NPARAM], dtype=np.int32).tofile(f)
+        p.tofile(f)
+
+
+def load_mlp_model(path):
+    with open(path, "rb") as f:
+        if f.read(8) != b"IPTMLP01":
+            raise ValueError("not an IPT MLP model file")
+        cnt = int(np.fromfile(f, dtype=np.int32, count=1)[0])
+        return np.fromfile(f, dtype=np.float32, count=cnt)

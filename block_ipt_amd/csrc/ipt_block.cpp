@@ -34,6 +34,10 @@
 
 namespace ipt {
 
+// learned preconditioner (ipt_mlp.hip)
+int mlp_correction(Context *ctx, const float *params, const double *R, long ldr, const double *d,
+                   const double *rowabs, const double *theta, const int *idx, int n, int nt, double *T, long ldt);
+
 namespace {
 
 double seconds_since(const std::chrono::steady_clock::time_point &t0)
@@ -54,8 +58,35 @@ struct BlockParams {
     double conv_tol = 1e-13;
     int lowp_mode = -1;     // -1: every product in FP64
     int anchor_every = 20;  // low-precision steps between exact FP64 steps
+    int precond_mlp = 0;    // Davidson corrections from the learned MLP
+    std::string mlp_model;  // its parameter file
     int verbose = 0;
 };
+
+constexpr int kMlpParams = 4481;
+
+// Parameter file: tag "IPTMLP01", int32 count, FP32 parameters.
+int load_mlp_model(const std::string &path, DevBuf<float> *out)
+{
+    FILE *f = fopen(path.c_str(), "rb");
+    if (f == nullptr) {
+        fprintf(stderr, "IPT: cannot open MLP model %s\n", path.c_str());
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    char tag[8];
+    int cnt = 0;
+    std::vector<float> h((size_t)kMlpParams);
+    const bool ok = fread(tag, 1, 8, f) == 8 && memcmp(tag, "IPTMLP01", 8) == 0 && fread(&cnt, sizeof(int), 1, f) == 1 &&
+                    cnt == kMlpParams && fread(h.data(), sizeof(float), h.size(), f) == h.size();
+    fclose(f);
+    if (!ok) {
+        fprintf(stderr, "IPT: %s is not an MLP model file with %d parameters\n", path.c_str(), kMlpParams);
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    IPT_STCHK(out->alloc(h.size()));
+    IPT_HIPCHK(hipMemcpy(out->p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    return IPT_CUDA_SUCCESS;
+}
 
 // Projected problems up to this order are solved on the host.
 constexpr int kHostEigMax = 128;
@@ -232,17 +263,22 @@ struct DavidsonStats {
     double clip = NAN;
     long long matvecs = 0;
     int basis_cols = 0;
+    int mlp_accepted = 0;  // learned correction steps kept
+    int mlp_rejected = 0;  // learned steps redone with the diagonal correction
 };
 
 double max_of(const std::vector<double> &v, int *idx)
 {
     double m = NAN;
-    *idx = -1;
+    int at = -1;
     for (size_t i = 0; i < v.size(); ++i) {
         if (!std::isfinite(m) || v[i] > m) {
             m = v[i];
-            *idx = (int)i;
+            at = (int)i;
         }
+    }
+    if (idx != nullptr) {
+        *idx = at;
     }
     return m;
 }
@@ -250,10 +286,13 @@ double max_of(const std::vector<double> &v, int *idx)
 // Residual-only Davidson refinement, IPT/src/ipt_cuda.cu:3318-3982 (HEAD):
 // corrections for every pair above conv_tol, thick restart to the k Ritz
 // vectors plus the last correction block, trial accepted when finite.
+// With `mlp` (device parameters; rowabs = off-diagonal row sums of |H|)
+// the corrections come from the learned preconditioner, and a trial step
+// that does not lower the largest residual is redone with the diagonal one.
 // V/HV: n x cap buffers holding the search space (first *cols columns).
-int davidson(Context *ctx, const ApplyFn &apply, const double *d, int n, int k, double normH, double *V,
-             double *HV, int *cols_io, int cap, double *theta, double *X, double *HX, const BlockParams &p,
-             DavidsonStats *st)
+int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *rowabs, const float *mlp, int n, int k,
+             double normH, double *V, double *HV, int *cols_io, int cap, double *theta, double *X, double *HX,
+             const BlockParams &p, DavidsonStats *st)
 {
     const int steps = std::max(1, std::max(0, p.davidson_steps) + std::max(0, p.davidson_extra));
     int kmax = std::min(n, k + std::max(16, (2 * k + 2) / 3));
@@ -348,27 +387,56 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, int n, int k, 
         }
         const int nt = (int)cand.size();
         IPT_HIPCHK(hipMemcpyAsync(idx.p, cand.data(), (size_t)nt * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        IPT_STCHK(diag_correction(ctx, R.p, n, d, theta, idx.p, nt, st->clip, T.p, n, n));
+        // one trial step: corrections (learned or diagonal), projection,
+        // products, Rayleigh-Ritz on the enlarged basis, trial residuals
+        std::vector<double> tres((size_t)k);
         int kept = 0;
-        IPT_STCHK(project_orthonormalize(ctx, V, n, cols, T.p, nt, &kept, work.p));
+        auto trial = [&](bool learned) -> int {
+            if (learned) {
+                IPT_STCHK(mlp_correction(ctx, mlp, R.p, n, d, rowabs, theta, idx.p, n, nt, T.p, n));
+            } else {
+                IPT_STCHK(diag_correction(ctx, R.p, n, d, theta, idx.p, nt, st->clip, T.p, n, n));
+            }
+            kept = 0;
+            IPT_STCHK(project_orthonormalize(ctx, V, n, cols, T.p, nt, &kept, work.p));
+            if (kept == 0) {
+                return IPT_CUDA_SUCCESS;
+            }
+            IPT_STCHK(apply(T.p, HT.p, kept));
+            st->matvecs += kept;
+            IPT_HIPCHK(hipMemcpyAsync(V + (size_t)cols * n, T.p, (size_t)n * kept * sizeof(double),
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+            IPT_HIPCHK(hipMemcpyAsync(HV + (size_t)cols * n, HT.p, (size_t)n * kept * sizeof(double),
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+            IPT_STCHK(rayleigh_ritz(ctx, V, HV, n, cols + kept, k, ttheta.p, tX.p, tHX.p));
+            // the current residual block R is still needed if the trial is
+            // redone, so the trial residual vectors go to `work` first
+            IPT_STCHK(residual_norms(ctx, tX.p, tHX.p, n, n, k, ttheta.p, normH, res.p, work.p));
+            IPT_STCHK(read_host(ctx, tres.data(), res.p, (size_t)k * sizeof(double)));
+            return IPT_CUDA_SUCCESS;
+        };
+        const double before = max_of(hres, nullptr);
+        bool learned = mlp != nullptr;
+        IPT_STCHK(trial(learned));
+        if (learned) {
+            // the diagonal preconditioner is the judge: a learned step that
+            // does not lower the largest residual is redone with it
+            const double after = kept > 0 ? max_of(tres, nullptr) : INFINITY;
+            if (!(after < before)) {
+                ++st->mlp_rejected;
+                learned = false;
+                IPT_STCHK(trial(false));
+            } else {
+                ++st->mlp_accepted;
+            }
+        }
         if (kept == 0) {
             ++st->rejected_steps;
             break;
         }
-        IPT_STCHK(apply(T.p, HT.p, kept));
-        st->matvecs += kept;
-        IPT_HIPCHK(hipMemcpyAsync(V + (size_t)cols * n, T.p, (size_t)n * kept * sizeof(double),
-                                  hipMemcpyDeviceToDevice, ctx->stream));
-        IPT_HIPCHK(hipMemcpyAsync(HV + (size_t)cols * n, HT.p, (size_t)n * kept * sizeof(double),
-                                  hipMemcpyDeviceToDevice, ctx->stream));
         const int tcols = cols + kept;
-        int rst = rayleigh_ritz(ctx, V, HV, n, tcols, k, ttheta.p, tX.p, tHX.p);
-        if (rst != IPT_CUDA_SUCCESS) {
-            return rst;
-        }
-        std::vector<double> tres((size_t)k);
-        IPT_STCHK(residual_norms(ctx, tX.p, tHX.p, n, n, k, ttheta.p, normH, res.p, R.p));
-        IPT_STCHK(read_host(ctx, tres.data(), res.p, (size_t)k * sizeof(double)));
+        IPT_HIPCHK(hipMemcpyAsync(R.p, work.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
         bool finite = true;
         for (double r : tres) {
             if (!std::isfinite(r)) {
@@ -421,6 +489,20 @@ struct BlockResult {
     DavidsonStats dav;
 };
 
+void record_stats(Context *ctx, const BlockResult &br)
+{
+    ctx->last_basis_cols = br.basis_cols;
+    ctx->last_steps = br.iterations;
+    ctx->last_fp64_products = br.fp64_products;
+    ctx->last_lowp_products = br.lowp_products;
+    ctx->last_davidson_steps = br.dav.accepted_steps;
+    ctx->last_davidson_rejected = br.dav.rejected_steps;
+    ctx->last_mlp_accepted = br.dav.mlp_accepted;
+    ctx->last_mlp_rejected = br.dav.mlp_rejected;
+    ctx->last_davidson_matvecs = br.dav.matvecs;
+    ctx->last_davidson_residual = br.dav.residual_after;
+}
+
 // Block-cluster IPT on a prepared (sorted-diagonal) operator.  With a
 // low-precision operator the iteration alternates `anchor_every` steps whose
 // product is the anchored correction (see ipt_gemm_lp.hip) with one exact
@@ -428,8 +510,9 @@ struct BlockResult {
 // is the FP64 one.  `identity_product` (optional) fills H * identity blocks
 // without a product (dense: gathered columns of H).
 int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_product, const LowpOperator *lowp,
-                   const double *d, int n, int k, const std::vector<std::pair<int, int>> &ranges, double normH,
-                   const BlockParams &p, BlockResult *out)
+                   const double *d, const double *rowabs, int n, int k,
+                   const std::vector<std::pair<int, int>> &ranges, double normH, const BlockParams &p,
+                   BlockResult *out)
 {
     BlockPlan plan;
     IPT_STCHK(block_plan_create(ctx, ranges, n, &plan));
@@ -572,8 +655,12 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     if (p.davidson) {
         auto t2 = std::chrono::steady_clock::now();
         int cols = bc;
-        IPT_STCHK(davidson(ctx, apply, d, n, k, normH, Ba.p, Y.p, &cols, cap, out->theta.p, out->X.p, out->HX.p, p,
-                           &out->dav));
+        DevBuf<float> mlp;
+        if (p.precond_mlp && rowabs != nullptr) {
+            IPT_STCHK(load_mlp_model(p.mlp_model, &mlp));
+        }
+        IPT_STCHK(davidson(ctx, apply, d, rowabs, mlp.p, n, k, normH, Ba.p, Y.p, &cols, cap, out->theta.p, out->X.p,
+                           out->HX.p, p, &out->dav));
         out->matvecs += out->dav.matvecs;
         out->basis_cols = cols;
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -618,6 +705,17 @@ BlockParams params_from_env(int maxiter, double tol, int use_tol)
         p.anchor_every = 20;
     }
     p.verbose = env_flag("IPT_HIP_VERBOSE");
+    // addition: IPT_DAVIDSON_PRECOND=mlp (model file IPT_DAVIDSON_MLP_MODEL)
+    const char *pc = getenv("IPT_DAVIDSON_PRECOND");
+    if (pc != nullptr && !strcmp(pc, "mlp")) {
+        const char *mf = getenv("IPT_DAVIDSON_MLP_MODEL");
+        if (mf != nullptr && mf[0] != '\0') {
+            p.precond_mlp = 1;
+            p.mlp_model = mf;
+        } else {
+            fprintf(stderr, "IPT: IPT_DAVIDSON_PRECOND=mlp without IPT_DAVIDSON_MLP_MODEL; diagonal corrections used\n");
+        }
+    }
     return p;
 }
 
@@ -682,6 +780,7 @@ struct DensePrep {
     DevBuf<double> Hs;                  // permuted copy when needed
     const double *H = nullptr;          // matrix used by the solver
     DevBuf<double> d;                   // sorted diagonal (device)
+    DevBuf<double> rowabs;              // sorted off-diagonal row sums of |H|
     DevBuf<int> dperm;
     std::vector<double> hd;             // sorted diagonal (host)
     double normH = 0.0;
@@ -750,11 +849,16 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
     IPT_STCHK(pr->dperm.alloc((size_t)n));
     IPT_HIPCHK(hipMemcpyAsync(pr->dperm.p, pr->perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
                               ctx->stream));
-    std::vector<double> sd((size_t)n), soff((size_t)n);
+    std::vector<double> sd((size_t)n), soff((size_t)n), sabs((size_t)n);
     for (int s = 0; s < n; ++s) {
         sd[(size_t)s] = hd[(size_t)pr->perm[(size_t)s]];
         soff[(size_t)s] = hmax[(size_t)pr->perm[(size_t)s]];
+        // column sums of |H| (= row sums, symmetric) minus the diagonal
+        sabs[(size_t)s] = std::max(0.0, hsum[(size_t)pr->perm[(size_t)s]] - std::fabs(sd[(size_t)s]));
     }
+    IPT_STCHK(pr->rowabs.alloc((size_t)n));
+    IPT_HIPCHK(hipMemcpyAsync(pr->rowabs.p, sabs.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
     if (pr->permuted) {
         IPT_STCHK(pr->Hs.alloc((size_t)n * n));
         IPT_STCHK(dense_permute(ctx, dH, n, pr->Hs.p, n, n, pr->dperm.p));
@@ -869,12 +973,9 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
         return IPT_CUDA_SUCCESS;
     };
     BlockResult br;
-    IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp, pr.d.p, n, k, pr.ranges, pr.normH, p, &br));
-    ctx->last_basis_cols = br.basis_cols;
-    ctx->last_steps = br.iterations;
-    ctx->last_fp64_products = br.fp64_products;
-    ctx->last_lowp_products = br.lowp_products;
-    ctx->last_davidson_steps = br.dav.accepted_steps;
+    IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp, pr.d.p, pr.rowabs.p, n, k, pr.ranges, pr.normH, p,
+                             &br));
+    record_stats(ctx, br);
     if (p.verbose) {
         fprintf(stderr,
                 "IPT block-cluster dense: n=%d k=%d clusters=%zu basis_cols=%d precision=%d steps=%d "
@@ -995,13 +1096,33 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         IPT_STCHK(vv.alloc((size_t)std::max(pn, 1)));
         IPT_STCHK(dd.alloc((size_t)n));
         IPT_STCHK(dperm.alloc((size_t)n));
-        IPT_HIPCHK(hipMemcpy(rp.p, a.col_ptr.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice));
-        if (pn > 0) {
-            IPT_HIPCHK(hipMemcpy(ci.p, a.row_ind.data(), (size_t)pn * sizeof(int), hipMemcpyHostToDevice));
-            IPT_HIPCHK(hipMemcpy(vv.p, a.values.data(), (size_t)pn * sizeof(double), hipMemcpyHostToDevice));
+        // off-diagonal absolute row sums (symmetric: column sums), for the
+        // learned Davidson preconditioner
+        std::vector<double> rabs((size_t)n, 0.0);
+        for (int c = 0; c < n; ++c) {
+            for (int e = a.col_ptr[(size_t)c]; e < a.col_ptr[(size_t)c + 1]; ++e) {
+                if (a.row_ind[(size_t)e] != c) {
+                    rabs[(size_t)c] += std::fabs(a.values[(size_t)e]);
+                }
+            }
         }
-        IPT_HIPCHK(hipMemcpy(dd.p, d.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
-        IPT_HIPCHK(hipMemcpy(dperm.p, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+        DevBuf<double> drabs;
+        IPT_STCHK(drabs.alloc((size_t)n));
+        // uploads on the library stream (device buffers may come from the
+        // allocation cache of that stream)
+        IPT_HIPCHK(hipMemcpyAsync(rp.p, a.col_ptr.data(), (size_t)(n + 1) * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        if (pn > 0) {
+            IPT_HIPCHK(hipMemcpyAsync(ci.p, a.row_ind.data(), (size_t)pn * sizeof(int), hipMemcpyHostToDevice,
+                                      ctx->stream));
+            IPT_HIPCHK(hipMemcpyAsync(vv.p, a.values.data(), (size_t)pn * sizeof(double), hipMemcpyHostToDevice,
+                                      ctx->stream));
+        }
+        IPT_HIPCHK(hipMemcpyAsync(dd.p, d.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(dperm.p, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(drabs.p, rabs.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
         result->transfer_setup_time_sec = seconds_since(t1);
         auto t2 = std::chrono::steady_clock::now();
         const BlockParams p = params_from_env(maxiter, tol, use_tol);
@@ -1022,12 +1143,8 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
             return IPT_CUDA_SUCCESS;
         };
         BlockResult br;
-        IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), nullptr, dd.p, n, k, ranges, normH, p, &br));
-        ctx->last_basis_cols = br.basis_cols;
-        ctx->last_steps = br.iterations;
-        ctx->last_fp64_products = br.fp64_products;
-        ctx->last_lowp_products = br.lowp_products;
-        ctx->last_davidson_steps = br.dav.accepted_steps;
+        IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), nullptr, dd.p, drabs.p, n, k, ranges, normH, p, &br));
+        record_stats(ctx, br);
         DevBuf<double> vec;
         IPT_STCHK(vec.alloc((size_t)n * k));
         IPT_STCHK(unpermute_rows(ctx, br.X.p, vec.p, n, n, k, dperm.p));
@@ -1066,5 +1183,22 @@ extern "C" int ipt_hip_last_solve_stats(void *handle, int *basis_cols, int *step
     if (fp64_products) *fp64_products = ctx->last_fp64_products;
     if (lowp_products) *lowp_products = ctx->last_lowp_products;
     if (davidson_steps) *davidson_steps = ctx->last_davidson_steps;
+    return IPT_CUDA_SUCCESS;
+}
+
+extern "C" int ipt_hip_last_davidson_stats(void *handle, int *accepted, int *rejected, int *mlp_accepted,
+                                           int *mlp_rejected, long long *matvecs, double *residual_after)
+{
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    if (accepted) *accepted = ctx->last_davidson_steps;
+    if (rejected) *rejected = ctx->last_davidson_rejected;
+    if (mlp_accepted) *mlp_accepted = ctx->last_mlp_accepted;
+    if (mlp_rejected) *mlp_rejected = ctx->last_mlp_rejected;
+    if (matvecs) *matvecs = ctx->last_davidson_matvecs;
+    if (residual_after) *residual_after = ctx->last_davidson_residual;
     return IPT_CUDA_SUCCESS;
 }

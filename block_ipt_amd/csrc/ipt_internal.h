@@ -49,6 +49,11 @@ struct Context {
     long long last_fp64_products = 0;
     long long last_lowp_products = 0;
     int last_davidson_steps = 0;
+    int last_davidson_rejected = 0;
+    int last_mlp_accepted = 0;
+    int last_mlp_rejected = 0;
+    long long last_davidson_matvecs = 0;
+    double last_davidson_residual = 0.0;
     size_t slot_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 

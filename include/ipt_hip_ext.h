@@ -42,6 +42,47 @@ int ipt_hip_profile_get(const char *kernel, double *total_ms, long long *launche
 int ipt_hip_last_solve_stats(void *handle, int *basis_cols, int *steps, long long *fp64_products,
                              long long *lowp_products, int *davidson_steps);
 
+/* Davidson refinement of the last block-cluster solve on `handle`: steps
+ * accepted and rejected, steps whose learned (MLP) correction was kept and
+ * those redone with the diagonal correction, H products of single columns,
+ * and the largest scaled residual at the end (the reference reports the
+ * same figures in IPTBlockClusterResult, IPT/src/ipt_cuda.cu:3318-3982). */
+int ipt_hip_last_davidson_stats(void *handle, int *accepted, int *rejected, int *mlp_accepted, int *mlp_rejected,
+                                long long *matvecs, double *residual_after);
+
+/* Learned Davidson preconditioner (per-component MLP 3 -> 64 -> 64 -> 1,
+ * GELU; 4481 FP32 parameters: W1[64][3], b1[64], W2[64][64], b2[64],
+ * W3[64], b3).  Correction t_i = (r_i / s_i) * y_i with y = MLP(x_i),
+ * s_i = sum_{j != i} |H_ij|, x_i = (asinh((D_i - theta)/s_i),
+ * r_i / max|r|, s_i / (|D_i - theta| + s_i)).  Enabled in the block path by
+ * IPT_DAVIDSON_PRECOND=mlp with IPT_DAVIDSON_MLP_MODEL=<file>; the diagonal
+ * correction stays the fallback (a step whose largest residual does not
+ * decrease is redone with it).  These entry points expose the kernels for
+ * training and testing. */
+
+/* Features (3 floats per (target q, component i), at (q n + i) * 3) and
+ * qf = r_i / s_i for the columns idx[0..nt) of the residual block R
+ * (n x ., column-major, leading dimension ldr) and Ritz values theta. */
+int ipt_hip_mlp_features(const double *d_R, int ldr, const double *d_diag, const double *d_rowabs,
+                         const double *d_theta, const int *d_idx, int n, int nt, float *d_feat, float *d_qf,
+                         void *handle);
+
+/* out[c] = qf[c] * MLP(feat[3c..3c+3)) (qf NULL: the bare MLP output);
+ * BF16 weights of the hidden layer, FP32 accumulation. */
+int ipt_hip_mlp_infer(const float *d_params, const float *d_feat, const float *d_qf, long long count, float *d_out,
+                      void *handle);
+
+/* Loss sum_c w_c (q_c MLP(x_c) - t_c)^2 and its gradient (4481 floats) in
+ * FP32 over `count` samples (feat 3 per sample, q, t, w one each). */
+int ipt_hip_mlp_gradient(const float *d_params, const float *d_feat, const float *d_q, const float *d_t,
+                         const float *d_w, long long count, float *d_grad, float *d_loss, void *handle);
+
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with gradient
+ * grad * gscale; m, v are the moment buffers (4481 floats each, zero at
+ * step 1). */
+int ipt_hip_mlp_adam_step(float *d_params, float *d_m, float *d_v, const float *d_grad, float gscale, float lr,
+                          int step, void *handle);
+
 #ifdef __cplusplus
 }
 #endif

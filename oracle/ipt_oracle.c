@@ -26,6 +26,10 @@
  *   orc_davidson       LAPACK eigenpairs + residual definition; control flow
  *                      (targets, restart sizes) checked by invariants only:
  *                      "parity unpinned" for the accept/restart sequence.
+ *   orc_mlp_forward/grad  the learned Davidson preconditioner (no reference
+ *                      counterpart): gradient checked against central
+ *                      finite differences of orc_mlp_loss; GELU against its
+ *                      closed form at known points.
  */
 #include <float.h>
 #include <math.h>
@@ -1336,6 +1340,103 @@ out:
     free(diag);
     free(B);
     return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* learned Davidson preconditioner (MLP 3 -> 64 -> 64 -> 1, GELU)      */
+/* ------------------------------------------------------------------ */
+
+/* Parameter layout (doubles here): W1[64][3] at 0, b1[64] at 192,
+ * W2[64][64] (out, in) at 256, b2[64] at 4352, W3[64] at 4416, b3 at 4480. */
+#define ORC_MLP_NPARAM 4481
+
+/* GELU, tanh form (Hendrycks & Gimpel 2016) */
+double orc_gelu(double x)
+{
+    const double c = 0.7978845608028654;
+    return 0.5 * x * (1.0 + tanh(c * (x + 0.044715 * x * x * x)));
+}
+
+static double orc_gelu_grad(double x)
+{
+    const double c = 0.7978845608028654;
+    const double u = c * (x + 0.044715 * x * x * x);
+    const double th = tanh(u);
+    return 0.5 * (1.0 + th) + 0.5 * x * (1.0 - th * th) * c * (1.0 + 3.0 * 0.044715 * x * x);
+}
+
+static double orc_mlp_one(const double *P, const double *x, double *z1, double *h1, double *z2)
+{
+    for (int h = 0; h < 64; ++h) {
+        z1[h] = P[3 * h] * x[0] + P[3 * h + 1] * x[1] + P[3 * h + 2] * x[2] + P[192 + h];
+        h1[h] = orc_gelu(z1[h]);
+    }
+    double y = P[4480];
+    for (int c = 0; c < 64; ++c) {
+        double z = P[4352 + c];
+        for (int k = 0; k < 64; ++k) {
+            z += P[256 + 64 * c + k] * h1[k];
+        }
+        z2[c] = z;
+        y += P[4416 + c] * orc_gelu(z);
+    }
+    return y;
+}
+
+/* y[i] = MLP(feat[3i..3i+3)) */
+void orc_mlp_forward(const double *P, const double *feat, long n, double *y)
+{
+    double z1[64], h1[64], z2[64];
+    for (long i = 0; i < n; ++i) {
+        y[i] = orc_mlp_one(P, feat + 3 * i, z1, h1, z2);
+    }
+}
+
+/* loss = sum_i w_i (q_i y_i - t_i)^2 */
+double orc_mlp_loss(const double *P, const double *feat, const double *q, const double *t, const double *w, long n)
+{
+    double z1[64], h1[64], z2[64], l = 0.0;
+    for (long i = 0; i < n; ++i) {
+        const double e = q[i] * orc_mlp_one(P, feat + 3 * i, z1, h1, z2) - t[i];
+        l += w[i] * e * e;
+    }
+    return l;
+}
+
+/* gradient of orc_mlp_loss by backpropagation */
+double orc_mlp_grad(const double *P, const double *feat, const double *q, const double *t, const double *w, long n,
+                    double *G)
+{
+    double z1[64], h1[64], z2[64], d2[64], l = 0.0;
+    memset(G, 0, sizeof(double) * ORC_MLP_NPARAM);
+    for (long i = 0; i < n; ++i) {
+        const double *x = feat + 3 * i;
+        const double y = orc_mlp_one(P, x, z1, h1, z2);
+        const double e = q[i] * y - t[i];
+        l += w[i] * e * e;
+        const double dy = 2.0 * w[i] * e * q[i];
+        G[4480] += dy;
+        for (int c = 0; c < 64; ++c) {
+            G[4416 + c] += dy * orc_gelu(z2[c]);
+            d2[c] = dy * P[4416 + c] * orc_gelu_grad(z2[c]);
+            G[4352 + c] += d2[c];
+            for (int k = 0; k < 64; ++k) {
+                G[256 + 64 * c + k] += d2[c] * h1[k];
+            }
+        }
+        for (int k = 0; k < 64; ++k) {
+            double a = 0.0;
+            for (int c = 0; c < 64; ++c) {
+                a += P[256 + 64 * c + k] * d2[c];
+            }
+            const double d1 = a * orc_gelu_grad(z1[k]);
+            G[192 + k] += d1;
+            G[3 * k] += d1 * x[0];
+            G[3 * k + 1] += d1 * x[1];
+            G[3 * k + 2] += d1 * x[2];
+        }
+    }
+    return l;
 }
 
 int orc_num_threads(void)

@@ -317,3 +317,41 @@ def block_pipeline(k, dense=None, csc=None, opts=DEFAULT_DEG_OPTS, max_block=DEF
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# ------------------------------------------------------------------ learned preconditioner
+
+MLP_NPARAM = 4481
+
+
+def gelu(x):
+    f = lib().orc_gelu
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double]
+    return f(float(x))
+
+
+def mlp_forward(params, feat):
+    params, feat = _f64(params), _f64(feat)
+    n = feat.size // 3
+    y = np.zeros(n)
+    f = lib().orc_mlp_forward
+    f.restype = None
+    f(_d(params), _d(feat), ctypes.c_long(n), _d(y))
+    return y
+
+
+def mlp_loss(params, feat, q, t, w):
+    params, feat, q, t, w = (_f64(a) for a in (params, feat, q, t, w))
+    f = lib().orc_mlp_loss
+    f.restype = ctypes.c_double
+    return f(_d(params), _d(feat), _d(q), _d(t), _d(w), ctypes.c_long(q.size))
+
+
+def mlp_grad(params, feat, q, t, w):
+    params, feat, q, t, w = (_f64(a) for a in (params, feat, q, t, w))
+    g = np.zeros(MLP_NPARAM)
+    f = lib().orc_mlp_grad
+    f.restype = ctypes.c_double
+    loss = f(_d(params), _d(feat), _d(q), _d(t), _d(w), ctypes.c_long(q.size), _d(g))
+    return g, loss

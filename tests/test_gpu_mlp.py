@@ -1,0 +1,227 @@
+"""Learned Davidson preconditioner on the GPU: the feature, inference,
+gradient and Adam kernels against the oracle and torch, training on the
+GPU lowering the loss, and the guarded use inside the block solver."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import torch
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+from block_ipt_amd import api  # noqa: E402
+
+P = api.MLP_NPARAM
+MODEL = os.path.join(os.path.dirname(api.__file__), "davidson_mlp.bin")
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(P, dtype=np.float32)
+    p[0:192] = rng.standard_normal(192) * 0.8
+    p[192:256] = rng.standard_normal(64) * 0.3
+    p[256:4352] = rng.standard_normal(4096) / 8.0
+    p[4352:4416] = rng.standard_normal(64) * 0.3
+    p[4416:4480] = rng.standard_normal(64) / 8.0
+    p[4480] = 0.1
+    return p
+
+
+class _Bufs:
+    def __init__(self):
+        self.hip = api.hip()
+        self.ptrs = []
+
+    def put(self, a):
+        p = self.hip.to_device(np.ascontiguousarray(a))
+        self.ptrs.append(p)
+        return p
+
+    def empty(self, nbytes):
+        p = self.hip.malloc(nbytes)
+        self.ptrs.append(p)
+        return p
+
+    def get(self, p, n, dtype=np.float32):
+        return self.hip.to_host(p, (n,), dtype=dtype)
+
+    def close(self):
+        for p in self.ptrs:
+            self.hip.free(p)
+
+
+@pytest.fixture
+def bufs():
+    b = _Bufs()
+    yield b
+    b.close()
+
+
+def test_features_match_definition(bufs):
+    n, k = 300, 6
+    rng = np.random.default_rng(11)
+    R = np.asfortranarray(rng.standard_normal((n, k)) * 1e-3)
+    d = np.sort(rng.uniform(0, 50, n))
+    s = rng.uniform(0.5, 7.0, n)
+    th = np.sort(rng.uniform(0, 1, k))
+    idx = np.array([4, 1, 5], dtype=np.int32)
+    nt = idx.size
+    dR, dd, ds, dth, di = bufs.put(R.ravel(order="F")), bufs.put(d), bufs.put(s), bufs.put(th), bufs.put(idx)
+    df, dq = bufs.empty(nt * n * 3 * 4), bufs.empty(nt * n * 4)
+    api.mlp_features(dR, n, dd, ds, dth, di, n, nt, df, dq)
+    F = bufs.get(df, nt * n * 3).reshape(nt, n, 3)
+    Q = bufs.get(dq, nt * n).reshape(nt, n)
+    for q, j in enumerate(idx):
+        a = d - th[j]
+        r = R[:, j]
+        ref = np.stack([np.arcsinh(a / s), r / np.abs(r).max(), s / (np.abs(a) + s)], axis=1)
+        err = float(np.abs(F[q] - ref).max())
+        assert err <= 2e-6 * np.abs(ref).max(), err
+        err = float(np.abs(Q[q] - r / s).max())
+        assert err <= 2e-7 * np.abs(r / s).max(), err
+
+
+@pytest.mark.parametrize("count", [16, 1000, 40961])
+def test_infer_matches_oracle(bufs, count):
+    """BF16 hidden layer with FP32 accumulation: within 1% of the output
+    scale of the FP64 oracle; with qf the output is qf * y exactly."""
+    p = _params(3)
+    rng = np.random.default_rng(count)
+    feat = (rng.standard_normal((count, 3)) * np.array([2.0, 0.5, 0.3]) + np.array([0, 0, 0.5])).astype(np.float32)
+    qf = rng.standard_normal(count).astype(np.float32)
+    dp, dfe, dq = bufs.put(p), bufs.put(feat.ravel()), bufs.put(qf)
+    dy, dyq = bufs.empty(count * 4), bufs.empty(count * 4)
+    api.mlp_infer(dp, dfe, 0, count, dy)
+    api.mlp_infer(dp, dfe, dq, count, dyq)
+    y = bufs.get(dy, count)
+    yq = bufs.get(dyq, count)
+    ref = orc.mlp_forward(p.astype(np.float64), feat.astype(np.float64).ravel())
+    assert np.abs(y - ref).max() <= 1e-2 * max(1.0, np.abs(ref).max())
+    assert np.corrcoef(y, ref)[0, 1] > 0.9999
+    assert np.abs(yq - qf * y).max() <= 1e-6 * np.abs(qf * y).max()
+
+
+@pytest.mark.parametrize("count", [1, 63, 5003, 70000])
+def test_gradient_matches_oracle(bufs, count):
+    p = _params(4)
+    rng = np.random.default_rng(50 + count)
+    feat = rng.standard_normal((count, 3)).astype(np.float32)
+    q = rng.standard_normal(count).astype(np.float32)
+    t = rng.standard_normal(count).astype(np.float32)
+    w = rng.uniform(0.1, 2.0, count).astype(np.float32)
+    dp, dfe, dq, dt, dw = (bufs.put(a) for a in (p, feat.ravel(), q, t, w))
+    dg, dl = bufs.empty(P * 4), bufs.empty(4)
+    api.mlp_gradient(dp, dfe, dq, dt, dw, count, dg, dl)
+    g = bufs.get(dg, P)
+    loss = bufs.get(dl, 1)[0]
+    gref, lref = orc.mlp_grad(*(a.astype(np.float64) for a in (p, feat.ravel(), q, t, w)))
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert np.linalg.norm(g - gref) <= 1e-4 * np.linalg.norm(gref)
+    assert np.abs(g - gref).max() <= 1e-4 * np.abs(gref).max()
+
+
+def test_adam_matches_torch(bufs):
+    p0 = _params(6)
+    rng = np.random.default_rng(6)
+    grads = [rng.standard_normal(P).astype(np.float32) for _ in range(3)]
+    m0, v0 = np.zeros(P, np.float32), np.zeros(P, np.float32)
+    dp, dm, dv = bufs.put(p0), bufs.put(m0), bufs.put(v0)
+    dg = bufs.empty(P * 4)
+    pt = torch.tensor(p0.copy(), requires_grad=True)
+    opt = torch.optim.Adam([pt], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    for i, g in enumerate(grads, start=1):
+        bufs.hip.h.hipMemcpy(dg, g.ctypes.data, g.nbytes, 1)
+        api.mlp_adam_step(dp, dm, dv, dg, 0.5, 1e-2, i)
+        pt.grad = torch.tensor(g * 0.5)
+        opt.step()
+    got = bufs.get(dp, P)
+    ref = pt.detach().numpy()
+    assert np.abs(got - ref).max() <= 1e-6
+
+
+def test_training_on_gpu_lowers_loss(bufs):
+    """Full-batch Adam with the GPU gradient fits a smooth target map."""
+    count = 8192
+    rng = np.random.default_rng(9)
+    feat = rng.standard_normal((count, 3)).astype(np.float32)
+    q = rng.uniform(0.5, 2.0, count).astype(np.float32)
+    t = (q * (np.tanh(feat[:, 0]) + 0.5 * feat[:, 1] * feat[:, 2])).astype(np.float32)
+    w = np.full(count, 1.0 / count, np.float32)
+    p = _params(10) * 0.5
+    dp, dfe, dq, dt, dw = (bufs.put(a) for a in (p, feat.ravel(), q, t, w))
+    dm, dv = bufs.put(np.zeros(P, np.float32)), bufs.put(np.zeros(P, np.float32))
+    dg, dl = bufs.empty(P * 4), bufs.empty(4)
+    losses = []
+    for step in range(1, 301):
+        api.mlp_gradient(dp, dfe, dq, dt, dw, count, dg, dl)
+        losses.append(float(bufs.get(dl, 1)[0]))
+        api.mlp_adam_step(dp, dm, dv, dg, 1.0, 3e-3, step)
+    assert losses[-1] < 0.05 * losses[0], (losses[0], losses[-1])
+    # the GPU loss is the oracle's loss at the same parameters
+    pf = bufs.get(dp, P)
+    api.mlp_gradient(dp, dfe, dq, dt, dw, count, dg, dl)
+    lref = orc.mlp_loss(pf.astype(np.float64), feat.astype(np.float64).ravel(), q.astype(np.float64),
+                        t.astype(np.float64), w.astype(np.float64))
+    assert abs(float(bufs.get(dl, 1)[0]) - lref) <= 1e-4 * lref
+
+
+# ------------------------------------------------------------------ guarded use in the block solver
+
+@pytest.fixture
+def dav_env(monkeypatch):
+    for v in ("IPT_PRECISION", "IPT_MIXED_ANCHOR_EVERY", "IPT_BLOCK_CLUSTER_TOL", "IPT_DAVIDSON_EXTRA_STEPS"):
+        monkeypatch.delenv(v, raising=False)
+    monkeypatch.setenv("IPT_BLOCK_CLUSTER", "1")
+    monkeypatch.setenv("IPT_BLOCK_CLUSTER_MAXITER", "3")
+    monkeypatch.setenv("IPT_DAVIDSON_ENRICH", "1")
+    monkeypatch.setenv("IPT_DAVIDSON_STEPS", "60")
+    monkeypatch.setenv("IPT_DAVIDSON_CONVERGED_TOL", "1e-11")
+    monkeypatch.delenv("IPT_DAVIDSON_PRECOND", raising=False)
+    monkeypatch.delenv("IPT_DAVIDSON_MLP_MODEL", raising=False)
+    return monkeypatch
+
+
+def test_zero_model_falls_back_to_diagonal(dav_env, tmp_path):
+    """A model whose corrections are all zero is rejected on every step and
+    the solve is the diagonal-preconditioned one, bit for bit."""
+    n, k = 2048, 16
+    H = orc.rand_dense(n, 0, 0.0, 10.0, 0.05)
+    vd, ld, _ = api.ipt_cuda_tol(H, k, 1e-12, 3)
+    sd = api.last_davidson_stats()
+    zero = str(tmp_path / "zero.bin")
+    api.save_mlp_model(zero, np.zeros(P, np.float32))
+    dav_env.setenv("IPT_DAVIDSON_PRECOND", "mlp")
+    dav_env.setenv("IPT_DAVIDSON_MLP_MODEL", zero)
+    vz, lz, _ = api.ipt_cuda_tol(H, k, 1e-12, 3)
+    sz = api.last_davidson_stats()
+    assert sd["accepted"] > 0
+    assert sz["mlp_accepted"] == 0 and sz["mlp_rejected"] >= sz["accepted"]
+    assert sz["accepted"] == sd["accepted"]
+    assert np.array_equal(lz, ld) and np.array_equal(vz, vd)
+
+
+def test_missing_model_is_an_error(dav_env, tmp_path):
+    dav_env.setenv("IPT_DAVIDSON_PRECOND", "mlp")
+    dav_env.setenv("IPT_DAVIDSON_MLP_MODEL", str(tmp_path / "absent.bin"))
+    H = orc.rand_dense(256, 0, 0.0, 10.0, 0.05)
+    with pytest.raises(api.IptError):
+        api.ipt_cuda_tol(H, 4, 1e-12, 3)
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="trained model file not present")
+def test_trained_model_solves_to_tolerance(dav_env):
+    """With the shipped model the refined pairs are LAPACK's to 1e-10 with
+    residual <= 1e-10, and learned steps are used."""
+    n, k = 2048, 16
+    H = orc.rand_dense(n, 101, 0.0, 50.0, 0.1)
+    dav_env.setenv("IPT_DAVIDSON_PRECOND", "mlp")
+    dav_env.setenv("IPT_DAVIDSON_MLP_MODEL", MODEL)
+    v, lam, _ = api.ipt_cuda_tol(H, k, 1e-12, 3)
+    st = api.last_davidson_stats()
+    w = sl.eigh(H, eigvals_only=True, subset_by_index=[0, k - 1])
+    assert np.abs(lam - w).max() <= 1e-10 * max(1.0, np.abs(w).max())
+    assert orc.residuals(v, lam, dense=H).max() <= 1e-10
+    assert st["mlp_accepted"] > 0
