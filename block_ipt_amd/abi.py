@@ -196,6 +196,9 @@ SIGNATURES = {
     "ipt_hip_profile_get": (_c.c_int, [_c.c_char_p, _dp, _c.POINTER(_c.c_longlong)]),
     "ipt_hip_last_solve_stats": (_c.c_int, [_vp, _ip, _ip, _c.POINTER(_c.c_longlong),
                                             _c.POINTER(_c.c_longlong), _ip]),
+    "ipt_hip_generate_orbital_csr": (_c.c_int, [_c.c_int, _c.c_int, _c.c_ulonglong, _c.c_int, _c.c_int, _c.c_double,
+                                                _c.c_double, _c.c_double, _vp, _vp, _vp, _c.c_longlong,
+                                                _c.POINTER(_c.c_longlong), _vp]),
     "ipt_hip_last_davidson_stats": (_c.c_int, [_vp, _ip, _ip, _ip, _ip, _c.POINTER(_c.c_longlong), _dp]),
     "ipt_hip_mlp_features": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int, _vp, _vp, _vp]),
     "ipt_hip_mlp_infer": (_c.c_int, [_vp, _vp, _vp, _c.c_longlong, _vp, _vp]),

@@ -284,6 +284,18 @@ def generate_dense(d_H, n, seed, lo, hi, vscale, handle=None):
         raise IptError(st, "ipt_hip_generate_dense")
 
 
+def generate_orbital_csr(n, m, seed, d_row_ptr, d_col_ind, d_values, capacity, norb=40, nsub=8, elo=-20.0,
+                         ehi=5.0, sigma=0.02, handle=None):
+    """Device CSR of the orbital-sum sparse model; returns nnz."""
+    nnz = ctypes.c_longlong(0)
+    st = lib().ipt_hip_generate_orbital_csr(int(n), int(m), int(seed), int(norb), int(nsub), float(elo), float(ehi),
+                                            float(sigma), _vpp(d_row_ptr), _vpp(d_col_ind), _vpp(d_values),
+                                            int(capacity), ctypes.byref(nnz), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_generate_orbital_csr")
+    return nnz.value
+
+
 def lowp_product(d_H, n, mode, d_B, d_Ba, d_Ya, d_diag, nc, d_Y, handle=None):
     st = lib().ipt_hip_lowp_product(ctypes.c_void_p(d_H), int(n), LOWP_MODES.get(mode, mode),
                                     ctypes.c_void_p(d_B), ctypes.c_void_p(d_Ba), ctypes.c_void_p(d_Ya),

@@ -24,6 +24,7 @@ SOURCES = [
     "ipt_block_kernels.hip",
     "ipt_gemm_lp.hip",
     "ipt_gen.hip",
+    "ipt_gen_sparse.hip",
     "ipt_mlp.hip",
     "ipt_profile.cpp",
 ]

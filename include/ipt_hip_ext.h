@@ -21,6 +21,21 @@ extern "C" {
 int ipt_hip_generate_dense(double *d_H, int n, unsigned long long seed, double lo, double hi, double vscale,
                            void *handle);
 
+/* Sparse synthetic workload (orbital-sum model), CSR = CSC (symmetric) with
+ * 32-bit indices and ascending column indices per row:
+ *   norb orbital energies U[elo, ehi] (sorted); row i's diagonal is the sum
+ *   of a random distinct nsub-subset, the n diagonals sorted ascending; row
+ *   i keeps the first m distinct columns of its candidate stream
+ *   floor(U (n-1)) (shifted past i), each with the value sigma * N(0,1), and
+ *   H_ji = H_ij (mirror); when i drew j and j drew i, the larger row's draw
+ *   sets both entries.  U as in ipt_hip_generate_dense, one tagged stream per
+ *   purpose.  The caller provides row_ptr (n + 1 ints) and col_ind / values
+ *   of `capacity` entries (n (2m + 1) always suffices); nnz_out receives the
+ *   entry count.  m <= 512 and m <= n - 1. */
+int ipt_hip_generate_orbital_csr(int n, int m, unsigned long long seed, int norb, int nsub, double elo, double ehi,
+                                 double sigma, int *d_row_ptr, int *d_col_ind, double *d_values, long long capacity,
+                                 long long *nnz_out, void *handle);
+
 /* Y = Ya + diag(d)(B - Ba) + E (B - Ba), E = H - diag(H) rounded to
  * precision `mode` (0 fp16, 1 bf16, 2 bf16x3, 3 fp32, 4 fp8 E4M3 with one
  * scale per 128 x 128 tile of E), FP32-accumulated MFMA.  All device,

@@ -156,6 +156,104 @@ void orc_gen_rand_dense(double *H, int n, uint64_t seed, double lo, double hi,
     free(d);
 }
 
+/* Orbital-sum sparse model (the N = 1e6 sparse workload), restated
+ * sequentially: no reference counterpart (the reference builds its sparse
+ * test matrices from PySCF integrals); the recipe is the one the device
+ * generator documents.  Tagged counter streams, first-m-distinct rejection
+ * sampling, mirror with the larger source row winning a collision.
+ * Returns nnz, or -1 if `cap` entries do not suffice. */
+#define ORC_TAG_ORBITAL 0x6f72626974616c73ULL
+#define ORC_TAG_SUBSET 0x0073756273657473ULL
+#define ORC_TAG_COLUMN 0x00636f6c756d6e73ULL
+#define ORC_TAG_VALUE 0x000076616c756573ULL
+
+typedef struct {
+    int row, col, src;
+    double v;
+} orc_entry;
+
+static int orc_cmp_entry(const void *a, const void *b)
+{
+    const orc_entry *x = (const orc_entry *)a, *y = (const orc_entry *)b;
+    if (x->row != y->row) return (x->row > y->row) - (x->row < y->row);
+    if (x->col != y->col) return (x->col > y->col) - (x->col < y->col);
+    return (x->src > y->src) - (x->src < y->src);
+}
+
+void orc_orbital_diagonal(int n, uint64_t seed, int norb, int nsub, double elo, double ehi, double *d)
+{
+    double *e = (double *)malloc((size_t)norb * sizeof(double));
+    int *perm = (int *)malloc((size_t)norb * sizeof(int));
+    for (int o = 0; o < norb; ++o) {
+        e[o] = elo + (ehi - elo) * orc_uniform01(seed ^ ORC_TAG_ORBITAL, n, o, 0);
+    }
+    qsort(e, (size_t)norb, sizeof(double), orc_cmp_double);
+    for (int i = 0; i < n; ++i) {
+        for (int o = 0; o < norb; ++o) perm[o] = o;
+        double s = 0.0;
+        for (int t = 0; t < nsub; ++t) {
+            int r = t + (int)(orc_uniform01(seed ^ ORC_TAG_SUBSET, n, i, t) * (double)(norb - t));
+            if (r > norb - 1) r = norb - 1;
+            int tmp = perm[t];
+            perm[t] = perm[r];
+            perm[r] = tmp;
+            s += e[perm[t]];
+        }
+        d[i] = s;
+    }
+    qsort(d, (size_t)n, sizeof(double), orc_cmp_double);
+    free(e);
+    free(perm);
+}
+
+long long orc_gen_orbital_csr(int n, int m, uint64_t seed, int norb, int nsub, double elo, double ehi,
+                              double sigma, int *row_ptr, int *col_ind, double *values, long long cap)
+{
+    const long long ne = (long long)n * (2LL * m + 1);
+    orc_entry *E = (orc_entry *)malloc((size_t)ne * sizeof(orc_entry));
+    double *d = (double *)malloc((size_t)n * sizeof(double));
+    int *taken = (int *)malloc((size_t)(m > 0 ? m : 1) * sizeof(int));
+    long long cnt = 0;
+    orc_orbital_diagonal(n, seed, norb, nsub, elo, ehi, d);
+    for (int i = 0; i < n; ++i) {
+        E[cnt].row = i; E[cnt].col = i; E[cnt].src = i; E[cnt].v = d[i]; ++cnt;
+        int got = 0;
+        for (long long t = 0; got < m; ++t) {
+            int c = (int)(orc_uniform01(seed ^ ORC_TAG_COLUMN, n, i, t) * (double)(n - 1));
+            if (c > n - 2) c = n - 2;
+            if (c >= i) c += 1;
+            int dup = 0;
+            for (int q = 0; q < got; ++q) dup |= taken[q] == c;
+            if (dup) continue;
+            taken[got++] = c;
+            double u1 = orc_uniform01(seed ^ ORC_TAG_VALUE, n, i, 2 * t);
+            double u2 = orc_uniform01(seed ^ ORC_TAG_VALUE, n, i, 2 * t + 1);
+            double v = sigma * sqrt(-2.0 * log1p(-u1)) * cos(6.283185307179586 * u2);
+            E[cnt].row = i; E[cnt].col = c; E[cnt].src = i; E[cnt].v = v; ++cnt;
+            E[cnt].row = c; E[cnt].col = i; E[cnt].src = i; E[cnt].v = v; ++cnt;
+        }
+    }
+    qsort(E, (size_t)cnt, sizeof(orc_entry), orc_cmp_entry);
+    long long nnz = 0;
+    int r = 0;
+    row_ptr[0] = 0;
+    for (long long e = 0; e < cnt; ++e) {
+        if (e + 1 < cnt && E[e + 1].row == E[e].row && E[e + 1].col == E[e].col) continue;  /* keep last */
+        if (nnz >= cap) { nnz = -1; break; }
+        while (r < E[e].row) row_ptr[++r] = (int)nnz;
+        col_ind[nnz] = E[e].col;
+        values[nnz] = E[e].v;
+        ++nnz;
+    }
+    if (nnz >= 0) {
+        while (r < n) row_ptr[++r] = (int)nnz;
+    }
+    free(E);
+    free(d);
+    free(taken);
+    return nnz;
+}
+
 /* tridiagonal banded CSC, IPT/test/sparse/runtests_sparse.cu:224-268 */
 int orc_gen_banded_csc(int n, int *col_ptr, int *row_ind, double *values)
 {

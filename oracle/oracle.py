@@ -140,6 +140,31 @@ def rand_dense(n, seed, lo, hi, vscale):
     return H
 
 
+def orbital_csr(n, m, seed, norb=40, nsub=8, elo=-20.0, ehi=5.0, sigma=0.02):
+    """CSR (= CSC) of the orbital-sum sparse model: (row_ptr, col_ind, values)."""
+    cap = n * (2 * m + 1)
+    rp = np.zeros(n + 1, dtype=np.int32)
+    ci = np.zeros(cap, dtype=np.int32)
+    v = np.zeros(cap)
+    f = lib().orc_gen_orbital_csr
+    f.restype = ctypes.c_longlong
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                  ctypes.c_double, ctypes.c_double, _ip, _ip, _dp, ctypes.c_longlong]
+    nnz = f(n, m, int(seed), norb, nsub, float(elo), float(ehi), float(sigma), _i(rp), _i(ci), _d(v), cap)
+    if nnz < 0:
+        raise RuntimeError("orc_gen_orbital_csr failed")
+    return rp, ci[:nnz].copy(), v[:nnz].copy()
+
+
+def orbital_diagonal(n, seed, norb=40, nsub=8, elo=-20.0, ehi=5.0):
+    d = np.zeros(n)
+    f = lib().orc_orbital_diagonal
+    f.restype = None
+    f.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+    f(n, int(seed), norb, nsub, float(elo), float(ehi), _d(d))
+    return d
+
+
 def banded_csc(n):
     """Tridiagonal CSC of IPT/test/sparse/runtests_sparse.cu:224-268."""
     nnz = n + 2 * (n - 1)
