@@ -296,6 +296,13 @@ def generate_orbital_csr(n, m, seed, d_row_ptr, d_col_ind, d_values, capacity, n
     return nnz.value
 
 
+def csr_spmm(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, handle=None):
+    st = lib().ipt_hip_csr_spmm(_vpp(d_row_ptr), _vpp(d_col_ind), _vpp(d_values), int(n), _vpp(d_X), int(ldx),
+                                _vpp(d_Y), int(ldy), int(k), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_csr_spmm")
+
+
 def lowp_product(d_H, n, mode, d_B, d_Ba, d_Ya, d_diag, nc, d_Y, handle=None):
     st = lib().ipt_hip_lowp_product(ctypes.c_void_p(d_H), int(n), LOWP_MODES.get(mode, mode),
                                     ctypes.c_void_p(d_B), ctypes.c_void_p(d_Ba), ctypes.c_void_p(d_Ya),

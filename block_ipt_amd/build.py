@@ -25,6 +25,7 @@ SOURCES = [
     "ipt_gemm_lp.hip",
     "ipt_gen.hip",
     "ipt_gen_sparse.hip",
+    "ipt_sparse.hip",
     "ipt_mlp.hip",
     "ipt_profile.cpp",
 ]

@@ -808,6 +808,76 @@ struct WindowCoupling : CouplingSource {
     void offdiag_strengths(std::vector<double> *s) const override { *s = *off; }
 };
 
+// Fetches H(r, c) and H(c, r) for the pairs rc[2t], rc[2t+1] (device
+// arrays) into out[2t], out[2t+1].
+using GatherFn = std::function<int(const int *rc, int npairs, double *out)>;
+
+// Cluster detection on the sorted diagonal `sd` and the per-row largest
+// off-diagonal magnitudes `soff`: the (row, target) pairs inside each
+// target's detection window are fetched with `gather`, then union-find
+// (IPT/src/ipt_preparation.cu:501-645).
+int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vector<double> &soff, int n, int k,
+                    const GatherFn &gather, std::vector<std::pair<int, int>> *ranges)
+{
+    // candidate (row, target) pairs of the detection windows
+    const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
+    const double eps = DBL_EPSILON;
+    double maxabs = 0.0;
+    for (double x : sd) {
+        maxabs = std::max(maxabs, std::fabs(x));
+    }
+    const double tau_num = o.eps_multiplier * eps * std::max(1.0, maxabs);
+    const double ratio_tol = std::max(o.ratio_tol, eps);
+    std::vector<int> rc;
+    for (int col = 0; col < k; ++col) {
+        const double dc = sd[(size_t)col];
+        const double scale = std::max(1.0, std::fabs(dc));
+        const double fl = std::max(std::max(o.absolute_gap, tau_num), o.rel_gap_tol * scale);
+        const double tau_c = std::max(fl, soff[(size_t)col] / ratio_tol);
+        const int lo = (int)(std::lower_bound(sd.begin(), sd.end(), dc - tau_c) - sd.begin());
+        const int hi = (int)(std::upper_bound(sd.begin(), sd.end(), dc + tau_c) - sd.begin());
+        for (int r = lo; r < hi; ++r) {
+            if (r != col) {
+                rc.push_back(r);
+                rc.push_back(col);
+            }
+        }
+    }
+    WindowCoupling wc;
+    wc.nn = n;
+    wc.dg = &sd;
+    wc.off = &soff;
+    const int npairs = (int)(rc.size() / 2);
+    if (npairs > 0) {
+        DevBuf<int> drc;
+        DevBuf<double> dv;
+        IPT_STCHK(drc.alloc(rc.size()));
+        IPT_STCHK(dv.alloc(rc.size()));
+        IPT_HIPCHK(hipMemcpyAsync(drc.p, rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        IPT_STCHK(gather(drc.p, npairs, dv.p));
+        std::vector<double> hv(rc.size());
+        IPT_STCHK(read_host(ctx, hv.data(), dv.p, hv.size() * sizeof(double)));
+        for (int i = 0; i < npairs; ++i) {
+            const int r = rc[(size_t)2 * i], c = rc[(size_t)2 * i + 1];
+            wc.lut.emplace_back((long)r * n + c, hv[(size_t)2 * i]);
+            wc.lut.emplace_back((long)c * n + r, hv[(size_t)2 * i + 1]);
+        }
+        std::sort(wc.lut.begin(), wc.lut.end());
+    }
+    bool too_large = false;
+    *ranges = target_clusters(wc, sd, k, o, &too_large);
+    if (too_large) {
+        int worst = 0;
+        for (const auto &r : *ranges) {
+            worst = std::max(worst, r.second - r.first + 1);
+        }
+        fprintf(stderr, "IPT block-cluster invalid argument at block detection: block_size=%d max_block_size=%d\n",
+                worst, o.max_block);
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
 // `early` (may be empty) is called as soon as the working matrix and its
 // largest off-diagonal magnitude are known, before the host-side cluster
 // detection, so device work it enqueues overlaps that host work.
@@ -876,63 +946,10 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
     if (early) {
         IPT_STCHK(early(*pr));
     }
-    // candidate (row, target) pairs of the detection windows
-    const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
-    const double eps = DBL_EPSILON;
-    double maxabs = 0.0;
-    for (double x : sd) {
-        maxabs = std::max(maxabs, std::fabs(x));
-    }
-    const double tau_num = o.eps_multiplier * eps * std::max(1.0, maxabs);
-    const double ratio_tol = std::max(o.ratio_tol, eps);
-    std::vector<int> rc;
-    for (int col = 0; col < k; ++col) {
-        const double dc = sd[(size_t)col];
-        const double scale = std::max(1.0, std::fabs(dc));
-        const double fl = std::max(std::max(o.absolute_gap, tau_num), o.rel_gap_tol * scale);
-        const double tau_c = std::max(fl, soff[(size_t)col] / ratio_tol);
-        const int lo = (int)(std::lower_bound(sd.begin(), sd.end(), dc - tau_c) - sd.begin());
-        const int hi = (int)(std::upper_bound(sd.begin(), sd.end(), dc + tau_c) - sd.begin());
-        for (int r = lo; r < hi; ++r) {
-            if (r != col) {
-                rc.push_back(r);
-                rc.push_back(col);
-            }
-        }
-    }
-    WindowCoupling wc;
-    wc.nn = n;
-    wc.dg = &sd;
-    wc.off = &soff;
-    const int npairs = (int)(rc.size() / 2);
-    if (npairs > 0) {
-        DevBuf<int> drc;
-        DevBuf<double> dv;
-        IPT_STCHK(drc.alloc(rc.size()));
-        IPT_STCHK(dv.alloc(rc.size()));
-        IPT_HIPCHK(hipMemcpyAsync(drc.p, rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        IPT_STCHK(gather_pairs(ctx, pr->H, n, drc.p, npairs, dv.p));
-        std::vector<double> hv(rc.size());
-        IPT_STCHK(read_host(ctx, hv.data(), dv.p, hv.size() * sizeof(double)));
-        for (int i = 0; i < npairs; ++i) {
-            const int r = rc[(size_t)2 * i], c = rc[(size_t)2 * i + 1];
-            wc.lut.emplace_back((long)r * n + c, hv[(size_t)2 * i]);
-            wc.lut.emplace_back((long)c * n + r, hv[(size_t)2 * i + 1]);
-        }
-        std::sort(wc.lut.begin(), wc.lut.end());
-    }
-    bool too_large = false;
-    pr->ranges = target_clusters(wc, sd, k, o, &too_large);
-    if (too_large) {
-        int worst = 0;
-        for (const auto &r : pr->ranges) {
-            worst = std::max(worst, r.second - r.first + 1);
-        }
-        fprintf(stderr, "IPT block-cluster invalid argument at block detection: block_size=%d max_block_size=%d\n",
-                worst, o.max_block);
-        return IPT_CUDA_INVALID_ARGUMENT;
-    }
-    return IPT_CUDA_SUCCESS;
+    GatherFn gather = [&](const int *drc, int npairs, double *dv) {
+        return gather_pairs(ctx, pr->H, n, drc, npairs, dv);
+    };
+    return detect_clusters(ctx, sd, soff, n, k, gather, &pr->ranges);
 }
 
 }  // namespace
@@ -998,6 +1015,106 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
         info->preparation_time_sec = prep;
         info->solve_time_sec = seconds_since(t0);
     }
+    *d_vec_out = vec.take();
+    *d_val_out = val.take();
+    return IPT_CUDA_SUCCESS;
+}
+
+// Sparse block-cluster solve with the matrix resident on the device (CSC of
+// a symmetric matrix = its CSR).  Runs when every row's indices ascend and
+// the diagonal is already sorted (no permutation needed); otherwise sets
+// *host_path and returns, and the caller takes the host preparation route
+// (IPT/src/ipt_cuda.cu:4551-4683 permutes on the host as well).
+int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const double *d_v, int n, int k,
+                        int maxiter, double tol, int use_tol, double **d_vec_out, double **d_val_out,
+                        int *iters_out, double *fp_out, bool *host_path)
+{
+    *host_path = false;
+    auto t0 = std::chrono::steady_clock::now();
+    const BlockParams p = params_from_env(maxiter, tol, use_tol);
+    DevBuf<double> d, asum, omax, rowabs;
+    DevBuf<int> flags;
+    IPT_STCHK(d.alloc((size_t)n));
+    IPT_STCHK(asum.alloc((size_t)n));
+    IPT_STCHK(omax.alloc((size_t)n));
+    IPT_STCHK(rowabs.alloc((size_t)n));
+    IPT_STCHK(flags.alloc(1));
+    IPT_HIPCHK(hipMemsetAsync(flags.p, 0, sizeof(int), ctx->stream));
+    IPT_STCHK(csr_scan(ctx, d_rp, d_ci, d_v, n, d.p, asum.p, omax.p, flags.p));
+    int hflags = 0;
+    IPT_STCHK(read_host(ctx, &hflags, flags.p, sizeof(int)));
+    if (hflags & 1) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    if (hflags != 0) {
+        *host_path = true;
+        return IPT_CUDA_SUCCESS;
+    }
+    IPT_STCHK(csr_symcheck(ctx, d_rp, d_ci, d_v, n, flags.p));
+    std::vector<double> hd((size_t)n), hsum((size_t)n), hmax((size_t)n);
+    IPT_HIPCHK(hipMemcpyAsync(hd.data(), d.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipMemcpyAsync(hsum.data(), asum.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipMemcpyAsync(hmax.data(), omax.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_STCHK(read_host(ctx, &hflags, flags.p, sizeof(int)));
+    if (hflags & 4) {
+        fprintf(stderr, "IPT block-cluster path currently requires a real Hermitian CSC matrix\n");
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i + 1 < n; ++i) {
+        if (hd[(size_t)i + 1] < hd[(size_t)i]) {
+            *host_path = true;
+            return IPT_CUDA_SUCCESS;
+        }
+    }
+    double normH = 0.0;
+    std::vector<double> rabs((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        normH = std::max(normH, hsum[(size_t)i]);
+        rabs[(size_t)i] = std::max(0.0, hsum[(size_t)i] - std::fabs(hd[(size_t)i]));
+    }
+    IPT_HIPCHK(hipMemcpyAsync(rowabs.p, rabs.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<std::pair<int, int>> ranges;
+    GatherFn gather = [&](const int *drc, int npairs, double *dv) {
+        return csr_gather_pairs(ctx, d_rp, d_ci, d_v, drc, npairs, dv);
+    };
+    IPT_STCHK(detect_clusters(ctx, hd, hmax, n, k, gather, &ranges));
+    const double prep = seconds_since(t0);
+    int maxcols = 0;
+    for (const auto &r : ranges) {
+        maxcols += r.second - r.first + 1;
+    }
+    maxcols = std::max(maxcols, std::min(n, k + std::max(16, (2 * k + 2) / 3)) + k + 1);
+    const int chunk = std::min(maxcols, 128);
+    DevBuf<double> xr, yr;
+    IPT_STCHK(xr.alloc((size_t)n * chunk));
+    IPT_STCHK(yr.alloc((size_t)n * chunk));
+    ApplyFn apply = [&](const double *X, double *Y, int nc) -> int {
+        for (int c0 = 0; c0 < nc; c0 += chunk) {
+            const int w = std::min(chunk, nc - c0);
+            IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));
+            IPT_STCHK(spmm_csr_rowmajor(ctx, d_rp, d_ci, d_v, n, xr.p, w, yr.p, w, w));
+            IPT_STCHK(rowmajor_to_colmajor(ctx, yr.p, w, Y + (size_t)c0 * n, n, n, w));
+        }
+        return IPT_CUDA_SUCCESS;
+    };
+    BlockResult br;
+    IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), nullptr, d.p, rowabs.p, n, k, ranges, normH, p, &br));
+    record_stats(ctx, br);
+    if (p.verbose) {
+        fprintf(stderr,
+                "IPT block-cluster sparse: n=%d k=%d clusters=%zu basis_cols=%d steps=%d fp64_products=%lld "
+                "davidson_steps=%d max_fp=%.3e prep=%.3fs\n",
+                n, k, ranges.size(), br.basis_cols, br.iterations, br.fp64_products, br.dav.accepted_steps, br.fp_max,
+                prep);
+    }
+    DevBuf<double> vec, val;
+    IPT_STCHK(vec.alloc((size_t)n * k));
+    IPT_STCHK(val.alloc((size_t)k));
+    IPT_HIPCHK(hipMemcpyAsync(vec.p, br.X.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    IPT_HIPCHK(hipMemcpyAsync(val.p, br.theta.p, (size_t)k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (iters_out) *iters_out = br.iterations;
+    if (fp_out) *fp_out = br.fp_max;
     *d_vec_out = vec.take();
     *d_val_out = val.take();
     return IPT_CUDA_SUCCESS;

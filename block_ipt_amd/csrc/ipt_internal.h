@@ -175,6 +175,15 @@ int plain_update(Context *ctx, const double *Y, const double *Z, double *Znew, c
 // values[j] = Y[j*rs + j*cs]
 int gather_diagonal(Context *ctx, const double *Y, int k, long rs, long cs, double *values);
 
+// Symmetric sparse preparation (ipt_sparse.hip), CSR with ascending columns.
+// flags |= 1 index out of range, 2 columns not strictly ascending (scan),
+// 4 not symmetric (symcheck; call only when the scan set no flag).
+int csr_scan(Context *ctx, const int *rp, const int *ci, const double *v, int n, double *diag, double *abssum,
+             double *offmax, int *flags);
+int csr_symcheck(Context *ctx, const int *rp, const int *ci, const double *v, int n, int *flags);
+int csr_gather_pairs(Context *ctx, const int *rp, const int *ci, const double *v, const int *rc, int npairs,
+                     double *out);
+
 // CSR SpMM: Y(n x k, row-major ld ldy) = A(CSR) * X(n x k, row-major ld ldx).
 int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,
                       int n, const double *X, long ldx, double *Y, long ldy, int k);

@@ -22,6 +22,9 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
                             IPTCudaResult *result);
 int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter, double tol, int use_tol,
                        double **d_vec_out, double **d_val_out, IPTCudaResult *info);
+int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const double *d_v, int n, int k,
+                        int maxiter, double tol, int use_tol, double **d_vec_out, double **d_val_out,
+                        int *iters_out, double *fp_out, bool *host_path);
 
 namespace {
 
@@ -386,6 +389,41 @@ int sparse_with_initial_impl(const int *cp, const int *ri, const double *vals, c
     return st;
 }
 
+// Block path for a device CSC whose rows are not ascending or whose
+// diagonal is not sorted: host preparation (permutation), results copied
+// back to library-allocated device buffers.
+int block_sparse_via_host(Context *ctx, const int *d_cp, const int *d_ri, const double *d_v, int n, int k, int nnz,
+                          int maxiter, double tol, int use_tol, double **d_vec_out, double **d_val_out,
+                          int *iters_out, double *fp_out)
+{
+    std::vector<int> cp((size_t)n + 1), ri((size_t)std::max(nnz, 1));
+    std::vector<double> v((size_t)std::max(nnz, 1));
+    IPT_HIPCHK(hipMemcpy(cp.data(), d_cp, cp.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (nnz > 0) {
+        IPT_HIPCHK(hipMemcpy(ri.data(), d_ri, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost));
+        IPT_HIPCHK(hipMemcpy(v.data(), d_v, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    IPTCudaResult r;
+    memset(&r, 0, sizeof(r));
+    IPT_STCHK(block_sparse_host_entry(cp.data(), ri.data(), v.data(), n, k, nnz, maxiter, tol, use_tol, &r));
+    DevBuf<double> vec, val;
+    int st = vec.alloc((size_t)n * k);
+    if (st == IPT_CUDA_SUCCESS) st = val.alloc((size_t)k);
+    if (st == IPT_CUDA_SUCCESS &&
+        (hipMemcpy(vec.p, r.vectors, (size_t)n * k * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(val.p, r.values, (size_t)k * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)) {
+        st = IPT_CUDA_CUDA_ERROR;
+    }
+    if (iters_out) *iters_out = r.iterations;
+    if (fp_out) *fp_out = r.fixed_point_residual;
+    ipt_cuda_free_result(&r);
+    IPT_STCHK(st);
+    (void)ctx;
+    *d_vec_out = vec.take();
+    *d_val_out = val.take();
+    return IPT_CUDA_SUCCESS;
+}
+
 int sparse_device_impl(const int *d_cp, const int *d_ri, const double *d_v, int n, int k, int nnz,
                        int maxiter, double tol, int use_tol, void *handle, double **d_vec_out,
                        double **d_val_out, int *iters_out, double *fp_out)
@@ -402,6 +440,27 @@ int sparse_device_impl(const int *d_cp, const int *d_ri, const double *d_v, int 
     Context *ctx = context_from_handle(handle, &st);
     if (ctx == nullptr) {
         return st;
+    }
+    if (env_flag("IPT_BLOCK_CLUSTER")) {
+        // symmetric input: the CSC is the CSR, used in place when its rows
+        // ascend and its diagonal is sorted
+        std::vector<int> hcp((size_t)n + 1);
+        IPT_HIPCHK(hipMemcpy(hcp.data(), d_cp, hcp.size() * sizeof(int), hipMemcpyDeviceToHost));
+        bool ok = hcp[0] == 0 && hcp[(size_t)n] == nnz;
+        for (int i = 0; ok && i < n; ++i) {
+            ok = hcp[(size_t)i] <= hcp[(size_t)i + 1];
+        }
+        if (!ok) {
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        bool host_path = false;
+        IPT_STCHK(block_sparse_device(ctx, d_cp, d_ri, d_v, n, k, maxiter, tol, use_tol, d_vec_out, d_val_out,
+                                      iters_out, fp_out, &host_path));
+        if (!host_path) {
+            return IPT_CUDA_SUCCESS;
+        }
+        return block_sparse_via_host(ctx, d_cp, d_ri, d_v, n, k, nnz, maxiter, tol, use_tol, d_vec_out, d_val_out,
+                                     iters_out, fp_out);
     }
     // The wavefront-per-row SpMM needs rows: build the CSR of H from the
     // caller's CSC (= CSR of H^T) on the host.

@@ -36,6 +36,12 @@ int ipt_hip_generate_orbital_csr(int n, int m, unsigned long long seed, int norb
                                  double sigma, int *d_row_ptr, int *d_col_ind, double *d_values, long long capacity,
                                  long long *nnz_out, void *handle);
 
+/* Y = A X for a CSR matrix A (n x n, 32-bit indices) and row-major X, Y
+ * (n x k, leading dimensions ldx, ldy >= k): the sparse product of the block
+ * path (the reference calls cusparseSpMM, IPT/src/ipt_cuda.cu:1535-1559). */
+int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n, const double *d_X,
+                     int ldx, double *d_Y, int ldy, int k, void *handle);
+
 /* Y = Ya + diag(d)(B - Ba) + E (B - Ba), E = H - diag(H) rounded to
  * precision `mode` (0 fp16, 1 bf16, 2 bf16x3, 3 fp32, 4 fp8 E4M3 with one
  * scale per 128 x 128 tile of E), FP32-accumulated MFMA.  All device,

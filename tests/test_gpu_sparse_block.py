@@ -1,0 +1,115 @@
+"""Sparse block path with the matrix resident on the device
+(ipt_cuda_sparse_csc_device_tol, IPT_BLOCK_CLUSTER=1) on the orbital-sum
+model, the standalone CSR product, and the host-preparation route for
+input whose diagonal is not sorted."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import scipy.sparse as sp
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+from block_ipt_amd import api  # noqa: E402
+
+
+@pytest.fixture
+def block_env(monkeypatch):
+    monkeypatch.setenv("IPT_BLOCK_CLUSTER", "1")
+    for v in ("IPT_DAVIDSON_ENRICH", "IPT_DAVIDSON_STEPS", "IPT_BLOCK_CLUSTER_MAXITER", "IPT_BLOCK_CLUSTER_TOL",
+              "IPT_COUPLING_GAP_RATIO_TOL", "IPT_MAX_LIFT_BLOCK_SIZE", "IPT_PRECISION", "IPT_DAVIDSON_PRECOND"):
+        monkeypatch.delenv(v, raising=False)
+    return monkeypatch
+
+
+class _Dev:
+    def __init__(self):
+        self.hip = api.hip()
+        self.ptrs = []
+
+    def put(self, a):
+        p = self.hip.to_device(np.ascontiguousarray(a))
+        self.ptrs.append(p)
+        return p
+
+    def close(self):
+        for p in self.ptrs:
+            self.hip.free(p)
+
+
+@pytest.fixture
+def dev():
+    d = _Dev()
+    yield d
+    d.close()
+
+
+@pytest.mark.parametrize("k", [1, 7, 32, 33, 64, 100])
+def test_spmm_matches_scipy(dev, k):
+    n, m = 5000, 30
+    rp, ci, v = orc.orbital_csr(n, m, 3)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    X = np.random.default_rng(k).standard_normal((n, k))
+    drp, dci, dv, dX = dev.put(rp), dev.put(ci), dev.put(v), dev.put(X)
+    dY = dev.hip.malloc(n * k * 8)
+    dev.ptrs.append(dY)
+    api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
+    Y = dev.hip.to_host(dY, (n, k), order="C")
+    ref = A @ X
+    assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def _solve_device(dev, rp, ci, v, n, k, tol=1e-12, maxiter=300):
+    drp, dci, dv = dev.put(rp), dev.put(ci), dev.put(v)
+    dvec, dval, it, fp = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, ci.size, tol, maxiter)
+    try:
+        vec = dev.hip.to_host(dvec, (n, k))
+        val = dev.hip.to_host(dval, (k,))
+    finally:
+        api.free_device_result(dvec, dval)
+    return vec, val, it, fp
+
+
+@pytest.mark.parametrize("n,m,k,seed", [(3000, 20, 8, 1), (4000, 25, 16, 5)])
+def test_device_sparse_block_orbital(block_env, dev, n, m, k, seed):
+    """Eigenpairs of the orbital-sum model from the device-resident path:
+    LAPACK's eigenvalues to 1e-10, residual <= 1e-10, and the oracle's
+    block pipeline on the same CSC gives the same values."""
+    rp, ci, v = orc.orbital_csr(n, m, seed)
+    vec, val, it, fp = _solve_device(dev, rp, ci, v, n, k)
+    H = sp.csr_matrix((v, ci, rp), shape=(n, n)).toarray()
+    w = sl.eigh(H, eigvals_only=True, subset_by_index=[0, k - 1])
+    assert np.abs(val - w).max() <= 1e-10 * max(1.0, np.abs(w).max())
+    assert orc.residuals(vec, val, dense=H).max() <= 1e-10
+    ov, _, oinfo = orc.block_pipeline(k, csc=(rp, ci, v), maxiter=300, tol=1e-12)
+    assert np.abs(val - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
+    assert it == oinfo["max_iterations"]
+
+
+def test_device_sparse_block_unsorted_diagonal(block_env, dev):
+    """A permuted matrix (diagonal not sorted) takes the host preparation
+    route and returns the same pairs in the caller's row order."""
+    n, m, k = 1500, 12, 6
+    rp, ci, v = orc.orbital_csr(n, m, 8)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    p = np.random.default_rng(1).permutation(n)
+    Ap = sp.csc_matrix(A.toarray()[np.ix_(p, p)])
+    Ap.sort_indices()
+    vec, val, it, fp = _solve_device(dev, Ap.indptr.astype(np.int32), Ap.indices.astype(np.int32), Ap.data, n, k)
+    vec0, val0, it0, fp0 = _solve_device(dev, rp, ci, v, n, k)
+    assert np.abs(val - val0).max() <= 1e-12 * max(1.0, np.abs(val0).max())
+    # same vectors up to sign, rows permuted back
+    for j in range(k):
+        a, b = vec[:, j], vec0[p, j]
+        assert min(np.abs(a - b).max(), np.abs(a + b).max()) <= 1e-9
+
+
+def test_device_sparse_block_nonsymmetric_rejected(block_env, dev):
+    n, m = 500, 6
+    rp, ci, v = orc.orbital_csr(n, m, 2)
+    v = v.copy()
+    off = np.nonzero(ci != np.repeat(np.arange(n), np.diff(rp)))[0]
+    v[off[3]] += 0.1
+    with pytest.raises(api.IptError):
+        _solve_device(dev, rp, ci, v, n, 4)
