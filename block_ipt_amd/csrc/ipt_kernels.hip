@@ -220,9 +220,12 @@ int gather_diagonal(Context *ctx, const double *Y, int k, long rs, long cs, doub
 
 // Wavefront-per-row CSR SpMM with row-major X / Y.  The 64 lanes are split
 // into G = 64/S groups of S lanes (S = next power of two >= k, capped at
-// 64); group g walks nonzeros g, g+G, ... of the row and lane s of the
-// group accumulates column s (and s+S, ...) -- every nonzero gathers one
-// contiguous row of X.  Groups are combined with cross-lane shuffles.
+// 64); lane s of a group accumulates column s (and s+S, ...).  The row's
+// (column, value) pairs are read 64 at a time, one per lane (coalesced),
+// and handed to the groups by cross-lane shuffles; each step gathers G
+// contiguous rows of X, and the unrolled steps keep several gathers in
+// flight per lane -- the gathers, not the compulsory bytes, bound this
+// kernel when X (n x k FP64) is larger than the L2.
 template <int S>
 __global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ rp,
                                                        const int *__restrict__ ci,
@@ -231,6 +234,7 @@ __global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ r
                                                        double *__restrict__ Y, long ldy, int k)
 {
     constexpr int G = 64 / S;
+    constexpr int STEPS = 64 / G;
     const int lane = threadIdx.x & 63;
     const int g = lane / S;
     const int s = lane % S;
@@ -240,19 +244,41 @@ __global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ r
         const int pb = rp[row], pe = rp[row + 1];
         for (int c0 = 0; c0 < k; c0 += S) {
             const int col = c0 + s;
+            const bool active = col < k;
+            const double *xc = X + (active ? col : 0);
             double acc = 0.0;
-            for (int p = pb + g; p < pe; p += G) {
-                const int c = ci[p];
-                const double a = vals[p];
-                if (col < k) {
-                    acc += a * X[(long)c * ldx + col];
+            for (int base = pb; base < pe; base += 64) {
+                const int cnt = min(64, pe - base);
+                // padding entries: column 0 (a valid row of X), value 0
+                const int myc = lane < cnt ? ci[base + lane] : 0;
+                const double myv = lane < cnt ? vals[base + lane] : 0.0;
+                // U gathers issued back to back, then their FMAs; the step
+                // count is padded to a multiple of U (U * G divides 64)
+                constexpr int U = STEPS < 8 ? STEPS : 8;
+                const int steps = ((cnt + G * U - 1) / (G * U)) * U;
+                for (int j0 = 0; j0 < steps; j0 += U) {
+                    int c[U];
+                    double v[U], x[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        c[u] = __shfl(myc, (j0 + u) * G + g, 64);
+                        v[u] = __shfl(myv, (j0 + u) * G + g, 64);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        x[u] = xc[(long)c[u] * ldx];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        acc += v[u] * x[u];
+                    }
                 }
             }
 #pragma unroll
             for (int off = S; off < 64; off <<= 1) {
                 acc += __shfl_xor(acc, off, 64);
             }
-            if (g == 0 && col < k) {
+            if (g == 0 && active) {
                 Y[row * ldy + col] = acc;
             }
         }
