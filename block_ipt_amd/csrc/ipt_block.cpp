@@ -106,27 +106,22 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
     int *info = reinterpret_cast<int *>(E + cols);
     IPT_STCHK(gemm_tn_tall(ctx, V, n, V, n, G, cols, n, cols, cols));
     IPT_STCHK(gemm_tn_tall(ctx, V, n, HV, n, P, cols, n, cols, cols));
-    if (cols <= kHostEigMax) {
-        // small projected problem: solved on the host (one round trip of
-        // 2 cols^2 doubles) instead of a few hundred tiny solver launches
-        std::vector<double> hg((size_t)cols * cols), hp((size_t)cols * cols), hc((size_t)cols * cols),
-            ht((size_t)cols);
-        IPT_HIPCHK(hipMemcpyAsync(hg.data(), G, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        IPT_HIPCHK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-        const int st = sym_generalized_eig(cols, hg.data(), hp.data(), ht.data(), hc.data());
-        if (st != 0) {
-            fprintf(stderr, "IPT Rayleigh-Ritz failed: %s\n",
-                    st == 1 ? "Gram matrix not positive definite" : "eigenvalue iteration did not converge");
-            return IPT_CUDA_CUSOLVER_ERROR;
-        }
-        IPT_HIPCHK(hipMemcpyAsync(P, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        IPT_HIPCHK(hipMemcpyAsync(theta, ht.data(), ht.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        // the host vectors must stay alive until the copies have run
-        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    } else {
+    // small projected problems are solved on the host (one round trip of
+    // 2 cols^2 doubles instead of a few hundred tiny solver launches); a
+    // Gram matrix that is not positive definite (nearly dependent basis)
+    // falls back to canonical orthogonalisation on the host at any size
+    bool host = cols <= kHostEigMax;
+    if (!host) {
         void *blas = nullptr;
         IPT_STCHK(context_blas(ctx, &blas));
+        // rocsolver overwrites G and P: keep copies for the fallback
+        DevBuf<double> g0, p0;
+        IPT_STCHK(g0.alloc((size_t)cols * cols));
+        IPT_STCHK(p0.alloc((size_t)cols * cols));
+        IPT_HIPCHK(hipMemcpyAsync(g0.p, G, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(p0.p, P, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
         rocblas_status rs = rocsolver_dsygvd((rocblas_handle)blas, rocblas_eform_ax, rocblas_evect_original,
                                              rocblas_fill_lower, cols, P, cols, G, cols, theta, E, info);
         if (rs != rocblas_status_success) {
@@ -136,10 +131,34 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
         int hinfo = 0;
         IPT_HIPCHK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (hinfo != 0) {
+        if (hinfo > cols) {
+            // leading minor of G not positive definite
+            IPT_HIPCHK(hipMemcpyAsync(G, g0.p, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            IPT_HIPCHK(hipMemcpyAsync(P, p0.p, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            host = true;
+        } else if (hinfo != 0) {
             fprintf(stderr, "IPT Rayleigh-Ritz failed: info=%d\n", hinfo);
             return IPT_CUDA_CUSOLVER_ERROR;
         }
+    }
+    if (host) {
+        std::vector<double> hg((size_t)cols * cols), hp((size_t)cols * cols), hc((size_t)cols * cols),
+            ht((size_t)cols);
+        IPT_HIPCHK(hipMemcpyAsync(hg.data(), G, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int st = sym_generalized_eig_robust(cols, hg.data(), hp.data(), ht.data(), hc.data(), k);
+        if (st != 0) {
+            fprintf(stderr, "IPT Rayleigh-Ritz failed: %s\n",
+                    st == 1 ? "basis of rank below k" : "eigenvalue iteration did not converge");
+            return IPT_CUDA_CUSOLVER_ERROR;
+        }
+        IPT_HIPCHK(hipMemcpyAsync(P, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(theta, ht.data(), ht.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        // the host vectors must stay alive until the copies have run
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     IPT_STCHK(gemm_nn_small(ctx, V, n, P, cols, X, n, n, cols, k, 1.0, 0.0));
     IPT_STCHK(gemm_nn_small(ctx, HV, n, P, cols, HX, n, n, cols, k, 1.0, 0.0));

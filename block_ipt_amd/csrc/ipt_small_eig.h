@@ -275,4 +275,96 @@ inline int sym_generalized_eig(int m, const double *G, const double *P, double *
     return 0;
 }
 
+// As sym_generalized_eig, but a Gram matrix G that is not numerically
+// positive definite (a nearly dependent basis) is handled by canonical
+// orthogonalisation: G = W diag(g) W^T, the directions with
+// g <= 1e-13 max(g) are dropped, B = W_kept diag(g_kept)^-1/2, and
+// (B^T P B) z = theta z gives C = B Z.  The first `need` columns of C and
+// entries of theta are defined (the rest are zero / +inf).  Returns 0, 1 if
+// fewer than `need` directions survive, 2 if an eigenvalue iteration did
+// not converge.
+inline int sym_generalized_eig_robust(int m, const double *G, const double *P, double *theta, double *C, int need)
+{
+    using namespace small_eig_detail;
+    const int st = sym_generalized_eig(m, G, P, theta, C);
+    if (st != 1) {
+        return st;
+    }
+    std::vector<double> W(G, G + (size_t)m * m), g((size_t)m), e((size_t)m);
+    tridiagonalize(W.data(), m, g.data(), e.data());
+    if (!tridiagonal_ql(g.data(), e.data(), W.data(), m)) {
+        return 2;
+    }
+    double gmax = 0.0;
+    for (double x : g) {
+        gmax = std::max(gmax, x);
+    }
+    std::vector<int> keep;
+    for (int i = 0; i < m; ++i) {
+        if (g[(size_t)i] > 1e-13 * gmax) {
+            keep.push_back(i);
+        }
+    }
+    const int r = (int)keep.size();
+    if (r < need || !(gmax > 0.0)) {
+        return 1;
+    }
+    // B (m x r) = W_kept / sqrt(g_kept)
+    std::vector<double> B((size_t)m * r), PB((size_t)m * r), A((size_t)r * r), d((size_t)r), ee((size_t)r);
+    for (int j = 0; j < r; ++j) {
+        const double sc = 1.0 / std::sqrt(g[(size_t)keep[(size_t)j]]);
+        for (int i = 0; i < m; ++i) {
+            B[i + (size_t)j * m] = W[i + (size_t)keep[(size_t)j] * m] * sc;
+        }
+    }
+    for (int j = 0; j < r; ++j) {
+        for (int i = 0; i < m; ++i) {
+            double acc = 0.0;
+            for (int l = 0; l < m; ++l) {
+                acc += P[i + (size_t)l * m] * B[l + (size_t)j * m];
+            }
+            PB[i + (size_t)j * m] = acc;
+        }
+    }
+    for (int j = 0; j < r; ++j) {
+        for (int i = 0; i < r; ++i) {
+            double acc = 0.0;
+            for (int l = 0; l < m; ++l) {
+                acc += B[l + (size_t)i * m] * PB[l + (size_t)j * m];
+            }
+            A[i + (size_t)j * r] = acc;
+        }
+    }
+    for (int j = 0; j < r; ++j) {
+        for (int i = j + 1; i < r; ++i) {
+            const double v = 0.5 * (A[i + (size_t)j * r] + A[j + (size_t)i * r]);
+            A[i + (size_t)j * r] = v;
+            A[j + (size_t)i * r] = v;
+        }
+    }
+    tridiagonalize(A.data(), r, d.data(), ee.data());
+    if (!tridiagonal_ql(d.data(), ee.data(), A.data(), r)) {
+        return 2;
+    }
+    std::vector<int> order((size_t)r);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return d[(size_t)a] < d[(size_t)b]; });
+    std::fill(C, C + (size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j) {
+        theta[j] = INFINITY;
+    }
+    for (int j = 0; j < r; ++j) {
+        const int src = order[(size_t)j];
+        theta[j] = d[(size_t)src];
+        for (int i = 0; i < m; ++i) {
+            double acc = 0.0;
+            for (int l = 0; l < r; ++l) {
+                acc += B[i + (size_t)l * m] * A[l + (size_t)src * r];
+            }
+            C[i + (size_t)j * m] = acc;
+        }
+    }
+    return 0;
+}
+
 }  // namespace ipt

@@ -1,13 +1,15 @@
 // Test driver for block_ipt_amd/csrc/ipt_small_eig.h: reads m, G, P
-// (column-major doubles) from a binary file, writes theta then C.
+// (column-major doubles) from a binary file, writes theta then C.  With a
+// third argument `need` it calls the rank-revealing variant.
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "../../block_ipt_amd/csrc/ipt_small_eig.h"
 
 int main(int argc, char **argv)
 {
-    if (argc != 3) {
+    if (argc != 3 && argc != 4) {
         return 2;
     }
     FILE *f = fopen(argv[1], "rb");
@@ -23,7 +25,9 @@ int main(int argc, char **argv)
         return 2;
     }
     fclose(f);
-    const int st = ipt::sym_generalized_eig(m, G.data(), P.data(), theta.data(), C.data());
+    const int st = argc == 4 ? ipt::sym_generalized_eig_robust(m, G.data(), P.data(), theta.data(), C.data(),
+                                                               atoi(argv[3]))
+                             : ipt::sym_generalized_eig(m, G.data(), P.data(), theta.data(), C.data());
     FILE *o = fopen(argv[2], "wb");
     fwrite(&st, sizeof(int), 1, o);
     fwrite(theta.data(), sizeof(double), theta.size(), o);

@@ -225,3 +225,39 @@ def test_trained_model_solves_to_tolerance(dav_env):
     assert np.abs(lam - w).max() <= 1e-10 * max(1.0, np.abs(w).max())
     assert orc.residuals(v, lam, dense=H).max() <= 1e-10
     assert st["mlp_accepted"] > 0
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="trained model file not present")
+def test_learned_preconditioner_needs_fewer_steps(dav_env):
+    """The N=16384, k=64 dense class (d ~ U[0,50], V scale 0.1/sqrt(N)), test
+    seeds outside the training seeds: one block IPT step, then Davidson to
+    the 1e-10 scaled residual.  The learned corrections (guarded) need at
+    least 30% fewer Davidson steps and fewer block products than the
+    diagonal ones, and every learned run reaches the tolerance."""
+    n, k = 16384, 64
+    hip = api.hip()
+    dav_env.setenv("IPT_BLOCK_CLUSTER_MAXITER", "1")
+    dav_env.setenv("IPT_DAVIDSON_STEPS", "120")
+    dav_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "1e-10")
+    dH = hip.malloc(n * n * 8)
+    tot = {"diag": [0, 0], "mlp": [0, 0]}
+    try:
+        for seed in (100, 101, 104):
+            api.generate_dense(dH, n, seed, 0.0, 50.0, 0.1)
+            for mode in ("diag", "mlp"):
+                if mode == "mlp":
+                    dav_env.setenv("IPT_DAVIDSON_PRECOND", "mlp")
+                    dav_env.setenv("IPT_DAVIDSON_MLP_MODEL", MODEL)
+                else:
+                    dav_env.delenv("IPT_DAVIDSON_PRECOND", raising=False)
+                dv, dl, _, _ = api.ipt_cuda_device_tol(dH, n, k, 1e-12, 1)
+                api.free_device_result(dv, dl)
+                st = api.last_davidson_stats()
+                tot[mode][0] += st["accepted"]
+                tot[mode][1] += st["accepted"] + st["mlp_rejected"]
+                if mode == "mlp":
+                    assert st["residual_after"] <= 1e-10
+    finally:
+        hip.free(dH)
+    assert tot["mlp"][0] <= 0.7 * tot["diag"][0], tot
+    assert tot["mlp"][1] < tot["diag"][1], tot

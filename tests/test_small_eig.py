@@ -20,7 +20,7 @@ def driver(tmp_path_factory):
     return exe, d
 
 
-def _run(driver, G, P):
+def _run(driver, G, P, need=None):
     exe, d = driver
     m = G.shape[0]
     inp, out = d / "in.bin", d / "out.bin"
@@ -28,7 +28,7 @@ def _run(driver, G, P):
         np.array([m], dtype=np.int32).tofile(f)
         np.asfortranarray(G).ravel(order="F").tofile(f)
         np.asfortranarray(P).ravel(order="F").tofile(f)
-    subprocess.run([str(exe), str(inp), str(out)], check=True)
+    subprocess.run([str(exe), str(inp), str(out)] + ([] if need is None else [str(need)]), check=True)
     raw = np.fromfile(out, dtype=np.uint8)
     st = int(raw[:4].view(np.int32)[0])
     vals = raw[4:].view(np.float64)
@@ -87,3 +87,34 @@ def test_identity_metric_is_plain_eigh(driver):
     w, V = np.linalg.eigh(A)
     assert np.abs(theta - w).max() <= 1e-12 * np.abs(w).max()
     assert np.abs(np.abs(np.sum(C * V, axis=0)) - 1.0).max() <= 1e-10
+
+
+@pytest.mark.parametrize("m,dup", [(12, 1), (60, 3)])
+def test_dependent_basis_falls_back_to_canonical_orthogonalisation(driver, m, dup):
+    """A basis with exactly repeated columns (G singular): the rank-revealing
+    variant returns the Ritz pairs of the spanned subspace (LAPACK eigh of
+    the projection onto an orthonormal basis of it), with V C orthonormal."""
+    rng = np.random.default_rng(m)
+    n = 5 * m
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    V = rng.standard_normal((n, m))
+    for j in range(dup):
+        V[:, m - 1 - j] = V[:, j]
+    G = V.T @ V
+    P = V.T @ A @ V
+    st, _, _ = _run(driver, G, P)
+    assert st == 1
+    r = m - dup
+    st, theta, C = _run(driver, G, P, need=r)
+    assert st == 0
+    Qv, _ = np.linalg.qr(V[:, :r])
+    w = np.linalg.eigvalsh(Qv.T @ A @ Qv)
+    assert np.abs(theta[:r] - w).max() <= 1e-9 * np.abs(w).max()
+    X = V @ C[:, :r]
+    assert np.abs(X.T @ X - np.eye(r)).max() <= 1e-9
+    # Galerkin condition: the residuals are orthogonal to the subspace
+    assert np.abs(Qv.T @ (A @ X - X * theta[:r])).max() <= 1e-9 * np.abs(w).max()
+    # more pairs than the rank: an error
+    st, _, _ = _run(driver, G, P, need=r + 1)
+    assert st == 1
