@@ -37,13 +37,31 @@ def parse():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "bf16x3"])
+    ap.add_argument("--precision", default="fp16", choices=["fp16", "bf16", "bf16x3", "fp32", "fp8"])
     ap.add_argument("--anchor-every", type=int, default=20)
     ap.add_argument("--block-tol", type=float, default=1e-8)
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed solves of the plain FP64 path (0: skip)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=2)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy kernel of known size: the reference
+    point that scales the FETCH_SIZE / WRITE_SIZE counters of the kernels
+    measured here."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}), flush=True)
 
 
 def solver_env(precision, anchor_every, block_tol):
@@ -111,6 +129,9 @@ def main():
         print("bench.py: this build runs on one GPU only (requested --gpus %d); multi-GPU is not built"
               % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe_mib > 0:
+        copy_probe(a.copy_probe_mib)
+        return
     from block_ipt_amd import api
 
     hip = api.hip()
@@ -126,11 +147,12 @@ def main():
         solver_env("fp64", a.anchor_every, a.block_tol)
         _, dv_w, _ = run_solves(api, hip, dH, n, k, 1, maxiter)  # warm-up
         api.free_device_result(dv_w, 0)
-        t, dv, vals = run_solves(api, hip, dH, n, k, a.fp64_steps, maxiter)
+        t, dv, vals64 = run_solves(api, hip, dH, n, k, a.fp64_steps, maxiter)
+        X64 = hip.to_host(dv, (n, k))
         api.free_device_result(dv, 0)
         st64 = api.last_solve_stats()
         fp64_iters = st64["steps"]
-        fp64 = {"ms_per_solve": 1e3 * t / a.fp64_steps, "stats": st64}
+        fp64 = {"ms_per_solve": 1e3 * t / a.fp64_steps, "stats": st64, "X": X64, "vals": vals64}
 
     # ---------------- mixed path (timed)
     solver_env(a.precision, a.anchor_every, a.block_tol)
@@ -154,6 +176,10 @@ def main():
     normH = float(np.abs(H).sum(axis=0).max())  # symmetric: column sums = row sums
     res = residuals_host(H, X, vals, normH)
     converged = int(np.sum(res <= 1e-10))
+    converged64 = None
+    if fp64 is not None:
+        res64 = residuals_host(H, fp64.pop("X"), fp64.pop("vals"), normH)
+        converged64 = int(np.sum(res64 <= 1e-10))
     del H
     value = converged / (ms_per_step / 1e3)
 
@@ -182,7 +208,10 @@ def main():
     }
     if lp_launches > 0:
         bc = stats["basis_cols"]
-        alg_bytes = 2.0 * n * n + 2.0 * 2.0 * n * bc  # E fp16 + (B-Ba) fp16 in, partials counted out
+        # E (bytes per element by precision) + the converted (B - B_a) operand
+        # in + FP32-pair partial sums out, each counted once
+        e_b = {"fp16": 2, "bf16": 2, "bf16x3": 4, "fp32": 4, "fp8": 1}[a.precision]
+        alg_bytes = e_b * float(n) * n + (e_b + 2.0) * n * bc
         avg_ms = lp_ms / lp_launches
         achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -192,9 +221,9 @@ def main():
     if f64_launches > 0:
         out["fp64_gemm_in_mixed"] = {"avg_ms": f64_ms / f64_launches, "launches": f64_launches}
     if fp64 is not None:
-        fp64_value = converged / (fp64["ms_per_solve"] / 1e3)
+        fp64_value = converged64 / (fp64["ms_per_solve"] / 1e3)
         out["fp64_path"] = {"pairs_per_sec": fp64_value, "ms_per_solve": fp64["ms_per_solve"],
-                            "stats": fp64["stats"]}
+                            "converged_pairs": converged64, "stats": fp64["stats"]}
         out["mixed_vs_fp64"] = value / fp64_value
     if a.cpu_baseline:
         try:

@@ -30,7 +30,9 @@ struct BlockPlan {
     DevBuf<double> fp;        // last fixed-point residual per cluster
     DevBuf<int> flags;        // [0] singular, [1] status bits, [2] active count
     int partial_stride = 0;   // max nblocks over clusters
-    int fused_blocks = 0;     // row blocks of all clusters of width <= 32
+    int w1_count = 0;         // clusters of width 1 (their own kernel)
+    DevBuf<int> w1_list;      // their ids
+    int fused_blocks = 0;     // row blocks of all clusters of width 2..32
     DevBuf<int> blk_cluster;  // per fused block: cluster id
     DevBuf<int> blk_local;    // per fused block: row block within the cluster
 };

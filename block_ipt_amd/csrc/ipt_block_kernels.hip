@@ -203,6 +203,85 @@ __global__ __launch_bounds__(256) void k_block_update_fused(
     }
 }
 
+// Width-1 clusters (most of the targets): the update is the scalar
+//     x = (d_r B_r - (HB)_r) / (d_r - h),  h = (HB)[first, off],
+// one thread per row and kClustersPerBlock clusters per block, so the
+// registers (and occupancy) are those of a scalar update and a row's d_r is
+// loaded once.  Same arithmetic as the P = 1 path of block_update_rows
+// (pivot test, product with the reciprocal, damping, finite check).  The
+// norm partials are per 64-row wavefront (nblocks = ceil(n / 64)).
+constexpr int kClustersPerBlock = 4;
+
+__global__ __launch_bounds__(256) void k_block_update_w1(
+    const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
+    const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
+    const int *__restrict__ w1_list, int nw1, const int *__restrict__ active, double damping,
+    double *__restrict__ partials, int pstride, int *__restrict__ flags)
+{
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = (int)(row >> 6);
+    const bool in = row < n;
+    const double dr = in ? d[row] : 0.0;
+    int bad = 0;
+    for (int j = 0; j < kClustersPerBlock; ++j) {
+        const int q = blockIdx.y * kClustersPerBlock + j;
+        if (q >= nw1) {
+            break;
+        }
+        const int cid = w1_list[q];
+        const BlockCluster cl = clusters[cid];
+        const bool live = active[cid] != 0;
+        const long col = (long)cl.off * ld;
+        double sdn = 0.0, sbn = 0.0;
+        if (in) {
+            const double ov = B[row + col];
+            if (!live) {
+                Bn[row + col] = ov;
+            } else if (row == cl.first) {
+                Bn[row + col] = 1.0;
+                sdn = (1.0 - ov) * (1.0 - ov);
+                sbn = ov * ov;
+            } else {
+                const double a = dr - Y[cl.first + col];
+                const double rhs = dr * ov - Y[row + col];
+                if (!(fabs(a) > 64.0 * DBL_EPSILON) || !isfinite(a)) {
+                    bad = 1;
+                } else {
+                    double v = rhs * (1.0 / a);
+                    if (damping < 1.0) {
+                        v = (1.0 - damping) * ov + damping * v;
+                    }
+                    if (!isfinite(v)) {
+                        bad = 1;
+                        v = 0.0;
+                    }
+                    Bn[row + col] = v;
+                    sdn = (v - ov) * (v - ov);
+                    sbn = ov * ov;
+                }
+            }
+        }
+        // a wavefront entirely past the last row has no partial slot
+        if (live && (long)chunk * 64 < n) {
+            for (int o = 32; o > 0; o >>= 1) {
+                sdn += __shfl_down(sdn, o, 64);
+                sbn += __shfl_down(sbn, o, 64);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                const long slot = (long)cid * pstride + chunk;
+                partials[2 * slot] = sdn;
+                partials[2 * slot + 1] = sbn;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        bad |= __shfl_down(bad, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && bad) {
+        atomicOr(&flags[0], 1);
+    }
+}
+
 // Per-cluster fixed-point residual fp = ||B'-B||_F / max(1, ||B||_F)
 // (IPT/src/ipt_cuda.cu:2984-3003); clusters reaching fp <= tol are frozen.
 // One block per cluster, fixed-order reduction of the update's partials.
@@ -313,7 +392,8 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
         if (cls > 6) {
             return IPT_CUDA_INVALID_ARGUMENT;
         }
-        const int rows = 256 >> cls;
+        // width 1: partials per 64-row wavefront; wider: per row block
+        const int rows = cl.m == 1 ? 64 : 256 >> cls;
         cl.nblocks = (n + rows - 1) / rows;
         cl.pad = 0;
         plan->partial_stride = std::max(plan->partial_stride, cl.nblocks);
@@ -335,11 +415,18 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
     IPT_STCHK(plan->iters.alloc((size_t)nc));
     IPT_STCHK(plan->fp.alloc((size_t)nc));
     IPT_STCHK(plan->flags.alloc(4));
-    // fused launch table: row blocks of every cluster of width <= 32, widest
+    // width-1 clusters: their own kernel
+    plan->w1_count = (int)lists[0].size();
+    IPT_STCHK(plan->w1_list.alloc(std::max<size_t>(lists[0].size(), 1)));
+    if (!lists[0].empty()) {
+        IPT_HIPCHK(hipMemcpyAsync(plan->w1_list.p, lists[0].data(), lists[0].size() * sizeof(int),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    }
+    // fused launch table: row blocks of every cluster of width 2..32, widest
     // clusters first (their blocks take longest)
     {
         std::vector<int> bc, bl;
-        for (int cls = 5; cls >= 0; --cls) {
+        for (int cls = 5; cls >= 1; --cls) {
             for (int c : lists[(size_t)cls]) {
                 for (int b = 0; b < plan->host[(size_t)c].nblocks; ++b) {
                     bc.push_back(c);
@@ -391,6 +478,13 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
                               (size_t)std::max(plan.nclusters, 1) * plan.partial_stride * 2 * sizeof(double),
                               (void **)&partials));
     const ProfileToken prof = profile_begin(ctx, PROF_BLOCK_UPDATE);
+    if (plan.w1_count > 0) {
+        dim3 grid((n + 255) / 256, (plan.w1_count + kClustersPerBlock - 1) / kClustersPerBlock);
+        hipLaunchKernelGGL(k_block_update_w1, grid, dim3(256), 0, ctx->stream, B, Y, Bn, ld, d, n, plan.clusters.p,
+                           plan.w1_list.p, plan.w1_count, plan.active.p, damping, partials, plan.partial_stride,
+                           plan.flags.p);
+        IPT_HIPCHK(hipGetLastError());
+    }
     if (plan.fused_blocks > 0) {
         hipLaunchKernelGGL(k_block_update_fused, dim3(plan.fused_blocks), dim3(256), 0, ctx->stream, B, Y, Bn, ld,
                            d, n, plan.clusters.p, plan.blk_cluster.p, plan.blk_local.p, plan.active.p, damping,
