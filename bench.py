@@ -28,6 +28,13 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_lowp_gemm launch measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counters (one counter per run), each scaled by what the counter
+# reads for a plain 1 GiB device copy (FETCH_SIZE x2.000, WRITE_SIZE x1.000);
+# runs and tables in profiles/lowp_gemm_n16384_k64_hbm_counters.md.  Keyed by
+# (n, precision, basis columns); other configurations report null.
+MEASURED_TRAFFIC = {(16384, "fp16", 70): 613315551.0}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -43,6 +50,9 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed solves of the plain FP64 path (0: skip)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=2)
+    ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products"],
+                    help="dense: the headline solve; sparse: the N=1e6 orbital-sum CSR solve and its SpMM; "
+                         "products: N=65536 k=256 correction products by precision")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -123,6 +133,119 @@ def cpu_baseline(n, k, seed, sample_iters, full_iters):
     }
 
 
+def sparse_workload(a):
+    """N=1e6 orbital-sum model (40 orbital energies U[-20,5], diagonal = sums
+    of 8, 200 drawn off-diagonal columns per row with N(0, 0.02^2) values,
+    mirrored; nnz about 4.01e8), k=32 lowest pairs, FP64 block path on the
+    device-resident CSR.  One step = one complete solve; SpMM timed with HIP
+    events on the library stream."""
+    from block_ipt_amd import api
+
+    hip = api.hip()
+    n, m, k = 1000000, 200, 32
+    cap = n * (2 * m + 1)
+    drp, dci, dv = hip.malloc((n + 1) * 4), hip.malloc(cap * 4), hip.malloc(cap * 8)
+    nnz = api.generate_orbital_csr(n, m, a.seed, drp, dci, dv, cap)
+    os.environ.update({"IPT_BLOCK_CLUSTER": "1", "IPT_DAVIDSON_ENRICH": "1", "IPT_DAVIDSON_STEPS": "10",
+                       "IPT_DAVIDSON_CONVERGED_TOL": "1e-10"})
+    for v in ("IPT_PRECISION", "IPT_BLOCK_CLUSTER_TOL", "IPT_DAVIDSON_PRECOND"):
+        os.environ.pop(v, None)
+    for _ in range(a.warmup):
+        dvec, dval, _, _ = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, nnz, 1e-12, 200)
+        api.free_device_result(dvec, dval)
+    api.profile_reset()
+    api.profile_enable(True)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        dvec, dval, it, fp = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, nnz, 1e-12, 200)
+        vals = hip.to_host(dval, (k,))
+        if i < a.steps - 1:
+            api.free_device_result(dvec, dval)
+    hip.synchronize()
+    t = time.perf_counter() - t0
+    api.profile_enable(False)
+    sp_ms, sp_launches = api.profile_get("spmm")
+    stats = api.last_solve_stats()
+    # independent residuals: H X with the library's CSR product, norms on the host
+    X = hip.to_host(dvec, (n, k))
+    api.free_device_result(dvec, dval)
+    Xr = np.ascontiguousarray(X)
+    dX, dY = hip.to_device(Xr), hip.malloc(n * k * 8)
+    api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
+    HX = hip.to_host(dY, (n, k), order="C")
+    hip.free(dX)
+    hip.free(dY)
+    # ||H||_inf = max row sum of |H|: |values| summed per row on the device copy
+    vals_abs = np.abs(hip.to_host(dv, (nnz,)))
+    rp = hip.to_host(drp, (n + 1,), dtype=np.int32)
+    normH = float(np.add.reduceat(vals_abs, rp[:-1]).max())
+    del vals_abs
+    for p in (drp, dci, dv):
+        hip.free(p)
+    res = np.linalg.norm(HX - X * vals, axis=0) / (max(1.0, normH) * np.maximum(1.0, np.linalg.norm(X, axis=0)))
+    converged = int(np.sum(res <= 1e-10))
+    ms = 1e3 * t / a.steps
+    bc = stats["basis_cols"]
+    alg = 12.0 * nnz + 4.0 * (n + 1) + 2.0 * 8.0 * n * bc
+    avg = sp_ms / max(1, sp_launches)
+    out = {
+        "metric": "converged eigenpairs/sec at residual<=1e-10 (sparse FP64 block path) on 1 MI355X",
+        "value": converged / (ms / 1e3), "unit": "eigenpairs/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "sparse orbital-sum model N=%d, nnz=%d, k=%d lowest, seed %d" % (n, nnz, k, a.seed)},
+        "converged_pairs": converged, "max_residual": float(res.max()), "solve_stats": stats,
+        "roofline": {"bound": "hbm", "achieved": alg / (avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "k_spmm_rowmajor",
+                     "avg_ms": avg, "launches": sp_launches, "algorithmic_bytes": alg,
+                     "gathered_bytes": 8.0 * nnz * bc},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def products_workload(a):
+    """Correction products at N=65536, k=256 (C2 recipe): the anchored
+    low-precision product E (B - B_a) per precision (one launch per 128
+    columns) against the FP64 product H Z of the plain path."""
+    from block_ipt_amd import api
+
+    hip = api.hip()
+    n, k = 65536, 256
+    dH = hip.malloc(n * n * 8)
+    api.generate_dense(dH, n, a.seed, 0.0, 50.0, 0.1)
+    rng = np.random.default_rng(0)
+    Ba = np.asfortranarray(rng.standard_normal((n, k)))
+    B = np.asfortranarray(Ba + 1e-3 * rng.standard_normal((n, k)))
+    dB, dBa = hip.to_device(B.ravel(order="F")), hip.to_device(Ba.ravel(order="F"))
+    dYa, dY, dd = hip.malloc(n * k * 8), hip.malloc(n * k * 8), hip.malloc(n * 8)
+    out = {"workload": "dense N=%d, k=%d, d~U[0,50] sorted, V~N(0,1)*0.1/sqrt(N), seed %d" % (n, k, a.seed)}
+    for mode, eb in (("fp8", 1), ("fp16", 2), ("bf16x3", 4)):
+        api.profile_reset()
+        api.profile_enable(True)
+        for _ in range(max(1, a.steps)):
+            api.lowp_product(dH, n, mode, dB, dBa, dYa, dd, k, dY)
+        api.profile_enable(False)
+        ms, cnt = api.profile_get("lowp_gemm")
+        alg = eb * float(n) * n + (eb + 2.0) * n * 128
+        out[mode] = {"avg_ms_per_launch": ms / cnt, "launches": cnt, "GB/s": alg / (ms / cnt * 1e-3) / 1e9,
+                     "frac": alg / (ms / cnt * 1e-3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": alg}
+    for v in ("IPT_BLOCK_CLUSTER", "IPT_PRECISION"):
+        os.environ.pop(v, None)
+    api.profile_reset()
+    api.profile_enable(True)
+    dvec, dval = api.ipt_cuda_device(dH, n, k, max(1, a.steps))
+    api.free_device_result(dvec, dval)
+    api.profile_enable(False)
+    ms, cnt = api.profile_get("gemm_f64")
+    per_product = ms / max(1, a.steps)
+    out["fp64"] = {"ms_per_product": per_product, "launches": cnt,
+                   "TF/s": 2.0 * n * n * k / (per_product * 1e-3) / 1e12}
+    for p in (dH, dB, dBa, dYa, dY, dd):
+        hip.free(p)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -131,6 +254,12 @@ def main():
         sys.exit(2)
     if a.copy_probe_mib > 0:
         copy_probe(a.copy_probe_mib)
+        return
+    if a.workload == "sparse":
+        sparse_workload(a)
+        return
+    if a.workload == "products":
+        products_workload(a)
         return
     from block_ipt_amd import api
 
@@ -215,7 +344,8 @@ def main():
         avg_ms = lp_ms / lp_launches
         achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": MEASURED_TRAFFIC.get((n, a.precision, bc)),
                            "kernel": "k_lowp_gemm", "avg_ms": avg_ms, "launches": lp_launches,
                            "algorithmic_bytes": alg_bytes}
     if f64_launches > 0:

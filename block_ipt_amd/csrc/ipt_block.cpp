@@ -1069,7 +1069,17 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
         *host_path = true;
         return IPT_CUDA_SUCCESS;
     }
-    IPT_STCHK(csr_symcheck(ctx, d_rp, d_ci, d_v, n, flags.p));
+    // exact symmetry by fingerprint (one streaming pass); only a mismatch
+    // runs the entry-by-entry check with the reference's tolerance
+    DevBuf<unsigned long long> fps;
+    IPT_STCHK(fps.alloc(2));
+    IPT_HIPCHK(hipMemsetAsync(fps.p, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    IPT_STCHK(csr_symhash(ctx, d_rp, d_ci, d_v, n, fps.p));
+    unsigned long long hfp[2] = {0, 0};
+    IPT_STCHK(read_host(ctx, hfp, fps.p, sizeof(hfp)));
+    if (hfp[0] != hfp[1]) {
+        IPT_STCHK(csr_symcheck(ctx, d_rp, d_ci, d_v, n, flags.p));
+    }
     std::vector<double> hd((size_t)n), hsum((size_t)n), hmax((size_t)n);
     IPT_HIPCHK(hipMemcpyAsync(hd.data(), d.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     IPT_HIPCHK(hipMemcpyAsync(hsum.data(), asum.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

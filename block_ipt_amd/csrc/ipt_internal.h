@@ -181,6 +181,10 @@ int gather_diagonal(Context *ctx, const double *Y, int k, long rs, long cs, doub
 int csr_scan(Context *ctx, const int *rp, const int *ci, const double *v, int n, double *diag, double *abssum,
              double *offmax, int *flags);
 int csr_symcheck(Context *ctx, const int *rp, const int *ci, const double *v, int n, int *flags);
+// sums[0] / sums[1]: order-independent 64-bit fingerprints of the entries
+// (r, c, v) and of their transposes (c, r, v); equal for an exactly
+// symmetric matrix (sums must be zero on entry).
+int csr_symhash(Context *ctx, const int *rp, const int *ci, const double *v, int n, unsigned long long *sums);
 int csr_gather_pairs(Context *ctx, const int *rp, const int *ci, const double *v, const int *rc, int npairs,
                      double *out);
 

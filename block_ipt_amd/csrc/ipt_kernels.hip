@@ -1,4 +1,6 @@
 // Elementwise / reduction kernels of the plain IPT path and the CSR SpMM.
+#include <algorithm>
+
 #include "ipt_internal.h"
 
 namespace ipt {
@@ -313,40 +315,82 @@ int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, cons
 
 // ------------------------------------------------------------------ layouts
 
-__global__ void k_col_to_row(const double *__restrict__ A, long lda, double *__restrict__ B, long ldb,
-                             int n, int k)
+// Layout changes between column-major (ld >= n) and row-major (ld >= k)
+// n x k blocks, k <= 128 per launch column chunk: 32-row tiles staged in
+// LDS so both the column-major side (32 consecutive rows of a column) and
+// the row-major side (whole rows) are accessed contiguously.
+constexpr int kTileRows = 32;
+constexpr int kTileCols = 128;
+
+__global__ __launch_bounds__(256) void k_col_to_row(const double *__restrict__ A, long lda,
+                                                    double *__restrict__ B, long ldb, int n, int k)
 {
-    const long total = (long)n * k;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long)gridDim.x * blockDim.x) {
-        const long r = idx / k, c = idx % k;
-        B[r * ldb + c] = A[r + c * lda];
+    __shared__ double t[kTileRows][kTileCols + 1];
+    const int c0 = blockIdx.y * kTileCols;
+    const int kc = min(kTileCols, k - c0);
+    for (long r0 = (long)blockIdx.x * kTileRows; r0 < n; r0 += (long)gridDim.x * kTileRows) {
+        const int rows = (int)min((long)kTileRows, n - r0);
+        // read: lane -> row within the tile, wave-strided over columns
+        for (int e = threadIdx.x; e < kTileRows * kc; e += 256) {
+            const int rr = e % kTileRows, cc = e / kTileRows;
+            if (rr < rows) {
+                t[rr][cc] = A[r0 + rr + (long)(c0 + cc) * lda];
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < rows * kc; e += 256) {
+            const int rr = e / kc, cc = e % kc;
+            B[(r0 + rr) * ldb + c0 + cc] = t[rr][cc];
+        }
+        __syncthreads();
     }
 }
 
-__global__ void k_row_to_col(const double *__restrict__ A, long lda, double *__restrict__ B, long ldb,
-                             int n, int k)
+__global__ __launch_bounds__(256) void k_row_to_col(const double *__restrict__ A, long lda,
+                                                    double *__restrict__ B, long ldb, int n, int k)
 {
-    const long total = (long)n * k;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long)gridDim.x * blockDim.x) {
-        const long r = idx % n, c = idx / n;
-        B[r + c * ldb] = A[r * lda + c];
+    __shared__ double t[kTileRows][kTileCols + 1];
+    const int c0 = blockIdx.y * kTileCols;
+    const int kc = min(kTileCols, k - c0);
+    for (long r0 = (long)blockIdx.x * kTileRows; r0 < n; r0 += (long)gridDim.x * kTileRows) {
+        const int rows = (int)min((long)kTileRows, n - r0);
+        for (int e = threadIdx.x; e < rows * kc; e += 256) {
+            const int rr = e / kc, cc = e % kc;
+            t[rr][cc] = A[(r0 + rr) * lda + c0 + cc];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < kTileRows * kc; e += 256) {
+            const int rr = e % kTileRows, cc = e / kTileRows;
+            if (rr < rows) {
+                B[r0 + rr + (long)(c0 + cc) * ldb] = t[rr][cc];
+            }
+        }
+        __syncthreads();
     }
+}
+
+static dim3 tile_grid(int n, int k)
+{
+    const long tiles = ((long)n + kTileRows - 1) / kTileRows;
+    return dim3((unsigned)std::min<long>(tiles, 16384), (unsigned)((k + kTileCols - 1) / kTileCols));
 }
 
 int colmajor_to_rowmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k)
 {
-    hipLaunchKernelGGL(k_col_to_row, dim3(blocks_for((long)n * k, 256, 16384)), dim3(256), 0,
-                       ctx->stream, A, lda, B, ldb, n, k);
+    if (n <= 0 || k <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    hipLaunchKernelGGL(k_col_to_row, tile_grid(n, k), dim3(256), 0, ctx->stream, A, lda, B, ldb, n, k);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }
 
 int rowmajor_to_colmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k)
 {
-    hipLaunchKernelGGL(k_row_to_col, dim3(blocks_for((long)n * k, 256, 16384)), dim3(256), 0,
-                       ctx->stream, A, lda, B, ldb, n, k);
+    if (n <= 0 || k <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    hipLaunchKernelGGL(k_row_to_col, tile_grid(n, k), dim3(256), 0, ctx->stream, A, lda, B, ldb, n, k);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

@@ -78,6 +78,50 @@ __global__ __launch_bounds__(256) void k_csr_scan(const int *__restrict__ rp, co
     }
 }
 
+__device__ inline unsigned long long mix64(unsigned long long x)
+{
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+__device__ inline unsigned long long entry_hash(unsigned r, unsigned c, double v)
+{
+    return mix64(mix64(((unsigned long long)r << 32) | c) ^ (unsigned long long)__double_as_longlong(v));
+}
+
+// Exact-symmetry fingerprint in one streaming pass: sums[0] = sum of
+// hash(r, c, v) and sums[1] = sum of hash(c, r, v) over the stored entries
+// (wrapping 64-bit sums, order independent).  A symmetric matrix has equal
+// sums; unequal sums send the caller to the tolerance check below.
+__global__ __launch_bounds__(256) void k_csr_symhash(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                     const double *__restrict__ v, int n,
+                                                     unsigned long long *__restrict__ sums)
+{
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long waves = ((long)gridDim.x * blockDim.x) >> 6;
+    unsigned long long a = 0, b = 0;
+    for (long row = wave; row < n; row += waves) {
+        const int pb = rp[row], pe = rp[row + 1];
+        for (int p = pb + lane; p < pe; p += 64) {
+            const unsigned c = (unsigned)ci[p];
+            const double x = v[p];
+            a += entry_hash((unsigned)row, c, x);
+            b += entry_hash(c, (unsigned)row, x);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&sums[0], a);
+        atomicAdd(&sums[1], b);
+    }
+}
+
 // |H_rc - H_cr| <= 1e-12 max(1, |H_rc|, |H_cr|) for every stored entry; the
 // mirror is found by binary search in the (ascending) row c.
 __global__ __launch_bounds__(256) void k_csr_symcheck(const int *__restrict__ rp, const int *__restrict__ ci,
@@ -138,6 +182,13 @@ int csr_scan(Context *ctx, const int *rp, const int *ci, const double *v, int n,
 {
     hipLaunchKernelGGL(k_csr_scan, dim3(grid_waves(n)), dim3(256), 0, ctx->stream, rp, ci, v, n, diag, abssum, offmax,
                        flags);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+int csr_symhash(Context *ctx, const int *rp, const int *ci, const double *v, int n, unsigned long long *sums)
+{
+    hipLaunchKernelGGL(k_csr_symhash, dim3(grid_waves(n)), dim3(256), 0, ctx->stream, rp, ci, v, n, sums);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }
