@@ -55,6 +55,29 @@ __device__ __forceinline__ void block_update_rows(
     const long row = (long)lblk * GROUPS + grp;
     const bool live = active[cid] != 0;
     double sdn = 0.0, sbn = 0.0;
+    // Off-diagonal absolute row sums of the cluster matrix h, shared by
+    // every row of the block: a row whose system matrix (d_r I - h^T) is
+    // column diagonally dominant by a factor of 2 keeps its pivots on the
+    // diagonal through the whole elimination (dominance survives Gaussian
+    // elimination, and the factor 2 absorbs rounding), so partial pivoting
+    // would pick exactly those lanes: the arg-max search is skipped and the
+    // arithmetic is the same.
+    __shared__ double hsum[P];
+    if (P > 1) {
+        if (threadIdx.x < P) {
+            double acc = 0.0;
+            if (threadIdx.x < m && live) {
+                const double *Yc = Y + (long)cl.off * ld;
+                for (int u = 0; u < m; ++u) {
+                    if (u != (int)threadIdx.x) {
+                        acc += fabs(Yc[cl.first + threadIdx.x + (long)u * ld]);
+                    }
+                }
+            }
+            hsum[threadIdx.x] = acc;
+        }
+        __syncthreads();
+    }
 
     if (row < n) {
         const double *Bc = B + (long)cl.off * ld;
@@ -87,18 +110,34 @@ __device__ __forceinline__ void block_update_rows(
             bool used = false;
             int mypc = -1;
             bool bad = false;
+            // lane t checks column t: |d_r - h_tt| > 2 sum_{u != t} |h_tu|
+            bool dominant = true;
+            if (P > 1) {
+                const bool mine = t >= m || fabs(dr - Yc[cl.first + t + (long)t * ld]) > 2.0 * hsum[t];
+                const unsigned long long ball = __ballot(mine);
+                const int base = (threadIdx.x & 63) & ~(P - 1);
+                const unsigned long long gmask = (P == 64) ? ~0ULL : (((1ULL << P) - 1ULL) << base);
+                dominant = (ball & gmask) == gmask;
+            }
 #pragma unroll
             for (int pc = 0; pc < P; ++pc) {
                 if (pc < m && !bad) {
-                    double cand = (t < m && !used) ? fabs(a[pc]) : -1.0;
-                    int cidx = t;
+                    double cand;
+                    int cidx;
+                    if (dominant) {
+                        cidx = pc;
+                        cand = fabs(__shfl(a[pc], pc, P));
+                    } else {
+                        cand = (t < m && !used) ? fabs(a[pc]) : -1.0;
+                        cidx = t;
 #pragma unroll
-                    for (int o = 1; o < P; o <<= 1) {
-                        const double oc = __shfl_xor(cand, o, P);
-                        const int oi = __shfl_xor(cidx, o, P);
-                        if (oc > cand || (oc == cand && oi < cidx)) {
-                            cand = oc;
-                            cidx = oi;
+                        for (int o = 1; o < P; o <<= 1) {
+                            const double oc = __shfl_xor(cand, o, P);
+                            const int oi = __shfl_xor(cidx, o, P);
+                            if (oc > cand || (oc == cand && oi < cidx)) {
+                                cand = oc;
+                                cidx = oi;
+                            }
                         }
                     }
                     if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
