@@ -303,6 +303,12 @@ def csr_spmm(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, handle=No
         raise IptError(st, "ipt_hip_csr_spmm")
 
 
+def cholqr_mixed(d_Q, n, k, handle=None):
+    st = lib().ipt_hip_cholqr_mixed(_vpp(d_Q), int(n), int(k), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_cholqr_mixed")
+
+
 def lowp_product(d_H, n, mode, d_B, d_Ba, d_Ya, d_diag, nc, d_Y, handle=None):
     st = lib().ipt_hip_lowp_product(ctypes.c_void_p(d_H), int(n), LOWP_MODES.get(mode, mode),
                                     ctypes.c_void_p(d_B), ctypes.c_void_p(d_Ba), ctypes.c_void_p(d_Ya),

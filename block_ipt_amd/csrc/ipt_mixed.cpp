@@ -13,15 +13,19 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "../../include/ipt_hip_mixed.h"
 #include "ipt_block.h"
 #include "ipt_lowp.h"
 #include "ipt_plain.h"
+#include "ipt_small_eig.h"
 
 namespace ipt {
 
@@ -154,10 +158,134 @@ int orthogonalize_qr(Context *ctx, double *Q, int n, int k)
     return IPT_CUDA_SUCCESS;
 }
 
+__global__ void k_to_float(const double *__restrict__ in, float *__restrict__ out, long total)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        out[i] = (float)in[i];
+    }
+}
+
+// Inverse of the k x k upper triangular R (column-major, leading dimension
+// ldr) into Rinv (k x k); false if a diagonal entry is zero or not finite.
+bool upper_inverse(const double *R, int ldr, int k, std::vector<double> *Rinv)
+{
+    Rinv->assign((size_t)k * k, 0.0);
+    for (int j = 0; j < k; ++j) {
+        const double rjj = R[j + (size_t)j * ldr];
+        if (!(std::fabs(rjj) > 0.0) || !std::isfinite(rjj)) {
+            return false;
+        }
+        // column j of R^-1: solve R x = e_j (x_i = 0 for i > j)
+        double *x = Rinv->data() + (size_t)j * k;
+        x[j] = 1.0 / rjj;
+        for (int i = j - 1; i >= 0; --i) {
+            double acc = 0.0;
+            for (int l = i + 1; l <= j; ++l) {
+                acc += R[i + (size_t)l * ldr] * x[l];
+            }
+            x[i] = -acc / R[i + (size_t)i * ldr];
+        }
+    }
+    return true;
+}
+
 }  // namespace
+
+// FP32-preconditioned Cholesky QR (IPT/mixed/ipt_rayleigh_mixed_mpqr_cuda.cu:
+// 108-386): columns normalised, R from an FP32 Householder QR, V = Q R^-1,
+// G = V^T V, G = L L^T, Q <- V L^-T, all but the FP32 QR in FP64.  The FP32
+// R brings the condition number of V down to about 1/eps_32, so one Cholesky
+// QR in FP64 finishes the orthogonalisation.  Q: n x k, column-major, ld n.
+int orthogonalize_mpqr(Context *ctx, double *Q, int n, int k)
+{
+    if (Q == nullptr || n <= 0 || k <= 0 || k > n) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    const size_t nk = (size_t)n * (size_t)k;
+    DevBuf<double> sq, V, W;
+    DevBuf<float> Q32, tau;
+    IPT_STCHK(sq.alloc((size_t)k));
+    IPT_STCHK(column_sqnorms(ctx, Q, n, n, k, sq.p));
+    std::vector<double> hsq((size_t)k);
+    IPT_HIPCHK(hipMemcpyAsync(hsq.data(), sq.p, hsq.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (double x : hsq) {
+        if (!(x > 0.0) || !std::isfinite(x)) {
+            return IPT_CUDA_INVALID_ARGUMENT;  // zero or non-finite column
+        }
+    }
+    IPT_STCHK(scale_columns_inv_sqrt(ctx, Q, nullptr, n, n, k, sq.p));
+    IPT_STCHK(Q32.alloc(nk));
+    IPT_STCHK(tau.alloc((size_t)k));
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)std::min<size_t>((nk + 255) / 256, 16384)), dim3(256), 0,
+                       ctx->stream, Q, Q32.p, (long)nk);
+    IPT_HIPCHK(hipGetLastError());
+    void *blas = nullptr;
+    IPT_STCHK(context_blas(ctx, &blas));
+    if (rocsolver_sgeqrf((rocblas_handle)blas, n, k, Q32.p, n, tau.p) != rocblas_status_success) {
+        return IPT_CUDA_CUSOLVER_ERROR;
+    }
+    // R: upper triangle of the first k rows
+    std::vector<float> r32((size_t)k * k);
+    IPT_HIPCHK(hipMemcpy2DAsync(r32.data(), (size_t)k * sizeof(float), Q32.p, (size_t)n * sizeof(float),
+                                (size_t)k * sizeof(float), (size_t)k, hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<double> R((size_t)k * k, 0.0), Rinv;
+    for (int j = 0; j < k; ++j) {
+        for (int i = 0; i <= j; ++i) {
+            R[i + (size_t)j * k] = (double)r32[i + (size_t)j * k];
+        }
+    }
+    if (!upper_inverse(R.data(), k, k, &Rinv)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    IPT_STCHK(V.alloc(nk));
+    IPT_STCHK(W.alloc((size_t)k * k));
+    IPT_HIPCHK(hipMemcpyAsync(W.p, Rinv.data(), Rinv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    IPT_STCHK(gemm_nn_small(ctx, Q, n, W.p, k, V.p, n, n, k, k, 1.0, 0.0));
+    // Cholesky QR of V in FP64
+    DevBuf<double> G;
+    IPT_STCHK(G.alloc((size_t)k * k));
+    IPT_STCHK(gemm_tn_tall(ctx, V.p, n, V.p, n, G.p, k, n, k, k));
+    std::vector<double> hg((size_t)k * k);
+    IPT_HIPCHK(hipMemcpyAsync(hg.data(), G.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (!small_eig_detail::cholesky_lower(hg.data(), k)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    // L^-T = (L^T)^-1: L^T is upper triangular
+    std::vector<double> LT((size_t)k * k, 0.0), LTinv;
+    for (int j = 0; j < k; ++j) {
+        for (int i = j; i < k; ++i) {
+            LT[j + (size_t)i * k] = hg[i + (size_t)j * k];
+        }
+    }
+    if (!upper_inverse(LT.data(), k, k, &LTinv)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    IPT_HIPCHK(hipMemcpyAsync(W.p, LTinv.data(), LTinv.size() * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+    IPT_STCHK(gemm_nn_small(ctx, V.p, n, W.p, k, Q, n, n, k, k, 1.0, 0.0));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
+
 }  // namespace ipt
 
 using namespace ipt;
+
+extern "C" int ipt_hip_cholqr_mixed(double *d_Q, int n, int k, void *handle)
+{
+    if (d_Q == nullptr || n <= 0 || k <= 0 || k > n) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    return orthogonalize_mpqr(ctx, d_Q, n, k);
+}
 
 #define IPT_TWO_PHASE_ENTRY(NAME, FREE, TYPE, MODE, LO_IT, LO_TIME, LO_FP)                                        \
     extern "C" void FREE(TYPE *result)                                                                            \
@@ -241,7 +369,15 @@ extern "C" int ipt_rayleigh_mixed_tf32_cuda_device_tol(const double *dH, int n, 
     result->tf32_time_sec = now_seconds() - t;
 
     t = now_seconds();
-    IPT_STCHK(orthogonalize_qr(ctx, Q.p, n, k));
+    // Householder QR (ipt_rayleigh_mixed_cuda.cu) or, with
+    // IPT_RAYLEIGH_QR=mpqr, the FP32-preconditioned Cholesky QR of the
+    // reference's mpqr build (ipt_rayleigh_mixed_mpqr_cuda.cu:456-460)
+    const char *qr = getenv("IPT_RAYLEIGH_QR");
+    if (qr != nullptr && !strcmp(qr, "mpqr")) {
+        IPT_STCHK(orthogonalize_mpqr(ctx, Q.p, n, k));
+    } else {
+        IPT_STCHK(orthogonalize_qr(ctx, Q.p, n, k));
+    }
     result->orthogonalize_time_sec = now_seconds() - t;
 
     // B = Q^T (H Q), symmetrised (ipt_rayleigh_mixed_cuda.cu:319-335)

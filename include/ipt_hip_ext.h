@@ -42,6 +42,15 @@ int ipt_hip_generate_orbital_csr(int n, int m, unsigned long long seed, int norb
 int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n, const double *d_X,
                      int ldx, double *d_Y, int ldy, int k, void *handle);
 
+/* In place: Q (n x k, column-major, leading dimension n, k <= n) <- an
+ * orthonormal basis of its column space by FP32-preconditioned Cholesky QR
+ * (IPT/mixed/ipt_rayleigh_mixed_mpqr_cuda.cu:169-386: columns normalised,
+ * R of an FP32 Householder QR, V = Q R^-1, V^T V = L L^T, Q = V L^-T, in
+ * FP64 except the FP32 QR).  Status 1 for a zero column or a basis whose
+ * Gram matrix is not numerically positive definite after preconditioning.
+ * ipt_rayleigh_mixed_tf32_cuda_device_tol uses it with IPT_RAYLEIGH_QR=mpqr. */
+int ipt_hip_cholqr_mixed(double *d_Q, int n, int k, void *handle);
+
 /* Y = Ya + diag(d)(B - Ba) + E (B - Ba), E = H - diag(H) rounded to
  * precision `mode` (0 fp16, 1 bf16, 2 bf16x3, 3 fp32, 4 fp8 E4M3 with one
  * scale per 128 x 128 tile of E), FP32-accumulated MFMA.  All device,
