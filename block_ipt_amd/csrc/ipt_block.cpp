@@ -836,7 +836,8 @@ using GatherFn = std::function<int(const int *rc, int npairs, double *out)>;
 // target's detection window are fetched with `gather`, then union-find
 // (IPT/src/ipt_preparation.cu:501-645).
 int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vector<double> &soff, int n, int k,
-                    const GatherFn &gather, std::vector<std::pair<int, int>> *ranges)
+                    const GatherFn &gather, std::vector<std::pair<int, int>> *ranges,
+                    const std::function<int()> &overlap = std::function<int()>())
 {
     // candidate (row, target) pairs of the detection windows
     const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
@@ -867,15 +868,41 @@ int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vect
     wc.dg = &sd;
     wc.off = &soff;
     const int npairs = (int)(rc.size() / 2);
+    // the entries are fetched and copied back first; work enqueued by
+    // `overlap` (dense: packing the low-precision operator) runs behind
+    // the copy while the host waits only for the copy
+    DevBuf<int> drc;
+    DevBuf<double> dv;
+    double *hv = nullptr;
+    hipEvent_t ready = nullptr;
     if (npairs > 0) {
-        DevBuf<int> drc;
-        DevBuf<double> dv;
         IPT_STCHK(drc.alloc(rc.size()));
         IPT_STCHK(dv.alloc(rc.size()));
+        IPT_STCHK(context_host_pinned(ctx, rc.size() * sizeof(double), (void **)&hv));
         IPT_HIPCHK(hipMemcpyAsync(drc.p, rc.data(), rc.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         IPT_STCHK(gather(drc.p, npairs, dv.p));
-        std::vector<double> hv(rc.size());
-        IPT_STCHK(read_host(ctx, hv.data(), dv.p, hv.size() * sizeof(double)));
+        IPT_HIPCHK(hipMemcpyAsync(hv, dv.p, rc.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        IPT_HIPCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        const hipError_t er = hipEventRecord(ready, ctx->stream);
+        if (er != hipSuccess) {
+            (void)hipEventDestroy(ready);
+            return IPT_CUDA_CUDA_ERROR;
+        }
+    }
+    if (overlap) {
+        const int st = overlap();
+        if (st != IPT_CUDA_SUCCESS) {
+            if (ready != nullptr) {
+                (void)hipEventSynchronize(ready);
+                (void)hipEventDestroy(ready);
+            }
+            return st;
+        }
+    }
+    if (npairs > 0) {
+        const hipError_t es = hipEventSynchronize(ready);
+        (void)hipEventDestroy(ready);
+        IPT_HIPCHK(es);
         for (int i = 0; i < npairs; ++i) {
             const int r = rc[(size_t)2 * i], c = rc[(size_t)2 * i + 1];
             wc.lut.emplace_back((long)r * n + c, hv[(size_t)2 * i]);
@@ -962,13 +989,14 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
     for (double x : soff) {
         pr->max_offdiag = std::max(pr->max_offdiag, x);
     }
-    if (early) {
-        IPT_STCHK(early(*pr));
-    }
     GatherFn gather = [&](const int *drc, int npairs, double *dv) {
         return gather_pairs(ctx, pr->H, n, drc, npairs, dv);
     };
-    return detect_clusters(ctx, sd, soff, n, k, gather, &pr->ranges);
+    std::function<int()> overlap;
+    if (early) {
+        overlap = [&]() { return early(*pr); };
+    }
+    return detect_clusters(ctx, sd, soff, n, k, gather, &pr->ranges, overlap);
 }
 
 }  // namespace

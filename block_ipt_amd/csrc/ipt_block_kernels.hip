@@ -324,61 +324,77 @@ __global__ __launch_bounds__(256) void k_block_update_w1(
 // Per-cluster fixed-point residual fp = ||B'-B||_F / max(1, ||B||_F)
 // (IPT/src/ipt_cuda.cu:2984-3003); clusters reaching fp <= tol are frozen.
 // One block per cluster, fixed-order reduction of the update's partials.
-// flags[1] |= 1 on a non-finite norm.
+// flags[1] |= 1 on a non-finite norm.  The last block to finish (ticket in
+// flags[3]) also sets flags[1] |= 2 when a solve was singular (flags[0],
+// then cleared) and flags[2] = clusters still active, and resets the
+// ticket.
 __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict__ partials, int pstride,
                                                         const BlockCluster *__restrict__ clusters,
                                                         int *__restrict__ active, double *__restrict__ fp_out,
-                                                        int *__restrict__ iters, double tol,
+                                                        int *__restrict__ iters, double tol, int nclusters,
                                                         int *__restrict__ flags)
 {
     const int c = blockIdx.x;
-    if (!active[c]) {
-        return;
-    }
-    const int nb = clusters[c].nblocks;
-    double dn = 0.0, bn = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        dn += partials[2 * ((long)c * pstride + b)];
-        bn += partials[2 * ((long)c * pstride + b) + 1];
-    }
     __shared__ double sa[4], sb[4];
-    for (int o = 32; o > 0; o >>= 1) {
-        dn += __shfl_down(dn, o, 64);
-        bn += __shfl_down(bn, o, 64);
+    __shared__ int wl[4];
+    __shared__ int last;
+    if (active[c]) {
+        const int nb = clusters[c].nblocks;
+        double dn = 0.0, bn = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            dn += partials[2 * ((long)c * pstride + b)];
+            bn += partials[2 * ((long)c * pstride + b) + 1];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            dn += __shfl_down(dn, o, 64);
+            bn += __shfl_down(bn, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            sa[threadIdx.x >> 6] = dn;
+            sb[threadIdx.x >> 6] = bn;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            dn = sqrt(sa[0] + sa[1] + sa[2] + sa[3]);
+            bn = sqrt(sb[0] + sb[1] + sb[2] + sb[3]);
+            if (!isfinite(dn) || !isfinite(bn) || bn <= 0.0) {
+                atomicOr(&flags[1], 1);
+            }
+            const double fp = dn / fmax(1.0, bn);
+            fp_out[c] = fp;
+            iters[c] += 1;
+            if (fp <= tol) {
+                active[c] = 0;
+            }
+        }
     }
-    if ((threadIdx.x & 63) == 0) {
-        sa[threadIdx.x >> 6] = dn;
-        sb[threadIdx.x >> 6] = bn;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&flags[3], 1) == nclusters - 1;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        dn = sqrt(sa[0] + sa[1] + sa[2] + sa[3]);
-        bn = sqrt(sb[0] + sb[1] + sb[2] + sb[3]);
-        if (!isfinite(dn) || !isfinite(bn) || bn <= 0.0) {
-            atomicOr(&flags[1], 1);
+    if (last) {
+        __threadfence();
+        int live = 0;
+        for (int q = threadIdx.x; q < nclusters; q += blockDim.x) {
+            live += __atomic_load_n(&active[q], __ATOMIC_RELAXED) != 0;
         }
-        const double fp = dn / fmax(1.0, bn);
-        fp_out[c] = fp;
-        iters[c] += 1;
-        if (fp <= tol) {
-            active[c] = 0;
+        for (int o = 32; o > 0; o >>= 1) {
+            live += __shfl_down(live, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            wl[threadIdx.x >> 6] = live;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (__atomic_load_n(&flags[0], __ATOMIC_RELAXED)) {
+                atomicOr(&flags[1], 2);
+            }
+            flags[0] = 0;
+            flags[2] = wl[0] + wl[1] + wl[2] + wl[3];
+            flags[3] = 0;
         }
     }
-}
-
-// flags[1] |= 2 when a solve was singular (flags[0], then cleared);
-// flags[2] = clusters still active.  Single thread.
-__global__ void k_block_flags(const int *__restrict__ active, int nclusters, int *__restrict__ flags)
-{
-    if (flags[0]) {
-        flags[1] |= 2;
-    }
-    flags[0] = 0;
-    int live = 0;
-    for (int c = 0; c < nclusters; ++c) {
-        live += active[c] != 0;
-    }
-    flags[2] = live;
 }
 
 __global__ void k_block_reset(int *__restrict__ active, int *__restrict__ iters, double *__restrict__ fp,
@@ -393,6 +409,7 @@ __global__ void k_block_reset(int *__restrict__ active, int *__restrict__ iters,
         flags[0] = 0;
         flags[1] = 0;
         flags[2] = nclusters;
+        flags[3] = 0;
     }
 }
 
@@ -543,9 +560,7 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
     profile_end(ctx, prof);
     hipLaunchKernelGGL(k_block_finalize, dim3(plan.nclusters), dim3(256), 0, ctx->stream, partials,
                        plan.partial_stride, plan.clusters.p, plan.active.p, plan.fp.p, plan.iters.p, tol,
-                       plan.flags.p);
-    IPT_HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_block_flags, dim3(1), dim3(1), 0, ctx->stream, plan.active.p, plan.nclusters, plan.flags.p);
+                       plan.nclusters, plan.flags.p);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }
@@ -843,79 +858,120 @@ namespace ipt {
 //  * symmetry: |H_rc - H_cr| <= 1e-12 max(1,|H_rc|,|H_cr|) (IPT/src/ipt_preparation.cu:860-883)
 //  * per-column partial abs sums over each 64-row block (for ||H||_inf of
 //    a symmetric matrix) written to colsum_part[rowblock * n + col]
-//  * per-column max |offdiag| via atomicMax on the IEEE bits (order free)
+//  * per-column partial max |offdiag| over each 64-row block, colmax_part
 __global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H, long ld, int n, int nb,
                                                     double *__restrict__ colsum_part,
-                                                    unsigned long long *__restrict__ colmax_bits,
-                                                    int *__restrict__ asym)
+                                                    double *__restrict__ colmax_part, int *__restrict__ asym)
 {
     __shared__ double tt[64][65];
-    // decode blockIdx.x -> (bi, bj), bi <= bj, row-major over the upper triangle
-    long idx = blockIdx.x;
-    int bi = 0;
-    while (idx >= nb - bi) {
-        idx -= nb - bi;
+    // blockIdx.x -> (bi, bj), bi <= bj, row-major over the upper triangle of
+    // tile pairs: row bi starts at bi*nb - bi*(bi-1)/2
+    const long idx = blockIdx.x;
+    const double b2 = 2.0 * nb + 1.0;
+    int bi = (int)((b2 - sqrt(b2 * b2 - 8.0 * (double)idx)) * 0.5);
+    bi = max(0, min(bi, nb - 1));
+    while (bi > 0 && (long)bi * nb - (long)bi * (bi - 1) / 2 > idx) {
+        --bi;
+    }
+    while ((long)(bi + 1) * nb - (long)(bi + 1) * bi / 2 <= idx) {
         ++bi;
     }
-    const int bj = bi + (int)idx;
-    const int r = threadIdx.x & 63;
-    const int cg = threadIdx.x >> 6;
+    const int bj = bi + (int)(idx - ((long)bi * nb - (long)bi * (bi - 1) / 2));
+    // thread -> column c of a tile, rows 16q .. 16q+15 (one 128-byte segment)
+    const int c = threadIdx.x >> 2;
+    const int q = threadIdx.x & 3;
     const long R0 = (long)bi * 64, C0 = (long)bj * 64;
     int bad = 0;
-    // T' = H[bj rows, bi cols] -> LDS, with its column stats
-    for (int q = 0; q < 16; ++q) {
-        const int c = cg * 16 + q;
-        const long gr = C0 + r, gc = R0 + c;
-        const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
-        tt[r][c] = v;
-        if (bi != bj) {
-            double s = fabs(v);
-            double mx = (gr != gc) ? fabs(v) : 0.0;
-            for (int o = 32; o > 0; o >>= 1) {
-                s += __shfl_down(s, o, 64);
-                mx = fmax(mx, __shfl_down(mx, o, 64));
-            }
-            if (r == 0 && gc < n) {
-                colsum_part[(long)bj * n + gc] = s;
-                atomicMax(&colmax_bits[gc], (unsigned long long)__double_as_longlong(mx));
-            }
+    if (bi != bj) {
+        // T' = H[bj rows, bi cols]: stats of columns R0 + c, kept in LDS
+        const long gc = R0 + c;
+        double s = 0.0, mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = 16 * q + i;
+            const long gr = C0 + r;
+            const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+            tt[r][c] = v;
+            s += fabs(v);
+            mx = fmax(mx, fabs(v));  // rows of block bj never meet column gc's diagonal
+        }
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        mx = fmax(mx, __shfl_xor(mx, 1, 64));
+        mx = fmax(mx, __shfl_xor(mx, 2, 64));
+        if (q == 0 && gc < n) {
+            colsum_part[(long)bj * n + gc] = s;
+            colmax_part[(long)bj * n + gc] = mx;
         }
     }
     __syncthreads();
-    // T = H[bi rows, bj cols]: compare with T'^T, column stats
-    for (int q = 0; q < 16; ++q) {
-        const int c = cg * 16 + q;
-        const long gr = R0 + r, gc = C0 + c;
-        const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
-        const double w = tt[c][r];
-        const double scale = fmax(1.0, fmax(fabs(v), fabs(w)));
-        if (fabs(v - w) > 1e-12 * scale) {
-            bad = 1;
+    // T = H[bi rows, bj cols]: compare with T'^T, stats of columns C0 + c
+    {
+        const long gc = C0 + c;
+        double s = 0.0, mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = 16 * q + i;
+            const long gr = R0 + r;
+            const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+            if (bi != bj) {
+                const double w = tt[c][r];
+                const double scale = fmax(1.0, fmax(fabs(v), fabs(w)));
+                if (fabs(v - w) > 1e-12 * scale) {
+                    bad = 1;
+                }
+            }
+            s += fabs(v);
+            if (gr != gc) {
+                mx = fmax(mx, fabs(v));
+            }
         }
-        double s = fabs(v);
-        double mx = (gr != gc) ? fabs(v) : 0.0;
-        for (int o = 32; o > 0; o >>= 1) {
-            s += __shfl_down(s, o, 64);
-            mx = fmax(mx, __shfl_down(mx, o, 64));
+        if (bi == bj) {
+            // diagonal tile: symmetry within the tile through LDS
+            for (int i = 0; i < 16; ++i) {
+                const int r = 16 * q + i;
+                const long gr = R0 + r;
+                tt[r][c] = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+            }
+            __syncthreads();
+            for (int i = 0; i < 16; ++i) {
+                const int r = 16 * q + i;
+                const double v = tt[r][c], w = tt[c][r];
+                const double scale = fmax(1.0, fmax(fabs(v), fabs(w)));
+                if (fabs(v - w) > 1e-12 * scale) {
+                    bad = 1;
+                }
+            }
         }
-        if (r == 0 && gc < n) {
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        mx = fmax(mx, __shfl_xor(mx, 1, 64));
+        mx = fmax(mx, __shfl_xor(mx, 2, 64));
+        if (q == 0 && gc < n) {
             colsum_part[(long)bi * n + gc] = s;
-            atomicMax(&colmax_bits[gc], (unsigned long long)__double_as_longlong(mx));
+            colmax_part[(long)bi * n + gc] = mx;
         }
     }
-    if (bad) {
+    for (int o = 32; o > 0; o >>= 1) {
+        bad |= __shfl_xor(bad, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && bad) {
         atomicOr(asym, 1);
     }
 }
 
-__global__ void k_colsum_reduce(const double *__restrict__ part, int nb, int n, double *__restrict__ colsum)
+// colsum[c] / colmax[c]: fixed-order reductions over the nb row-block partials
+__global__ void k_colsum_reduce(const double *__restrict__ part, const double *__restrict__ mpart, int nb, int n,
+                                double *__restrict__ colsum, double *__restrict__ colmax)
 {
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
-        double s = 0.0;
+        double s = 0.0, m = 0.0;
         for (int b = 0; b < nb; ++b) {
             s += part[(long)b * n + c];
+            m = fmax(m, mpart[(long)b * n + c]);
         }
         colsum[c] = s;
+        colmax[c] = m;
     }
 }
 
@@ -923,15 +979,14 @@ int dense_scan(Context *ctx, const double *H, long ld, int n, double *colsum, do
 {
     const int nb = (n + 63) / 64;
     double *part = nullptr;
-    IPT_STCHK(context_scratch(ctx, SLOT_SOLVER, (size_t)nb * n * sizeof(double), (void **)&part));
-    IPT_HIPCHK(hipMemsetAsync(colmax, 0, (size_t)n * sizeof(double), ctx->stream));
+    IPT_STCHK(context_scratch(ctx, SLOT_SOLVER, (size_t)2 * nb * n * sizeof(double), (void **)&part));
     IPT_HIPCHK(hipMemsetAsync(asym, 0, sizeof(int), ctx->stream));
     const long pairs = (long)nb * (nb + 1) / 2;
     hipLaunchKernelGGL(k_dense_scan, dim3((unsigned)pairs), dim3(256), 0, ctx->stream, H, ld, n, nb, part,
-                       (unsigned long long *)colmax, asym);
+                       part + (size_t)nb * n, asym);
     IPT_HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_colsum_reduce, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, ctx->stream, part, nb, n,
-                       colsum);
+    hipLaunchKernelGGL(k_colsum_reduce, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, ctx->stream, part,
+                       part + (size_t)nb * n, nb, n, colsum, colmax);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

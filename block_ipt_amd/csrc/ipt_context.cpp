@@ -8,6 +8,8 @@
 #include <mutex>
 #include <unordered_map>
 
+#include <algorithm>
+
 #include "ipt_internal.h"
 
 namespace ipt {
@@ -180,6 +182,9 @@ void context_destroy(Context *ctx)
     if (ctx->host_flags != nullptr) {
         (void)hipHostFree(ctx->host_flags);
     }
+    if (ctx->host_pinned != nullptr) {
+        (void)hipHostFree(ctx->host_pinned);
+    }
     if (ctx->own_stream && ctx->stream != nullptr) {
         (void)hipStreamDestroy(ctx->stream);
     }
@@ -229,6 +234,23 @@ int context_blas(Context *ctx, void **blas_out)
         ctx->blas = h;
     }
     *blas_out = ctx->blas;
+    return IPT_CUDA_SUCCESS;
+}
+
+int context_host_pinned(Context *ctx, size_t bytes, void **out)
+{
+    if (bytes > ctx->host_pinned_bytes) {
+        if (ctx->host_pinned != nullptr) {
+            IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+            IPT_HIPCHK(hipHostFree(ctx->host_pinned));
+            ctx->host_pinned = nullptr;
+            ctx->host_pinned_bytes = 0;
+        }
+        const size_t want = std::max(bytes, (size_t)1 << 20);
+        IPT_HIPCHK(hipHostMalloc(&ctx->host_pinned, want, hipHostMallocDefault));
+        ctx->host_pinned_bytes = want;
+    }
+    *out = ctx->host_pinned;
     return IPT_CUDA_SUCCESS;
 }
 

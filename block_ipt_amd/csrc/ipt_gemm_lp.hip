@@ -270,11 +270,11 @@ int lowp_operator_create(Context *ctx, const double *H, long ld, int n, int mode
 
 // ------------------------------------------------------------------ delta
 
-// colmax[c] = max_r |B - Ba| (Ba NULL: zero): blockIdx.y = column,
-// blockIdx.x = row slice; one atomicMax on the IEEE bits per block (order
-// free: the maximum does not depend on it).
+// colmax_part[b * nc + c] = max over row slice b of |B - Ba| (Ba NULL:
+// zero): blockIdx.y = column, blockIdx.x = row slice b.  The converter
+// takes the maximum over the slices.
 __global__ __launch_bounds__(256) void k_delta_colmax(const double *__restrict__ B, const double *__restrict__ Ba,
-                                                      long ld, int n, int nc, unsigned long long *__restrict__ colmax)
+                                                      long ld, int n, int nc, double *__restrict__ colmax_part)
 {
     const int c = blockIdx.y;
     double m = 0.0;
@@ -290,10 +290,8 @@ __global__ __launch_bounds__(256) void k_delta_colmax(const double *__restrict__
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        m = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
-        if (m > 0.0) {
-            atomicMax(&colmax[c], (unsigned long long)__double_as_longlong(m));
-        }
+        // one partial per block: no zero-initialised accumulator needed
+        colmax_part[(long)blockIdx.x * nc + c] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
     }
 }
 
@@ -305,7 +303,7 @@ __global__ __launch_bounds__(256) void k_delta_colmax(const double *__restrict__
 // NULL (zero anchor).
 __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict__ B, const double *__restrict__ Ba,
                                                        long ld, int n, int nc, int nkb, int nb, int mode,
-                                                       const unsigned long long *__restrict__ colmax,
+                                                       const double *__restrict__ colmax_part, int nparts,
                                                        double *__restrict__ colscale, uint4 *__restrict__ zhi,
                                                        uint4 *__restrict__ zlo)
 {
@@ -317,7 +315,10 @@ __global__ __launch_bounds__(256) void k_delta_convert(const double *__restrict_
         const int c = cb * 16 + (l & 15);
         double s = 1.0;
         if (c < nc) {
-            const double mx = __longlong_as_double((long long)colmax[c]);
+            double mx = 0.0;
+            for (int q = 0; q < nparts; ++q) {
+                mx = fmax(mx, colmax_part[(long)q * nc + c]);
+            }
             if (mx > 0.0) {
                 int e = 0;
                 (void)frexp(mx, &e);  // mx = f * 2^e, f in [0.5,1)
@@ -495,16 +496,18 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         const int groups = std::max(1, std::min(op.nrb, (256 + splits - 1) / splits));
         const int rbw = (op.nrb + groups - 1) / groups;
         const int gx = (op.nrb + rbw - 1) / rbw;
-        // scratch: colmax (w u64) | colscale (w) | Z hi/lo (fragment-major) | partials
+        // scratch: colmax partials (nparts x w) | colscale (w) | Z hi/lo
+        // (fragment-major) | partial sums
+        const int nparts = std::min(8, (n + 255) / 256);
         const size_t zvec = (size_t)op.nkb * NB * 64;  // uint4 per plane
-        const size_t zoff = 2 * (size_t)w * 8;
+        const size_t zoff = (size_t)(nparts + 1) * w * 8;
         const size_t zoff_al = (zoff + 255) / 256 * 256;
         const size_t poff = zoff_al + 2 * zvec * 16;
         const size_t bytes = poff + (size_t)splits * op.nrb * NB * 1024 + 256;
         char *base = nullptr;
         IPT_STCHK(context_scratch(ctx, SLOT_LOWP, bytes, (void **)&base));
-        unsigned long long *colmax = reinterpret_cast<unsigned long long *>(base);
-        double *colscale = reinterpret_cast<double *>(base + (size_t)w * 8);
+        double *colmax = reinterpret_cast<double *>(base);
+        double *colscale = reinterpret_cast<double *>(base + (size_t)nparts * w * 8);
         uint4 *zhi = reinterpret_cast<uint4 *>(base + zoff_al);
         uint4 *zlo = zhi + zvec;
         float *P = reinterpret_cast<float *>(base + poff);
@@ -512,13 +515,11 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         // a NULL anchor stays NULL in every column chunk (zero anchor)
         const double *Bac = Ba != nullptr ? Ba + (size_t)c0 * ld : nullptr;
         const double *Yac = Ya != nullptr ? Ya + (size_t)c0 * ld : nullptr;
-        IPT_HIPCHK(hipMemsetAsync(colmax, 0, (size_t)w * 8, ctx->stream));
-        hipLaunchKernelGGL(k_delta_colmax, dim3(std::min(8, (n + 255) / 256), w), dim3(256), 0, ctx->stream, Bc, Bac,
-                           ld, n, w, colmax);
+        hipLaunchKernelGGL(k_delta_colmax, dim3(nparts, w), dim3(256), 0, ctx->stream, Bc, Bac, ld, n, w, colmax);
         IPT_HIPCHK(hipGetLastError());
         const size_t zconv = op.mode == LOWP_FP8 ? zvec / 2 : zvec;  // FP8: one entry per k-block pair
         hipLaunchKernelGGL(k_delta_convert, dim3(blocks_for((long)zconv, 256, 16384)), dim3(256), 0, ctx->stream, Bc,
-                           Bac, ld, n, w, op.nkb, NB, op.mode, colmax, colscale, zhi, zlo);
+                           Bac, ld, n, w, op.nkb, NB, op.mode, colmax, nparts, colscale, zhi, zlo);
         IPT_HIPCHK(hipGetLastError());
         dim3 grid(gx, splits);
         const uint4 *Ehi = reinterpret_cast<const uint4 *>(op.hi.p);

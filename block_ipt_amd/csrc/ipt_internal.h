@@ -42,6 +42,8 @@ struct Context {
     void *blas = nullptr;        // rocblas_handle, created on first use
     double *host_scalars = nullptr;  // pinned, 4096 doubles
     int *host_flags = nullptr;       // pinned, 4096 ints
+    void *host_pinned = nullptr;     // pinned staging for device-to-host reads, grows
+    size_t host_pinned_bytes = 0;
     void *slots[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // statistics of the last block-cluster solve
     int last_basis_cols = 0;
@@ -70,6 +72,9 @@ enum {
     SLOT_MISC = 7
 };
 int context_scratch(Context *ctx, int slot, size_t bytes, void **out);
+// Pinned host staging buffer of at least `bytes` (grows; freed with the
+// context), so that device-to-host reads can run asynchronously.
+int context_host_pinned(Context *ctx, size_t bytes, void **out);
 
 // NULL handle -> library context of the current device.
 Context *context_from_handle(void *handle, int *status);

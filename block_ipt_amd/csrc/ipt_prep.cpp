@@ -66,8 +66,11 @@ void stable_sort_permutation(const std::vector<double> &diag, std::vector<int> *
     perm->resize((size_t)n);
     inverse->resize((size_t)n);
     std::iota(perm->begin(), perm->end(), 0);
-    std::stable_sort(perm->begin(), perm->end(),
-                     [&](int a, int b) { return diag[(size_t)a] < diag[(size_t)b]; });
+    // already sorted (the common case): identity, no sort
+    if (!std::is_sorted(diag.begin(), diag.end())) {
+        std::stable_sort(perm->begin(), perm->end(),
+                         [&](int a, int b) { return diag[(size_t)a] < diag[(size_t)b]; });
+    }
     for (int s = 0; s < n; ++s) {
         (*inverse)[(size_t)(*perm)[(size_t)s]] = s;
     }
