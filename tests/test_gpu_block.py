@@ -116,6 +116,19 @@ def test_device_entry_block_path(block_env):
     assert fp <= 1e-12
 
 
+@pytest.mark.parametrize("n,hi,k,seed", [(1024, 8.0, 48, 1), (1024, 6.0, 40, 4)])
+def test_wide_cluster_class(block_env, n, hi, k, seed):
+    """A 49- / 46-column cluster (the 33..64 lane class with its own launch
+    and a separate finalize): same pairs as LAPACK and the oracle, same
+    iteration count as the oracle."""
+    H = orc.rand_dense(n, seed, 0.0, hi, 0.1)
+    gv, gl, info = api.ipt_cuda_tol(H, k, 1e-12, 400)
+    _check_eigs(H, gv, gl, k)
+    ov, ovec, oinfo = orc.block_pipeline(k, dense=H, maxiter=400, tol=1e-12)
+    assert np.abs(gl - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
+    assert info["iterations"] == oinfo["max_iterations"]
+
+
 def test_too_large_cluster_rejected(block_env):
     H = orc.rand_dense(2048, 1, 0.0, 6.25, 0.1)
     with pytest.raises(IptError) as e:
