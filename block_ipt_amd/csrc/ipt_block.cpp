@@ -306,8 +306,9 @@ double max_of(const std::vector<double> &v, int *idx)
 // corrections for every pair above conv_tol, thick restart to the k Ritz
 // vectors plus the last correction block, trial accepted when finite.
 // With `mlp` (device parameters; rowabs = off-diagonal row sums of |H|)
-// the corrections come from the learned preconditioner, and a trial step
-// that does not lower the largest residual is redone with the diagonal one.
+// the corrections come from the learned preconditioner; a trial step that
+// does not lower the largest residual is redone with the diagonal one, which
+// also serves the next two steps.
 // V/HV: n x cap buffers holding the search space (first *cols columns).
 int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *rowabs, const float *mlp, int n, int k,
              double normH, double *V, double *HV, int *cols_io, int cap, double *theta, double *X, double *HX,
@@ -336,6 +337,7 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
     IPT_STCHK(hrecent.alloc((size_t)n * k));
     IPT_STCHK(idx.alloc((size_t)k));
     int recent_count = 0;
+    int diag_hold = 0;  // steps left on the diagonal correction after a rejected learned step
 
     std::vector<double> htheta((size_t)k), hres((size_t)k);
     IPT_STCHK(read_host(ctx, htheta.data(), theta, (size_t)k * sizeof(double)));
@@ -435,15 +437,20 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
             return IPT_CUDA_SUCCESS;
         };
         const double before = max_of(hres, nullptr);
-        bool learned = mlp != nullptr;
+        bool learned = mlp != nullptr && diag_hold == 0;
+        if (diag_hold > 0) {
+            --diag_hold;
+        }
         IPT_STCHK(trial(learned));
         if (learned) {
             // the diagonal preconditioner is the judge: a learned step that
-            // does not lower the largest residual is redone with it
+            // does not lower the largest residual is redone with it, and the
+            // next two steps use the diagonal correction directly
             const double after = kept > 0 ? max_of(tres, nullptr) : INFINITY;
             if (!(after < before)) {
                 ++st->mlp_rejected;
                 learned = false;
+                diag_hold = 2;
                 IPT_STCHK(trial(false));
             } else {
                 ++st->mlp_accepted;

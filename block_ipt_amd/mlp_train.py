@@ -1,9 +1,10 @@
 """Train the learned Davidson preconditioner and measure it against the
 diagonal one.
 
-Data: for each training seed, a dense matrix of the N=16384 class
-(d ~ U[0,50] sorted, symmetric N(0,1) * 0.1/sqrt(N) off-diagonal, built by
-the library's device generator), its exact lowest pairs from the library's
+Data: for each training seed, a matrix of the class (dense, default: the
+N=16384 class d ~ U[0,50] sorted with symmetric N(0,1) * 0.1/sqrt(N)
+off-diagonal; or, with --matrix sparse, the orbital-sum CSR model), built by
+the library's device generators, its exact lowest pairs from the library's
 block solver, and the states of a Davidson run from the identity start
 (the library's restart rule and residual scaling, re-expressed with torch
 on the device).  At every state and for every unconverged target j the
@@ -22,9 +23,12 @@ Davidson steps and single-column products to reach the residual
 tolerance, learned vs diagonal.
 
     python -m block_ipt_amd.mlp_train --out block_ipt_amd/davidson_mlp.bin
+    python -m block_ipt_amd.mlp_train --matrix sparse --n 200000 --k 32 --test-n 1000000 \
+        --out block_ipt_amd/davidson_mlp_sparse.bin
 
 The script needs a GPU; torch provides device memory and the dense algebra
-of the data generation only.
+of the data generation only (sparse products go through the library's CSR
+product).
 """
 import argparse
 import json
@@ -53,38 +57,109 @@ def ptr(t):
     return t.data_ptr()
 
 
-class Problem:
-    """One matrix of the class, its diagonal, off-diagonal row sums and its
-    exact lowest pairs."""
+def _reference_env():
+    # reference pairs: block solve with Davidson refinement to 1e-12
+    os.environ.update({"IPT_BLOCK_CLUSTER": "1", "IPT_DAVIDSON_ENRICH": "1", "IPT_DAVIDSON_STEPS": "200",
+                       "IPT_DAVIDSON_CONVERGED_TOL": "1e-12"})
+    for v in ("IPT_PRECISION", "IPT_BLOCK_CLUSTER_MAXITER", "IPT_DAVIDSON_PRECOND"):
+        os.environ.pop(v, None)
 
-    def __init__(self, n, k, seed, vscale, extra=16):
-        self.n, self.k = n, k
+
+def _clear_reference_env():
+    for v in ("IPT_DAVIDSON_ENRICH", "IPT_DAVIDSON_STEPS", "IPT_DAVIDSON_CONVERGED_TOL"):
+        os.environ.pop(v, None)
+
+
+class DenseMatrix:
+    """Dense class: diag(sorted U[0,50]) + symmetric N(0,1) vscale/sqrt(N)."""
+
+    def __init__(self, n, seed, vscale):
+        self.n = n
         self.H = torch.empty((n, n), dtype=torch.float64, device="cuda")  # column-major == H^T == H
         sync()
         api.generate_dense(ptr(self.H), n, seed, 0.0, 50.0, vscale)
         self.d = torch.diagonal(self.H).clone()
         self.s = self.H.abs().sum(0) - self.d.abs()
         self.normH = float(self.H.abs().sum(0).max())
-        # reference pairs: block solve with Davidson refinement to 1e-12
-        os.environ.update({"IPT_BLOCK_CLUSTER": "1", "IPT_DAVIDSON_ENRICH": "1", "IPT_DAVIDSON_STEPS": "200",
-                           "IPT_DAVIDSON_CONVERGED_TOL": "1e-12"})
-        for v in ("IPT_PRECISION", "IPT_BLOCK_CLUSTER_MAXITER", "IPT_DAVIDSON_PRECOND"):
-            os.environ.pop(v, None)
+
+    def apply(self, X):
+        return self.H @ X
+
+    def solve(self, k, maxiter):
+        """library block solve on the device: (vectors n x k column-major, values)"""
         sync()
+        dv, dl, _, _ = api.ipt_cuda_device_tol(ptr(self.H), self.n, k, 1e-12, maxiter)
+        return dv, dl
+
+
+class SparseMatrix:
+    """Sparse orbital-sum class (40 orbital energies U[-20,5], diagonal = sums
+    of 8, m drawn columns per row with N(0, 0.02^2) values, mirrored)."""
+
+    def __init__(self, n, seed, m):
+        self.n = n
+        cap = n * (2 * m + 1)
+        self.rp = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+        ci = torch.empty(cap, dtype=torch.int32, device="cuda")
+        vv = torch.empty(cap, dtype=torch.float64, device="cuda")
+        sync()
+        self.nnz = api.generate_orbital_csr(n, m, seed, ptr(self.rp), ptr(ci), ptr(vv), cap)
+        self.ci = ci[:self.nnz].clone()
+        self.v = vv[:self.nnz].clone()
+        del ci, vv
+        rows = torch.repeat_interleave(torch.arange(n, device="cuda"), (self.rp[1:] - self.rp[:-1]).long())
+        absum = torch.zeros(n, dtype=torch.float64, device="cuda").index_add_(0, rows, self.v.abs())
+        self.d = torch.zeros(n, dtype=torch.float64, device="cuda")
+        diag = rows == self.ci.long()
+        self.d[rows[diag]] = self.v[diag]
+        self.s = absum - self.d.abs()
+        self.normH = float(absum.max())
+
+    def apply(self, X):
+        X = X.contiguous()  # row-major n x c: the layout of the CSR product
+        Y = torch.empty_like(X)
+        sync()
+        api.csr_spmm(ptr(self.rp), ptr(self.ci), ptr(self.v), self.n, ptr(X), X.shape[1], ptr(Y), X.shape[1],
+                     X.shape[1])
+        return Y
+
+    def solve(self, k, maxiter):
+        sync()
+        dv, dl, _, _ = api.ipt_cuda_sparse_csc_device_tol(ptr(self.rp), ptr(self.ci), ptr(self.v), self.n, k,
+                                                          self.nnz, 1e-12, maxiter)
+        return dv, dl
+
+
+def make_matrix(args, seed):
+    if args.matrix == "sparse":
+        return SparseMatrix(args.n, seed, args.draws)
+    return DenseMatrix(args.n, seed, args.vscale)
+
+
+class Problem:
+    """One matrix of the class with its exact lowest pairs."""
+
+    def __init__(self, args, k, seed, extra=16):
+        self.mat = make_matrix(args, seed)
+        self.n, self.k = self.mat.n, k
+        self.d, self.s, self.normH = self.mat.d, self.mat.s, self.mat.normH
+        _reference_env()
         ke = k + extra
-        dv, dl, _, _ = api.ipt_cuda_device_tol(ptr(self.H), n, ke, 1e-12, 400)
+        dv, dl = self.mat.solve(ke, 400)
         hip = api.hip()
-        U = hip.to_host(dv, (n, ke))
+        U = hip.to_host(dv, (self.n, ke))
         lam = hip.to_host(dl, (ke,))
         api.free_device_result(dv, dl)
+        _clear_reference_env()
         self.U = torch.tensor(U, device="cuda")
         # the reference pairs used for matching must be eigenpairs (scaled
         # residual); the last few of the k + extra may be less accurate
         lt = torch.tensor(lam, device="cuda")
-        res = torch.linalg.norm(self.H @ self.U - self.U * lt, dim=0) / max(1.0, self.normH)
+        res = torch.linalg.norm(self.mat.apply(self.U) - self.U * lt, dim=0) / max(1.0, self.normH)
         self.ok = bool(res[:k + extra // 2].max() <= 1e-9)
-        for v in ("IPT_DAVIDSON_ENRICH", "IPT_DAVIDSON_STEPS", "IPT_DAVIDSON_CONVERGED_TOL"):
-            os.environ.pop(v, None)
+
+    def apply(self, X):
+        return self.mat.apply(X)
 
 
 def rayleigh_ritz(V, HV, k):
@@ -165,17 +240,18 @@ def davidson_states(pb, k, net, conv_tol, max_steps):
     """Davidson from the identity start, the library's rules; returns the
     recorded (R, theta, X, cand) states and the step / trial counts."""
     n = pb.n
-    H, d, s = pb.H, pb.d, pb.s
+    d, s = pb.d, pb.s
     scale = max(1.0, pb.normH)
     X0 = torch.zeros((n, k), dtype=torch.float64, device="cuda")
     X0[torch.arange(k), torch.arange(k)] = 1.0
     V = X0
-    HV = H @ V
+    HV = pb.apply(V)
     th, X, HX = rayleigh_ritz(V, HV, k)
     kmax = min(n, k + max(16, (2 * k + 2) // 3))
     retain = max(0, kmax - k - 1)
     clip = np.finfo(float).eps ** (2.0 / 3.0) * max(scale, float(th.abs().max()))
     recent = None
+    hold = 0
     states, trials, steps = [], 0, 0
 
     def resid(X, HX, th):
@@ -193,7 +269,7 @@ def davidson_states(pb, k, net, conv_tol, max_steps):
                 Q = orth_against(V, recent[:, :retain])
                 if Q.shape[1] > 0:
                     V = torch.cat([V, Q], 1)
-                    HV = torch.cat([HV, H @ Q], 1)
+                    HV = torch.cat([HV, pb.apply(Q)], 1)
         lim = min(k, max(0, kmax - V.shape[1]))
         cand = cand[:lim]
         if cand.numel() == 0:
@@ -201,7 +277,9 @@ def davidson_states(pb, k, net, conv_tol, max_steps):
         states.append((R[:, cand].clone(), th[cand].clone(), X[:, cand].clone(), X.clone()))
         before = float(res.max())
         accepted = None
-        for learned in ([True, False] if net is not None else [False]):
+        use_net = net is not None and hold == 0
+        hold = max(0, hold - 1)
+        for learned in ([True, False] if use_net else [False]):
             if learned:
                 T = net.correction(R[:, cand], d, s, th[cand])
             else:
@@ -211,12 +289,13 @@ def davidson_states(pb, k, net, conv_tol, max_steps):
             if Q.shape[1] == 0:
                 continue
             V2 = torch.cat([V, Q], 1)
-            HV2 = torch.cat([HV, H @ Q], 1)
+            HV2 = torch.cat([HV, pb.apply(Q)], 1)
             th2, X2, HX2 = rayleigh_ritz(V2, HV2, k)
             R2, res2 = resid(X2, HX2, th2)
             if not learned or float(res2.max()) < before:
                 accepted = (Q, V2, HV2, th2, X2, HX2, R2, res2)
                 break
+            hold = 2  # the library's rule: two diagonal steps after a rejection
         if accepted is None:
             break
         recent, V, HV, th, X, HX, R, res = accepted
@@ -226,7 +305,7 @@ def davidson_states(pb, k, net, conv_tol, max_steps):
             Q = orth_against(V, recent[:, :retain]) if retain > 0 else recent[:, :0]
             if Q.shape[1] > 0:
                 V = torch.cat([V, Q], 1)
-                HV = torch.cat([HV, H @ Q], 1)
+                HV = torch.cat([HV, pb.apply(Q)], 1)
     return states, steps, trials, float(res.max())
 
 
@@ -271,9 +350,7 @@ def measure(args, model, seeds):
     step, then Davidson to the tolerance), diagonal vs learned corrections."""
     rows = []
     for seed in seeds:
-        H = torch.empty((args.n, args.n), dtype=torch.float64, device="cuda")
-        sync()
-        api.generate_dense(ptr(H), args.n, seed, 0.0, 50.0, args.vscale)
+        mat = make_matrix(args, seed)
         rec = {"seed": seed}
         for mode in ("diag", "mlp"):
             env = {"IPT_BLOCK_CLUSTER": "1", "IPT_BLOCK_CLUSTER_MAXITER": str(args.ipt_steps),
@@ -286,7 +363,7 @@ def measure(args, model, seeds):
                 os.environ["IPT_DAVIDSON_PRECOND"] = "mlp"
                 os.environ["IPT_DAVIDSON_MLP_MODEL"] = model
             t0 = time.time()
-            dv, dl, _, _ = api.ipt_cuda_device_tol(ptr(H), args.n, args.k, 1e-12, args.ipt_steps)
+            dv, dl = mat.solve(args.k, args.ipt_steps)
             el = time.time() - t0
             api.free_device_result(dv, dl)
             st = api.last_davidson_stats()
@@ -297,7 +374,7 @@ def measure(args, model, seeds):
             os.environ.pop(kk, None)
         log("  seed %d  diag %s  mlp %s" % (seed, rec["diag"], rec["mlp"]))
         rows.append(rec)
-        del H
+        del mat
     tot = {m: {f: sum(r[m][f] for r in rows) for f in ("steps", "trials", "matvecs")} for m in ("diag", "mlp")}
     return rows, tot
 
@@ -308,6 +385,10 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--vscale", type=float, default=0.1)
+    ap.add_argument("--matrix", default="dense", choices=["dense", "sparse"],
+                    help="dense: d ~ U[0,50] + N(0,1) vscale/sqrt(N); sparse: the orbital-sum CSR model")
+    ap.add_argument("--draws", type=int, default=200, help="sparse: drawn columns per row")
+    ap.add_argument("--test-n", type=int, default=0, help="matrix size of the measurement (default: --n)")
     ap.add_argument("--train-seeds", default="1-20")
     ap.add_argument("--test-seeds", default="100-109")
     ap.add_argument("--rounds", type=int, default=2)
@@ -333,7 +414,11 @@ def main(argv=None):
             t0 = time.time()
             parts = []
             for sd in seeds(args.train_seeds):
-                pb = Problem(args.n, args.k, sd, args.vscale)
+                try:
+                    pb = Problem(args, args.k, sd)
+                except api.IptError as e:
+                    log("  round %d seed %d: reference solve failed (%s), seed skipped" % (rnd, sd, e))
+                    continue
                 if not pb.ok:
                     log("  round %d seed %d: reference pairs not converged, seed skipped" % (rnd, sd))
                     continue
@@ -350,6 +435,8 @@ def main(argv=None):
                       args.train_steps, args.batch, args.lr, seed=rnd)
         api.save_mlp_model(args.out, net.p.cpu().numpy())
         log("model written to %s" % args.out)
+    if args.test_n > 0:
+        args.n = args.test_n
     rows, tot = measure(args, args.out, seeds(args.test_seeds))
     red_steps = 1.0 - tot["mlp"]["steps"] / max(1, tot["diag"]["steps"])
     red_trials = 1.0 - tot["mlp"]["trials"] / max(1, tot["diag"]["trials"])
@@ -358,7 +445,8 @@ def main(argv=None):
         % (tot["diag"], tot["mlp"], 100 * red_steps, 100 * red_trials, 100 * red_mv))
     if args.report:
         with open(args.report, "w") as f:
-            json.dump({"n": args.n, "k": args.k, "conv_tol": args.conv_tol, "ipt_steps": args.ipt_steps,
+            json.dump({"matrix": args.matrix, "n": args.n, "k": args.k, "conv_tol": args.conv_tol,
+                       "ipt_steps": args.ipt_steps,
                        "rows": rows, "total": tot, "fewer_steps": red_steps, "fewer_trials": red_trials,
                        "fewer_column_products": red_mv}, f, indent=1)
 

@@ -185,8 +185,8 @@ def dav_env(monkeypatch):
 
 
 def test_zero_model_falls_back_to_diagonal(dav_env, tmp_path):
-    """A model whose corrections are all zero is rejected on every step and
-    the solve is the diagonal-preconditioned one, bit for bit."""
+    """A model whose corrections are all zero is rejected whenever it is
+    tried, and the solve is the diagonal-preconditioned one, bit for bit."""
     n, k = 2048, 16
     H = orc.rand_dense(n, 0, 0.0, 10.0, 0.05)
     vd, ld, _ = api.ipt_cuda_tol(H, k, 1e-12, 3)
@@ -198,7 +198,9 @@ def test_zero_model_falls_back_to_diagonal(dav_env, tmp_path):
     vz, lz, _ = api.ipt_cuda_tol(H, k, 1e-12, 3)
     sz = api.last_davidson_stats()
     assert sd["accepted"] > 0
-    assert sz["mlp_accepted"] == 0 and sz["mlp_rejected"] >= sz["accepted"]
+    # rejected on every learned attempt; two diagonal steps follow each one
+    assert sz["mlp_accepted"] == 0 and sz["mlp_rejected"] >= 1
+    assert sz["mlp_rejected"] >= sz["accepted"] // 3
     assert sz["accepted"] == sd["accepted"]
     assert np.array_equal(lz, ld) and np.array_equal(vz, vd)
 
