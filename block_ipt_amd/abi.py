@@ -155,6 +155,15 @@ class IPTRayleighMixedCudaDeviceResult(_c.Structure):
         ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]
 
 
+class ACXCudaResult(_c.Structure):
+    """Reference IPT/src/ACX/acx_cuda.cu:20-33."""
+    _fields_ = [
+        ("n", _c.c_int), ("k", _c.c_int), ("iterations", _c.c_int), ("f_calls", _c.c_int),
+        ("converged", _c.c_int), ("time_sec", _c.c_double), ("iteration_time_sec", _c.c_double),
+        ("finalization_time_sec", _c.c_double), ("max_residual", _c.c_double),
+        ("fixed_point_residual", _c.c_double), ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]
+
+
 _dp = _c.POINTER(_c.c_double)
 _ip = _c.POINTER(_c.c_int)
 _dpp = _c.POINTER(_dp)
@@ -223,6 +232,12 @@ SIGNATURES = {
                                                            _c.c_double, _c.c_int, _vp,
                                                            _c.POINTER(IPTRayleighMixedCudaDeviceResult)]),
     "ipt_rayleigh_mixed_cuda_free_device_result": (None, [_c.POINTER(IPTRayleighMixedCudaDeviceResult)]),
+    # include/ipt_hip_acx.h
+    "acx_cuda_status_string": (_c.c_char_p, [_c.c_int]),
+    "acx_cuda_reset_result": (None, [_c.POINTER(ACXCudaResult)]),
+    "acx_cuda_free_result": (None, [_c.POINTER(ACXCudaResult)]),
+    "acx_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _vp,
+                                       _c.POINTER(ACXCudaResult)]),
 }
 
 _lib = None

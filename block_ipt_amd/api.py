@@ -270,6 +270,30 @@ def mixed_device_tol(kind, d_H, n, k, lo_tol, lo_maxiter, hi_tol, hi_maxiter, ha
     return vec, val, info
 
 
+def acx_cuda_device_tol(d_H, n, k, tol, maxiter, handle=None, copy=True):
+    """ACX-accelerated plain IPT (include/ipt_hip_acx.h).  Returns
+    (vectors, values, info) copied to the host with the device result freed,
+    or with copy=False the ACXCudaResult itself (free with acx_free_result)."""
+    r = abi.ACXCudaResult()
+    st = lib().acx_cuda_device_tol(ctypes.c_void_p(d_H), int(n), int(k), float(tol), int(maxiter), handle,
+                                   ctypes.byref(r))
+    if st != 0:
+        raise IptError(st, "acx_cuda_device_tol")
+    if not copy:
+        return r
+    info = {f[0]: getattr(r, f[0]) for f in abi.ACXCudaResult._fields_ if not f[0].startswith("d_")}
+    try:
+        vec = hip().to_host(r.d_vectors, (n, k))
+        val = hip().to_host(r.d_values, (k,))
+    finally:
+        lib().acx_cuda_free_result(ctypes.byref(r))
+    return vec, val, info
+
+
+def acx_free_result(r):
+    lib().acx_cuda_free_result(ctypes.byref(r))
+
+
 # ------------------------------------------------------------------ extensions
 
 LOWP_MODES = {"fp16": 0, "bf16": 1, "bf16x3": 2, "fp32": 3, "fp8": 4}

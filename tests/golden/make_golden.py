@@ -123,9 +123,30 @@ def cluster_ranges():
                                    "cases": cases})
 
 
+def acx_sweep():
+    """ACX vs plain IPT: IPT/results/ACX/acx_vs_original_ipt_summary.csv, the
+    configuration of IPT/test/ACX/benchmark_acx_cuda_sweep.cu:20-25 (k = N,
+    tol 1e-12, maxiter 1000, seed 20260615 + N + a per-epsilon offset)."""
+    rows = []
+    with open(os.path.join(REF, "results/ACX/acx_vs_original_ipt_summary.csv")) as f:
+        for r in csv.DictReader(f):
+            rows.append({"n": int(r["N"]), "epsilon": float(r["epsilon"]),
+                         "acx_iterations": int(r["acx_iterations"]), "acx_f_calls": int(r["acx_f_calls"]),
+                         "ipt_iterations": int(r["original_ipt_iterations"]),
+                         "acx_fixed_point_residual": float(r["acx_fixed_point_residual"]),
+                         "ipt_fixed_point_residual": float(r["original_ipt_fixed_point_residual"]),
+                         "acx_time_sec": float(r["acx_time_sec"]),
+                         "ipt_time_sec": float(r["original_ipt_time_sec"]),
+                         "status": r["acx_status"] + "/" + r["original_ipt_status"]})
+    _write("acx_sweep.json", {"seed_base": 20260615, "tol": 1e-12, "maxiter": 1000,
+                              "gpu": "NVIDIA A100-SXM4-80GB",
+                              "source": "IPT/results/ACX/acx_vs_original_ipt_summary.csv", "rows": rows})
+
+
 if __name__ == "__main__":
     runtests_kat()
     sparse_kat()
     dense_eps_sweep()
     h2o_csc()
     cluster_ranges()
+    acx_sweep()

@@ -26,7 +26,7 @@ def _declared_functions():
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(",
                              text, flags=re.M):
-            if not m.group(1).startswith("ipt"):
+            if not m.group(1).startswith(("ipt", "acx")):
                 continue
             names.add(m.group(1))
     return names
@@ -149,3 +149,38 @@ def test_mixed_invalid_arguments_rejected_before_gpu():
     st = abi.lib().ipt_mixed_precision_cuda_device_tol(ctypes.c_void_p(8), 4, 2, 0.0, 10, 1e-12, 10, None,
                                                        ctypes.byref(r))
     assert st == 1
+
+
+def test_acx_result_and_arguments_before_gpu(tmp_path):
+    """ACXCudaResult layout equals the C layout; reset gives NaN residuals;
+    invalid arguments return status 1 before any device call
+    (IPT/src/ACX/acx_cuda.cu:420-432)."""
+    src = tmp_path / "probe.c"
+    src.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "ipt_hip_acx.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\\n", sizeof(ACXCudaResult), offsetof(ACXCudaResult, time_sec),
+         offsetof(ACXCudaResult, fixed_point_residual), offsetof(ACXCudaResult, d_values));
+  return 0; }
+""")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    R = abi.ACXCudaResult
+    assert got == [ctypes.sizeof(R), R.time_sec.offset, R.fixed_point_residual.offset, R.d_values.offset]
+    l = abi.lib()
+    assert l.acx_cuda_status_string(5) == b"zero diagonal denominator"
+    assert l.acx_cuda_status_string(0) == b"success"
+    r = R()
+    r.iterations = 3
+    l.acx_cuda_reset_result(ctypes.byref(r))
+    assert r.iterations == 0 and np.isnan(r.max_residual) and np.isnan(r.fixed_point_residual)
+    p = ctypes.c_void_p(8)
+    for args in [(None, 4, 2, 1e-12, 10), (p, 0, 1, 1e-12, 10), (p, 4, 0, 1e-12, 10), (p, 4, 5, 1e-12, 10),
+                 (p, 4, 2, 0.0, 10), (p, 4, 2, 1e-12, 0)]:
+        r.iterations = 7
+        assert l.acx_cuda_device_tol(*args, None, ctypes.byref(r)) == 1
+        assert r.iterations == 0 and not r.d_vectors
+    assert l.acx_cuda_device_tol(p, 4, 2, 1e-12, 10, None, None) == 1
