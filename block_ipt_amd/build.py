@@ -22,6 +22,12 @@ SOURCES = [
     "ipt_kernels.hip",
     "ipt_gemm_f64.hip",
     "ipt_block_kernels.hip",
+    "ipt_block_w1.hip",
+    "ipt_block_w2.hip",
+    "ipt_block_w3.hip",
+    "ipt_block_w4.hip",
+    "ipt_block_w5.hip",
+    "ipt_block_w6.hip",
     "ipt_gemm_lp.hip",
     "ipt_gen.hip",
     "ipt_gen_sparse.hip",

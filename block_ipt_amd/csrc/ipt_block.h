@@ -32,9 +32,10 @@ struct BlockPlan {
     int partial_stride = 0;   // max nblocks over clusters
     int w1_count = 0;         // clusters of width 1 (their own kernel)
     DevBuf<int> w1_list;      // their ids
-    int fused_blocks = 0;     // row blocks of all clusters of width 2..32
-    DevBuf<int> blk_cluster;  // per fused block: cluster id
-    DevBuf<int> blk_local;    // per fused block: row block within the cluster
+    int wide_count = 0;          // clusters of width 2..32
+    int width_count[33] = {};    // clusters of each width 2..32
+    int width_offset[33] = {};   // their position in width_lists
+    DevBuf<int> width_lists;     // cluster ids grouped by width
 };
 
 // Build the plan for ranges [first,last] (sorted by first) on n rows.

@@ -16,6 +16,7 @@
 #include <cfloat>
 
 #include "ipt_block.h"
+#include "ipt_block_width.h"
 
 namespace ipt {
 
@@ -213,32 +214,74 @@ __global__ __launch_bounds__(256) void k_block_update(
                          partials, pstride, flags);
 }
 
-// All clusters of widths up to 32 in one launch (their row blocks run
-// concurrently instead of one class after another): block b updates row
-// block blk_local[b] of cluster blk_cluster[b].
-__global__ __launch_bounds__(256) void k_block_update_fused(
-    const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
-    const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
-    const int *__restrict__ blk_cluster, const int *__restrict__ blk_local, const int *__restrict__ active,
-    double damping, double *__restrict__ partials, int pstride, int *__restrict__ flags)
+// Widths 2..32: one kernel per width (ipt_block_width.h).
+IPT_BLOCK_WIDTH_EXTERN(2)
+IPT_BLOCK_WIDTH_EXTERN(3)
+IPT_BLOCK_WIDTH_EXTERN(4)
+IPT_BLOCK_WIDTH_EXTERN(5)
+IPT_BLOCK_WIDTH_EXTERN(6)
+IPT_BLOCK_WIDTH_EXTERN(7)
+IPT_BLOCK_WIDTH_EXTERN(8)
+IPT_BLOCK_WIDTH_EXTERN(9)
+IPT_BLOCK_WIDTH_EXTERN(10)
+IPT_BLOCK_WIDTH_EXTERN(11)
+IPT_BLOCK_WIDTH_EXTERN(12)
+IPT_BLOCK_WIDTH_EXTERN(13)
+IPT_BLOCK_WIDTH_EXTERN(14)
+IPT_BLOCK_WIDTH_EXTERN(15)
+IPT_BLOCK_WIDTH_EXTERN(16)
+IPT_BLOCK_WIDTH_EXTERN(17)
+IPT_BLOCK_WIDTH_EXTERN(18)
+IPT_BLOCK_WIDTH_EXTERN(19)
+IPT_BLOCK_WIDTH_EXTERN(20)
+IPT_BLOCK_WIDTH_EXTERN(21)
+IPT_BLOCK_WIDTH_EXTERN(22)
+IPT_BLOCK_WIDTH_EXTERN(23)
+IPT_BLOCK_WIDTH_EXTERN(24)
+IPT_BLOCK_WIDTH_EXTERN(25)
+IPT_BLOCK_WIDTH_EXTERN(26)
+IPT_BLOCK_WIDTH_EXTERN(27)
+IPT_BLOCK_WIDTH_EXTERN(28)
+IPT_BLOCK_WIDTH_EXTERN(29)
+IPT_BLOCK_WIDTH_EXTERN(30)
+IPT_BLOCK_WIDTH_EXTERN(31)
+IPT_BLOCK_WIDTH_EXTERN(32)
+
+int block_update_width(int m, const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream)
 {
-    const int cid = blk_cluster[blockIdx.x];
-    const int lblk = blk_local[blockIdx.x];
-    const int m = clusters[cid].m;
-    if (m <= 1) {
-        block_update_rows<1>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
-    } else if (m <= 2) {
-        block_update_rows<2>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
-    } else if (m <= 4) {
-        block_update_rows<4>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
-    } else if (m <= 8) {
-        block_update_rows<8>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride, flags);
-    } else if (m <= 16) {
-        block_update_rows<16>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride,
-                              flags);
-    } else {
-        block_update_rows<32>(cid, lblk, B, Y, Bn, ld, d, n, clusters, active, damping, partials, pstride,
-                              flags);
+    switch (m) {
+    case 2: return launch_block_width<2>(list, count, g, stream);
+    case 3: return launch_block_width<3>(list, count, g, stream);
+    case 4: return launch_block_width<4>(list, count, g, stream);
+    case 5: return launch_block_width<5>(list, count, g, stream);
+    case 6: return launch_block_width<6>(list, count, g, stream);
+    case 7: return launch_block_width<7>(list, count, g, stream);
+    case 8: return launch_block_width<8>(list, count, g, stream);
+    case 9: return launch_block_width<9>(list, count, g, stream);
+    case 10: return launch_block_width<10>(list, count, g, stream);
+    case 11: return launch_block_width<11>(list, count, g, stream);
+    case 12: return launch_block_width<12>(list, count, g, stream);
+    case 13: return launch_block_width<13>(list, count, g, stream);
+    case 14: return launch_block_width<14>(list, count, g, stream);
+    case 15: return launch_block_width<15>(list, count, g, stream);
+    case 16: return launch_block_width<16>(list, count, g, stream);
+    case 17: return launch_block_width<17>(list, count, g, stream);
+    case 18: return launch_block_width<18>(list, count, g, stream);
+    case 19: return launch_block_width<19>(list, count, g, stream);
+    case 20: return launch_block_width<20>(list, count, g, stream);
+    case 21: return launch_block_width<21>(list, count, g, stream);
+    case 22: return launch_block_width<22>(list, count, g, stream);
+    case 23: return launch_block_width<23>(list, count, g, stream);
+    case 24: return launch_block_width<24>(list, count, g, stream);
+    case 25: return launch_block_width<25>(list, count, g, stream);
+    case 26: return launch_block_width<26>(list, count, g, stream);
+    case 27: return launch_block_width<27>(list, count, g, stream);
+    case 28: return launch_block_width<28>(list, count, g, stream);
+    case 29: return launch_block_width<29>(list, count, g, stream);
+    case 30: return launch_block_width<30>(list, count, g, stream);
+    case 31: return launch_block_width<31>(list, count, g, stream);
+    case 32: return launch_block_width<32>(list, count, g, stream);
+    default: return IPT_CUDA_INVALID_ARGUMENT;
     }
 }
 
@@ -478,26 +521,28 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
         IPT_HIPCHK(hipMemcpyAsync(plan->w1_list.p, lists[0].data(), lists[0].size() * sizeof(int),
                                   hipMemcpyHostToDevice, ctx->stream));
     }
-    // fused launch table: row blocks of every cluster of width 2..32, widest
-    // clusters first (their blocks take longest)
+    // clusters of width 2..32 grouped by width (one launch per width)
     {
-        std::vector<int> bc, bl;
-        for (int cls = 5; cls >= 1; --cls) {
-            for (int c : lists[(size_t)cls]) {
-                for (int b = 0; b < plan->host[(size_t)c].nblocks; ++b) {
-                    bc.push_back(c);
-                    bl.push_back(b);
+        std::vector<int> wl;
+        plan->wide_count = 0;
+        for (int m = 0; m <= 32; ++m) {
+            plan->width_offset[m] = (int)wl.size();
+            plan->width_count[m] = 0;
+            if (m < 2) {
+                continue;
+            }
+            for (int c = 0; c < plan->nclusters; ++c) {
+                if (plan->host[(size_t)c].m == m) {
+                    wl.push_back(c);
+                    plan->width_count[m] += 1;
                 }
             }
+            plan->wide_count += plan->width_count[m];
         }
-        plan->fused_blocks = (int)bc.size();
-        IPT_STCHK(plan->blk_cluster.alloc(std::max<size_t>(bc.size(), 1)));
-        IPT_STCHK(plan->blk_local.alloc(std::max<size_t>(bl.size(), 1)));
-        if (!bc.empty()) {
-            IPT_HIPCHK(hipMemcpyAsync(plan->blk_cluster.p, bc.data(), bc.size() * sizeof(int),
-                                      hipMemcpyHostToDevice, ctx->stream));
-            IPT_HIPCHK(hipMemcpyAsync(plan->blk_local.p, bl.data(), bl.size() * sizeof(int),
-                                      hipMemcpyHostToDevice, ctx->stream));
+        IPT_STCHK(plan->width_lists.alloc(std::max<size_t>(wl.size(), 1)));
+        if (!wl.empty()) {
+            IPT_HIPCHK(hipMemcpyAsync(plan->width_lists.p, wl.data(), wl.size() * sizeof(int), hipMemcpyHostToDevice,
+                                      ctx->stream));
         }
     }
     if (plan->nclusters > 0) {
@@ -541,11 +586,16 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
                            plan.flags.p);
         IPT_HIPCHK(hipGetLastError());
     }
-    if (plan.fused_blocks > 0) {
-        hipLaunchKernelGGL(k_block_update_fused, dim3(plan.fused_blocks), dim3(256), 0, ctx->stream, B, Y, Bn, ld,
-                           d, n, plan.clusters.p, plan.blk_cluster.p, plan.blk_local.p, plan.active.p, damping,
-                           partials, plan.partial_stride, plan.flags.p);
-        IPT_HIPCHK(hipGetLastError());
+    if (plan.wide_count > 0) {
+        const BlockUpdateArgs g{B, Y, Bn, ld, d, n, plan.clusters.p, plan.active.p, damping, partials,
+                                plan.partial_stride, plan.flags.p};
+        // widest first: their blocks take longest
+        for (int m = 32; m >= 2; --m) {
+            if (plan.width_count[m] > 0) {
+                IPT_STCHK(block_update_width(m, plan.width_lists.p + plan.width_offset[m], plan.width_count[m], g,
+                                             ctx->stream));
+            }
+        }
     }
     if (plan.class_count[6] > 0) {
         // widths 33..64 keep their own launch (their registers would lower

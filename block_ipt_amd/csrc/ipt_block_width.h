@@ -1,0 +1,209 @@
+// Block-cluster update with the cluster width M (2..32) fixed at compile
+// time, one launch per width present in a solve.
+//
+// The elimination is that of block_update_rows<P> in ipt_block_kernels.hip
+// (row r owned by P = next_pow2(M) lanes, lane t holding equation t of
+// (d_r I - h^T) x = d_r B_r - (HB)_r, implicit partial pivoting by a
+// shuffle arg-max, the column-dominance shortcut) with the same operation
+// order per element, so the results are identical; the loops run to M
+// instead of P.  A width's kernel then holds only the pivot steps and
+// columns it uses: one kernel for every width up to 32 with the loops run
+// to P needs ~120 KB of instructions (twice the 64 KB instruction cache a
+// CU pair shares) and the registers of its widest path in every block.
+#pragma once
+
+#include <cfloat>
+
+#include "ipt_block.h"
+
+namespace ipt {
+
+struct BlockUpdateArgs {
+    const double *B;
+    const double *Y;
+    double *Bn;
+    long ld;
+    const double *d;
+    int n;
+    const BlockCluster *clusters;
+    const int *active;
+    double damping;
+    double *partials;
+    int pstride;
+    int *flags;
+};
+
+template <int M>
+__device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const BlockUpdateArgs &g)
+{
+    constexpr int P = M <= 2 ? 2 : M <= 4 ? 4 : M <= 8 ? 8 : M <= 16 ? 16 : 32;
+    constexpr int GROUPS = 256 / P;
+    const BlockCluster cl = g.clusters[cid];
+    const long ld = g.ld;
+    const int t = threadIdx.x % P;
+    const int grp = threadIdx.x / P;
+    const long row = (long)lblk * GROUPS + grp;
+    const bool live = g.active[cid] != 0;
+    double sdn = 0.0, sbn = 0.0;
+    // off-diagonal absolute row sums of h (see block_update_rows)
+    __shared__ double hsum[P];
+    if (threadIdx.x < P) {
+        double acc = 0.0;
+        if (threadIdx.x < M && live) {
+            const double *Yc = g.Y + (long)cl.off * ld;
+#pragma unroll
+            for (int u = 0; u < M; ++u) {
+                if (u != (int)threadIdx.x) {
+                    acc += fabs(Yc[cl.first + threadIdx.x + (long)u * ld]);
+                }
+            }
+        }
+        hsum[threadIdx.x] = acc;
+    }
+    __syncthreads();
+
+    if (row < g.n) {
+        const double *Bc = g.B + (long)cl.off * ld;
+        const double *Yc = g.Y + (long)cl.off * ld;
+        double *Bnc = g.Bn + (long)cl.off * ld;
+        if (!live) {
+            if (t < M) {
+                Bnc[row + (long)t * ld] = Bc[row + (long)t * ld];
+            }
+        } else if (row >= cl.first && row <= cl.last) {
+            if (t < M) {
+                const double nv = (row == cl.first + t) ? 1.0 : 0.0;
+                const double ov = Bc[row + (long)t * ld];
+                Bnc[row + (long)t * ld] = nv;
+                sdn += (nv - ov) * (nv - ov);
+                sbn += ov * ov;
+            }
+        } else {
+            const double dr = g.d[row];
+            // Lanes t >= M (padding up to P) load the equation of lane M-1:
+            // they are never the pivot lane, nothing reads their values, and
+            // unpredicated loads keep the register count of a power-of-two
+            // width.
+            const int te = t < M ? t : M - 1;
+            double a[M + 1];
+#pragma unroll
+            for (int u = 0; u < M; ++u) {
+                // h(u, t) = Y[first+u, off+t]
+                a[u] = (u == t ? dr : 0.0) - Yc[cl.first + u + (long)te * ld];
+            }
+            a[M] = dr * Bc[row + (long)te * ld] - Yc[row + (long)te * ld];
+            bool used = false;
+            int mypc = -1;
+            bool bad = false;
+            const bool mine = t >= M || fabs(dr - Yc[cl.first + t + (long)t * ld]) > 2.0 * hsum[t];
+            const unsigned long long ball = __ballot(mine);
+            const int base = (threadIdx.x & 63) & ~(P - 1);
+            const unsigned long long gmask = ((1ULL << P) - 1ULL) << base;
+            const bool dominant = (ball & gmask) == gmask;
+#pragma unroll
+            for (int pc = 0; pc < M; ++pc) {
+                if (!bad) {
+                    double cand;
+                    int cidx;
+                    if (dominant) {
+                        cidx = pc;
+                        cand = fabs(__shfl(a[pc], pc, P));
+                    } else {
+                        cand = (t < M && !used) ? fabs(a[pc]) : -1.0;
+                        cidx = t;
+#pragma unroll
+                        for (int o = 1; o < P; o <<= 1) {
+                            const double oc = __shfl_xor(cand, o, P);
+                            const int oi = __shfl_xor(cidx, o, P);
+                            if (oc > cand || (oc == cand && oi < cidx)) {
+                                cand = oc;
+                                cidx = oi;
+                            }
+                        }
+                    }
+                    if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
+                        bad = true;
+                    } else {
+                        const double inv = 1.0 / __shfl(a[pc], cidx, P);
+                        const double f = a[pc];
+#pragma unroll
+                        for (int c = pc; c <= M; ++c) {
+                            const double pr = __shfl(a[c], cidx, P) * inv;
+                            if (t == cidx) {
+                                a[c] = pr;
+                            } else {
+                                a[c] -= f * pr;
+                            }
+                        }
+                        if (t == cidx) {
+                            used = true;
+                            mypc = pc;
+                        }
+                    }
+                }
+            }
+            if (bad) {
+                if (t == 0) {
+                    atomicOr(&g.flags[0], 1);
+                }
+            } else if (mypc >= 0) {
+                double v = a[M];
+                const double ov = Bc[row + (long)mypc * ld];
+                if (g.damping < 1.0) {
+                    v = (1.0 - g.damping) * ov + g.damping * v;
+                }
+                if (!isfinite(v)) {
+                    atomicOr(&g.flags[0], 1);
+                    v = 0.0;
+                }
+                Bnc[row + (long)mypc * ld] = v;
+                sdn += (v - ov) * (v - ov);
+                sbn += ov * ov;
+            }
+        }
+    }
+    __shared__ double sa[4], sb[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        sdn += __shfl_down(sdn, o, 64);
+        sbn += __shfl_down(sbn, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = sdn;
+        sb[threadIdx.x >> 6] = sbn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long slot = (long)cid * g.pstride + lblk;
+        g.partials[2 * slot] = sa[0] + sa[1] + sa[2] + sa[3];
+        g.partials[2 * slot + 1] = sb[0] + sb[1] + sb[2] + sb[3];
+    }
+}
+
+// blockIdx.y = cluster list[blockIdx.y] (all of width M), blockIdx.x = row block.
+template <int M>
+__global__ __launch_bounds__(256) void k_block_update_m(BlockUpdateArgs g, const int *__restrict__ list)
+{
+    block_update_rows_m<M>(list[blockIdx.y], blockIdx.x, g);
+}
+
+// Launch the width-M update of `count` clusters listed at device `list`.
+template <int M>
+int launch_block_width(const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream)
+{
+    constexpr int P = M <= 2 ? 2 : M <= 4 ? 4 : M <= 8 ? 8 : M <= 16 ? 16 : 32;
+    constexpr int rows = 256 / P;
+    const dim3 grid((unsigned)((g.n + rows - 1) / rows), (unsigned)count);
+    hipLaunchKernelGGL(k_block_update_m<M>, grid, dim3(256), 0, stream, g, list);
+    return hipGetLastError() == hipSuccess ? IPT_CUDA_SUCCESS : IPT_CUDA_CUDA_ERROR;
+}
+
+// Width 2..32 dispatch (ipt_block_kernels.hip); the instantiations live in
+// ipt_block_w*.hip so that they compile in parallel.
+int block_update_width(int m, const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream);
+
+#define IPT_BLOCK_WIDTH_INSTANTIATE(W) \
+    template int launch_block_width<W>(const int *, int, const BlockUpdateArgs &, hipStream_t);
+#define IPT_BLOCK_WIDTH_EXTERN(W) \
+    extern template int launch_block_width<W>(const int *, int, const BlockUpdateArgs &, hipStream_t);
+
+}  // namespace ipt
