@@ -4,9 +4,10 @@
 // The elimination is that of block_update_rows<P> in ipt_block_kernels.hip
 // (row r owned by P = next_pow2(M) lanes, lane t holding equation t of
 // (d_r I - h^T) x = d_r B_r - (HB)_r, implicit partial pivoting by a
-// shuffle arg-max, the column-dominance shortcut) with the same operation
-// order per element, so the results are identical; the loops run to M
-// instead of P.  A width's kernel then holds only the pivot steps and
+// shuffle arg-max, the column-dominance shortcut); the loops run to M
+// instead of P, and a non-pivot lane applies the eliminated multiple as
+// a -= (f / piv) p (one FMA) where block_update_rows forms a -= f (p / piv):
+// the same pivots, results within a few units of rounding.  A width's kernel then holds only the pivot steps and
 // columns it uses: one kernel for every width up to 32 with the loops run
 // to P needs ~120 KB of instructions (twice the 64 KB instruction cache a
 // CU pair shares) and the registers of its widest path in every block.
@@ -124,16 +125,16 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
                     if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
                         bad = true;
                     } else {
+                        // pivot lane: a = p / piv; other lanes: a -= (f / piv) p,
+                        // one multiply and one FMA per column for every lane
                         const double inv = 1.0 / __shfl(a[pc], cidx, P);
-                        const double f = a[pc];
+                        const bool me = t == cidx;
+                        const double alpha = me ? inv : -(a[pc] * inv);
+                        const double keep = me ? 0.0 : 1.0;
 #pragma unroll
                         for (int c = pc; c <= M; ++c) {
-                            const double pr = __shfl(a[c], cidx, P) * inv;
-                            if (t == cidx) {
-                                a[c] = pr;
-                            } else {
-                                a[c] -= f * pr;
-                            }
+                            const double p = __shfl(a[c], cidx, P);
+                            a[c] = fma(alpha, p, keep * a[c]);
                         }
                         if (t == cidx) {
                             used = true;
