@@ -46,16 +46,27 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
     const long row = (long)lblk * GROUPS + grp;
     const bool live = g.active[cid] != 0;
     double sdn = 0.0, sbn = 0.0;
-    // off-diagonal absolute row sums of h (see block_update_rows)
+    // h staged in LDS once per block (coalesced: column u of h is M
+    // consecutive rows of Y), hs[t][u] = h(u, t), so that the equations of
+    // every row of the block read it from there
+    __shared__ double hs[M][M + 1];
     __shared__ double hsum[P];
+    if (live) {
+        const double *Yc = g.Y + (long)cl.off * ld;
+        for (int e = threadIdx.x; e < M * M; e += 256) {
+            const int u = e % M, tc = e / M;
+            hs[tc][u] = Yc[cl.first + u + (long)tc * ld];
+        }
+    }
+    __syncthreads();
+    // off-diagonal absolute row sums of h (see block_update_rows)
     if (threadIdx.x < P) {
         double acc = 0.0;
         if (threadIdx.x < M && live) {
-            const double *Yc = g.Y + (long)cl.off * ld;
 #pragma unroll
             for (int u = 0; u < M; ++u) {
                 if (u != (int)threadIdx.x) {
-                    acc += fabs(Yc[cl.first + threadIdx.x + (long)u * ld]);
+                    acc += fabs(hs[u][threadIdx.x]);
                 }
             }
         }
@@ -90,13 +101,13 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
 #pragma unroll
             for (int u = 0; u < M; ++u) {
                 // h(u, t) = Y[first+u, off+t]
-                a[u] = (u == t ? dr : 0.0) - Yc[cl.first + u + (long)te * ld];
+                a[u] = (u == t ? dr : 0.0) - hs[te][u];
             }
             a[M] = dr * Bc[row + (long)te * ld] - Yc[row + (long)te * ld];
             bool used = false;
             int mypc = -1;
             bool bad = false;
-            const bool mine = t >= M || fabs(dr - Yc[cl.first + t + (long)t * ld]) > 2.0 * hsum[t];
+            const bool mine = t >= M || fabs(dr - hs[te][te]) > 2.0 * hsum[t];
             const unsigned long long ball = __ballot(mine);
             const int base = (threadIdx.x & 63) & ~(P - 1);
             const unsigned long long gmask = ((1ULL << P) - 1ULL) << base;
