@@ -5,9 +5,11 @@
 // (row r owned by P = next_pow2(M) lanes, lane t holding equation t of
 // (d_r I - h^T) x = d_r B_r - (HB)_r, implicit partial pivoting by a
 // shuffle arg-max, the column-dominance shortcut); the loops run to M
-// instead of P, and a non-pivot lane applies the eliminated multiple as
-// a -= (f / piv) p (one FMA) where block_update_rows forms a -= f (p / piv):
-// the same pivots, results within a few units of rounding.  A width's kernel then holds only the pivot steps and
+// instead of P, the pivot rows are not scaled (each unknown is divided by
+// its pivot once at the end, one FMA per eliminated entry), and when every
+// column of a row's system is dominant the pivot broadcasts are DPP moves
+// instead of LDS shuffles: the same pivots, results within a few units of
+// rounding.  A width's kernel then holds only the pivot steps and
 // columns it uses: one kernel for every width up to 32 with the loops run
 // to P needs ~120 KB of instructions (twice the 64 KB instruction cache a
 // CU pair shares) and the registers of its widest path in every block.
@@ -33,6 +35,56 @@ struct BlockUpdateArgs {
     int pstride;
     int *flags;
 };
+
+// Broadcast of lane L of each P-lane group (P <= 16) by DPP: row_newbcast
+// (lane L of every 16-lane row) for P = 16 and, with a select on lane bit 3,
+// for P = 8; quad_perm for P = 4 and 2.  P = 32 groups span two rows: a
+// shuffle.  No LDS traffic for P <= 16.
+template <int P, int L>
+__device__ __forceinline__ double group_bcast(double v)
+{
+    if constexpr (P == 16) {
+        return __builtin_amdgcn_update_dpp(v, v, 0x150 + L, 0xf, 0xf, false);
+    } else if constexpr (P == 8) {
+        const double lo = __builtin_amdgcn_update_dpp(v, v, 0x150 + L, 0xf, 0xf, false);
+        const double hi = __builtin_amdgcn_update_dpp(v, v, 0x150 + 8 + L, 0xf, 0xf, false);
+        return (threadIdx.x & 8) ? hi : lo;
+    } else if constexpr (P == 4) {
+        return __builtin_amdgcn_update_dpp(v, v, L | (L << 2) | (L << 4) | (L << 6), 0xf, 0xf, false);
+    } else if constexpr (P == 2) {
+        return __builtin_amdgcn_update_dpp(v, v, L | (L << 2) | ((2 + L) << 4) | ((2 + L) << 6), 0xf, 0xf, false);
+    } else {
+        return __shfl(v, L, P);
+    }
+}
+
+// Pivot steps PC..M-1 of a system whose columns are diagonally dominant
+// (pivot of step pc on lane pc, known at compile time).  Gauss-Jordan
+// without row scaling: the pivot lane keeps its row, every other lane
+// subtracts (a_pc / piv) times it; the unknown of the lane that pivoted at
+// step pc is its right-hand side over its pivot at the end.
+template <int M, int P, int PC>
+__device__ __forceinline__ void dominant_steps(double (&a)[M + 1], int t, bool &bad, int &mypc, double &mypiv)
+{
+    if constexpr (PC < M) {
+        const double piv = group_bcast<P, PC>(a[PC]);
+        if (!(fabs(piv) > 64.0 * DBL_EPSILON) || !isfinite(piv)) {
+            bad = true;
+            return;
+        }
+        const bool me = t == PC;
+        const double alpha = me ? 0.0 : -(a[PC] / piv);
+#pragma unroll
+        for (int c = PC + 1; c <= M; ++c) {
+            a[c] = fma(alpha, group_bcast<P, PC>(a[c]), a[c]);
+        }
+        if (me) {
+            mypc = PC;
+            mypiv = piv;
+        }
+        dominant_steps<M, P, PC + 1>(a, t, bad, mypc, mypiv);
+    }
+}
 
 template <int M>
 __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const BlockUpdateArgs &g)
@@ -104,25 +156,23 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
                 a[u] = (u == t ? dr : 0.0) - hs[te][u];
             }
             a[M] = dr * Bc[row + (long)te * ld] - Yc[row + (long)te * ld];
-            bool used = false;
             int mypc = -1;
+            double mypiv = 1.0;
             bool bad = false;
             const bool mine = t >= M || fabs(dr - hs[te][te]) > 2.0 * hsum[t];
             const unsigned long long ball = __ballot(mine);
             const int base = (threadIdx.x & 63) & ~(P - 1);
             const unsigned long long gmask = ((1ULL << P) - 1ULL) << base;
             const bool dominant = (ball & gmask) == gmask;
+            if (dominant) {
+                dominant_steps<M, P, 0>(a, t, bad, mypc, mypiv);
+            } else {
+                bool used = false;
 #pragma unroll
-            for (int pc = 0; pc < M; ++pc) {
-                if (!bad) {
-                    double cand;
-                    int cidx;
-                    if (dominant) {
-                        cidx = pc;
-                        cand = fabs(__shfl(a[pc], pc, P));
-                    } else {
-                        cand = (t < M && !used) ? fabs(a[pc]) : -1.0;
-                        cidx = t;
+                for (int pc = 0; pc < M; ++pc) {
+                    if (!bad) {
+                        double cand = (t < M && !used) ? fabs(a[pc]) : -1.0;
+                        int cidx = t;
 #pragma unroll
                         for (int o = 1; o < P; o <<= 1) {
                             const double oc = __shfl_xor(cand, o, P);
@@ -132,24 +182,21 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
                                 cidx = oi;
                             }
                         }
-                    }
-                    if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
-                        bad = true;
-                    } else {
-                        // pivot lane: a = p / piv; other lanes: a -= (f / piv) p,
-                        // one multiply and one FMA per column for every lane
-                        const double inv = 1.0 / __shfl(a[pc], cidx, P);
-                        const bool me = t == cidx;
-                        const double alpha = me ? inv : -(a[pc] * inv);
-                        const double keep = me ? 0.0 : 1.0;
+                        if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
+                            bad = true;
+                        } else {
+                            const double piv = __shfl(a[pc], cidx, P);
+                            const bool me = t == cidx;
+                            const double alpha = me ? 0.0 : -(a[pc] / piv);
 #pragma unroll
-                        for (int c = pc; c <= M; ++c) {
-                            const double p = __shfl(a[c], cidx, P);
-                            a[c] = fma(alpha, p, keep * a[c]);
-                        }
-                        if (t == cidx) {
-                            used = true;
-                            mypc = pc;
+                            for (int c = pc + 1; c <= M; ++c) {
+                                a[c] = fma(alpha, __shfl(a[c], cidx, P), a[c]);
+                            }
+                            if (me) {
+                                used = true;
+                                mypc = pc;
+                                mypiv = piv;
+                            }
                         }
                     }
                 }
@@ -159,7 +206,7 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
                     atomicOr(&g.flags[0], 1);
                 }
             } else if (mypc >= 0) {
-                double v = a[M];
+                double v = a[M] / mypiv;
                 const double ov = Bc[row + (long)mypc * ld];
                 if (g.damping < 1.0) {
                     v = (1.0 - g.damping) * ov + g.damping * v;
