@@ -50,9 +50,11 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed solves of the plain FP64 path (0: skip)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=2)
-    ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products"],
+    ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products", "acx"],
                     help="dense: the headline solve; sparse: the N=1e6 orbital-sum CSR solve and its SpMM; "
-                         "products: N=65536 k=256 correction products by precision")
+                         "products: N=65536 k=256 correction products by precision; "
+                         "acx: ACX vs plain IPT on the epsilon-sweep matrix (k = N, --n, --eps)")
+    ap.add_argument("--eps", type=float, default=0.12, help="acx workload: coupling scale epsilon")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
     return ap.parse_args()
@@ -246,6 +248,54 @@ def products_workload(a):
     print(json.dumps(out), flush=True)
 
 
+def sweep_seed(n, eps, base=20260615):
+    """Seed of one epsilon-sweep case: base + N + splitmix64(bits of eps)
+    mod 1000000007 (the reference's ACX sweep benchmark)."""
+    import struct
+
+    m = 0xFFFFFFFFFFFFFFFF
+    x = (struct.unpack("<Q", struct.pack("<d", float(eps)))[0] + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return base + n + (x ^ (x >> 31)) % 1000000007
+
+
+def acx_workload(a):
+    """ACX-accelerated vs plain FP64 IPT for all N eigenpairs of the
+    epsilon-sweep matrix diag(i+1) + eps U (tol 1e-12), generated on the
+    device; FP64 MFMA products throughout."""
+    from block_ipt_amd import api
+
+    hip = api.hip()
+    n = a.n
+    dH = hip.malloc(n * n * 8)
+    api.generate_eps(dH, n, a.eps, sweep_seed(n, a.eps))
+    out = {"workload": "epsilon sweep N=%d, eps=%g, k=N, tol 1e-12" % (n, a.eps)}
+    # warm-up: one short run of each path
+    r = api.acx_cuda_device_tol(dH, n, n, 1e-12, 1, copy=False)
+    api.acx_free_result(r)
+    api.profile_reset()
+    api.profile_enable(True)
+    t0 = time.perf_counter()
+    r = api.acx_cuda_device_tol(dH, n, n, 1e-12, 1000, copy=False)
+    t_acx = time.perf_counter() - t0
+    api.profile_enable(False)
+    ms, cnt = api.profile_get("gemm_f64")
+    out["acx"] = {"seconds": t_acx, "iterations": r.iterations, "f_calls": r.f_calls, "converged": r.converged,
+                  "fixed_point_residual": r.fixed_point_residual, "max_residual": r.max_residual,
+                  "finalization_seconds": r.finalization_time_sec,
+                  "gemm_f64_TF/s": 2.0 * n * n * n * r.f_calls / (ms * 1e-3) / 1e12}
+    api.acx_free_result(r)
+    t0 = time.perf_counter()
+    dv, dl, it, fp = api.ipt_cuda_device_tol(dH, n, n, 1e-12, 1000)
+    t_plain = time.perf_counter() - t0
+    api.free_device_result(dv, dl)
+    out["plain"] = {"seconds": t_plain, "iterations": it, "fixed_point_residual": fp}
+    out["acx_speedup_vs_plain"] = t_plain / t_acx
+    hip.free(dH)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -260,6 +310,9 @@ def main():
         return
     if a.workload == "products":
         products_workload(a)
+        return
+    if a.workload == "acx":
+        acx_workload(a)
         return
     from block_ipt_amd import api
 

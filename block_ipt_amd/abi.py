@@ -199,6 +199,7 @@ SIGNATURES = {
     # include/ipt_hip_ext.h
     "ipt_hip_generate_dense": (_c.c_int, [_vp, _c.c_int, _c.c_ulonglong, _c.c_double, _c.c_double,
                                           _c.c_double, _vp]),
+    "ipt_hip_generate_eps": (_c.c_int, [_vp, _c.c_int, _c.c_double, _c.c_ulonglong, _vp]),
     "ipt_hip_lowp_product": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _vp, _vp]),
     "ipt_hip_profile_enable": (None, [_c.c_int]),
     "ipt_hip_profile_reset": (None, []),

@@ -320,6 +320,13 @@ def generate_orbital_csr(n, m, seed, d_row_ptr, d_col_ind, d_values, capacity, n
     return nnz.value
 
 
+def generate_eps(d_H, n, eps, seed, handle=None):
+    """diag(i+1) + eps U sweep matrix on the device (include/ipt_hip_ext.h)."""
+    st = lib().ipt_hip_generate_eps(ctypes.c_void_p(d_H), int(n), float(eps), int(seed), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_generate_eps")
+
+
 def csr_spmm(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, handle=None):
     st = lib().ipt_hip_csr_spmm(_vpp(d_row_ptr), _vpp(d_col_ind), _vpp(d_values), int(n), _vpp(d_X), int(ldx),
                                 _vpp(d_Y), int(ldy), int(k), handle)

@@ -43,9 +43,50 @@ __global__ void k_gen_dense(double *__restrict__ H, int n, unsigned long long se
     }
 }
 
+// diag(i + 1) + eps U: H_ii = i + 1 + eps u(i, i), H_ij = H_ji =
+// eps (u(i, j) + u(j, i)) / 2 (IPT/test/ACX/benchmark_acx_cuda_sweep.cu:284-312,
+// the same recipe as benchmark_tf32_cusolver_sweep.cu:472-503).
+__global__ void k_gen_eps(double *__restrict__ H, int n, unsigned long long seed, double eps)
+{
+    const long total = (long)n * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const long r = t % n, c = t / n;
+        double v;
+        if (r == c) {
+            // separately rounded product and sum, as the host generators
+            // compute it (no contraction into an FMA)
+            v = __dadd_rn((double)(r + 1), __dmul_rn(eps, uniform01(seed, n, r, c)));
+        } else {
+            // the reference writes the pair from (row <= col): the same value
+            // both ways
+            const long a = r < c ? r : c, b = r < c ? c : r;
+            v = 0.5 * eps * (uniform01(seed, n, a, b) + uniform01(seed, n, b, a));
+        }
+        H[t] = v;
+    }
+}
+
 }  // namespace ipt
 
 using namespace ipt;
+
+extern "C" int ipt_hip_generate_eps(double *d_H, int n, double eps, unsigned long long seed, void *handle)
+{
+    if (d_H == nullptr || n <= 0) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    const long total = (long)n * n;
+    const long blocks = std::min((total + 255) / 256, 65536L);
+    hipLaunchKernelGGL(k_gen_eps, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d_H, n, seed, eps);
+    IPT_HIPCHK(hipGetLastError());
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
 
 extern "C" int ipt_hip_generate_dense(double *d_H, int n, unsigned long long seed, double lo, double hi,
                                       double vscale, void *handle)

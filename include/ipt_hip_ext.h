@@ -21,6 +21,13 @@ extern "C" {
 int ipt_hip_generate_dense(double *d_H, int n, unsigned long long seed, double lo, double hi, double vscale,
                            void *handle);
 
+/* The reference's epsilon-sweep matrix on the device (column-major n x n):
+ * H_ii = i + 1 + eps U(seed, n, i, i), H_ij = H_ji = eps (U(seed,n,i,j) +
+ * U(seed,n,j,i)) / 2, U as above; the generator of
+ * IPT/test/ACX/benchmark_acx_cuda_sweep.cu:284-312 (seed there: 20260615 +
+ * n + a per-epsilon offset) and benchmark_tf32_cusolver_sweep.cu:472-503. */
+int ipt_hip_generate_eps(double *d_H, int n, double eps, unsigned long long seed, void *handle);
+
 /* Sparse synthetic workload (orbital-sum model), CSR = CSC (symmetric) with
  * 32-bit indices and ascending column indices per row:
  *   norb orbital energies U[elo, ehi] (sorted); row i's diagonal is the sum

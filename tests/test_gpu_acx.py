@@ -111,3 +111,17 @@ def test_repeatable():
     b = _run(H, 100, 1e-12, 1000)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert a[2]["f_calls"] == b[2]["f_calls"]
+
+
+def test_device_eps_generator_matches_oracle():
+    """The device sweep-matrix generator equals the oracle's bit for bit."""
+    hip = api.hip()
+    n, eps = 700, 0.12
+    seed = acx.sweep_seed(n, eps)
+    dH = hip.malloc(n * n * 8)
+    try:
+        api.generate_eps(dH, n, eps, seed)
+        G = hip.to_host(dH, (n, n))
+    finally:
+        hip.free(dH)
+    assert np.array_equal(G, orc.eps_dense(n, eps, seed))
