@@ -36,6 +36,8 @@ struct BlockPlan {
     int width_count[33] = {};    // clusters of each width 2..32
     int width_offset[33] = {};   // their position in width_lists
     DevBuf<int> width_lists;     // cluster ids grouped by width
+    std::vector<int> wide_host;  // clusters of width 65..142 (one launch each)
+    DevBuf<int> wide_list;       // their ids
 };
 
 // Build the plan for ranges [first,last] (sorted by first) on n rows.

@@ -478,6 +478,7 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
 {
     plan->nclusters = (int)ranges.size();
     plan->host.clear();
+    plan->wide_host.clear();
     std::vector<std::vector<int>> lists(7);
     int off = 0;
     plan->partial_stride = 1;
@@ -488,16 +489,22 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
         cl.m = cl.last - cl.first + 1;
         cl.off = off;
         const int cls = lane_class(cl.m);
-        if (cls > 6) {
+        if (cl.m > kMaxWideCluster) {
+            fprintf(stderr, "IPT block-cluster: a cluster of width %d exceeds the widest supported update (%d)\n",
+                    cl.m, kMaxWideCluster);
             return IPT_CUDA_INVALID_ARGUMENT;
         }
         // width 1: partials per 64-row wavefront; wider: per row block
-        const int rows = cl.m == 1 ? 64 : 256 >> cls;
+        const int rows = cl.m == 1 ? 64 : cl.m > 64 ? block_wide_rows() : 256 >> cls;
         cl.nblocks = (n + rows - 1) / rows;
         cl.pad = 0;
         plan->partial_stride = std::max(plan->partial_stride, cl.nblocks);
         off += cl.m;
-        lists[(size_t)cls].push_back(c);
+        if (cl.m > 64) {
+            plan->wide_host.push_back(c);
+        } else {
+            lists[(size_t)cls].push_back(c);
+        }
         plan->host.push_back(cl);
     }
     plan->basis_cols = off;
@@ -519,6 +526,11 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
     IPT_STCHK(plan->w1_list.alloc(std::max<size_t>(lists[0].size(), 1)));
     if (!lists[0].empty()) {
         IPT_HIPCHK(hipMemcpyAsync(plan->w1_list.p, lists[0].data(), lists[0].size() * sizeof(int),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    }
+    IPT_STCHK(plan->wide_list.alloc(std::max<size_t>(plan->wide_host.size(), 1)));
+    if (!plan->wide_host.empty()) {
+        IPT_HIPCHK(hipMemcpyAsync(plan->wide_list.p, plan->wide_host.data(), plan->wide_host.size() * sizeof(int),
                                   hipMemcpyHostToDevice, ctx->stream));
     }
     // clusters of width 2..32 grouped by width (one launch per width)
@@ -606,6 +618,14 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
                            plan.clusters.p, plan.class_lists.p + plan.class_offset[6], plan.active.p, damping,
                            partials, plan.partial_stride, plan.flags.p);
         IPT_HIPCHK(hipGetLastError());
+    }
+    if (!plan.wide_host.empty()) {
+        const BlockUpdateArgs g{B, Y, Bn, ld, d, n, plan.clusters.p, plan.active.p, damping, partials,
+                                plan.partial_stride, plan.flags.p};
+        for (size_t i = 0; i < plan.wide_host.size(); ++i) {
+            const int m = plan.host[(size_t)plan.wide_host[i]].m;
+            IPT_STCHK(launch_block_wide(m, plan.wide_list.p + i, 1, g, ctx->stream));
+        }
     }
     profile_end(ctx, prof);
     hipLaunchKernelGGL(k_block_finalize, dim3(plan.nclusters), dim3(256), 0, ctx->stream, partials,

@@ -256,6 +256,12 @@ int launch_block_width(const int *list, int count, const BlockUpdateArgs &g, hip
     return hipGetLastError() == hipSuccess ? IPT_CUDA_SUCCESS : IPT_CUDA_CUDA_ERROR;
 }
 
+// Widths 65..kMaxWideCluster (ipt_block_wide.hip): one workgroup per row,
+// the augmented system in LDS; block_wide_rows() rows per workgroup.
+constexpr int kMaxWideCluster = 142;
+int block_wide_rows();
+int launch_block_wide(int m, const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream);
+
 // Width 2..32 dispatch (ipt_block_kernels.hip); the instantiations live in
 // ipt_block_w*.hip so that they compile in parallel.
 int block_update_width(int m, const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream);

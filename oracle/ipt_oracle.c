@@ -789,6 +789,10 @@ int orc_target_clusters(const double *dense, const int *cp, const int *ri,
 /* block-cluster (Riccati) iteration                                   */
 /* ------------------------------------------------------------------ */
 
+/* Widest cluster the block iteration accepts (the reference has no fixed
+ * bound; its block-size option IPT_MAX_LIFT_BLOCK_SIZE defaults to 64). */
+#define ORC_MAX_CLUSTER 142
+
 /*
  * Solve (d_r I - h^T) x = rhs with partial-pivot Gauss-Jordan.  The pivot
  * rule and the singular threshold 64*DBL_EPSILON follow the GPU kernel
@@ -871,7 +875,8 @@ static int orc_block_solve_core(const OrcMat *M, const double *diag, int first, 
         }
 #pragma omp parallel for schedule(static) reduction(| : singular)
         for (int row = 0; row < n; ++row) {
-            double aug[65 * 64];
+            /* m x (m+1) system of this row (m <= ORC_MAX_CLUSTER) */
+            double aug[ORC_MAX_CLUSTER * (ORC_MAX_CLUSTER + 1)];
             if (row >= first && row <= last) {
                 for (int c = 0; c < m; ++c) {
                     Bn[(size_t)row + (size_t)c * n] = (row == first + c) ? 1.0 : 0.0;
@@ -935,7 +940,7 @@ int orc_block_cluster_solve(const double *dense, const int *cp, const int *ri,
     OrcMat M;
     double *diag;
     int st;
-    if (first < 0 || last < first || last >= n || last - first + 1 > 64) {
+    if (first < 0 || last < first || last >= n || last - first + 1 > ORC_MAX_CLUSTER) {
         return ORC_INVALID;
     }
     orc_mat_init(&M, dense, cp, ri, v, n);

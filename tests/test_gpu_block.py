@@ -129,6 +129,32 @@ def test_wide_cluster_class(block_env, n, hi, k, seed):
     assert info["iterations"] == oinfo["max_iterations"]
 
 
+@pytest.mark.parametrize("n,m,k", [(1024, 80, 80), (1024, 100, 96), (2048, 130, 128)])
+def test_clusters_wider_than_64(block_env, n, m, k):
+    """IPT_MAX_LIFT_BLOCK_SIZE raised: an 80- / 100- / 130-column cluster runs
+    through the LDS-resident update; same pairs as LAPACK and the oracle,
+    same iteration count as the oracle."""
+    from test_oracle import wide_cluster_matrix
+
+    block_env.setenv("IPT_MAX_LIFT_BLOCK_SIZE", "142")
+    H = wide_cluster_matrix(n, m, 7)
+    gv, gl, info = api.ipt_cuda_tol(H, k, 1e-12, 100)
+    _check_eigs(H, gv, gl, k)
+    ov, ovec, oinfo = orc.block_pipeline(k, dense=H, max_block=142, maxiter=100, tol=1e-12)
+    assert np.abs(gl - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
+    assert info["iterations"] == oinfo["max_iterations"]
+
+
+def test_cluster_wider_than_supported_rejected(block_env):
+    from test_oracle import wide_cluster_matrix
+
+    block_env.setenv("IPT_MAX_LIFT_BLOCK_SIZE", "200")
+    H = wide_cluster_matrix(1024, 150, 7)
+    with pytest.raises(IptError) as e:
+        api.ipt_cuda_tol(H, 140, 1e-12, 100)
+    assert e.value.status == 1
+
+
 def test_too_large_cluster_rejected(block_env):
     H = orc.rand_dense(2048, 1, 0.0, 6.25, 0.1)
     with pytest.raises(IptError) as e:

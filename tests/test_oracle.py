@@ -201,6 +201,32 @@ def test_block_cluster_solve_invariant_subspace():
     assert ang < 1e-9
 
 
+def wide_cluster_matrix(n, m, seed):
+    """m nearly degenerate states (spacing 1e-3, couplings 2e-2) below the
+    rest (U[5,10], couplings 1e-3): one cluster of width m, weakly coupled
+    to the others."""
+    rng = np.random.default_rng(seed)
+    d = np.concatenate([1.0 + 1e-3 * np.arange(m), np.sort(rng.uniform(5.0, 10.0, n - m))])
+    V = 1e-3 * rng.standard_normal((n, n))
+    V[:m, :m] = 2e-2 * rng.standard_normal((m, m))
+    V = np.triu(V, 1)
+    return np.asfortranarray(np.diag(d) + V + V.T)
+
+
+def test_wide_cluster_invariant_subspace():
+    """A 100-wide cluster (beyond the 64 default, IPT_MAX_LIFT_BLOCK_SIZE
+    raised): the converged basis spans the LAPACK invariant subspace of the
+    100 lowest eigenvalues."""
+    H = wide_cluster_matrix(600, 100, 7)
+    ranges, tl = orc.target_clusters(96, dense=H, max_block=142)
+    assert not tl and max(b - a + 1 for a, b in ranges) == 100
+    B, it, fp = orc.block_cluster_solve(0, 99, 100, 1e-13, dense=H)
+    assert fp <= 1e-13
+    w, V = np.linalg.eigh(H)
+    Q, _ = np.linalg.qr(B)
+    assert sl.subspace_angles(Q, V[:, :100]).max() < 1e-9
+
+
 def test_width_one_cluster_equals_plain_ipt_fixed_point():
     H = orc.kat_dense(48, 1)
     B, it, fp = orc.block_cluster_solve(3, 3, 200, 1e-14, dense=H)
