@@ -233,6 +233,10 @@ SIGNATURES = {
                                                            _c.c_double, _c.c_int, _vp,
                                                            _c.POINTER(IPTRayleighMixedCudaDeviceResult)]),
     "ipt_rayleigh_mixed_cuda_free_device_result": (None, [_c.POINTER(IPTRayleighMixedCudaDeviceResult)]),
+    "ipt_hip_prepare_sparse_csc": (_c.c_int, [_ip, _ip, _dp, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _dp, _ip,
+                                              _ip]),
+    "ipt_hip_prepared_vectors_to_original": (_c.c_int, [_ip, _ip, _dp, _c.c_int, _c.c_int, _c.c_int, _c.c_double,
+                                                        _dp, _c.c_int]),
     # include/ipt_hip_acx.h
     "acx_cuda_status_string": (_c.c_char_p, [_c.c_int]),
     "acx_cuda_reset_result": (None, [_c.POINTER(ACXCudaResult)]),

@@ -311,28 +311,13 @@ int sparse_with_initial_impl(const int *cp, const int *ri, const double *vals, c
     const size_t nk = (size_t)n * (size_t)k;
     auto run = [&]() -> int {
         auto t0 = std::chrono::steady_clock::now();
-        // Preparation, IPT/src/ipt_preparation.cu:1375-1420: stable sort of
-        // the diagonal and symmetric permutation of the matrix.
-        std::vector<double> dg;
-        csc_diagonal_host(cp, ri, vals, n, &dg);
-        std::vector<int> perm, inv;
-        stable_sort_permutation(dg, &perm, &inv);
-        bool permuted = false;
-        for (int i = 0; i < n; ++i) {
-            if (perm[(size_t)i] != i) {
-                permuted = true;
-                break;
-            }
-        }
-        HostCsc sorted;
-        const int *ucp = cp, *uri = ri;
-        const double *uv = vals;
-        if (permuted) {
-            permute_csc(cp, ri, vals, n, inv, &sorted);
-            ucp = sorted.col_ptr.data();
-            uri = sorted.row_ind.data();
-            uv = sorted.values.data();
-        }
+        // Preparation, IPT/src/ipt_preparation.cu:1375-1648 (the reference's
+        // plain CSC route, IPT/src/ipt_cuda.cu:5076-5092): stable sort of the
+        // diagonal and lifting of degenerate target subspaces.
+        LiftedCsc prep;
+        IPT_STCHK(prepare_lifted_csc(cp, ri, vals, n, k, nnz, env_double("IPT_DEGENERACY_THRESHOLD", 0.0), &prep));
+        const int *ucp = prep.a.col_ptr.data(), *uri = prep.a.row_ind.data();
+        const double *uv = prep.a.values.data();
         result->preparation_time_sec = seconds_since(t0);
         result->vectors = (double *)calloc(nk, sizeof(double));
         result->values = (double *)calloc((size_t)k, sizeof(double));
@@ -361,17 +346,8 @@ int sparse_with_initial_impl(const int *cp, const int *ri, const double *vals, c
         ipt_cuda_free_device_result(dv, dl);
         IPT_HIPCHK(e1);
         IPT_HIPCHK(e2);
-        if (permuted) {
-            // back to the caller's ordering, IPT/src/ipt_preparation.cu:1275-1291
-            std::vector<double> tmp((size_t)n);
-            for (int c = 0; c < k; ++c) {
-                double *vcol = result->vectors + (size_t)c * n;
-                for (int s = 0; s < n; ++s) {
-                    tmp[(size_t)perm[(size_t)s]] = vcol[s];
-                }
-                memcpy(vcol, tmp.data(), (size_t)n * sizeof(double));
-            }
-        }
+        // back to the caller's basis, IPT/src/ipt_preparation.cu:1222-1294
+        lifted_to_original(prep, result->vectors, k);
         result->n = n;
         result->k = k;
         result->iterations = iters;
@@ -573,4 +549,47 @@ extern "C" int ipt_cuda_sparse_csc_device_tol(const int *d_cp, const int *d_ri, 
 {
     return sparse_device_impl(d_cp, d_ri, d_v, n, k, nnz, maxiter, tol, 1, handle, d_vectors_out,
                               d_values_out, iterations_done_out, fixed_point_residual_out);
+}
+
+extern "C" int ipt_hip_prepare_sparse_csc(const int *col_ptr, const int *row_ind, const double *values, int n, int k,
+                                          int nnz, double degeneracy_threshold, double *dense_out, int *perm_out,
+                                          int *rotations_out)
+{
+    if (col_ptr == nullptr || row_ind == nullptr || values == nullptr || n <= 0 || k <= 0 || k > n || nnz < 0 ||
+        !csc_valid(col_ptr, row_ind, n, nnz)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    LiftedCsc prep;
+    IPT_STCHK(prepare_lifted_csc(col_ptr, row_ind, values, n, k, nnz, degeneracy_threshold, &prep));
+    if (dense_out != nullptr) {
+        memset(dense_out, 0, (size_t)n * n * sizeof(double));
+        for (int c = 0; c < n; ++c) {
+            for (int p = prep.a.col_ptr[(size_t)c]; p < prep.a.col_ptr[(size_t)c + 1]; ++p) {
+                dense_out[(size_t)prep.a.row_ind[(size_t)p] + (size_t)c * n] += prep.a.values[(size_t)p];
+            }
+        }
+    }
+    if (perm_out != nullptr) {
+        for (int i = 0; i < n; ++i) {
+            perm_out[i] = prep.perm.empty() ? i : prep.perm[(size_t)i];
+        }
+    }
+    if (rotations_out != nullptr) {
+        *rotations_out = (int)prep.rot_first.size();
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
+extern "C" int ipt_hip_prepared_vectors_to_original(const int *col_ptr, const int *row_ind, const double *values,
+                                                    int n, int k, int nnz, double degeneracy_threshold,
+                                                    double *vectors, int kv)
+{
+    if (col_ptr == nullptr || row_ind == nullptr || values == nullptr || vectors == nullptr || n <= 0 || k <= 0 ||
+        k > n || nnz < 0 || kv <= 0 || !csc_valid(col_ptr, row_ind, n, nnz)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    LiftedCsc prep;
+    IPT_STCHK(prepare_lifted_csc(col_ptr, row_ind, values, n, k, nnz, degeneracy_threshold, &prep));
+    lifted_to_original(prep, vectors, kv);
+    return IPT_CUDA_SUCCESS;
 }

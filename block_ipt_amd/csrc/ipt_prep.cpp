@@ -23,6 +23,8 @@ DegeneracyOptions degeneracy_options_from_env(double degeneracy_threshold)
     o.eta = env_double("IPT_COUPLING_ETA", 0.02);
     int mb = env_int("IPT_MAX_LIFT_BLOCK_SIZE", 64);
     o.max_block = mb > 0 ? mb : 64;
+    int mr = env_int("IPT_MAX_LIFT_ROUNDS", 3);
+    o.max_lift_rounds = mr > 0 ? mr : 3;
     return o;
 }
 
@@ -305,8 +307,14 @@ double gap_floor(double a, double b, double tau_num, const DegeneracyOptions &o)
 
 }  // namespace
 
-std::vector<std::pair<int, int>> target_clusters(const CouplingSource &m, const std::vector<double> &diag,
-                                                 int k, const DegeneracyOptions &o, bool *too_large)
+namespace {
+
+// Pairs (row, col) of the cluster rule for target column col: rows whose
+// diagonal lies within the target window (IPT/src/ipt_preparation.cu:426-441)
+// and that the pairwise test joins (443-478).
+template <typename F>
+void for_joined_pairs(const CouplingSource &m, const std::vector<double> &diag, int k, const DegeneracyOptions &o,
+                      F &&visit)
 {
     const int n = m.n();
     const double eps = DBL_EPSILON;
@@ -331,8 +339,6 @@ std::vector<std::pair<int, int>> target_clusters(const CouplingSource &m, const 
     for (int i = 0; i < n; ++i) {
         sd[(size_t)i] = diag[(size_t)order[(size_t)i]];
     }
-
-    UnionFind uf(n);
     for (int col = 0; col < k; ++col) {
         const double dc = diag[(size_t)col];
         const double tau_c = std::max(gap_floor(dc, dc, tau_num, o), off[(size_t)col] / ratio_tol);
@@ -352,10 +358,29 @@ std::vector<std::pair<int, int>> target_clusters(const CouplingSource &m, const 
             const double fl = gap_floor(dr, dc, tau_num, o);
             const double ratio = coupling > 0.0 ? coupling / std::max(gap, fl) : 0.0;
             if (gap <= fl || (coupling > 0.0 && ratio >= ratio_tol)) {
-                uf.unite(row, col);
+                visit(row, col);
             }
         }
     }
+}
+
+}  // namespace
+
+int target_gap_violations(const CouplingSource &m, const std::vector<double> &diag, int k,
+                          const DegeneracyOptions &o)
+{
+    int bad = 0;
+    for_joined_pairs(m, diag, k, o, [&](int, int) { ++bad; });
+    return bad;
+}
+
+std::vector<std::pair<int, int>> target_clusters(const CouplingSource &m, const std::vector<double> &diag,
+                                                 int k, const DegeneracyOptions &o, bool *too_large,
+                                                 bool add_targets)
+{
+    const int n = m.n();
+    UnionFind uf(n);
+    for_joined_pairs(m, diag, k, o, [&](int row, int col) { uf.unite(row, col); });
 
     std::vector<int> cmin((size_t)n, n), cmax((size_t)n, -1);
     std::vector<char> has((size_t)n, 0);
@@ -392,13 +417,312 @@ std::vector<std::pair<int, int>> target_clusters(const CouplingSource &m, const 
             covered[(size_t)i] = 1;
         }
     }
-    for (int t = 0; t < k; ++t) {
-        if (!covered[(size_t)t]) {
-            merged.emplace_back(t, t);
+    if (add_targets) {
+        for (int t = 0; t < k; ++t) {
+            if (!covered[(size_t)t]) {
+                merged.emplace_back(t, t);
+            }
+        }
+        std::sort(merged.begin(), merged.end());
+    }
+    return merged;
+}
+
+void sym_block_eigenvectors(const std::vector<double> &a, int m, std::vector<double> *v)
+{
+    // Classical Jacobi: rotate away the largest off-diagonal entry until
+    // all are <= 1e-14 (or 64 m^2 rotations); the same rotation formulas,
+    // stop rule and ordering as the reference, so that the lifted matrix
+    // and the vectors' normalisation agree with it.
+    std::vector<double> w = a;
+    v->assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) {
+        (*v)[(size_t)i * m + i] = 1.0;
+    }
+    auto at = [m](std::vector<double> &x, int r, int c) -> double & { return x[(size_t)r + (size_t)c * m]; };
+    const int max_rot = 64 * m * m;
+    for (int it = 0; it < max_rot; ++it) {
+        int p = 0, q = 1;
+        double big = 0.0;
+        for (int c = 1; c < m; ++c) {
+            for (int r = 0; r < c; ++r) {
+                const double x = std::fabs(at(w, r, c));
+                if (x > big) {
+                    big = x;
+                    p = r;
+                    q = c;
+                }
+            }
+        }
+        if (big <= 1.0e-14) {
+            break;
+        }
+        const double app = at(w, p, p), aqq = at(w, q, q), apq = at(w, p, q);
+        const double tau = (aqq - app) / (2.0 * apq);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t);
+        const double sn = t * c;
+        for (int r = 0; r < m; ++r) {
+            if (r == p || r == q) {
+                continue;
+            }
+            const double arp = at(w, r, p), arq = at(w, r, q);
+            const double np = c * arp - sn * arq, nq = sn * arp + c * arq;
+            at(w, r, p) = np;
+            at(w, p, r) = np;
+            at(w, r, q) = nq;
+            at(w, q, r) = nq;
+        }
+        at(w, p, p) = c * c * app - 2.0 * sn * c * apq + sn * sn * aqq;
+        at(w, q, q) = sn * sn * app + 2.0 * sn * c * apq + c * c * aqq;
+        at(w, p, q) = 0.0;
+        at(w, q, p) = 0.0;
+        for (int r = 0; r < m; ++r) {
+            const double vp = at(*v, r, p), vq = at(*v, r, q);
+            at(*v, r, p) = c * vp - sn * vq;
+            at(*v, r, q) = sn * vp + c * vq;
         }
     }
-    std::sort(merged.begin(), merged.end());
-    return merged;
+    // order by the Rayleigh quotients v^T a v (stable)
+    std::vector<double> lam((size_t)m, 0.0);
+    for (int j = 0; j < m; ++j) {
+        double x = 0.0;
+        for (int r = 0; r < m; ++r) {
+            for (int u = 0; u < m; ++u) {
+                x += (*v)[(size_t)r + (size_t)j * m] * a[(size_t)r + (size_t)u * m] * (*v)[(size_t)u + (size_t)j * m];
+            }
+        }
+        lam[(size_t)j] = x;
+    }
+    std::vector<int> order((size_t)m);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return lam[(size_t)x] < lam[(size_t)y]; });
+    std::vector<double> sorted((size_t)m * m);
+    for (int j = 0; j < m; ++j) {
+        std::copy(v->begin() + (size_t)order[(size_t)j] * m, v->begin() + (size_t)(order[(size_t)j] + 1) * m,
+                  sorted.begin() + (size_t)j * m);
+    }
+    v->swap(sorted);
+}
+
+namespace {
+
+// value at (row, col), duplicates summed (IPT/src/ipt_preparation.cu:830-844)
+double csc_value_sum(const HostCsc &a, int row, int col)
+{
+    double v = 0.0;
+    for (int p = a.col_ptr[(size_t)col]; p < a.col_ptr[(size_t)col + 1]; ++p) {
+        if (a.row_ind[(size_t)p] == row) {
+            v += a.values[(size_t)p];
+        }
+    }
+    return v;
+}
+
+void csc_diagonal_sum(const HostCsc &a, std::vector<double> *d)
+{
+    d->assign((size_t)a.n, 0.0);
+    for (int i = 0; i < a.n; ++i) {
+        (*d)[(size_t)i] = csc_value_sum(a, i, i);
+    }
+}
+
+// A <- S^-1 A S with S = I except S[first..last, first..last] = R (left =
+// R^-1 = R^T here), entries of each column summed by row, zeros dropped
+// (IPT/src/ipt_preparation.cu:1109-1220).
+void local_similarity(HostCsc *a, int first, int last, const std::vector<double> &right,
+                      const std::vector<double> &left)
+{
+    const int n = a->n, m = last - first + 1;
+    std::vector<std::vector<std::pair<int, double>>> cols((size_t)n);
+    for (int col = 0; col < n; ++col) {
+        const int cl = col - first;
+        const bool cin = cl >= 0 && cl < m;
+        for (int p = a->col_ptr[(size_t)col]; p < a->col_ptr[(size_t)col + 1]; ++p) {
+            const int row = a->row_ind[(size_t)p];
+            const int rl = row - first;
+            const bool rin = rl >= 0 && rl < m;
+            const double v = a->values[(size_t)p];
+            if (!rin && !cin) {
+                cols[(size_t)col].emplace_back(row, v);
+            } else if (!rin) {
+                for (int j = 0; j < m; ++j) {
+                    const double x = v * right[(size_t)cl + (size_t)j * m];
+                    if (x != 0.0) {
+                        cols[(size_t)(first + j)].emplace_back(row, x);
+                    }
+                }
+            } else if (!cin) {
+                for (int i = 0; i < m; ++i) {
+                    const double x = left[(size_t)i + (size_t)rl * m] * v;
+                    if (x != 0.0) {
+                        cols[(size_t)col].emplace_back(first + i, x);
+                    }
+                }
+            } else {
+                for (int j = 0; j < m; ++j) {
+                    const double rv = right[(size_t)cl + (size_t)j * m];
+                    if (rv == 0.0) {
+                        continue;
+                    }
+                    for (int i = 0; i < m; ++i) {
+                        const double x = left[(size_t)i + (size_t)rl * m] * v * rv;
+                        if (x != 0.0) {
+                            cols[(size_t)(first + j)].emplace_back(first + i, x);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    a->col_ptr.assign((size_t)n + 1, 0);
+    a->row_ind.clear();
+    a->values.clear();
+    for (int col = 0; col < n; ++col) {
+        auto &e = cols[(size_t)col];
+        std::stable_sort(e.begin(), e.end(), [](const std::pair<int, double> &x, const std::pair<int, double> &y) {
+            return x.first < y.first;
+        });
+        a->col_ptr[(size_t)col] = (int)a->row_ind.size();
+        for (size_t i = 0; i < e.size(); ++i) {
+            const int row = e[i].first;
+            double v = e[i].second;
+            while (i + 1 < e.size() && e[i + 1].first == row) {
+                ++i;
+                v += e[i].second;
+            }
+            if (v != 0.0) {
+                a->row_ind.push_back(row);
+                a->values.push_back(v);
+            }
+        }
+    }
+    a->col_ptr[(size_t)n] = (int)a->row_ind.size();
+}
+
+}  // namespace
+
+int prepare_lifted_csc(const int *col_ptr, const int *row_ind, const double *values, int n, int k, int nnz,
+                       double degeneracy_threshold, LiftedCsc *out)
+{
+    out->perm.clear();
+    out->rot_first.clear();
+    out->rot_m.clear();
+    out->rot_q.clear();
+    std::vector<double> dg;
+    csc_diagonal_host(col_ptr, row_ind, values, n, &dg);
+    std::vector<int> perm, inv;
+    stable_sort_permutation(dg, &perm, &inv);
+    bool permuted = false;
+    for (int i = 0; i < n; ++i) {
+        if (perm[(size_t)i] != i) {
+            permuted = true;
+            break;
+        }
+    }
+    if (permuted) {
+        permute_csc(col_ptr, row_ind, values, n, inv, &out->a);
+        out->perm = perm;
+    } else {
+        out->a.n = n;
+        out->a.col_ptr.assign(col_ptr, col_ptr + (size_t)n + 1);
+        out->a.row_ind.assign(row_ind, row_ind + (size_t)nnz);
+        out->a.values.assign(values, values + (size_t)nnz);
+    }
+    const DegeneracyOptions o = degeneracy_options_from_env(degeneracy_threshold);
+    const bool hermitian = csc_is_symmetric(out->a);
+    std::vector<double> diag;
+    bool clean = false;
+    for (int round = 0; round < o.max_lift_rounds; ++round) {
+        csc_diagonal_sum(out->a, &diag);
+        const CscCoupling cm(&out->a);
+        bool too_large = false;
+        const auto subspaces = target_clusters(cm, diag, k, o, &too_large, false);
+        if (too_large) {
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        if (subspaces.empty()) {
+            clean = target_gap_violations(cm, diag, k, o) == 0;
+            break;
+        }
+        if (!hermitian) {
+            // the reference lifts a non-symmetric block with LAPACK dgeev
+            // (and fails where that is absent); not built here
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        for (const auto &sp : subspaces) {
+            const int first = sp.first, m = sp.second - sp.first + 1;
+            std::vector<double> block((size_t)m * m, 0.0);
+            for (int c = 0; c < m; ++c) {
+                const int gc = first + c;
+                for (int p = out->a.col_ptr[(size_t)gc]; p < out->a.col_ptr[(size_t)gc + 1]; ++p) {
+                    const int gr = out->a.row_ind[(size_t)p];
+                    if (gr >= first && gr < first + m) {
+                        block[(size_t)(gr - first) + (size_t)c * m] += out->a.values[(size_t)p];
+                    }
+                }
+            }
+            std::vector<double> q, qt((size_t)m * m);
+            sym_block_eigenvectors(block, m, &q);
+            for (int c = 0; c < m; ++c) {
+                for (int r = 0; r < m; ++r) {
+                    qt[(size_t)r + (size_t)c * m] = q[(size_t)c + (size_t)r * m];
+                }
+            }
+            local_similarity(&out->a, first, sp.second, q, qt);
+            out->rot_first.push_back(first);
+            out->rot_m.push_back(m);
+            out->rot_q.insert(out->rot_q.end(), q.begin(), q.end());
+        }
+        csc_diagonal_sum(out->a, &diag);
+        const CscCoupling cm2(&out->a);
+        if (target_gap_violations(cm2, diag, k, o) == 0) {
+            clean = true;
+            break;
+        }
+    }
+    if (!clean) {
+        csc_diagonal_sum(out->a, &diag);
+        const CscCoupling cm(&out->a);
+        if (target_gap_violations(cm, diag, k, o) > 0) {
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
+void lifted_to_original(const LiftedCsc &p, double *vectors, int k)
+{
+    const int n = p.a.n;
+    size_t off = p.rot_q.size();
+    std::vector<double> tmp;
+    for (int r = (int)p.rot_first.size() - 1; r >= 0; --r) {
+        const int first = p.rot_first[(size_t)r], m = p.rot_m[(size_t)r];
+        off -= (size_t)m * m;
+        const double *q = p.rot_q.data() + off;
+        tmp.assign((size_t)m, 0.0);
+        for (int c = 0; c < k; ++c) {
+            double *x = vectors + (size_t)c * n + first;
+            for (int i = 0; i < m; ++i) {
+                double sum = 0.0;
+                for (int u = 0; u < m; ++u) {
+                    sum += q[(size_t)i + (size_t)u * m] * x[u];
+                }
+                tmp[(size_t)i] = sum;
+            }
+            std::copy(tmp.begin(), tmp.end(), x);
+        }
+    }
+    if (!p.perm.empty()) {
+        tmp.assign((size_t)n, 0.0);
+        for (int c = 0; c < k; ++c) {
+            double *x = vectors + (size_t)c * n;
+            for (int s = 0; s < n; ++s) {
+                tmp[(size_t)p.perm[(size_t)s]] = x[s];
+            }
+            std::copy(tmp.begin(), tmp.end(), x);
+        }
+    }
 }
 
 }  // namespace ipt

@@ -22,6 +22,7 @@ struct DegeneracyOptions {
     double ratio_tol = 0.2;
     double eta = 0.02;
     int max_block = 64;
+    int max_lift_rounds = 3;
 };
 DegeneracyOptions degeneracy_options_from_env(double degeneracy_threshold);
 
@@ -79,6 +80,36 @@ struct DenseCoupling : CouplingSource {
 // uncovered targets (IPT/src/ipt_preparation.cu:501-645,
 // IPT/src/ipt_cuda.cu:4114-4132).  `diag` is the sorted diagonal.
 std::vector<std::pair<int, int>> target_clusters(const CouplingSource &m, const std::vector<double> &diag,
-                                                 int k, const DegeneracyOptions &o, bool *too_large);
+                                                 int k, const DegeneracyOptions &o, bool *too_large,
+                                                 bool add_targets = true);
+
+// Number of (row, target column) pairs that the cluster rule would still
+// join, IPT/src/ipt_preparation.cu:658-757 (0: every target is separated).
+int target_gap_violations(const CouplingSource &m, const std::vector<double> &diag, int k,
+                          const DegeneracyOptions &o);
+
+// Sparse preparation of the plain path, IPT/src/ipt_preparation.cu:1375-1648:
+// stable sort of the diagonal (symmetric permutation) and, for symmetric
+// input, lifting of the degenerate target subspaces by local eigenvector
+// similarities, up to max_lift_rounds rounds, until no target pair is joined.
+struct LiftedCsc {
+    HostCsc a;                  // prepared matrix
+    std::vector<int> perm;      // perm[sorted] = original (empty: not permuted)
+    std::vector<int> rot_first; // rotations in the order applied
+    std::vector<int> rot_m;
+    std::vector<double> rot_q;  // their m x m eigenvector blocks, column-major
+};
+// Status: 0 ok; 1 invalid (a subspace wider than max_block, a non-symmetric
+// matrix needing a lift, or targets still joined after the last round).
+int prepare_lifted_csc(const int *col_ptr, const int *row_ind, const double *values, int n, int k, int nnz,
+                       double degeneracy_threshold, LiftedCsc *out);
+// vectors (n x k column-major, prepared basis) -> caller's basis: the
+// rotations in reverse order, then the inverse permutation,
+// IPT/src/ipt_preparation.cu:1222-1294.
+void lifted_to_original(const LiftedCsc &p, double *vectors, int k);
+// Eigenvectors of a symmetric m x m block (column-major) by classical
+// Jacobi, columns ordered by ascending Rayleigh quotient,
+// IPT/src/ipt_preparation.cu:759-828, 921-977.
+void sym_block_eigenvectors(const std::vector<double> &a, int m, std::vector<double> *v);
 
 }  // namespace ipt

@@ -120,6 +120,23 @@ int ipt_hip_mlp_gradient(const float *d_params, const float *d_feat, const float
 int ipt_hip_mlp_adam_step(float *d_params, float *d_m, float *d_v, const float *d_grad, float gscale, float lr,
                           int step, void *handle);
 
+/* The host preparation of the plain CSC entry points (no IPT_BLOCK_CLUSTER),
+ * IPT/src/ipt_preparation.cu:1375-1648: stable sort of the diagonal and
+ * lifting of degenerate target subspaces (symmetric input) by local
+ * eigenvector similarities.  Writes the prepared matrix densely (n x n
+ * column-major, may be NULL), perm_out[sorted] = original (may be NULL) and
+ * the number of rotations.  Status 1 where the reference's preparation
+ * fails (a subspace wider than IPT_MAX_LIFT_BLOCK_SIZE, a non-symmetric
+ * matrix that needs a lift, targets still joined after the last round).
+ * Host only; no device work. */
+int ipt_hip_prepare_sparse_csc(const int *col_ptr, const int *row_ind, const double *values, int n, int k, int nnz,
+                               double degeneracy_threshold, double *dense_out, int *perm_out, int *rotations_out);
+/* vectors (n x kv, column-major) from the prepared basis of the same call to
+ * the caller's: the rotations in reverse order, then the inverse
+ * permutation, IPT/src/ipt_preparation.cu:1222-1294.  Host only. */
+int ipt_hip_prepared_vectors_to_original(const int *col_ptr, const int *row_ind, const double *values, int n, int k,
+                                         int nnz, double degeneracy_threshold, double *vectors, int kv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -272,6 +272,49 @@ def ref_target_clusters(cp, ri, v, k, degeneracy_threshold=0.0):
     return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(nr)], bool(tl.value)
 
 
+def _ref_csc_args(cp, ri, v):
+    return (np.ascontiguousarray(cp, dtype=np.int32), np.ascontiguousarray(ri, dtype=np.int32),
+            np.ascontiguousarray(v, dtype=np.float64))
+
+
+def ref_prepare_sparse_csc(cp, ri, v, k, degeneracy_threshold=0.0):
+    """The reference's own plain-CSC preparation (sort + degeneracy lifting,
+    IPT/src/ipt_preparation.cu:1375-1648) from oracle/_ref: returns
+    (status, prepared dense n x n, perm[sorted] = original, rotations), or
+    None when oracle/_ref is not built."""
+    if not os.path.exists(_REF_PATH):
+        return None
+    rl = ctypes.CDLL(_REF_PATH)
+    f = rl.ref_prepare_sparse_csc
+    f.restype = ctypes.c_int
+    f.argtypes = [_ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _ip]
+    cp, ri, v = _ref_csc_args(cp, ri, v)
+    n = len(cp) - 1
+    dense = np.zeros((n, n), order="F")
+    perm = np.zeros(n, dtype=np.int32)
+    rot = ctypes.c_int(0)
+    st = f(_i(cp), _i(ri), _d(v), n, int(k), int(cp[-1]), float(degeneracy_threshold), _d(dense), _i(perm),
+           ctypes.byref(rot))
+    return st, dense, perm, rot.value
+
+
+def ref_prepared_vectors_to_original(cp, ri, v, k, X, degeneracy_threshold=0.0):
+    """The reference's ipt_apply_prepared_q_to_vectors on a copy of X (n x kv)."""
+    if not os.path.exists(_REF_PATH):
+        return None
+    rl = ctypes.CDLL(_REF_PATH)
+    f = rl.ref_prepared_vectors_to_original
+    f.restype = ctypes.c_int
+    f.argtypes = [_ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, ctypes.c_int]
+    cp, ri, v = _ref_csc_args(cp, ri, v)
+    X = np.array(X, dtype=np.float64, order="F", copy=True)
+    st = f(_i(cp), _i(ri), _d(v), len(cp) - 1, int(k), int(cp[-1]), float(degeneracy_threshold), _d(X),
+           X.shape[1])
+    if st != 0:
+        raise OracleError(st)
+    return X
+
+
 def block_cluster_solve(first, last, maxiter, tol, damping=1.0, dense=None, csc=None):
     dn, cp, ri, v, n = _mat_args(dense, csc)
     m = last - first + 1

@@ -169,3 +169,32 @@ def test_sparse_device_entry():
     ov, ol, oit, _ = orc.ipt_csc(cp, ri, v, k, tol=1e-13, maxiter=200)
     assert it == oit
     assert np.abs(vec - ov).max() <= 1e-12 and np.abs(val - ol).max() <= 1e-12
+
+
+@pytest.mark.parametrize("n,k,seed", [(60, 12, 1), (120, 30, 2)])
+def test_sparse_plain_lifts_degeneracies(monkeypatch, n, k, seed):
+    """The plain CSC route with exactly repeated diagonal entries: the
+    library's preparation lifts them (the unprepared map has zero gaps), the
+    solve converges to the LAPACK pairs, and the vectors equal the oracle's
+    plain IPT on the reference's own prepared matrix mapped back with the
+    reference's routine (oracle/_ref), when that is built."""
+    from test_lift import degenerate_csc
+
+    for v in ("IPT_BLOCK_CLUSTER", "IPT_MAX_LIFT_ROUNDS", "IPT_MAX_LIFT_BLOCK_SIZE", "IPT_DEGENERACY_THRESHOLD"):
+        monkeypatch.delenv(v, raising=False)
+    csc, A = degenerate_csc(n, seed, shuffle=True)
+    gv, gl, info = api.ipt_cuda_sparse_csc_tol(*csc, k, 1e-12, 200)
+    assert info["fixed_point_residual"] <= 1e-12
+    w = np.linalg.eigvalsh(A)
+    assert np.abs(np.sort(gl) - w[:k]).max() <= 1e-11
+    res = np.linalg.norm(A @ gv - gv * gl, axis=0) / np.linalg.norm(gv, axis=0)
+    assert res.max() <= 1e-10
+    ref = orc.ref_prepare_sparse_csc(*csc, k)
+    if ref is not None:
+        st, D, perm, rot = ref
+        assert st == 0 and rot >= 1
+        ov, ol, oit, ofp = orc.ipt_dense(D, k, tol=1e-12, maxiter=200)
+        ov = orc.ref_prepared_vectors_to_original(*csc, k, ov)
+        assert oit == info["iterations"]
+        assert np.abs(gv - ov).max() <= 1e-10
+        assert np.abs(gl - ol).max() <= 1e-11
