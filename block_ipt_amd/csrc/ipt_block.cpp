@@ -796,6 +796,35 @@ void result_init_block(IPTCudaResult *r)
     r->cluster_residual_max = NAN;
 }
 
+// Target-count and adaptive-expansion fields, IPT/src/ipt_cuda.cu:4613,
+// 4636-4667.
+void fill_target_fields(IPTCudaResult *r, const BlockTargets &bt, const std::vector<std::pair<int, int>> &ranges,
+                        int k, const std::vector<int> &added, bool limit_hit)
+{
+    if (r == nullptr) {
+        return;
+    }
+    r->oversample = bt.oversample;
+    r->adaptive_block_enabled = bt.adaptive ? 1 : 0;
+    r->adaptive_coupling_tau = bt.tau;
+    r->adaptive_limit_hit = limit_hit ? 1 : 0;
+    std::string text;
+    for (int i : added) {
+        if (!text.empty()) {
+            text += ";";
+        }
+        text += std::to_string(i);
+    }
+    snprintf(r->adaptive_added_indices, sizeof(r->adaptive_added_indices), "%s",
+             text.empty() ? "none" : text.c_str());
+    for (const auto &rg : ranges) {
+        if (rg.first <= k - 1 && k - 1 <= rg.second) {
+            r->adaptive_target_block_start = rg.first;
+            r->adaptive_target_block_end = rg.second;
+        }
+    }
+}
+
 // Dense matrix on the device: GPU-side preparation (diagonal, symmetry,
 // off-diagonal strengths, infinity norm), cluster detection on the host
 // from the few coupling values it needs, permutation to sorted order when
@@ -1003,7 +1032,9 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
     if (early) {
         overlap = [&]() { return early(*pr); };
     }
-    return detect_clusters(ctx, sd, soff, n, k, gather, &pr->ranges, overlap);
+    // IPT_BLOCK_CLUSTER_OVERSAMPLE widens the targets; adaptive expansion
+    // reads CSC columns and applies to sparse input only
+    return detect_clusters(ctx, sd, soff, n, block_targets_from_env(n, k).target_k, gather, &pr->ranges, overlap);
 }
 
 }  // namespace
@@ -1066,6 +1097,7 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     if (info != nullptr) {
         fill_result(info, br, n, k);
+        fill_target_fields(info, block_targets_from_env(n, k), pr.ranges, k, std::vector<int>(), false);
         info->preparation_time_sec = prep;
         info->solve_time_sec = seconds_since(t0);
     }
@@ -1141,7 +1173,7 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
     GatherFn gather = [&](const int *drc, int npairs, double *dv) {
         return csr_gather_pairs(ctx, d_rp, d_ci, d_v, drc, npairs, dv);
     };
-    IPT_STCHK(detect_clusters(ctx, hd, hmax, n, k, gather, &ranges));
+    IPT_STCHK(detect_clusters(ctx, hd, hmax, n, block_targets_from_env(n, k).target_k, gather, &ranges));
     const double prep = seconds_since(t0);
     int maxcols = 0;
     for (const auto &r : ranges) {
@@ -1254,7 +1286,8 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         const DegeneracyOptions o = degeneracy_options_from_env(env_double("IPT_DEGENERACY_THRESHOLD", 0.0));
         bool too_large = false;
         CscCoupling cc(&a);
-        std::vector<std::pair<int, int>> ranges = target_clusters(cc, d, k, o, &too_large);
+        const BlockTargets bt = block_targets_from_env(n, k);
+        std::vector<std::pair<int, int>> ranges = target_clusters(cc, d, bt.target_k, o, &too_large);
         if (too_large) {
             int worst = 0;
             for (const auto &r : ranges) {
@@ -1265,6 +1298,9 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
                     worst, o.max_block);
             return IPT_CUDA_INVALID_ARGUMENT;
         }
+        std::vector<int> added;
+        const bool limit_hit = bt.adaptive && adaptive_expand(a, d, k, o.max_block, bt.tau, &ranges, &added);
+        fill_target_fields(result, bt, ranges, k, added, limit_hit);
         const double normH = csc_inf_norm(col_ptr, row_ind, values, n);
         result->preparation_time_sec = seconds_since(t0);
         auto t1 = std::chrono::steady_clock::now();

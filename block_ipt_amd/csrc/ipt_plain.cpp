@@ -429,9 +429,12 @@ int sparse_device_impl(const int *d_cp, const int *d_ri, const double *d_v, int 
         if (!ok) {
             return IPT_CUDA_INVALID_ARGUMENT;
         }
-        bool host_path = false;
-        IPT_STCHK(block_sparse_device(ctx, d_cp, d_ri, d_v, n, k, maxiter, tol, use_tol, d_vec_out, d_val_out,
-                                      iters_out, fp_out, &host_path));
+        // adaptive expansion reads the CSC columns on the host
+        bool host_path = block_targets_from_env(n, k).adaptive;
+        if (!host_path) {
+            IPT_STCHK(block_sparse_device(ctx, d_cp, d_ri, d_v, n, k, maxiter, tol, use_tol, d_vec_out, d_val_out,
+                                          iters_out, fp_out, &host_path));
+        }
         if (!host_path) {
             return IPT_CUDA_SUCCESS;
         }

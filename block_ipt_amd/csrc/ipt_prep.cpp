@@ -725,4 +725,107 @@ void lifted_to_original(const LiftedCsc &p, double *vectors, int k)
     }
 }
 
+BlockTargets block_targets_from_env(int n, int k)
+{
+    BlockTargets t;
+    t.adaptive = env_flag("IPT_BLOCK_CLUSTER_ADAPTIVE") != 0;
+    t.tau = env_double("IPT_BLOCK_CLUSTER_COUPLING_TAU", 0.1);
+    t.oversample = env_int("IPT_BLOCK_CLUSTER_OVERSAMPLE", 0);
+    if (t.adaptive || t.oversample < 0) {
+        t.oversample = 0;
+    }
+    t.target_k = k;
+    if (t.oversample > 0) {
+        t.target_k = t.oversample > n - k ? n : k + t.oversample;
+    }
+    return t;
+}
+
+bool adaptive_expand(const HostCsc &a, const std::vector<double> &d, int k, int max_block, double tau,
+                     std::vector<std::pair<int, int>> *ranges, std::vector<int> *added)
+{
+    const double eps_gap = 1.0e-14;
+    const int n = (int)d.size();
+    bool limit_hit = false;
+    added->clear();
+    for (;;) {
+        int id = -1;
+        for (size_t i = 0; i < ranges->size(); ++i) {
+            if ((*ranges)[i].first <= k - 1 && k - 1 <= (*ranges)[i].second) {
+                id = (int)i;
+                break;
+            }
+        }
+        if (id < 0) {
+            break;
+        }
+        const int first = (*ranges)[(size_t)id].first, last = (*ranges)[(size_t)id].second;
+        int best = -1;
+        double best_ratio = 0.0;
+        for (int col = first; col <= last; ++col) {
+            for (int p = a.col_ptr[(size_t)col]; p < a.col_ptr[(size_t)col + 1]; ++p) {
+                const int row = a.row_ind[(size_t)p];
+                if (row >= first && row <= last) {
+                    continue;
+                }
+                const double ratio =
+                    std::fabs(a.values[(size_t)p]) / std::max(std::fabs(d[(size_t)col] - d[(size_t)row]), eps_gap);
+                if (std::max(last, row) - std::min(first, row) + 1 > max_block) {
+                    if (ratio > tau) {
+                        limit_hit = true;
+                    }
+                    continue;
+                }
+                if (ratio > best_ratio) {
+                    best_ratio = ratio;
+                    best = row;
+                }
+            }
+        }
+        if (best < 0 || best_ratio <= tau) {
+            break;
+        }
+        int lo = std::min(first, best), hi = std::max(last, best);
+        // rows whose diagonal equals a boundary row's to 1000 eps join too
+        auto same = [&](int i, int j) {
+            const double scale = std::max(1.0, std::max(std::fabs(d[(size_t)i]), std::fabs(d[(size_t)j])));
+            return std::fabs(d[(size_t)i] - d[(size_t)j]) <= std::max(eps_gap, 1000.0 * DBL_EPSILON * scale);
+        };
+        for (bool grew = true; grew;) {
+            grew = false;
+            if (lo > 0 && same(lo, lo - 1)) {
+                --lo;
+                grew = true;
+            }
+            if (hi + 1 < n && same(hi + 1, hi)) {
+                ++hi;
+                grew = true;
+            }
+        }
+        if (hi - lo + 1 > max_block) {
+            limit_hit = true;
+            break;
+        }
+        for (int i = lo; i <= hi; ++i) {
+            if (i < first || i > last) {
+                added->push_back(i);
+            }
+        }
+        (*ranges)[(size_t)id] = {lo, hi};
+        std::sort(ranges->begin(), ranges->end());
+        std::vector<std::pair<int, int>> merged;
+        for (const auto &r : *ranges) {
+            if (merged.empty() || r.first > merged.back().second) {
+                merged.push_back(r);
+            } else {
+                merged.back().second = std::max(merged.back().second, r.second);
+            }
+        }
+        ranges->swap(merged);
+    }
+    std::sort(added->begin(), added->end());
+    added->erase(std::unique(added->begin(), added->end()), added->end());
+    return limit_hit;
+}
+
 }  // namespace ipt

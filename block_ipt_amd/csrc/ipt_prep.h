@@ -112,4 +112,25 @@ void lifted_to_original(const LiftedCsc &p, double *vectors, int k);
 // IPT/src/ipt_preparation.cu:759-828, 921-977.
 void sym_block_eigenvectors(const std::vector<double> &a, int m, std::vector<double> *v);
 
+// Target count and adaptive expansion of the block-cluster path,
+// IPT/src/ipt_cuda.cu:4596-4613: IPT_BLOCK_CLUSTER_OVERSAMPLE (0) more
+// targets (all n when it exceeds n - k), IPT_BLOCK_CLUSTER_ADAPTIVE (off;
+// forces oversample 0) with coupling threshold IPT_BLOCK_CLUSTER_COUPLING_TAU
+// (0.1).
+struct BlockTargets {
+    int target_k = 0;
+    int oversample = 0;
+    bool adaptive = false;
+    double tau = 0.1;
+};
+BlockTargets block_targets_from_env(int n, int k);
+// Grow the cluster that holds target k-1 by its most strongly coupled
+// outside row (|a_rc| / max(|d_c - d_r|, 1e-14) > tau), with rows of
+// (numerically) equal diagonal alongside, while the widened range stays
+// within max_block; overlapping ranges merge.  added: the rows brought in
+// (sorted, unique).  Returns whether the width limit stopped a coupling
+// above tau.  IPT/src/ipt_cuda.cu:4134-4272.
+bool adaptive_expand(const HostCsc &a, const std::vector<double> &d, int k, int max_block, double tau,
+                     std::vector<std::pair<int, int>> *ranges, std::vector<int> *added);
+
 }  // namespace ipt

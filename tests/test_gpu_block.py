@@ -170,3 +170,68 @@ def test_nonsymmetric_rejected(block_env):
     with pytest.raises(IptError) as e:
         api.ipt_cuda_sparse_csc_tol(*_dense_to_csc(H), 4, 1e-12, 100)
     assert e.value.status == 1
+
+
+def _chain_csc(n, strong):
+    """Tridiagonal-plus-noise CSC with diagonal 0..n-1 and weak couplings,
+    plus the given strong couplings {(i, j): value}."""
+    rng = np.random.default_rng(4)
+    A = np.diag(np.arange(n, dtype=float))
+    for i in range(n - 1):
+        A[i, i + 1] = A[i + 1, i] = 1e-3 * rng.uniform(0.5, 1.0)
+    for (i, j), x in strong.items():
+        A[i, j] = A[j, i] = x
+    return _dense_to_csc(A), A
+
+
+def test_oversample_targets(block_env):
+    """IPT_BLOCK_CLUSTER_OVERSAMPLE=8: eight more target clusters in the
+    basis, the same k lowest pairs (IPT/src/ipt_cuda.cu:4603-4613)."""
+    block_env.setenv("IPT_BLOCK_CLUSTER_OVERSAMPLE", "8")
+    H = orc.rand_dense(500, 2, 0.0, 10.0, 0.05)
+    gv, gl, info = api.ipt_cuda_tol(H, 12, 1e-12, 100)
+    _check_eigs(H, gv, gl, 12)
+    assert info["oversample"] == 8 and info["basis_cols"] >= 20
+    csc, A = _chain_csc(200, {})
+    gv, gl, info = api.ipt_cuda_sparse_csc_tol(*csc, 10, 1e-12, 100)
+    _check_eigs(A, gv, gl, 10)
+    assert info["oversample"] == 8 and info["basis_cols"] >= 18
+
+
+def test_adaptive_expansion(block_env):
+    """IPT_BLOCK_CLUSTER_ADAPTIVE: the cluster of target k-1 = 9 takes in row
+    10 (coupling/gap 0.15: above tau 0.1, below the cluster rule's 0.2), then
+    row 11 of equal diagonal... here row 11 is 2 units away with ratio 0.05
+    and stays out; the fields report it and the pairs are LAPACK's."""
+    block_env.setenv("IPT_BLOCK_CLUSTER_ADAPTIVE", "1")
+    block_env.setenv("IPT_BLOCK_CLUSTER_OVERSAMPLE", "5")  # ignored when adaptive
+    csc, A = _chain_csc(300, {(9, 10): 0.15, (10, 12): 0.1})
+    gv, gl, info = api.ipt_cuda_sparse_csc_tol(*csc, 10, 1e-12, 200)
+    _check_eigs(A, gv, gl, 10)
+    assert info["adaptive_block_enabled"] == 1 and info["oversample"] == 0
+    assert abs(info["adaptive_coupling_tau"] - 0.1) < 1e-15
+    assert (info["adaptive_target_block_start"], info["adaptive_target_block_end"]) == (9, 10)
+    assert info["adaptive_added_indices"] == "10" and info["adaptive_limit_hit"] == 0
+    # the same through the device CSC entry (host preparation)
+    hip = api.hip()
+    cp, ri, v = csc
+    bufs = [hip.to_device(x) for x in (cp, ri, v)]
+    try:
+        dv, dl, it, fp = api.ipt_cuda_sparse_csc_device_tol(*bufs, 300, 10, len(v), 1e-12, 200)
+        vec, val = hip.to_host(dv, (300, 10)), hip.to_host(dl, (10,))
+        api.free_device_result(dv, dl)
+    finally:
+        for b in bufs:
+            hip.free(b)
+    _check_eigs(A, vec, val, 10)
+
+
+def test_adaptive_limit(block_env):
+    """A coupling above tau that would widen the target block past
+    IPT_MAX_LIFT_BLOCK_SIZE sets adaptive_limit_hit and is left out."""
+    block_env.setenv("IPT_BLOCK_CLUSTER_ADAPTIVE", "1")
+    block_env.setenv("IPT_MAX_LIFT_BLOCK_SIZE", "4")
+    csc, A = _chain_csc(100, {(9, 20): 1.5})
+    gv, gl, info = api.ipt_cuda_sparse_csc_tol(*csc, 10, 1e-12, 200)
+    _check_eigs(A, gv, gl, 10)
+    assert info["adaptive_limit_hit"] == 1 and info["adaptive_added_indices"] == "none"
