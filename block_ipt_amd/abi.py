@@ -164,6 +164,30 @@ class ACXCudaResult(_c.Structure):
         ("fixed_point_residual", _c.c_double), ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]
 
 
+class ACXMixedTF32CudaResult(_c.Structure):
+    """Reference IPT/src/ACX/acx_mixed_precision_cuda.cu:19-37."""
+    _fields_ = [
+        ("n", _c.c_int), ("k", _c.c_int), ("tf32_iterations", _c.c_int), ("fp64_iterations", _c.c_int),
+        ("tf32_f_calls", _c.c_int), ("fp64_f_calls", _c.c_int), ("tf32_converged", _c.c_int),
+        ("fp64_converged", _c.c_int), ("tf32_time_sec", _c.c_double), ("fp64_time_sec", _c.c_double),
+        ("total_time_sec", _c.c_double), ("finalization_time_sec", _c.c_double),
+        ("tf32_fixed_point_residual", _c.c_double), ("fp64_fixed_point_residual", _c.c_double),
+        ("fp64_max_residual", _c.c_double), ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]
+
+
+class ACXRayleighMixedTF32CudaResult(_c.Structure):
+    """Reference IPT/src/ACX/acx_rayleigh_mixed_precision_cuda.cu:19-40."""
+    _fields_ = [
+        ("n", _c.c_int), ("k", _c.c_int), ("tf32_iterations", _c.c_int), ("fp64_iterations", _c.c_int),
+        ("tf32_f_calls", _c.c_int), ("fp64_f_calls", _c.c_int), ("tf32_converged", _c.c_int),
+        ("fp64_converged", _c.c_int), ("tf32_time_sec", _c.c_double), ("orthogonalize_time_sec", _c.c_double),
+        ("rayleigh_transform_time_sec", _c.c_double), ("fp64_time_sec", _c.c_double),
+        ("finalization_time_sec", _c.c_double), ("backtransform_time_sec", _c.c_double),
+        ("total_time_sec", _c.c_double), ("tf32_fixed_point_residual", _c.c_double),
+        ("fp64_fixed_point_residual", _c.c_double), ("fp64_max_residual", _c.c_double),
+        ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]
+
+
 _dp = _c.POINTER(_c.c_double)
 _ip = _c.POINTER(_c.c_int)
 _dpp = _c.POINTER(_dp)
@@ -243,6 +267,16 @@ SIGNATURES = {
     "acx_cuda_free_result": (None, [_c.POINTER(ACXCudaResult)]),
     "acx_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _vp,
                                        _c.POINTER(ACXCudaResult)]),
+    "acx_mixed_precision_tf32_cuda_reset_result": (None, [_c.POINTER(ACXMixedTF32CudaResult)]),
+    "acx_mixed_precision_tf32_cuda_free_result": (None, [_c.POINTER(ACXMixedTF32CudaResult)]),
+    "acx_mixed_precision_tf32_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
+                                                            _c.c_double, _c.c_int, _vp,
+                                                            _c.POINTER(ACXMixedTF32CudaResult)]),
+    "acx_rayleigh_mixed_tf32_cuda_reset_result": (None, [_c.POINTER(ACXRayleighMixedTF32CudaResult)]),
+    "acx_rayleigh_mixed_tf32_cuda_free_result": (None, [_c.POINTER(ACXRayleighMixedTF32CudaResult)]),
+    "acx_rayleigh_mixed_tf32_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
+                                                           _c.c_double, _c.c_int, _vp,
+                                                           _c.POINTER(ACXRayleighMixedTF32CudaResult)]),
 }
 
 _lib = None

@@ -294,6 +294,32 @@ def acx_free_result(r):
     lib().acx_cuda_free_result(ctypes.byref(r))
 
 
+_ACX_MIXED = {
+    "tf32": ("acx_mixed_precision_tf32_cuda_device_tol", "acx_mixed_precision_tf32_cuda_free_result",
+             abi.ACXMixedTF32CudaResult),
+    "rayleigh_tf32": ("acx_rayleigh_mixed_tf32_cuda_device_tol", "acx_rayleigh_mixed_tf32_cuda_free_result",
+                      abi.ACXRayleighMixedTF32CudaResult),
+}
+
+
+def acx_mixed_device_tol(kind, d_H, n, k, lo_tol, lo_maxiter, hi_tol, hi_maxiter, handle=None):
+    """Two-phase ACX entry (include/ipt_hip_acx.h); kind is tf32 |
+    rayleigh_tf32.  Returns (vectors, values, info) copied to the host."""
+    name, free_name, R = _ACX_MIXED[kind]
+    r = R()
+    st = getattr(lib(), name)(ctypes.c_void_p(d_H), int(n), int(k), float(lo_tol), int(lo_maxiter),
+                              float(hi_tol), int(hi_maxiter), handle, ctypes.byref(r))
+    if st != 0:
+        raise IptError(st, name)
+    info = {f[0]: getattr(r, f[0]) for f in R._fields_ if not f[0].startswith("d_")}
+    try:
+        vec = hip().to_host(r.d_vectors, (n, k))
+        val = hip().to_host(r.d_values, (k,))
+    finally:
+        getattr(lib(), free_name)(ctypes.byref(r))
+    return vec, val, info
+
+
 # ------------------------------------------------------------------ extensions
 
 LOWP_MODES = {"fp16": 0, "bf16": 1, "bf16x3": 2, "fp32": 3, "fp8": 4}

@@ -15,8 +15,9 @@
 #include <cfloat>
 #include <chrono>
 #include <cstring>
+#include <functional>
 
-#include "ipt_internal.h"
+#include "ipt_acx.h"
 #include "../../include/ipt_hip_acx.h"
 
 namespace ipt {
@@ -268,7 +269,7 @@ __global__ void k_acx_diag_reduce(const double *__restrict__ part, int splits, i
 }
 
 struct AcxWork {
-    const double *H;
+    const std::function<int(const double *, double *)> *product;  // Y = H X, n x k
     int n, k, nchunks;
     const double *d;
     double *Y;
@@ -276,11 +277,11 @@ struct AcxWork {
     int *flag;
 };
 
-// F(X) -> Fout with the MODE sums (see k_acx_map); one FP64 product.
+// F(X) -> Fout with the MODE sums (see k_acx_map); one product.
 int acx_map(Context *ctx, const AcxWork &w, int mode, const double *X, double *Fout, const double *x0,
             const double *f1)
 {
-    IPT_STCHK(gemm_f64(ctx, w.H, w.n, X, w.n, w.Y, w.n, w.n, w.k));
+    IPT_STCHK((*w.product)(X, w.Y));
     const dim3 grid((unsigned)((long)w.k * w.nchunks));
     switch (mode) {
     case MAP_FIXED_POINT:
@@ -304,6 +305,8 @@ int acx_map(Context *ctx, const AcxWork &w, int mode, const double *X, double *F
     return IPT_CUDA_SUCCESS;
 }
 
+}  // namespace
+
 // values[c] = (H V)(c, c) for c < k.
 int acx_diag_product(Context *ctx, const double *H, int n, const double *V, int k, double *values)
 {
@@ -323,46 +326,43 @@ int acx_diag_product(Context *ctx, const double *H, int n, const double *V, int 
     return IPT_CUDA_SUCCESS;
 }
 
-double now_seconds()
+static double now_seconds()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-int acx_run(Context *ctx, const double *dH, int n, int k, double tol, int maxiter, ACXCudaResult *result)
+int acx_loop(Context *ctx, const double *d_diag, int n, int k, AcxRun *run)
 {
-    const double t_start = now_seconds();
     const size_t nk = (size_t)n * (size_t)k;
     const int nchunks = (n + kRowsPerChunk - 1) / kRowsPerChunk;
     if ((long)k * nchunks > 0x7fffffffL) {
         return ACX_CUDA_INVALID_ARGUMENT;
     }
-    const double stop_tol = tol < ACX_FIXED_POINT_RTOL ? tol : ACX_FIXED_POINT_RTOL;
-    DevBuf<double> x, f1, f2, f3, Y, d, part, vals, sums;
+    DevBuf<double> x, f1, f2, f3, Y, part, sums;
     DevBuf<int> flag;
     IPT_STCHK(x.alloc(nk));
     IPT_STCHK(f1.alloc(nk));
     IPT_STCHK(f2.alloc(nk));
     IPT_STCHK(f3.alloc(nk));
     IPT_STCHK(Y.alloc(nk));
-    IPT_STCHK(d.alloc((size_t)n));
     IPT_STCHK(part.alloc(3 * (size_t)k * nchunks));
-    IPT_STCHK(vals.alloc((size_t)k));
     IPT_STCHK(sums.alloc(3));
     IPT_STCHK(flag.alloc(1));
     IPT_HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
-    IPT_STCHK(dense_diagonal(ctx, dH, n, n, d.p));
-    IPT_STCHK(set_identity(ctx, x.p, n, n, k));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    const AcxWork w{dH, n, k, nchunks, d.p, Y.p, part.p, flag.p};
+    if (run->x0 != nullptr) {
+        IPT_HIPCHK(hipMemcpyAsync(x.p, run->x0, nk * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        IPT_STCHK(set_identity(ctx, x.p, n, n, k));
+    }
+    const AcxWork w{run->product, n, k, nchunks, d_diag, Y.p, part.p, flag.p};
     double *hs = ctx->host_scalars;
     int *hf = ctx->host_flags;
-
-    const double t_iter = now_seconds();
-    for (int iter = 1; iter <= maxiter; ++iter) {
+    bool final_f1 = true;
+    for (int iter = 1; iter <= run->maxiter; ++iter) {
         const int order = (iter % 2 == 1) ? 2 : 3;
-        result->iterations = iter;
+        run->iterations = iter;
         IPT_STCHK(acx_map(ctx, w, MAP_FIXED_POINT, x.p, f1.p, nullptr, nullptr));
-        result->f_calls += 1;
+        run->f_calls += 1;
         hipLaunchKernelGGL(k_acx_reduce, dim3(1), dim3(1024), 0, ctx->stream, part.p, k, nchunks, sums.p);
         IPT_HIPCHK(hipGetLastError());
         IPT_HIPCHK(hipMemcpyAsync(hs, sums.p, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -371,31 +371,33 @@ int acx_run(Context *ctx, const double *dH, int n, int k, double tol, int maxite
         if (hf[0] != 0) {
             return ACX_CUDA_ZERO_DENOMINATOR;
         }
-        result->max_residual = hs[0];
+        run->max_residual = hs[0];
         const double delta_norm = std::sqrt(hs[1]);
         const double current_norm = std::sqrt(hs[2]);
         if (!(current_norm > 0.0) || !std::isfinite(current_norm)) {
             return ACX_CUDA_INVALID_ARGUMENT;
         }
         const double fp = delta_norm / current_norm;
-        result->fixed_point_residual = fp;
-        if (fp < stop_tol) {
-            result->converged = 1;
+        run->fixed_point_residual = fp;
+        if (run->stop_inclusive ? fp <= run->stop_tol : fp < run->stop_tol) {
+            run->converged = 1;
+            final_f1 = true;
             break;
         }
         if (order == 2) {
             IPT_STCHK(acx_map(ctx, w, MAP_ORDER2, f1.p, f2.p, x.p, nullptr));
-            result->f_calls += 1;
+            run->f_calls += 1;
             hipLaunchKernelGGL(k_acx_extrapolate<2>, dim3((unsigned)((long)k * nchunks)), dim3(kThreads), 0,
                                ctx->stream, x.p, f1.p, f2.p, nullptr, n, nchunks, part.p);
         } else {
             IPT_STCHK(acx_map(ctx, w, MAP_PLAIN, f1.p, f2.p, nullptr, nullptr));
             IPT_STCHK(acx_map(ctx, w, MAP_ORDER3, f2.p, f3.p, x.p, f1.p));
-            result->f_calls += 2;
+            run->f_calls += 2;
             hipLaunchKernelGGL(k_acx_extrapolate<3>, dim3((unsigned)((long)k * nchunks)), dim3(kThreads), 0,
                                ctx->stream, x.p, f1.p, f2.p, f3.p, n, nchunks, part.p);
         }
         IPT_HIPCHK(hipGetLastError());
+        final_f1 = !run->final_x_unconverged;
     }
     // maps after the last stop test are checked here
     IPT_HIPCHK(hipMemcpyAsync(hf, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -403,14 +405,50 @@ int acx_run(Context *ctx, const double *dH, int n, int k, double tol, int maxite
     if (hf[0] != 0) {
         return ACX_CUDA_ZERO_DENOMINATOR;
     }
+    run->vectors.release();
+    if (final_f1) {
+        run->vectors.p = f1.take();
+    } else {
+        run->vectors.p = x.take();
+    }
+    run->vectors.count = nk;
+    return ACX_CUDA_SUCCESS;
+}
+
+namespace {
+
+int acx_run(Context *ctx, const double *dH, int n, int k, double tol, int maxiter, ACXCudaResult *result)
+{
+    const double t_start = now_seconds();
+    DevBuf<double> d, vals;
+    IPT_STCHK(d.alloc((size_t)n));
+    IPT_STCHK(vals.alloc((size_t)k));
+    IPT_STCHK(dense_diagonal(ctx, dH, n, n, d.p));
+    const std::function<int(const double *, double *)> product = [&](const double *X, double *Y) {
+        return gemm_f64(ctx, dH, n, X, n, Y, n, n, k);
+    };
+    AcxRun run;
+    run.product = &product;
+    run.stop_tol = tol < ACX_FIXED_POINT_RTOL ? tol : ACX_FIXED_POINT_RTOL;
+    run.maxiter = maxiter;
+    const double t_iter = now_seconds();
+    const int st = acx_loop(ctx, d.p, n, k, &run);
+    result->iterations = run.iterations;
+    result->f_calls = run.f_calls;
+    result->converged = run.converged;
+    result->max_residual = run.max_residual;
+    result->fixed_point_residual = run.fixed_point_residual;
+    if (st != ACX_CUDA_SUCCESS) {
+        return st;
+    }
     result->iteration_time_sec = now_seconds() - t_iter;
     const double t_fin = now_seconds();
-    IPT_STCHK(acx_diag_product(ctx, dH, n, f1.p, k, vals.p));
+    IPT_STCHK(acx_diag_product(ctx, dH, n, run.vectors.p, k, vals.p));
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     result->finalization_time_sec = now_seconds() - t_fin;
     result->n = n;
     result->k = k;
-    result->d_vectors = f1.take();
+    result->d_vectors = run.vectors.take();
     result->d_values = vals.take();
     result->time_sec = now_seconds() - t_start;
     return ACX_CUDA_SUCCESS;

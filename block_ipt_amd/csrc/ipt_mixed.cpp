@@ -21,7 +21,9 @@
 #include <cstring>
 #include <vector>
 
+#include "../../include/ipt_hip_acx.h"
 #include "../../include/ipt_hip_mixed.h"
+#include "ipt_acx.h"
 #include "ipt_block.h"
 #include "ipt_lowp.h"
 #include "ipt_plain.h"
@@ -422,4 +424,279 @@ extern "C" int ipt_rayleigh_mixed_tf32_cuda_device_tol(const double *dH, int n, 
     result->d_vectors = V.take();
     result->d_values = vals.take();
     return IPT_CUDA_SUCCESS;
+}
+
+// ------------------------------------------------------------------ two-phase ACX
+
+namespace ipt {
+namespace {
+
+// Low-precision ACX phase (IPT/src/ACX/acx_mixed_precision_cuda.cu:360-573):
+// ACX from the identity with H X = diag(d) X + E X, E X in BF16x3, stop at
+// fp <= tol; returns f1 when converged, else the last extrapolated x.
+int acx_lowp_phase(Context *ctx, const double *dH, int n, int k, double tol, int maxiter, const double *d_diag,
+                   AcxRun *run)
+{
+    double mx = 0.0;
+    IPT_STCHK(offdiag_absmax(ctx, dH, n, &mx));
+    LowpOperator op;
+    IPT_STCHK(lowp_operator_create(ctx, dH, n, n, LOWP_BF16X3, mx, &op));
+    const std::function<int(const double *, double *)> product = [&](const double *X, double *Y) {
+        return lowp_apply_delta(ctx, op, X, nullptr, nullptr, n, d_diag, k, Y);
+    };
+    run->product = &product;
+    run->stop_tol = tol;
+    run->stop_inclusive = true;
+    run->final_x_unconverged = true;
+    run->maxiter = maxiter;
+    return acx_loop(ctx, d_diag, n, k, run);
+}
+
+// FP64 ACX on a device matrix A (m x m, ld m) from x0 (NULL: identity),
+// stop at fp < min(tol, 1e-12); values diag(A vectors).
+int acx_fp64_phase(Context *ctx, const double *A, int m, int k, double tol, int maxiter, const double *x0,
+                   bool final_x_unconverged, AcxRun *run, DevBuf<double> *values, double *fin_time)
+{
+    DevBuf<double> d;
+    IPT_STCHK(d.alloc((size_t)m));
+    IPT_STCHK(dense_diagonal(ctx, A, m, m, d.p));
+    const std::function<int(const double *, double *)> product = [&](const double *X, double *Y) {
+        return gemm_f64(ctx, A, m, X, m, Y, m, m, k);
+    };
+    run->product = &product;
+    run->x0 = x0;
+    run->stop_tol = tol < 1.0e-12 ? tol : 1.0e-12;
+    run->stop_inclusive = false;
+    run->final_x_unconverged = final_x_unconverged;
+    run->maxiter = maxiter;
+    const int st = acx_loop(ctx, d.p, m, k, run);
+    if (st != IPT_CUDA_SUCCESS) {
+        return st;
+    }
+    const double t = now_seconds();
+    IPT_STCHK(values->alloc((size_t)k));
+    IPT_STCHK(acx_diag_product(ctx, A, m, run->vectors.p, k, values->p));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    *fin_time = now_seconds() - t;
+    return IPT_CUDA_SUCCESS;
+}
+
+bool acx_mixed_args_ok(const double *dH, int n, int k, double lo_tol, int lo_maxiter, double hi_tol, int hi_maxiter)
+{
+    // IPT/src/ACX/acx_mixed_precision_cuda.cu:832-836 (the handle may be NULL here)
+    return !(dH == nullptr || n <= 0 || k <= 0 || k > n || lo_tol <= 0.0 || hi_tol <= 0.0 || lo_maxiter <= 0 ||
+             hi_maxiter <= 0);
+}
+
+}  // namespace
+}  // namespace ipt
+
+extern "C" void acx_mixed_precision_tf32_cuda_reset_result(ACXMixedTF32CudaResult *r)
+{
+    if (r == nullptr) {
+        return;
+    }
+    memset(r, 0, sizeof(*r));
+    r->tf32_time_sec = NAN;
+    r->fp64_time_sec = NAN;
+    r->total_time_sec = NAN;
+    r->finalization_time_sec = NAN;
+    r->tf32_fixed_point_residual = NAN;
+    r->fp64_fixed_point_residual = NAN;
+    r->fp64_max_residual = NAN;
+}
+
+extern "C" void acx_mixed_precision_tf32_cuda_free_result(ACXMixedTF32CudaResult *r)
+{
+    if (r == nullptr) {
+        return;
+    }
+    if (r->d_vectors != nullptr) {
+        (void)hipFree(r->d_vectors);
+    }
+    if (r->d_values != nullptr) {
+        (void)hipFree(r->d_values);
+    }
+    acx_mixed_precision_tf32_cuda_reset_result(r);
+}
+
+extern "C" int acx_mixed_precision_tf32_cuda_device_tol(const double *dH, int n, int k, double tf32_tol,
+                                                        int tf32_maxiter, double fp64_tol, int fp64_maxiter,
+                                                        void *handle, ACXMixedTF32CudaResult *result)
+{
+    using namespace ipt;
+    if (result == nullptr) {
+        return ACX_CUDA_INVALID_ARGUMENT;
+    }
+    acx_mixed_precision_tf32_cuda_reset_result(result);
+    if (!acx_mixed_args_ok(dH, n, k, tf32_tol, tf32_maxiter, fp64_tol, fp64_maxiter)) {
+        return ACX_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    auto run = [&]() -> int {
+        const double t0 = now_seconds();
+        DevBuf<double> d;
+        IPT_STCHK(d.alloc((size_t)n));
+        IPT_STCHK(dense_diagonal(ctx, dH, n, n, d.p));
+        AcxRun lo;
+        const int s1 = acx_lowp_phase(ctx, dH, n, k, tf32_tol, tf32_maxiter, d.p, &lo);
+        result->tf32_iterations = lo.iterations;
+        result->tf32_f_calls = lo.f_calls;
+        result->tf32_converged = lo.converged;
+        result->tf32_fixed_point_residual = lo.fixed_point_residual;
+        if (s1 != IPT_CUDA_SUCCESS) {
+            return s1;
+        }
+        result->tf32_time_sec = now_seconds() - t0;
+        const double t1 = now_seconds();
+        AcxRun hi;
+        DevBuf<double> vals;
+        double fin = 0.0;
+        const int s2 = acx_fp64_phase(ctx, dH, n, k, fp64_tol, fp64_maxiter, lo.vectors.p, true, &hi, &vals, &fin);
+        result->fp64_iterations = hi.iterations;
+        result->fp64_f_calls = hi.f_calls;
+        result->fp64_converged = hi.converged;
+        result->fp64_fixed_point_residual = hi.fixed_point_residual;
+        result->fp64_max_residual = hi.max_residual;
+        if (s2 != IPT_CUDA_SUCCESS) {
+            return s2;
+        }
+        result->fp64_time_sec = now_seconds() - t1;
+        result->finalization_time_sec = fin;
+        result->total_time_sec = now_seconds() - t0;
+        result->n = n;
+        result->k = k;
+        result->d_vectors = hi.vectors.take();
+        result->d_values = vals.take();
+        return IPT_CUDA_SUCCESS;
+    };
+    st = run();
+    if (st != IPT_CUDA_SUCCESS) {
+        acx_mixed_precision_tf32_cuda_free_result(result);
+    }
+    return st;
+}
+
+extern "C" void acx_rayleigh_mixed_tf32_cuda_reset_result(ACXRayleighMixedTF32CudaResult *r)
+{
+    if (r == nullptr) {
+        return;
+    }
+    memset(r, 0, sizeof(*r));
+    r->tf32_time_sec = NAN;
+    r->orthogonalize_time_sec = NAN;
+    r->rayleigh_transform_time_sec = NAN;
+    r->fp64_time_sec = NAN;
+    r->finalization_time_sec = NAN;
+    r->backtransform_time_sec = NAN;
+    r->total_time_sec = NAN;
+    r->tf32_fixed_point_residual = NAN;
+    r->fp64_fixed_point_residual = NAN;
+    r->fp64_max_residual = NAN;
+}
+
+extern "C" void acx_rayleigh_mixed_tf32_cuda_free_result(ACXRayleighMixedTF32CudaResult *r)
+{
+    if (r == nullptr) {
+        return;
+    }
+    if (r->d_vectors != nullptr) {
+        (void)hipFree(r->d_vectors);
+    }
+    if (r->d_values != nullptr) {
+        (void)hipFree(r->d_values);
+    }
+    acx_rayleigh_mixed_tf32_cuda_reset_result(r);
+}
+
+extern "C" int acx_rayleigh_mixed_tf32_cuda_device_tol(const double *dH, int n, int k, double tf32_tol,
+                                                       int tf32_maxiter, double fp64_tol, int fp64_maxiter,
+                                                       void *handle, ACXRayleighMixedTF32CudaResult *result)
+{
+    using namespace ipt;
+    if (result == nullptr) {
+        return ACX_CUDA_INVALID_ARGUMENT;
+    }
+    acx_rayleigh_mixed_tf32_cuda_reset_result(result);
+    if (!acx_mixed_args_ok(dH, n, k, tf32_tol, tf32_maxiter, fp64_tol, fp64_maxiter)) {
+        return ACX_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    auto run = [&]() -> int {
+        double t = now_seconds();
+        DevBuf<double> d;
+        IPT_STCHK(d.alloc((size_t)n));
+        IPT_STCHK(dense_diagonal(ctx, dH, n, n, d.p));
+        AcxRun lo;
+        const int s1 = acx_lowp_phase(ctx, dH, n, k, tf32_tol, tf32_maxiter, d.p, &lo);
+        result->tf32_iterations = lo.iterations;
+        result->tf32_f_calls = lo.f_calls;
+        result->tf32_converged = lo.converged;
+        result->tf32_fixed_point_residual = lo.fixed_point_residual;
+        if (s1 != IPT_CUDA_SUCCESS) {
+            return s1;
+        }
+        result->tf32_time_sec = now_seconds() - t;
+        // Householder QR of the iterate (acx_rayleigh_mixed_precision_cuda.cu:
+        // 111-170; the column normalisation before it does not change Q)
+        t = now_seconds();
+        double *Q = lo.vectors.p;
+        IPT_STCHK(orthogonalize_qr(ctx, Q, n, k));
+        result->orthogonalize_time_sec = now_seconds() - t;
+        // B = Q^T (H Q), symmetrised
+        t = now_seconds();
+        const size_t nk = (size_t)n * (size_t)k;
+        DevBuf<double> HQ, B;
+        IPT_STCHK(HQ.alloc(nk));
+        IPT_STCHK(B.alloc((size_t)k * k));
+        IPT_STCHK(gemm_f64(ctx, dH, n, Q, n, HQ.p, n, n, k));
+        IPT_STCHK(gemm_tn_tall(ctx, Q, n, HQ.p, n, B.p, k, n, k, k));
+        IPT_STCHK(symmetrize(ctx, B.p, k, k));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        result->rayleigh_transform_time_sec = now_seconds() - t;
+        // FP64 ACX on the k x k quotient (the FP64 entry point's rules)
+        t = now_seconds();
+        AcxRun hi;
+        DevBuf<double> vals;
+        double fin = 0.0;
+        const int s2 = acx_fp64_phase(ctx, B.p, k, k, fp64_tol, fp64_maxiter, nullptr, false, &hi, &vals, &fin);
+        result->fp64_iterations = hi.iterations;
+        result->fp64_f_calls = hi.f_calls;
+        result->fp64_converged = hi.converged;
+        result->fp64_fixed_point_residual = hi.fixed_point_residual;
+        result->fp64_max_residual = hi.max_residual;
+        if (s2 != IPT_CUDA_SUCCESS) {
+            return s2;
+        }
+        result->fp64_time_sec = now_seconds() - t;
+        result->finalization_time_sec = fin;
+        // vectors = Q Z
+        t = now_seconds();
+        DevBuf<double> V;
+        IPT_STCHK(V.alloc(nk));
+        IPT_STCHK(gemm_nn_small(ctx, Q, n, hi.vectors.p, k, V.p, n, n, k, k, 1.0, 0.0));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        result->backtransform_time_sec = now_seconds() - t;
+        result->total_time_sec = result->tf32_time_sec + result->orthogonalize_time_sec +
+                                 result->rayleigh_transform_time_sec + result->fp64_time_sec +
+                                 result->backtransform_time_sec;
+        result->n = n;
+        result->k = k;
+        result->d_vectors = V.take();
+        result->d_values = vals.take();
+        return IPT_CUDA_SUCCESS;
+    };
+    st = run();
+    if (st != IPT_CUDA_SUCCESS) {
+        acx_rayleigh_mixed_tf32_cuda_free_result(result);
+    }
+    return st;
 }

@@ -64,6 +64,72 @@ void acx_cuda_free_result(ACXCudaResult *result);
 int acx_cuda_device_tol(const double *d_matrix_col_major, int n, int k, double tol, int maxiter, void *handle,
                         ACXCudaResult *result);
 
+/* Two-phase ACX (IPT/src/ACX/acx_mixed_precision_cuda.cu:19-52, 811-897):
+ * ACX with the off-diagonal product in low precision ("tf32" = BF16x3 on
+ * gfx950, which has no TF32; the iterate is kept in FP64) until
+ * ||f1 - x|| / ||x|| <= tf32_tol, then FP64 ACX restarted from that iterate
+ * (f1 when converged, else the last extrapolated x) to min(fp64_tol, 1e-12);
+ * the returned vectors are the FP64 phase's f1 when it converged, else its
+ * last x, and the values diag(H vectors). */
+typedef struct {
+    int n;
+    int k;
+    int tf32_iterations;
+    int fp64_iterations;
+    int tf32_f_calls;
+    int fp64_f_calls;
+    int tf32_converged;
+    int fp64_converged;
+    double tf32_time_sec;
+    double fp64_time_sec;
+    double total_time_sec;
+    double finalization_time_sec;
+    double tf32_fixed_point_residual;
+    double fp64_fixed_point_residual;
+    double fp64_max_residual;
+    double *d_vectors;
+    double *d_values;
+} ACXMixedTF32CudaResult;
+
+void acx_mixed_precision_tf32_cuda_reset_result(ACXMixedTF32CudaResult *result);
+void acx_mixed_precision_tf32_cuda_free_result(ACXMixedTF32CudaResult *result);
+int acx_mixed_precision_tf32_cuda_device_tol(const double *d_matrix_col_major, int n, int k, double tf32_tol,
+                                             int tf32_maxiter, double fp64_tol, int fp64_maxiter, void *handle,
+                                             ACXMixedTF32CudaResult *result);
+
+/* Rayleigh two-phase ACX (IPT/src/ACX/acx_rayleigh_mixed_precision_cuda.cu:
+ * 19-52, 289-434): low-precision ACX as above, Householder QR of its
+ * iterate Q, FP64 ACX on the k x k Rayleigh quotient Q^T H Q (symmetrised),
+ * vectors Q Z. */
+typedef struct {
+    int n;
+    int k;
+    int tf32_iterations;
+    int fp64_iterations;
+    int tf32_f_calls;
+    int fp64_f_calls;
+    int tf32_converged;
+    int fp64_converged;
+    double tf32_time_sec;
+    double orthogonalize_time_sec;
+    double rayleigh_transform_time_sec;
+    double fp64_time_sec;
+    double finalization_time_sec;
+    double backtransform_time_sec;
+    double total_time_sec;
+    double tf32_fixed_point_residual;
+    double fp64_fixed_point_residual;
+    double fp64_max_residual;
+    double *d_vectors;
+    double *d_values;
+} ACXRayleighMixedTF32CudaResult;
+
+void acx_rayleigh_mixed_tf32_cuda_reset_result(ACXRayleighMixedTF32CudaResult *result);
+void acx_rayleigh_mixed_tf32_cuda_free_result(ACXRayleighMixedTF32CudaResult *result);
+int acx_rayleigh_mixed_tf32_cuda_device_tol(const double *d_matrix_col_major, int n, int k, double tf32_tol,
+                                            int tf32_maxiter, double fp64_tol, int fp64_maxiter, void *handle,
+                                            ACXRayleighMixedTF32CudaResult *result);
+
 #ifdef __cplusplus
 }
 #endif

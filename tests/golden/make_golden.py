@@ -143,6 +143,29 @@ def acx_sweep():
                               "source": "IPT/results/ACX/acx_vs_original_ipt_summary.csv", "rows": rows})
 
 
+def acx_mixed_sweep():
+    """Two-phase ACX (TF32 then FP64): IPT/results/ACX/ACX_MIXED/
+    acx_mixed_tf32_cuda_sweep_epsilon_*.csv, the configuration of
+    IPT/test/ACX/ACX_MIXED/benchmark_acx_mixed_tf32_cuda_sweep.cu:17-26 (k = N,
+    the ACX sweep's matrices and seeds)."""
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(REF, "results/ACX/ACX_MIXED/acx_mixed_tf32_cuda_sweep_epsilon_0_*.csv"))):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                rows.append({"n": int(r["N"]), "epsilon": float(r["epsilon"]),
+                             "tf32_iterations": int(r["tf32_iterations"]), "tf32_f_calls": int(r["tf32_f_calls"]),
+                             "fp64_iterations": int(r["fp64_iterations"]), "fp64_f_calls": int(r["fp64_f_calls"]),
+                             "tf32_converged": int(r["tf32_converged"]), "fp64_converged": int(r["fp64_converged"]),
+                             "tf32_fixed_point_residual": float(r["tf32_fixed_point_residual"]),
+                             "fp64_fixed_point_residual": float(r["fp64_fixed_point_residual"]),
+                             "time_mixed_total_sec": float(r["time_mixed_total_sec"]), "status": r["status"]})
+    _write("acx_mixed_sweep.json", {"seed_base": 20260615, "tf32_tol": 1e-6, "fp64_tol": 1e-12, "maxiter": 1000,
+                                    "source": "IPT/results/ACX/ACX_MIXED/acx_mixed_tf32_cuda_sweep_epsilon_*.csv",
+                                    "rows": rows})
+
+
 if __name__ == "__main__":
     runtests_kat()
     sparse_kat()
@@ -150,3 +173,4 @@ if __name__ == "__main__":
     h2o_csc()
     cluster_ranges()
     acx_sweep()
+    acx_mixed_sweep()

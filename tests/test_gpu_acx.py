@@ -125,3 +125,67 @@ def test_device_eps_generator_matches_oracle():
     finally:
         hip.free(dH)
     assert np.array_equal(G, orc.eps_dense(n, eps, seed))
+
+
+def _mixed_golden():
+    with open(os.path.join(ROOT, "tests", "golden", "acx_mixed_sweep.json")) as f:
+        return json.load(f)
+
+
+def _run_mixed(kind, H, k, lo_tol, hi_tol, maxiter=1000):
+    hip = api.hip()
+    n = H.shape[0]
+    dH = hip.to_device(np.asfortranarray(H))
+    try:
+        return api.acx_mixed_device_tol(kind, dH, n, k, lo_tol, maxiter, hi_tol, maxiter)
+    finally:
+        hip.free(dH)
+
+
+@pytest.mark.parametrize("n,k,eps", [(512, 512, 0.1), (1024, 96, 0.12), (1500, 1500, 0.06)])
+def test_two_phase_equals_fp64(n, k, eps):
+    """The two-phase entry converges to the FP64 entry's fixed point: the
+    same unit-diagonal vectors and values to rounding."""
+    H = orc.eps_dense(n, eps, acx.sweep_seed(n, eps))
+    v64, l64, i64 = _run(H, k, 1e-12, 1000)
+    vec, val, info = _run_mixed("tf32", H, k, 1e-6, 1e-12)
+    assert info["tf32_converged"] == 1 and info["fp64_converged"] == 1
+    assert info["tf32_fixed_point_residual"] <= 1e-6 and info["fp64_fixed_point_residual"] < 1e-12
+    assert np.abs(vec - v64).max() <= 1e-9
+    assert np.abs(val - l64).max() <= 1e-11 * np.abs(l64).max()
+    # the low-precision phase saves FP64 map calls
+    assert info["fp64_f_calls"] < i64["f_calls"]
+
+
+@pytest.mark.parametrize("row", [r for r in _mixed_golden()["rows"] if r["n"] == 4096],
+                         ids=lambda r: "eps%g" % r["epsilon"])
+def test_two_phase_published_counts(row):
+    """IPT/results/ACX/ACX_MIXED (TF32 on the reference's GPU, BF16x3 here,
+    whose products are the more accurate): both phases converge, the
+    low-precision phase no later than the published one and the FP64 phase,
+    which starts from that more accurate iterate, no more than one iteration
+    later (observed: equal or shorter, e.g. 4 + 5 against 6 + 7 at eps 0.12)."""
+    g = _mixed_golden()
+    n, eps = row["n"], row["epsilon"]
+    print("acx two-phase published n=%d eps=%g" % (n, eps), flush=True)
+    H = orc.eps_dense(n, eps, acx.sweep_seed(n, eps, g["seed_base"]))
+    vec, val, info = _run_mixed("tf32", H, n, g["tf32_tol"], g["fp64_tol"], g["maxiter"])
+    assert info["tf32_converged"] == 1 and info["fp64_converged"] == 1
+    got = (info["tf32_iterations"], info["fp64_iterations"])
+    want = (row["tf32_iterations"], row["fp64_iterations"])
+    assert got[0] <= want[0] and got[1] <= want[1] + 1, (got, want)
+    assert info["fp64_fixed_point_residual"] < g["fp64_tol"]
+
+
+@pytest.mark.parametrize("n,k,eps", [(600, 40, 0.1), (800, 800, 0.08)])
+def test_rayleigh_two_phase(n, k, eps):
+    """Rayleigh two-phase ACX: values equal the FP64 entry's; vectors are the
+    Ritz vectors Q Z, whose residuals are set by the low-precision phase's
+    tolerance (1e-6) when k < N."""
+    H = orc.eps_dense(n, eps, acx.sweep_seed(n, eps))
+    v64, l64, _ = _run(H, k, 1e-12, 1000)
+    vec, val, info = _run_mixed("rayleigh_tf32", H, k, 1e-6, 1e-12)
+    assert info["tf32_converged"] == 1 and info["fp64_converged"] == 1
+    assert np.abs(np.sort(val) - np.sort(l64)).max() <= 1e-9 * np.abs(l64).max()
+    res = np.linalg.norm(H @ vec - vec * val, axis=0) / np.linalg.norm(vec, axis=0)
+    assert res.max() <= (1e-5 if k < n else 1e-9)
