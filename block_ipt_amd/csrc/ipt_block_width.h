@@ -36,10 +36,10 @@ struct BlockUpdateArgs {
     int *flags;
 };
 
-// Broadcast of lane L of each P-lane group (P <= 16) by DPP: row_newbcast
-// (lane L of every 16-lane row) for P = 16 and, with a select on lane bit 3,
-// for P = 8; quad_perm for P = 4 and 2.  P = 32 groups span two rows: a
-// shuffle.  No LDS traffic for P <= 16.
+// Broadcast of lane L of each P-lane group by DPP: row_newbcast (lane L of
+// every 16-lane row) for P = 16 and, with a select on lane bit 3, for P = 8;
+// quad_perm for P = 4 and 2; for P = 32, row_newbcast followed by gfx950's
+// permlane16_swap across the two rows of the group.  No LDS traffic.
 template <int P, int L>
 __device__ __forceinline__ double group_bcast(double v)
 {
@@ -54,7 +54,18 @@ __device__ __forceinline__ double group_bcast(double v)
     } else if constexpr (P == 2) {
         return __builtin_amdgcn_update_dpp(v, v, L | (L << 2) | ((2 + L) << 4) | ((2 + L) << 6), 0xf, 0xf, false);
     } else {
-        return __shfl(v, L, P);
+        // P = 32: row_newbcast gives each 16-lane row its own lane L % 16;
+        // permlane16_swap then hands the source row's value to the other
+        // row of the pair (the swapped-in operand for an even source row,
+        // the swapped-out one for an odd source row), per 32-bit half.
+        const double b = __builtin_amdgcn_update_dpp(v, v, 0x150 + (L & 15), 0xf, 0xf, false);
+        const unsigned long long u = (unsigned long long)__double_as_longlong(b);
+        const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+        const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        const unsigned olo = L < 16 ? rl[0] : rl[1];
+        const unsigned ohi = L < 16 ? rh[0] : rh[1];
+        return __longlong_as_double((long long)(((unsigned long long)ohi << 32) | olo));
     }
 }
 
