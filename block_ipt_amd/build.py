@@ -33,6 +33,7 @@ SOURCES = [
     "ipt_gen.hip",
     "ipt_gen_sparse.hip",
     "ipt_sparse.hip",
+    "ipt_sparse_lp.hip",
     "ipt_acx.hip",
     "ipt_mlp.hip",
     "ipt_profile.cpp",

@@ -29,6 +29,7 @@
 
 #include "ipt_block.h"
 #include "ipt_lowp.h"
+#include "ipt_sparse_lp.h"
 #include "ipt_small_eig.h"
 #include "ipt_prep.h"
 
@@ -47,6 +48,8 @@ double seconds_since(const std::chrono::steady_clock::time_point &t0)
 
 // Y(:, 0:nc) = H X(:, 0:nc), both column-major with leading dimension n.
 using ApplyFn = std::function<int(const double *X, double *Y, int nc)>;
+// Y = Ya + diag(d) (B - Ba) + E (B - Ba) with E (B - Ba) in low precision
+using DeltaFn = std::function<int(const double *B, const double *Ba, const double *Ya, int nc, double *Y)>;
 
 struct BlockParams {
     int maxiter = 100;
@@ -535,7 +538,7 @@ void record_stats(Context *ctx, const BlockResult &br)
 // FP64 step; clusters are frozen only on exact steps, so the stopping rule
 // is the FP64 one.  `identity_product` (optional) fills H * identity blocks
 // without a product (dense: gathered columns of H).
-int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_product, const LowpOperator *lowp,
+int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_product, const DeltaFn *lowp,
                    const double *d, const double *rowabs, int n, int k,
                    const std::vector<std::pair<int, int>> &ranges, double normH, const BlockParams &p,
                    BlockResult *out)
@@ -589,7 +592,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             out->fp64_products += 1;
             out->matvecs += bc;
         } else {
-            IPT_STCHK(lowp_apply_delta(ctx, *lowp, cur, Banc.p, Yanc.p, n, d, bc, Y.p));
+            IPT_STCHK((*lowp)(cur, Banc.p, Yanc.p, bc, Y.p));
             out->lowp_products += 1;
         }
         IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, exact ? p.tol : -1.0));
@@ -1074,9 +1077,12 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
         }
         return IPT_CUDA_SUCCESS;
     };
+    const DeltaFn delta = [&](const double *B, const double *Ba, const double *Ya, int nc, double *Y) {
+        return lowp_apply_delta(ctx, lowp, B, Ba, Ya, n, pr.d.p, nc, Y);
+    };
     BlockResult br;
-    IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp, pr.d.p, pr.rowabs.p, n, k, pr.ranges, pr.normH, p,
-                             &br));
+    IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp != nullptr ? &delta : nullptr, pr.d.p, pr.rowabs.p, n,
+                             k, pr.ranges, pr.normH, p, &br));
     record_stats(ctx, br);
     if (p.verbose) {
         fprintf(stderr,
@@ -1193,8 +1199,21 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
         }
         return IPT_CUDA_SUCCESS;
     };
+    // IPT_PRECISION on CSR input: the FP32 anchored product (any
+    // requested low precision maps to FP32 for sparse input)
+    SparseLowp slp;
+    DeltaFn delta;
+    if (p.lowp_mode >= 0) {
+        int nnz = 0;
+        IPT_HIPCHK(hipMemcpy(&nnz, d_rp + n, sizeof(int), hipMemcpyDeviceToHost));
+        IPT_STCHK(sparse_lowp_create(ctx, d_rp, d_ci, d_v, n, nnz, &slp));
+        delta = [&](const double *B, const double *Ba, const double *Ya, int nc, double *Y) {
+            return sparse_lowp_apply_delta(ctx, slp, B, Ba, Ya, n, d.p, nc, Y);
+        };
+    }
     BlockResult br;
-    IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), nullptr, d.p, rowabs.p, n, k, ranges, normH, p, &br));
+    IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), p.lowp_mode >= 0 ? &delta : nullptr, d.p, rowabs.p, n, k, ranges,
+                             normH, p, &br));
     record_stats(ctx, br);
     if (p.verbose) {
         fprintf(stderr,
@@ -1359,8 +1378,17 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
             }
             return IPT_CUDA_SUCCESS;
         };
+        SparseLowp slp;
+        DeltaFn delta;
+        if (p.lowp_mode >= 0) {
+            IPT_STCHK(sparse_lowp_create(ctx, rp.p, ci.p, vv.p, n, pn, &slp));
+            delta = [&](const double *B, const double *Ba, const double *Ya, int nc, double *Y) {
+                return sparse_lowp_apply_delta(ctx, slp, B, Ba, Ya, n, dd.p, nc, Y);
+            };
+        }
         BlockResult br;
-        IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), nullptr, dd.p, drabs.p, n, k, ranges, normH, p, &br));
+        IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), p.lowp_mode >= 0 ? &delta : nullptr, dd.p, drabs.p, n, k,
+                                 ranges, normH, p, &br));
         record_stats(ctx, br);
         DevBuf<double> vec;
         IPT_STCHK(vec.alloc((size_t)n * k));

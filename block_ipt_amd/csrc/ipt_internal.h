@@ -137,7 +137,7 @@ struct DevBuf {
 
 // ------------------------------------------------------------------ profiling
 
-enum { PROF_GEMM_F64 = 0, PROF_LOWP_GEMM = 1, PROF_BLOCK_UPDATE = 2, PROF_SPMM = 3, PROF_COUNT = 4 };
+enum { PROF_GEMM_F64 = 0, PROF_LOWP_GEMM = 1, PROF_BLOCK_UPDATE = 2, PROF_SPMM = 3, PROF_SPMM_F32 = 4, PROF_COUNT = 5 };
 struct ProfileToken {
     int id = -1;
     hipEvent_t start = nullptr;

@@ -9,7 +9,7 @@
 namespace ipt {
 
 namespace {
-const char *kNames[PROF_COUNT] = {"gemm_f64", "lowp_gemm", "block_update", "spmm"};
+const char *kNames[PROF_COUNT] = {"gemm_f64", "lowp_gemm", "block_update", "spmm", "spmm_f32"};
 std::mutex g_mu;
 bool g_on = false;
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev[PROF_COUNT];

@@ -68,7 +68,8 @@ int ipt_hip_lowp_product(const double *d_H, int n, int mode, const double *d_B, 
 /* Kernel timing with HIP events on the library stream.  enable(1) starts
  * recording; get() waits for the recorded events and returns the summed
  * duration (ms) and launch count of one kernel family: "gemm_f64",
- * "lowp_gemm", "block_update", "spmm". */
+ * "lowp_gemm", "block_update", "spmm" (FP64 CSR products), "spmm_f32"
+ * (the FP32 correction products of the sparse mixed path). */
 void ipt_hip_profile_enable(int on);
 void ipt_hip_profile_reset(void);
 int ipt_hip_profile_get(const char *kernel, double *total_ms, long long *launches);

@@ -113,3 +113,25 @@ def test_device_sparse_block_nonsymmetric_rejected(block_env, dev):
     v[off[3]] += 0.1
     with pytest.raises(api.IptError):
         _solve_device(dev, rp, ci, v, n, 4)
+
+
+@pytest.mark.parametrize("n,m,k,seed", [(3000, 20, 8, 1), (20000, 40, 32, 2)])
+def test_sparse_fp32_anchored(block_env, dev, n, m, k, seed):
+    """IPT_PRECISION on CSR input: FP32 correction products anchored by FP64
+    ones (the dense path's scheme); the pairs equal the FP64 block solve's
+    and the solve used low-precision products."""
+    rp, ci, v = orc.orbital_csr(n, m, seed)
+    vec64, val64, _, _ = _solve_device(dev, rp, ci, v, n, k)
+    st64 = api.last_solve_stats()
+    block_env.setenv("IPT_PRECISION", "fp32")
+    vec, val, it, fp = _solve_device(dev, rp, ci, v, n, k)
+    st = api.last_solve_stats()
+    assert st["lowp_products"] > 0 and st64["lowp_products"] == 0
+    assert np.abs(val - val64).max() <= 1e-10 * max(1.0, np.abs(val64).max())
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    normH = np.abs(A).sum(axis=1).max()
+    res = np.linalg.norm(A @ vec - vec * val, axis=0) / (max(1.0, normH) * np.maximum(1.0, np.linalg.norm(vec, axis=0)))
+    assert res.max() <= 1e-10
+    # host CSC entry (host preparation, same operator)
+    hv, hl, info = api.ipt_cuda_sparse_csc_tol(rp, ci, v, k, 1e-12, 300)
+    assert np.abs(hl - val64).max() <= 1e-10 * max(1.0, np.abs(val64).max())
