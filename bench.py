@@ -171,7 +171,7 @@ def sparse_workload(a):
         dvec, dval, _, _ = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, nnz, 1e-12, 200)
         api.free_device_result(dvec, dval)
     api.profile_reset()
-    api.profile_enable(True)
+    api.profile_enable(True, only=["spmm", "spmm_f32"])
     hip.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -372,7 +372,9 @@ def main():
         _, dv, _ = run_solves(api, hip, dH, n, k, 1, maxiter)
         api.free_device_result(dv, 0)
     api.profile_reset()
-    api.profile_enable(True)
+    # events only around the products: event records between the update
+    # kernels would lengthen the timed solves
+    api.profile_enable(True, only=["lowp_gemm", "gemm_f64"])
     t, dv, vals = run_solves(api, hip, dH, n, k, a.steps, maxiter)
     api.profile_enable(False)
     lp_ms, lp_launches = api.profile_get("lowp_gemm")

@@ -226,6 +226,7 @@ SIGNATURES = {
     "ipt_hip_generate_eps": (_c.c_int, [_vp, _c.c_int, _c.c_double, _c.c_ulonglong, _vp]),
     "ipt_hip_lowp_product": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _vp, _vp]),
     "ipt_hip_profile_enable": (None, [_c.c_int]),
+    "ipt_hip_profile_enable_only": (_c.c_int, [_c.c_char_p]),
     "ipt_hip_profile_reset": (None, []),
     "ipt_hip_profile_get": (_c.c_int, [_c.c_char_p, _dp, _c.POINTER(_c.c_longlong)]),
     "ipt_hip_last_solve_stats": (_c.c_int, [_vp, _ip, _ip, _c.POINTER(_c.c_longlong),

@@ -374,8 +374,15 @@ def lowp_product(d_H, n, mode, d_B, d_Ba, d_Ya, d_diag, nc, d_Y, handle=None):
         raise IptError(st, "ipt_hip_lowp_product")
 
 
-def profile_enable(on=True):
-    lib().ipt_hip_profile_enable(1 if on else 0)
+def profile_enable(on=True, only=None):
+    """Event timing of the library's kernel families; `only` (a list of
+    family names) times just those."""
+    if only is None:
+        lib().ipt_hip_profile_enable(1 if on else 0)
+        return
+    st = lib().ipt_hip_profile_enable_only(",".join(only).encode() if on else b"")
+    if st != 0:
+        raise IptError(st, "ipt_hip_profile_enable_only")
 
 
 def profile_reset():

@@ -585,6 +585,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     int t_prev = -1;
     std::vector<double> fp_prev((size_t)plan.nclusters, NAN), fp_now((size_t)plan.nclusters);
     std::vector<int> act((size_t)plan.nclusters);
+    int pending_steps = 0;
     for (int it = 0; it < p.maxiter; ++it) {
         const bool exact = !mixed || it == next_exact || it == p.maxiter - 1;
         if (exact) {
@@ -595,7 +596,14 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             IPT_STCHK((*lowp)(cur, Banc.p, Yanc.p, bc, Y.p));
             out->lowp_products += 1;
         }
-        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, exact ? p.tol : -1.0));
+        // low-precision steps freeze nothing: their norms are not reduced,
+        // the next exact step counts them
+        ++pending_steps;
+        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, exact ? p.tol : -1.0,
+                               exact ? pending_steps : 0));
+        if (exact) {
+            pending_steps = 0;
+        }
         if (mixed && exact) {
             // the exact pair (B, H B) becomes the anchor of the next steps
             IPT_HIPCHK(hipMemcpyAsync(Banc.p, cur, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice,

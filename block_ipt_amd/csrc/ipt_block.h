@@ -46,10 +46,14 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
 int block_plan_reset(Context *ctx, BlockPlan *plan);
 // B = identity blocks of every cluster (column-major, ld).
 int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int n);
-// One batched update B -> Bn from Y = H B; then per-cluster norms, freeze
-// converged clusters (fp <= tol), count iterations, and set flags.
+// One batched update B -> Bn from Y = H B; then, when count_steps > 0,
+// per-cluster norms of this update, freeze converged clusters (fp <= tol),
+// add count_steps to the iteration count of every cluster still active and
+// set flags.  count_steps = 0 skips that finalisation: the caller then
+// counts the step with the next finalised one (clusters only freeze in
+// finalised steps, so the active set is the same over those steps).
 int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
-                 const double *d, int n, double damping, double tol);
+                 const double *d, int n, double damping, double tol, int count_steps = 1);
 
 // C (m x nc, ldc) = A^T B for A (nrows x m, lda), B (nrows x nc, ldb), all
 // column-major FP64; fixed-order two-stage reduction.

@@ -374,8 +374,8 @@ __global__ __launch_bounds__(256) void k_block_update_w1(
 __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict__ partials, int pstride,
                                                         const BlockCluster *__restrict__ clusters,
                                                         int *__restrict__ active, double *__restrict__ fp_out,
-                                                        int *__restrict__ iters, double tol, int nclusters,
-                                                        int *__restrict__ flags)
+                                                        int *__restrict__ iters, double tol, int count_steps,
+                                                        int nclusters, int *__restrict__ flags)
 {
     const int c = blockIdx.x;
     __shared__ double sa[4], sb[4];
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
             }
             const double fp = dn / fmax(1.0, bn);
             fp_out[c] = fp;
-            iters[c] += 1;
+            iters[c] += count_steps;
             if (fp <= tol) {
                 active[c] = 0;
             }
@@ -584,7 +584,7 @@ int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int 
 }
 
 int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
-                 const double *d, int n, double damping, double tol)
+                 const double *d, int n, double damping, double tol, int count_steps)
 {
     double *partials = nullptr;
     IPT_STCHK(context_scratch(ctx, SLOT_REDUCE,
@@ -628,9 +628,12 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
         }
     }
     profile_end(ctx, prof);
+    if (count_steps <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
     hipLaunchKernelGGL(k_block_finalize, dim3(plan.nclusters), dim3(256), 0, ctx->stream, partials,
                        plan.partial_stride, plan.clusters.p, plan.active.p, plan.fp.p, plan.iters.p, tol,
-                       plan.nclusters, plan.flags.p);
+                       count_steps, plan.nclusters, plan.flags.p);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

@@ -11,7 +11,7 @@ namespace ipt {
 namespace {
 const char *kNames[PROF_COUNT] = {"gemm_f64", "lowp_gemm", "block_update", "spmm", "spmm_f32"};
 std::mutex g_mu;
-bool g_on = false;
+unsigned g_mask = 0;  // bit i: family i is timed
 std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev[PROF_COUNT];
 std::vector<hipEvent_t> g_pool;
 
@@ -30,13 +30,13 @@ hipEvent_t take_event()
 }
 }  // namespace
 
-bool profile_on() { return g_on; }
+bool profile_on() { return g_mask != 0; }
 
 ProfileToken profile_begin(Context *ctx, int id)
 {
     ProfileToken t;
     t.id = -1;
-    if (!g_on || id < 0 || id >= PROF_COUNT) {
+    if (id < 0 || id >= PROF_COUNT || !((g_mask >> id) & 1u)) {
         return t;
     }
     std::lock_guard<std::mutex> lock(g_mu);
@@ -64,7 +64,36 @@ void profile_end(Context *ctx, const ProfileToken &t)
 
 using namespace ipt;
 
-extern "C" void ipt_hip_profile_enable(int on) { g_on = on != 0; }
+extern "C" void ipt_hip_profile_enable(int on) { g_mask = on != 0 ? (1u << PROF_COUNT) - 1u : 0u; }
+
+extern "C" int ipt_hip_profile_enable_only(const char *kernels)
+{
+    if (kernels == nullptr) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    unsigned mask = 0;
+    const char *p = kernels;
+    while (*p != '\0') {
+        const char *e = strchr(p, ',');
+        const size_t len = e != nullptr ? (size_t)(e - p) : strlen(p);
+        int id = -1;
+        for (int i = 0; i < PROF_COUNT; ++i) {
+            if (strlen(kNames[i]) == len && !strncmp(p, kNames[i], len)) {
+                id = i;
+            }
+        }
+        if (id < 0) {
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        mask |= 1u << id;
+        p += len;
+        if (*p == ',') {
+            ++p;
+        }
+    }
+    g_mask = mask;
+    return IPT_CUDA_SUCCESS;
+}
 
 extern "C" void ipt_hip_profile_reset(void)
 {

@@ -71,6 +71,9 @@ int ipt_hip_lowp_product(const double *d_H, int n, int mode, const double *d_B, 
  * "lowp_gemm", "block_update", "spmm" (FP64 CSR products), "spmm_f32"
  * (the FP32 correction products of the sparse mixed path). */
 void ipt_hip_profile_enable(int on);
+/* Times only the listed families (comma-separated names as above), so the
+ * event records of the others stay out of the stream; "" stops timing. */
+int ipt_hip_profile_enable_only(const char *kernels);
 void ipt_hip_profile_reset(void);
 int ipt_hip_profile_get(const char *kernel, double *total_ms, long long *launches);
 
