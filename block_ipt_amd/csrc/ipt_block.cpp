@@ -648,6 +648,18 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             fp_prev = fp_now;
             t_prev = it;
             next_exact = it + steps;
+            if (p.verbose) {
+                double worst = 0.0;
+                int worst_c = -1;
+                for (int c = 0; c < plan.nclusters; ++c) {
+                    if (act[(size_t)c] && !(fp_now[(size_t)c] <= worst)) {
+                        worst = fp_now[(size_t)c];
+                        worst_c = c;
+                    }
+                }
+                fprintf(stderr, "IPT block-cluster mixed: exact step %d, %d clusters active, largest fp %.3e "
+                        "(cluster %d), next exact step %d\n", it, flags[2], worst, worst_c, next_exact);
+            }
         }
         if (flags[1] != 0) {
             fprintf(stderr,
