@@ -229,6 +229,7 @@ SIGNATURES = {
     "ipt_hip_profile_enable_only": (_c.c_int, [_c.c_char_p]),
     "ipt_hip_profile_reset": (None, []),
     "ipt_hip_profile_get": (_c.c_int, [_c.c_char_p, _dp, _c.POINTER(_c.c_longlong)]),
+    "ipt_hip_profile_get_columns": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_longlong)]),
     "ipt_hip_last_solve_stats": (_c.c_int, [_vp, _ip, _ip, _c.POINTER(_c.c_longlong),
                                             _c.POINTER(_c.c_longlong), _ip]),
     "ipt_hip_generate_orbital_csr": (_c.c_int, [_c.c_int, _c.c_int, _c.c_ulonglong, _c.c_int, _c.c_int, _c.c_double,

@@ -398,6 +398,15 @@ def profile_get(kernel):
     return ms.value, n.value
 
 
+def profile_columns(kernel):
+    """Columns processed by the timed launches of one kernel family."""
+    c = ctypes.c_longlong(0)
+    st = lib().ipt_hip_profile_get_columns(kernel.encode(), ctypes.byref(c))
+    if st != 0:
+        raise IptError(st, "ipt_hip_profile_get_columns")
+    return c.value
+
+
 def last_solve_stats(handle=None):
     b, s_, dav = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
     f64, lp = ctypes.c_longlong(0), ctypes.c_longlong(0)

@@ -60,7 +60,7 @@ struct BlockParams {
     int davidson_extra = 0;
     double conv_tol = 1e-13;
     int lowp_mode = -1;     // -1: every product in FP64
-    int anchor_every = 20;  // low-precision steps between exact FP64 steps
+    int anchor_every = 0;   // low-precision steps between exact FP64 steps (0: 20 dense, 5 CSR)
     int precond_mlp = 0;    // Davidson corrections from the learned MLP
     std::string mlp_model;  // its parameter file
     int verbose = 0;
@@ -541,7 +541,7 @@ void record_stats(Context *ctx, const BlockResult &br)
 int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_product, const DeltaFn *lowp,
                    const double *d, const double *rowabs, int n, int k,
                    const std::vector<std::pair<int, int>> &ranges, double normH, const BlockParams &p,
-                   BlockResult *out)
+                   BlockResult *out, bool sparse_products = false)
 {
     BlockPlan plan;
     IPT_STCHK(block_plan_create(ctx, ranges, n, &plan));
@@ -581,19 +581,81 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     // tol sooner (only exact steps may freeze a cluster, so steps after
     // convergence would otherwise be wasted).  The last step is always
     // exact, so the returned basis comes out of an exact step.
-    int next_exact = p.anchor_every;
+    // CSR products cost in proportion to their width and an exact one
+    // about twice a low-precision one: short spacing, so clusters freeze
+    // (and drop out of the products) soon after they converge
+    const int anchor_every = p.anchor_every > 0 ? p.anchor_every : (sparse_products ? 5 : 20);
+    int next_exact = anchor_every;
     int t_prev = -1;
     std::vector<double> fp_prev((size_t)plan.nclusters, NAN), fp_now((size_t)plan.nclusters);
-    std::vector<int> act((size_t)plan.nclusters);
+    std::vector<int> act((size_t)plan.nclusters, 1);
+    // Products cover the columns of the clusters still active (as of the
+    // last exact step): a frozen cluster's update copies its columns and
+    // reads nothing of H B, so its columns of Y (and of the anchor Y_a) may
+    // go stale.  A dense low-precision product streams all of E whatever
+    // its width, so it takes the hull of the active columns in one call.
+    // The other products take one call per run of adjacent active clusters
+    // when that is cheaper, by a cost in columns: a dense FP64 product is
+    // bound by streaming H below about 24 columns (cost max(24, w)); a CSR
+    // product streams the matrix once per call, worth about 12 columns of
+    // gathers (cost 12 + w).
+    const int kFloorCols = sparse_products ? 0 : 24;
+    const int kFixedCols = sparse_products ? 12 : 0;
+    const bool lowp_hull = !sparse_products;
+    std::vector<std::pair<int, int>> runs;
+    int run_cols = 0, hull_lo = 0, hull_w = bc;
+    bool per_run = false;
+    auto rebuild_runs = [&]() {
+        runs.clear();
+        run_cols = 0;
+        long run_cost = 0;
+        for (int c = 0; c < plan.nclusters; ++c) {
+            if (!act[(size_t)c]) {
+                continue;
+            }
+            const int o = plan.host[(size_t)c].off, m = plan.host[(size_t)c].m;
+            if (!runs.empty() && runs.back().first + runs.back().second == o) {
+                runs.back().second += m;
+            } else {
+                runs.emplace_back(o, m);
+            }
+            run_cols += m;
+        }
+        if (runs.empty()) {
+            hull_w = 0;
+            return;
+        }
+        for (const auto &r : runs) {
+            run_cost += kFixedCols + std::max(kFloorCols, r.second);
+        }
+        hull_lo = runs.front().first;
+        hull_w = runs.back().first + runs.back().second - hull_lo;
+        per_run = run_cost < kFixedCols + std::max(kFloorCols, hull_w);
+    };
+    rebuild_runs();
     int pending_steps = 0;
     for (int it = 0; it < p.maxiter; ++it) {
         const bool exact = !mixed || it == next_exact || it == p.maxiter - 1;
-        if (exact) {
-            IPT_STCHK(apply(cur, Y.p, bc));
-            out->fp64_products += 1;
-            out->matvecs += bc;
+        const size_t ho = (size_t)hull_lo * n;
+        if (exact && per_run) {
+            for (const auto &r : runs) {
+                const size_t o = (size_t)r.first * n;
+                IPT_STCHK(apply(cur + o, Y.p + o, r.second));
+            }
+        } else if (exact) {
+            IPT_STCHK(apply(cur + ho, Y.p + ho, hull_w));
+        } else if (lowp_hull || !per_run) {
+            IPT_STCHK((*lowp)(cur + ho, Banc.p + ho, Yanc.p + ho, hull_w, Y.p + ho));
         } else {
-            IPT_STCHK((*lowp)(cur, Banc.p, Yanc.p, bc, Y.p));
+            for (const auto &r : runs) {
+                const size_t o = (size_t)r.first * n;
+                IPT_STCHK((*lowp)(cur + o, Banc.p + o, Yanc.p + o, r.second, Y.p + o));
+            }
+        }
+        if (exact) {
+            out->fp64_products += 1;
+            out->matvecs += per_run ? run_cols : hull_w;
+        } else {
             out->lowp_products += 1;
         }
         // low-precision steps freeze nothing: their norms are not reduced,
@@ -616,17 +678,24 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             continue;
         }
         int flags[3];
+        IPT_HIPCHK(hipMemcpyAsync(act.data(), plan.active.p, act.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream));
         if (mixed) {
-            IPT_HIPCHK(hipMemcpyAsync(act.data(), plan.active.p, act.size() * sizeof(int), hipMemcpyDeviceToHost,
-                                      ctx->stream));
             IPT_HIPCHK(hipMemcpyAsync(fp_now.data(), plan.fp.p, fp_now.size() * sizeof(double),
                                       hipMemcpyDeviceToHost, ctx->stream));
         }
         IPT_STCHK(read_host(ctx, flags, plan.flags.p, 3 * sizeof(int)));
+        rebuild_runs();
         if (mixed) {
-            int steps = p.anchor_every + 1;
+            int steps = anchor_every + 1;
             if (t_prev >= 0) {
-                double need = 0.0;
+                // Dense input: the low-precision product costs the same
+                // whatever the active width, so the next exact step waits
+                // for the slowest cluster.  CSR input: products cost in
+                // proportion to the active width, so the next exact step
+                // comes when the first cluster is predicted to converge,
+                // which freezes it and narrows the following products.
+                double need = 0.0, need_first = INFINITY;
                 bool known = true;
                 for (int c = 0; c < plan.nclusters; ++c) {
                     if (!act[(size_t)c]) {
@@ -635,13 +704,17 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
                     const double f1 = fp_now[(size_t)c], f0 = fp_prev[(size_t)c];
                     if (!(f1 > 0.0) || !(f0 > f1) || !std::isfinite(f0)) {
                         known = false;
-                        break;
+                        continue;
                     }
                     const double rate = std::log(f1 / f0) / (double)(it - t_prev);  // < 0
-                    need = std::max(need, std::log(p.tol / f1) / rate);
+                    const double nc = std::log(p.tol / f1) / rate;
+                    need = std::max(need, nc);
+                    need_first = std::min(need_first, nc);
                 }
-                if (known) {
-                    // 10% margin: a late exact step costs less than an extra one
+                // 10% margin: a late exact step costs less than an extra one
+                if (sparse_products && std::isfinite(need_first)) {
+                    steps = std::min(steps, std::max(2, (int)std::ceil(1.1 * need_first) + 1));
+                } else if (!sparse_products && known) {
                     steps = std::min(steps, std::max(2, (int)std::ceil(1.1 * need) + 1));
                 }
             }
@@ -749,10 +822,7 @@ BlockParams params_from_env(int maxiter, double tol, int use_tol)
             p.lowp_mode = LOWP_FP8;
         }
     }
-    p.anchor_every = env_int("IPT_MIXED_ANCHOR_EVERY", 20);
-    if (p.anchor_every <= 0) {
-        p.anchor_every = 20;
-    }
+    p.anchor_every = std::max(0, env_int("IPT_MIXED_ANCHOR_EVERY", 0));
     p.verbose = env_flag("IPT_HIP_VERBOSE");
     // addition: IPT_DAVIDSON_PRECOND=mlp (model file IPT_DAVIDSON_MLP_MODEL)
     const char *pc = getenv("IPT_DAVIDSON_PRECOND");
@@ -1233,7 +1303,7 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
     }
     BlockResult br;
     IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), p.lowp_mode >= 0 ? &delta : nullptr, d.p, rowabs.p, n, k, ranges,
-                             normH, p, &br));
+                             normH, p, &br, true));
     record_stats(ctx, br);
     if (p.verbose) {
         fprintf(stderr,
@@ -1408,7 +1478,7 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         }
         BlockResult br;
         IPT_STCHK(block_pipeline(ctx, apply, ApplyFn(), p.lowp_mode >= 0 ? &delta : nullptr, dd.p, drabs.p, n, k,
-                                 ranges, normH, p, &br));
+                                 ranges, normH, p, &br, true));
         record_stats(ctx, br);
         DevBuf<double> vec;
         IPT_STCHK(vec.alloc((size_t)n * k));

@@ -189,7 +189,7 @@ static int gemm_f64_pass(Context *ctx, const double *H, long ldh, const double *
     case 7: launch_gemm<7>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
     default: launch_gemm<8>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
     }
-    profile_end(ctx, prof);
+    profile_end(ctx, prof, ncols);
     IPT_HIPCHK(hipGetLastError());
     if (splits > 1) {
         const long total = (long)n * ncols;

@@ -537,7 +537,7 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
         } else {
             launch_lowp<LOWP_BF16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
         }
-        profile_end(ctx, prof);
+        profile_end(ctx, prof, w);
         IPT_HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_lowp_epilogue, dim3(blocks_for((long)op.nrb * NB * 64, 256, 16384)), dim3(256), 0,
                            ctx->stream, reinterpret_cast<const f4_t *>(P), splits, n, w, op.nrb, NB,

@@ -145,7 +145,8 @@ struct ProfileToken {
 };
 bool profile_on();
 ProfileToken profile_begin(Context *ctx, int id);
-void profile_end(Context *ctx, const ProfileToken &t);
+// cols: the columns the timed launch processed (summed per family)
+void profile_end(Context *ctx, const ProfileToken &t, long long cols = 0);
 
 // ------------------------------------------------------------------ options
 

@@ -308,7 +308,7 @@ int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, cons
     } else {
         hipLaunchKernelGGL(k_spmm_rowmajor<64>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
     }
-    profile_end(ctx, prof);
+    profile_end(ctx, prof, k);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

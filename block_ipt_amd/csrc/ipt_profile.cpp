@@ -12,7 +12,12 @@ namespace {
 const char *kNames[PROF_COUNT] = {"gemm_f64", "lowp_gemm", "block_update", "spmm", "spmm_f32"};
 std::mutex g_mu;
 unsigned g_mask = 0;  // bit i: family i is timed
-std::vector<std::pair<hipEvent_t, hipEvent_t>> g_ev[PROF_COUNT];
+struct Timed {
+    hipEvent_t start;
+    hipEvent_t stop;
+    long long cols;
+};
+std::vector<Timed> g_ev[PROF_COUNT];
 std::vector<hipEvent_t> g_pool;
 
 hipEvent_t take_event()
@@ -50,14 +55,14 @@ ProfileToken profile_begin(Context *ctx, int id)
     return t;
 }
 
-void profile_end(Context *ctx, const ProfileToken &t)
+void profile_end(Context *ctx, const ProfileToken &t, long long cols)
 {
     if (t.id < 0) {
         return;
     }
     (void)hipEventRecord(t.stop, ctx->stream);
     std::lock_guard<std::mutex> lock(g_mu);
-    g_ev[t.id].emplace_back(t.start, t.stop);
+    g_ev[t.id].push_back(Timed{t.start, t.stop, cols});
 }
 
 }  // namespace ipt
@@ -100,9 +105,9 @@ extern "C" void ipt_hip_profile_reset(void)
     std::lock_guard<std::mutex> lock(g_mu);
     for (int i = 0; i < PROF_COUNT; ++i) {
         for (auto &p : g_ev[i]) {
-            (void)hipEventSynchronize(p.second);
-            g_pool.push_back(p.first);
-            g_pool.push_back(p.second);
+            (void)hipEventSynchronize(p.stop);
+            g_pool.push_back(p.start);
+            g_pool.push_back(p.stop);
         }
         g_ev[i].clear();
     }
@@ -126,7 +131,7 @@ extern "C" int ipt_hip_profile_get(const char *kernel, double *total_ms, long lo
     double sum = 0.0;
     for (auto &p : g_ev[id]) {
         float ms = 0.f;
-        if (hipEventSynchronize(p.second) != hipSuccess || hipEventElapsedTime(&ms, p.first, p.second) != hipSuccess) {
+        if (hipEventSynchronize(p.stop) != hipSuccess || hipEventElapsedTime(&ms, p.start, p.stop) != hipSuccess) {
             return IPT_CUDA_CUDA_ERROR;
         }
         sum += ms;
@@ -134,4 +139,23 @@ extern "C" int ipt_hip_profile_get(const char *kernel, double *total_ms, long lo
     *total_ms = sum;
     *launches = (long long)g_ev[id].size();
     return IPT_CUDA_SUCCESS;
+}
+
+extern "C" int ipt_hip_profile_get_columns(const char *kernel, long long *columns)
+{
+    if (kernel == nullptr || columns == nullptr) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < PROF_COUNT; ++i) {
+        if (!strcmp(kernel, kNames[i])) {
+            std::lock_guard<std::mutex> lock(g_mu);
+            long long c = 0;
+            for (auto &p : g_ev[i]) {
+                c += p.cols;
+            }
+            *columns = c;
+            return IPT_CUDA_SUCCESS;
+        }
+    }
+    return IPT_CUDA_INVALID_ARGUMENT;
 }

@@ -184,7 +184,7 @@ int sparse_lowp_apply_delta(Context *ctx, const SparseLowp &op, const double *B,
         } else {
             launch_spmm_f32<64>(g, ctx->stream, op.rp, op.ci, op.vals.p, n, D, w, Z);
         }
-        profile_end(ctx, prof);
+        profile_end(ctx, prof, w);
         IPT_HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_delta_epilogue, tiles, dim3(256), 0, ctx->stream, Z, w, n, Ya + o, B + o, Ba + o, ld, d,
                            Y + o);

@@ -76,6 +76,9 @@ void ipt_hip_profile_enable(int on);
 int ipt_hip_profile_enable_only(const char *kernels);
 void ipt_hip_profile_reset(void);
 int ipt_hip_profile_get(const char *kernel, double *total_ms, long long *launches);
+/* Columns processed by the timed launches of one family, summed (the
+ * algorithmic bytes of a product launch depend on its width). */
+int ipt_hip_profile_get_columns(const char *kernel, long long *columns);
 
 /* Statistics of the last block-cluster solve on `handle` (NULL: library
  * context of the current device): basis columns, iteration steps, number of

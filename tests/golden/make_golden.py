@@ -166,7 +166,28 @@ def acx_mixed_sweep():
                                     "rows": rows})
 
 
+def c2_lowest():
+    """The 65 lowest eigenvalues (LAPACK dsyevr) of the dense N=16384 matrix
+    of the headline workload (d ~ U[0,50] sorted, V ~ N(0,1) 0.1/sqrt(N),
+    seed 0; the oracle's generator, which the device generator equals bit
+    for bit).  About two minutes on 8 cores."""
+    import numpy as np
+    import scipy.linalg as sl
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    n, k = 16384, 64
+    H = orc.rand_dense(n, 0, 0.0, 50.0, 0.1)
+    w = sl.eigh(H, eigvals_only=True, subset_by_index=[0, k], driver="evr", overwrite_a=True, check_finite=False)
+    _write("c2_lowest.json", {"n": n, "seed": 0, "lo": 0.0, "hi": 50.0, "vscale": 0.1,
+                              "source": "scipy.linalg.eigh (LAPACK dsyevr) of oracle.rand_dense(16384, 0, 0, 50, 0.1)",
+                              "values": [float(x) for x in w]})
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            globals()[name]()
+        sys.exit(0)
     runtests_kat()
     sparse_kat()
     dense_eps_sweep()
@@ -174,3 +195,4 @@ if __name__ == "__main__":
     cluster_ranges()
     acx_sweep()
     acx_mixed_sweep()
+    c2_lowest()

@@ -85,6 +85,39 @@ def test_mixed_block_solve_matches_fp64(mixed_env, mode):
     assert im["iterations"] <= 2 * i64["iterations"] + 12
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp16", "bf16x3"])
+def test_headline_workload_lowest_pairs(mixed_env, mode):
+    """The headline workload (N=16384, k=64, seed 0) with the benchmark's
+    solver settings returns the 64 lowest eigenvalues: equal to LAPACK's
+    (tests/golden/c2_lowest.json) to 1e-12, on the FP64 and the mixed
+    paths."""
+    import json
+    import os
+    from conftest import ROOT
+    with open(os.path.join(ROOT, "tests", "golden", "c2_lowest.json")) as f:
+        g = json.load(f)
+    hip = api.hip()
+    n, k = g["n"], 64
+    dH = hip.malloc(n * n * 8)
+    try:
+        api.generate_dense(dH, n, g["seed"], g["lo"], g["hi"], g["vscale"])
+        if mode != "fp64":
+            mixed_env.setenv("IPT_PRECISION", mode)
+        mixed_env.setenv("IPT_MIXED_ANCHOR_EVERY", "20")
+        mixed_env.setenv("IPT_BLOCK_CLUSTER_TOL", "1e-8")
+        mixed_env.setenv("IPT_DAVIDSON_ENRICH", "1")
+        mixed_env.setenv("IPT_DAVIDSON_STEPS", "10")
+        mixed_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "1e-10")
+        print("headline workload, %s" % mode, flush=True)
+        dv, dl, it, fp = api.ipt_cuda_device_tol(dH, n, k, 1e-9, 400)
+        vals = np.sort(hip.to_host(dl, (k,)))
+        api.free_device_result(dv, dl)
+    finally:
+        hip.free(dH)
+    w = np.asarray(g["values"][:k])
+    assert np.abs(vals - w).max() <= 1e-12
+
+
 def test_fp8_block_solve_with_refinement(mixed_env):
     """FP8 E4M3 correction products (per-tile scaled E) anchored by FP64
     products, then the FP64 Rayleigh-Ritz + Davidson refinement: the same 16

@@ -60,6 +60,24 @@ def test_spmm_matches_scipy(dev, k):
     assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 8])
+def test_spmm_narrow_long_rows(dev, k):
+    """Widths up to 8 (the one-lane-per-pair kernel): rows of about 600
+    entries (several 256-entry rounds), strided X and Y rows."""
+    n, m = 3000, 300
+    rp, ci, v = orc.orbital_csr(n, m, 11)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    ldx, ldy = k + 3, k + 1
+    X = np.random.default_rng(k).standard_normal((n, ldx))
+    drp, dci, dv, dX = dev.put(rp), dev.put(ci), dev.put(v), dev.put(X)
+    dY = dev.hip.malloc(n * ldy * 8)
+    dev.ptrs.append(dY)
+    api.csr_spmm(drp, dci, dv, n, dX, ldx, dY, ldy, k)
+    Y = dev.hip.to_host(dY, (n, ldy), order="C")[:, :k]
+    ref = A @ X[:, :k]
+    assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
 def _solve_device(dev, rp, ci, v, n, k, tol=1e-12, maxiter=300):
     drp, dci, dv = dev.put(rp), dev.put(ci), dev.put(v)
     dvec, dval, it, fp = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, ci.size, tol, maxiter)
