@@ -711,11 +711,15 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
                     need = std::max(need, nc);
                     need_first = std::min(need_first, nc);
                 }
-                // 10% margin: a late exact step costs less than an extra one
+                // CSR: 10% margin, a late exact step costing less than an
+                // extra one.  Dense: none; by the time one cluster is left
+                // the exact product covers only its columns and costs about
+                // two low-precision steps, so an early exact step costs no
+                // more than a late one.
                 if (sparse_products && std::isfinite(need_first)) {
                     steps = std::min(steps, std::max(2, (int)std::ceil(1.1 * need_first) + 1));
                 } else if (!sparse_products && known) {
-                    steps = std::min(steps, std::max(2, (int)std::ceil(1.1 * need) + 1));
+                    steps = std::min(steps, std::max(2, (int)std::ceil(need)));
                 }
             }
             fp_prev = fp_now;
