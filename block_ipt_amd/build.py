@@ -28,7 +28,7 @@ SOURCES = [
     "ipt_block_w4.hip",
     "ipt_block_w5.hip",
     "ipt_block_w6.hip",
-    "ipt_block_wide.hip",
+    "ipt_block_class.hip", "ipt_block_wide.hip",
     "ipt_gemm_lp.hip",
     "ipt_gen.hip",
     "ipt_gen_sparse.hip",

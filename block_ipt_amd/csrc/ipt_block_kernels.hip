@@ -601,8 +601,24 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
     if (plan.wide_count > 0) {
         const BlockUpdateArgs g{B, Y, Bn, ld, d, n, plan.clusters.p, plan.active.p, damping, partials,
                                 plan.partial_stride, plan.flags.p};
-        // widest first: their blocks take longest
+        // widest first: their blocks take longest.  Widths 3..16 that share
+        // a lane class go out as one launch when more than one is present.
+        const bool merge = env_int("IPT_BLOCK_CLASS_LAUNCH", 1) != 0;
         for (int m = 32; m >= 2; --m) {
+            const int lo = m >= 9 && m <= 16 ? 9 : m >= 5 && m <= 8 ? 5 : m >= 3 && m <= 4 ? 3 : 0;
+            if (merge && lo != 0 && m == 2 * lo - 2) {
+                int present = 0, count = 0;
+                for (int q = lo; q <= m; ++q) {
+                    present += plan.width_count[q] > 0;
+                    count += plan.width_count[q];
+                }
+                if (present > 1) {
+                    IPT_STCHK(block_update_class(lo, plan.width_lists.p + plan.width_offset[lo], count, g,
+                                                 ctx->stream));
+                    m = lo;  // the loop's decrement moves on to lo - 1
+                    continue;
+                }
+            }
             if (plan.width_count[m] > 0) {
                 IPT_STCHK(block_update_width(m, plan.width_lists.p + plan.width_offset[m], plan.width_count[m], g,
                                              ctx->stream));

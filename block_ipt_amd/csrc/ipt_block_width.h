@@ -276,6 +276,9 @@ int launch_block_wide(int m, const int *list, int count, const BlockUpdateArgs &
 // Width 2..32 dispatch (ipt_block_kernels.hip); the instantiations live in
 // ipt_block_w*.hip so that they compile in parallel.
 int block_update_width(int m, const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream);
+// One launch for the clusters of widths lo..2lo-2 (lo = 3, 5, 9: lane
+// classes 4, 8, 16), list ordered as the per-width lists (ipt_block_class.hip).
+int block_update_class(int lo, const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream);
 
 #define IPT_BLOCK_WIDTH_INSTANTIATE(W) \
     template int launch_block_width<W>(const int *, int, const BlockUpdateArgs &, hipStream_t);

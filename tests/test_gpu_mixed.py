@@ -118,6 +118,33 @@ def test_headline_workload_lowest_pairs(mixed_env, mode):
     assert np.abs(vals - w).max() <= 1e-12
 
 
+@pytest.mark.parametrize("mode", ["fp16", "fp64"])
+def test_class_launch_bitwise(mixed_env, mode):
+    """Widths of one lane class updated in one launch give the iterates of
+    one launch per width bit for bit (headline workload: widths 11, 12 and
+    15 share a launch)."""
+    hip = api.hip()
+    n, k = 16384, 64
+    dH = hip.malloc(n * n * 8)
+    try:
+        api.generate_dense(dH, n, 0, 0.0, 50.0, 0.1)
+        if mode != "fp64":
+            mixed_env.setenv("IPT_PRECISION", mode)
+        mixed_env.setenv("IPT_BLOCK_CLUSTER_TOL", "1e-8")
+        out = []
+        for merge in ("0", "1"):
+            print("class launch %s, %s" % (merge, mode), flush=True)
+            mixed_env.setenv("IPT_BLOCK_CLASS_LAUNCH", merge)
+            dv, dl, it, fp = api.ipt_cuda_device_tol(dH, n, k, 1e-9, 400)
+            out.append((hip.to_host(dv, (n, k)), hip.to_host(dl, (k,)), api.last_solve_stats()))
+            api.free_device_result(dv, dl)
+    finally:
+        hip.free(dH)
+    (v0, l0, s0), (v1, l1, s1) = out
+    assert s0 == s1
+    assert np.array_equal(l0, l1) and np.array_equal(v0, v1)
+
+
 def test_fp8_block_solve_with_refinement(mixed_env):
     """FP8 E4M3 correction products (per-tile scaled E) anchored by FP64
     products, then the FP64 Rayleigh-Ritz + Davidson refinement: the same 16
