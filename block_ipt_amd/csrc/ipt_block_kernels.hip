@@ -971,6 +971,17 @@ __global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H
     const int q = threadIdx.x & 3;
     const long R0 = (long)bi * 64, C0 = (long)bj * 64;
     int bad = 0;
+    // T = H[bi rows, bj cols] is loaded first, into registers, so its loads
+    // are in flight together with those of T' below
+    double tv[16];
+    {
+        const long gc = C0 + c;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const long gr = R0 + 16 * q + i;
+            tv[i] = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+        }
+    }
     if (bi != bj) {
         // T' = H[bj rows, bi cols]: stats of columns R0 + c, kept in LDS
         const long gc = R0 + c;
@@ -1002,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H
         for (int i = 0; i < 16; ++i) {
             const int r = 16 * q + i;
             const long gr = R0 + r;
-            const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+            const double v = tv[i];
             if (bi != bj) {
                 const double w = tt[c][r];
                 const double scale = fmax(1.0, fmax(fabs(v), fabs(w)));
@@ -1018,9 +1029,7 @@ __global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H
         if (bi == bj) {
             // diagonal tile: symmetry within the tile through LDS
             for (int i = 0; i < 16; ++i) {
-                const int r = 16 * q + i;
-                const long gr = R0 + r;
-                tt[r][c] = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+                tt[16 * q + i][c] = tv[i];
             }
             __syncthreads();
             for (int i = 0; i < 16; ++i) {
