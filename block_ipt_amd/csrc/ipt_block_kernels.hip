@@ -1058,18 +1058,35 @@ __global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H
     }
 }
 
-// colsum[c] / colmax[c]: fixed-order reductions over the nb row-block partials
-__global__ void k_colsum_reduce(const double *__restrict__ part, const double *__restrict__ mpart, int nb, int n,
-                                double *__restrict__ colsum, double *__restrict__ colmax)
+// colsum[c] / colmax[c]: fixed-order reductions over the nb row-block
+// partials: 32 columns per block, 8 threads per column each summing every
+// 8th partial, the 8 sums then added in a fixed order (enough blocks to
+// keep HBM busy: the partials are 2 x nb x n doubles)
+__global__ __launch_bounds__(256) void k_colsum_reduce(const double *__restrict__ part,
+                                                       const double *__restrict__ mpart, int nb, int n,
+                                                       double *__restrict__ colsum, double *__restrict__ colmax)
 {
-    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
-        double s = 0.0, m = 0.0;
-        for (int b = 0; b < nb; ++b) {
+    __shared__ double ss[8][33], sm[8][33];
+    const int lc = threadIdx.x & 31, j = threadIdx.x >> 5;
+    const long c = (long)blockIdx.x * 32 + lc;
+    double s = 0.0, m = 0.0;
+    if (c < n) {
+        for (int b = j; b < nb; b += 8) {
             s += part[(long)b * n + c];
             m = fmax(m, mpart[(long)b * n + c]);
         }
-        colsum[c] = s;
-        colmax[c] = m;
+    }
+    ss[j][lc] = s;
+    sm[j][lc] = m;
+    __syncthreads();
+    if (j == 0 && c < n) {
+        double t = ss[0][lc], u = sm[0][lc];
+        for (int q = 1; q < 8; ++q) {
+            t += ss[q][lc];
+            u = fmax(u, sm[q][lc]);
+        }
+        colsum[c] = t;
+        colmax[c] = u;
     }
 }
 
@@ -1083,7 +1100,7 @@ int dense_scan(Context *ctx, const double *H, long ld, int n, double *colsum, do
     hipLaunchKernelGGL(k_dense_scan, dim3((unsigned)pairs), dim3(256), 0, ctx->stream, H, ld, n, nb, part,
                        part + (size_t)nb * n, asym);
     IPT_HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_colsum_reduce, dim3(blocks_for(n, 256, 1024)), dim3(256), 0, ctx->stream, part,
+    hipLaunchKernelGGL(k_colsum_reduce, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, ctx->stream, part,
                        part + (size_t)nb * n, nb, n, colsum, colmax);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
