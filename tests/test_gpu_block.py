@@ -172,6 +172,19 @@ def test_nonsymmetric_rejected(block_env):
     assert e.value.status == 1
 
 
+@pytest.mark.parametrize("n,i,j", [(300, 250, 10), (300, 70, 64), (1000, 999, 0)])
+def test_dense_scan_finds_asymmetry_off_the_diagonal_tiles(block_env, n, i, j):
+    """The dense scan compares every tile pair: one entry pair that differs
+    (a ragged size, the pair in tiles away from the diagonal or across a
+    tile edge) is reported; the symmetric matrix passes."""
+    H = orc.rand_dense(n, 2, 0.0, 10.0, 0.05)
+    api.ipt_cuda_tol(H, 4, 1e-12, 200)
+    H[i, j] += 1e-6
+    with pytest.raises(IptError) as e:
+        api.ipt_cuda_tol(H, 4, 1e-12, 200)
+    assert e.value.status == 1
+
+
 def _chain_csc(n, strong):
     """Tridiagonal-plus-noise CSC with diagonal 0..n-1 and weak couplings,
     plus the given strong couplings {(i, j): value}."""
