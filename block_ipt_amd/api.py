@@ -418,6 +418,20 @@ def last_solve_stats(handle=None):
             "davidson_steps": dav.value}
 
 
+def last_cluster_ranges(handle=None):
+    """(first, last) row ranges (sorted-diagonal order, singletons
+    included) of the last block-cluster solve."""
+    cnt = ctypes.c_int(0)
+    st = lib().ipt_hip_last_cluster_ranges(handle, None, 0, ctypes.byref(cnt))
+    if st != 0:
+        raise IptError(st, "ipt_hip_last_cluster_ranges")
+    out = np.zeros(2 * max(cnt.value, 1), dtype=np.int32)
+    st = lib().ipt_hip_last_cluster_ranges(handle, _i(out), cnt.value, ctypes.byref(cnt))
+    if st != 0:
+        raise IptError(st, "ipt_hip_last_cluster_ranges")
+    return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(cnt.value)]
+
+
 def last_davidson_stats(handle=None):
     acc, rej, ma, mr = (ctypes.c_int(0) for _ in range(4))
     mv, res = ctypes.c_longlong(0), ctypes.c_double(0.0)

@@ -93,6 +93,8 @@ int load_mlp_model(const std::string &path, DevBuf<float> *out)
 
 // Projected problems up to this order are solved on the host.
 constexpr int kHostEigMax = 128;
+// Internal status of rayleigh_ritz: the basis has numerical rank below k.
+constexpr int kRankDeficient = 100;
 
 // Rayleigh-Ritz over V (n x cols) with HV = H V: generalized problem
 // (V^T H V) c = theta (V^T V) c, Cholesky-based like the reference's
@@ -153,9 +155,12 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
         IPT_HIPCHK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
         const int st = sym_generalized_eig_robust(cols, hg.data(), hp.data(), ht.data(), hc.data(), k);
+        if (st == 1) {
+            // the caller may widen the basis and retry
+            return kRankDeficient;
+        }
         if (st != 0) {
-            fprintf(stderr, "IPT Rayleigh-Ritz failed: %s\n",
-                    st == 1 ? "basis of rank below k" : "eigenvalue iteration did not converge");
+            fprintf(stderr, "IPT Rayleigh-Ritz failed: eigenvalue iteration did not converge\n");
             return IPT_CUDA_CUSOLVER_ERROR;
         }
         IPT_HIPCHK(hipMemcpyAsync(P, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -287,6 +292,19 @@ struct DavidsonStats {
     int basis_cols = 0;
     int mlp_accepted = 0;  // learned correction steps kept
     int mlp_rejected = 0;  // learned steps redone with the diagonal correction
+    // lowest-largest-residual state seen (IPT/src/ipt_cuda.cu:485-701): the
+    // initial Ritz pairs or an accepted step; returned when a later accepted
+    // step raised the largest residual
+    int best_update_count = 0;
+    int best_updated = 0;
+    int best_step = -1;
+    const char *best_source = "none";
+    int best_basis_cols = 0;
+    double best_max = NAN;
+    int best_max_index = -1;
+    int returned_best = 0;
+    double pair28_best = NAN;
+    double pair29_best = NAN;
 };
 
 double max_of(const std::vector<double> &v, int *idx)
@@ -360,6 +378,44 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
     IPT_STCHK(read_host(ctx, hres.data(), res.p, (size_t)k * sizeof(double)));
     st->residual_before = max_of(hres, &st->target_index);
 
+    // best-so-far bookkeeping: the best state is either the current one
+    // (best_current) or a copy kept in bX / bHX / btheta
+    DevBuf<double> bX, bHX, btheta;
+    bool best_current = false;
+    auto note_pairs = [&](const std::vector<double> &r) {
+        // IPT/src/ipt_cuda.cu:538-555 (the two pairs the reference tracks)
+        if (r.size() > 28 && std::isfinite(r[28]) && !(r[28] >= st->pair28_best)) {
+            st->pair28_best = r[28];
+        }
+        if (r.size() > 29 && std::isfinite(r[29]) && !(r[29] >= st->pair29_best)) {
+            st->pair29_best = r[29];
+        }
+    };
+    // returns true when the state with residuals r becomes the best
+    auto note_best = [&](const std::vector<double> &r, const char *source, int step, int basis) -> bool {
+        note_pairs(r);
+        int at = -1;
+        const double m = max_of(r, &at);
+        if (!std::isfinite(m)) {
+            return false;
+        }
+        const bool has = st->best_update_count > 0;
+        if (has && !(m < st->best_max)) {
+            return false;
+        }
+        if (has) {
+            st->best_updated = 1;
+        }
+        ++st->best_update_count;
+        st->best_step = step;
+        st->best_source = source;
+        st->best_basis_cols = basis;
+        st->best_max = m;
+        st->best_max_index = at;
+        return true;
+    };
+    best_current = note_best(hres, "davidson_initial", 0, cols);
+
     auto restart = [&]() -> int {
         // basis = k Ritz vectors + up to `retain` previous corrections
         IPT_HIPCHK(hipMemcpyAsync(V, X, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -432,7 +488,12 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
                                       hipMemcpyDeviceToDevice, ctx->stream));
             IPT_HIPCHK(hipMemcpyAsync(HV + (size_t)cols * n, HT.p, (size_t)n * kept * sizeof(double),
                                       hipMemcpyDeviceToDevice, ctx->stream));
-            IPT_STCHK(rayleigh_ritz(ctx, V, HV, n, cols + kept, k, ttheta.p, tX.p, tHX.p));
+            int rs = rayleigh_ritz(ctx, V, HV, n, cols + kept, k, ttheta.p, tX.p, tHX.p);
+            if (rs == kRankDeficient) {
+                fprintf(stderr, "IPT Davidson: search space of rank below k\n");
+                rs = IPT_CUDA_CUSOLVER_ERROR;
+            }
+            IPT_STCHK(rs);
             // the current residual block R is still needed if the trial is
             // redone, so the trial residual vectors go to `work` first
             IPT_STCHK(residual_norms(ctx, tX.p, tHX.p, n, n, k, ttheta.p, normH, res.p, work.p));
@@ -480,6 +541,30 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
             break;
         }
         cols = tcols;
+        {
+            // an accepted step that does not lower the largest residual keeps
+            // the best state aside before it is overwritten
+            int tat = -1;
+            const double tmax = max_of(tres, &tat);
+            const bool improves = std::isfinite(tmax) && (st->best_update_count == 0 || tmax < st->best_max);
+            if (!improves && best_current) {
+                if (bX.p == nullptr) {
+                    IPT_STCHK(bX.alloc((size_t)n * k));
+                    IPT_STCHK(bHX.alloc((size_t)n * k));
+                    IPT_STCHK(btheta.alloc((size_t)k));
+                }
+                IPT_HIPCHK(hipMemcpyAsync(bX.p, X, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice,
+                                          ctx->stream));
+                IPT_HIPCHK(hipMemcpyAsync(bHX.p, HX, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice,
+                                          ctx->stream));
+                IPT_HIPCHK(hipMemcpyAsync(btheta.p, theta, (size_t)k * sizeof(double), hipMemcpyDeviceToDevice,
+                                          ctx->stream));
+                best_current = false;
+            }
+            if (note_best(tres, "davidson", step, tcols)) {
+                best_current = true;
+            }
+        }
         IPT_HIPCHK(hipMemcpyAsync(X, tX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         IPT_HIPCHK(hipMemcpyAsync(HX, tHX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         IPT_HIPCHK(hipMemcpyAsync(theta, ttheta.p, (size_t)tcols * sizeof(double), hipMemcpyDeviceToDevice,
@@ -496,6 +581,18 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
     int mi = -1;
     st->residual_after = max_of(hres, &mi);
     st->basis_cols = cols;
+    if (!best_current && st->best_update_count > 0) {
+        // IPT/src/ipt_cuda.cu:4941-4983: the best state is returned
+        IPT_HIPCHK(hipMemcpyAsync(X, bX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(HX, bHX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(theta, btheta.p, (size_t)k * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        st->returned_best = 1;
+        st->residual_after = st->best_max;
+        st->basis_cols = st->best_basis_cols;
+    }
     *cols_io = cols;
     return IPT_CUDA_SUCCESS;
 }
@@ -515,8 +612,18 @@ struct BlockResult {
     double davidson_time = 0.0;
     long long fp64_products = 0;  // H * block products in FP64
     long long lowp_products = 0;  // low-precision correction products
+    int widened = 0;              // final basis widened by unit vectors (rank below k)
     DavidsonStats dav;
 };
+
+void record_ranges(Context *ctx, const std::vector<std::pair<int, int>> &ranges)
+{
+    ctx->last_ranges.clear();
+    for (const auto &r : ranges) {
+        ctx->last_ranges.push_back(r.first);
+        ctx->last_ranges.push_back(r.second);
+    }
+}
 
 void record_stats(Context *ctx, const BlockResult &br)
 {
@@ -549,11 +656,12 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     if (bc < k) {
         return IPT_CUDA_INVALID_ARGUMENT;
     }
-    int cap = bc;
+    // room for k identity columns should the final basis be rank deficient
+    int cap = std::min(bc + k, std::max(bc, n));
     if (p.davidson) {
         const int kmax = std::min(n, k + std::max(16, (2 * k + 2) / 3));
         cap = std::max(bc, kmax) + k + 1;
-        cap = std::min(cap, std::max(bc, n));
+        cap = std::min(std::max(cap, bc + k), std::max(bc, n));
     }
     const bool mixed = lowp != nullptr;
     DevBuf<double> Ba, Bb, Y, Banc, Yanc;
@@ -633,10 +741,15 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         per_run = run_cost < kFixedCols + std::max(kFloorCols, hull_w);
     };
     rebuild_runs();
-    int pending_steps = 0;
     for (int it = 0; it < p.maxiter; ++it) {
         const bool exact = !mixed || it == next_exact || it == p.maxiter - 1;
         const size_t ho = (size_t)hull_lo * n;
+        if (mixed && exact) {
+            // clusters stopped by a low-precision step since the last exact
+            // one take this exact step (they are inside its product: they
+            // were active at the last exact step) and then freeze
+            IPT_STCHK(block_reactivate_tentative(ctx, plan));
+        }
         if (exact && per_run) {
             for (const auto &r : runs) {
                 const size_t o = (size_t)r.first * n;
@@ -658,14 +771,9 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         } else {
             out->lowp_products += 1;
         }
-        // low-precision steps freeze nothing: their norms are not reduced,
-        // the next exact step counts them
-        ++pending_steps;
-        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, exact ? p.tol : -1.0,
-                               exact ? pending_steps : 0));
-        if (exact) {
-            pending_steps = 0;
-        }
+        // a low-precision step stops clusters that reach tol there; the next
+        // exact step confirms them (block_update, k_block_finalize)
+        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, p.tol, 1, exact ? 0 : 1));
         if (mixed && exact) {
             // the exact pair (B, H B) becomes the anchor of the next steps
             IPT_HIPCHK(hipMemcpyAsync(Banc.p, cur, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice,
@@ -774,13 +882,43 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     IPT_STCHK(out->theta.alloc((size_t)cap));
     IPT_STCHK(out->X.alloc((size_t)n * k));
     IPT_STCHK(out->HX.alloc((size_t)n * k));
-    IPT_STCHK(rayleigh_ritz(ctx, Ba.p, Y.p, n, bc, k, out->theta.p, out->X.p, out->HX.p));
+    int rr_cols = bc;
+    int rs = rayleigh_ritz(ctx, Ba.p, Y.p, n, bc, k, out->theta.p, out->X.p, out->HX.p);
+    if (rs == kRankDeficient && bc + k <= cap) {
+        // Clusters that drifted onto overlapping subspaces leave a basis of
+        // rank below k: the unit vectors of the k target rows widen it, so
+        // that Rayleigh-Ritz (and Davidson after it) can still find k pairs.
+        if (p.verbose) {
+            fprintf(stderr, "IPT block-cluster: basis of rank below k; widened by %d unit vectors\n", k);
+        }
+        IPT_STCHK(set_identity(ctx, Ba.p + (size_t)bc * n, n, n, k));
+        IPT_STCHK(apply(Ba.p + (size_t)bc * n, Y.p + (size_t)bc * n, k));
+        out->matvecs += k;
+        out->fp64_products += 1;
+        out->widened = 1;
+        rr_cols = bc + k;
+        rs = rayleigh_ritz(ctx, Ba.p, Y.p, n, rr_cols, k, out->theta.p, out->X.p, out->HX.p);
+    }
+    if (rs == kRankDeficient) {
+        fprintf(stderr, "IPT Rayleigh-Ritz failed: basis of rank below k\n");
+        rs = IPT_CUDA_CUSOLVER_ERROR;
+    }
+    IPT_STCHK(rs);
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     out->rr_time = seconds_since(t1);
-    out->basis_cols = bc;
+    out->basis_cols = rr_cols;
     if (p.davidson) {
         auto t2 = std::chrono::steady_clock::now();
-        int cols = bc;
+        int cols = rr_cols;
+        if (out->widened) {
+            // the widened basis is far from orthonormal: search from the k
+            // Ritz vectors instead (the projections assume orthonormal columns)
+            IPT_HIPCHK(hipMemcpyAsync(Ba.p, out->X.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            IPT_HIPCHK(hipMemcpyAsync(Y.p, out->HX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+            cols = k;
+        }
         DevBuf<float> mlp;
         if (p.precond_mlp && rowabs != nullptr) {
             IPT_STCHK(load_mlp_model(p.mlp_model, &mlp));
@@ -788,7 +926,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         IPT_STCHK(davidson(ctx, apply, d, rowabs, mlp.p, n, k, normH, Ba.p, Y.p, &cols, cap, out->theta.p, out->X.p,
                            out->HX.p, p, &out->dav));
         out->matvecs += out->dav.matvecs;
-        out->basis_cols = cols;
+        out->basis_cols = out->dav.basis_cols;
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
         out->davidson_time = seconds_since(t2);
     }
@@ -862,6 +1000,20 @@ void fill_result(IPTCudaResult *r, const BlockResult &b, int n, int k)
     r->davidson_restart_count = b.dav.restarts;
     r->accepted_steps = b.dav.accepted_steps;
     r->rejected_steps = b.dav.rejected_steps;
+    // IPT/src/ipt_cuda.cu:678-701
+    const DavidsonStats &d = b.dav;
+    const bool has = d.best_update_count > 0;
+    r->best_so_far_enabled = 1;
+    r->best_so_far_updated = d.best_updated;
+    r->best_so_far_update_count = d.best_update_count;
+    r->best_so_far_step = has ? d.best_step : -1;
+    snprintf(r->best_so_far_source, sizeof(r->best_so_far_source), "%s", has ? d.best_source : "none");
+    r->best_so_far_basis_cols = has ? d.best_basis_cols : 0;
+    r->best_so_far_max_residual = has ? d.best_max : NAN;
+    r->best_so_far_max_residual_index = has ? d.best_max_index : -1;
+    r->final_returned_from_best_so_far = d.returned_best;
+    r->pair28_best_residual = d.pair28_best;
+    r->pair29_best_residual = d.pair29_best;
 }
 
 void result_init_block(IPTCudaResult *r)
@@ -1045,6 +1197,7 @@ int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vect
     }
     bool too_large = false;
     *ranges = target_clusters(wc, sd, k, o, &too_large);
+    record_ranges(ctx, *ranges);
     if (too_large) {
         int worst = 0;
         for (const auto &r : *ranges) {
@@ -1401,6 +1554,7 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         CscCoupling cc(&a);
         const BlockTargets bt = block_targets_from_env(n, k);
         std::vector<std::pair<int, int>> ranges = target_clusters(cc, d, bt.target_k, o, &too_large);
+        record_ranges(ctx, ranges);
         if (too_large) {
             int worst = 0;
             for (const auto &r : ranges) {
@@ -1414,6 +1568,7 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         std::vector<int> added;
         const bool limit_hit = bt.adaptive && adaptive_expand(a, d, k, o.max_block, bt.tau, &ranges, &added);
         fill_target_fields(result, bt, ranges, k, added, limit_hit);
+        record_ranges(ctx, ranges);
         const double normH = csc_inf_norm(col_ptr, row_ind, values, n);
         result->preparation_time_sec = seconds_since(t0);
         auto t1 = std::chrono::steady_clock::now();
@@ -1539,5 +1694,27 @@ extern "C" int ipt_hip_last_davidson_stats(void *handle, int *accepted, int *rej
     if (mlp_rejected) *mlp_rejected = ctx->last_mlp_rejected;
     if (matvecs) *matvecs = ctx->last_davidson_matvecs;
     if (residual_after) *residual_after = ctx->last_davidson_residual;
+    return IPT_CUDA_SUCCESS;
+}
+
+// Cluster ranges of the last block-cluster solve on `handle`: pairs
+// (first, last) of rows in the solver's sorted order, singletons included,
+// in basis-column order.  Writes min(count, capacity) pairs.
+extern "C" int ipt_hip_last_cluster_ranges(void *handle, int *first_last, int capacity, int *count)
+{
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    if (capacity < 0 || (capacity > 0 && first_last == nullptr)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    const int total = (int)(ctx->last_ranges.size() / 2);
+    if (count) *count = total;
+    for (int i = 0; i < std::min(total, capacity); ++i) {
+        first_last[2 * i] = ctx->last_ranges[(size_t)2 * i];
+        first_last[2 * i + 1] = ctx->last_ranges[(size_t)2 * i + 1];
+    }
     return IPT_CUDA_SUCCESS;
 }

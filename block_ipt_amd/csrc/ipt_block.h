@@ -28,6 +28,7 @@ struct BlockPlan {
     DevBuf<int> active;       // 1 while the cluster iterates
     DevBuf<int> iters;        // iterations done per cluster
     DevBuf<double> fp;        // last fixed-point residual per cluster
+    DevBuf<int> tentative;    // stopped after a low-precision step, awaiting an exact update
     DevBuf<int> flags;        // [0] singular, [1] status bits, [2] active count
     int partial_stride = 0;   // max nblocks over clusters
     int w1_count = 0;         // clusters of width 1 (their own kernel)
@@ -52,8 +53,13 @@ int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int 
 // set flags.  count_steps = 0 skips that finalisation: the caller then
 // counts the step with the next finalised one (clusters only freeze in
 // finalised steps, so the active set is the same over those steps).
+// lowp_step = 1: Y came from a low-precision product; clusters reaching
+// fp <= tol stop and are marked tentative (see block_reactivate_tentative).
 int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
-                 const double *d, int n, double damping, double tol, int count_steps = 1);
+                 const double *d, int n, double damping, double tol, int count_steps = 1, int lowp_step = 0);
+// Before an exact step: tentative clusters iterate once more (with the
+// exact product), after which the finalisation freezes them.
+int block_reactivate_tentative(Context *ctx, BlockPlan &plan);
 
 // C (m x nc, ldc) = A^T B for A (nrows x m, lda), B (nrows x nc, ldb), all
 // column-major FP64; fixed-order two-stage reduction.

@@ -366,6 +366,10 @@ __global__ __launch_bounds__(256) void k_block_update_w1(
 
 // Per-cluster fixed-point residual fp = ||B'-B||_F / max(1, ||B||_F)
 // (IPT/src/ipt_cuda.cu:2984-3003); clusters reaching fp <= tol are frozen.
+// After a low-precision step (lowp_step = 1) a cluster reaching fp <= tol
+// stops iterating at once but is marked tentative: the next exact step
+// updates it once more with an exact product and then freezes it, so no
+// cluster iterates on past its convergence waiting for an exact step.
 // One block per cluster, fixed-order reduction of the update's partials.
 // flags[1] |= 1 on a non-finite norm.  The last block to finish (ticket in
 // flags[3]) also sets flags[1] |= 2 when a solve was singular (flags[0],
@@ -375,7 +379,8 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
                                                         const BlockCluster *__restrict__ clusters,
                                                         int *__restrict__ active, double *__restrict__ fp_out,
                                                         int *__restrict__ iters, double tol, int count_steps,
-                                                        int nclusters, int *__restrict__ flags)
+                                                        int nclusters, int *__restrict__ flags,
+                                                        int *__restrict__ tentative, int lowp_step)
 {
     const int c = blockIdx.x;
     __shared__ double sa[4], sb[4];
@@ -406,8 +411,14 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
             const double fp = dn / fmax(1.0, bn);
             fp_out[c] = fp;
             iters[c] += count_steps;
-            if (fp <= tol) {
+            if (lowp_step) {
+                if (fp <= tol) {
+                    active[c] = 0;
+                    tentative[c] = 1;
+                }
+            } else if (tentative[c] || fp <= tol) {
                 active[c] = 0;
+                tentative[c] = 0;
             }
         }
     }
@@ -441,12 +452,13 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
 }
 
 __global__ void k_block_reset(int *__restrict__ active, int *__restrict__ iters, double *__restrict__ fp,
-                              int nclusters, int *__restrict__ flags)
+                              int nclusters, int *__restrict__ flags, int *__restrict__ tentative)
 {
     for (int c = threadIdx.x; c < nclusters; c += blockDim.x) {
         active[c] = 1;
         iters[c] = 0;
         fp[c] = NAN;
+        tentative[c] = 0;
     }
     if (threadIdx.x == 0) {
         flags[0] = 0;
@@ -520,6 +532,7 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
     IPT_STCHK(plan->active.alloc((size_t)nc));
     IPT_STCHK(plan->iters.alloc((size_t)nc));
     IPT_STCHK(plan->fp.alloc((size_t)nc));
+    IPT_STCHK(plan->tentative.alloc((size_t)nc));
     IPT_STCHK(plan->flags.alloc(4));
     // width-1 clusters: their own kernel
     plan->w1_count = (int)lists[0].size();
@@ -570,7 +583,24 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
 int block_plan_reset(Context *ctx, BlockPlan *plan)
 {
     hipLaunchKernelGGL(k_block_reset, dim3(1), dim3(256), 0, ctx->stream, plan->active.p, plan->iters.p,
-                       plan->fp.p, plan->nclusters, plan->flags.p);
+                       plan->fp.p, plan->nclusters, plan->flags.p, plan->tentative.p);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+__global__ void k_block_reactivate(int *__restrict__ active, const int *__restrict__ tentative, int nclusters)
+{
+    for (int c = threadIdx.x; c < nclusters; c += blockDim.x) {
+        if (tentative[c]) {
+            active[c] = 1;
+        }
+    }
+}
+
+int block_reactivate_tentative(Context *ctx, BlockPlan &plan)
+{
+    hipLaunchKernelGGL(k_block_reactivate, dim3(1), dim3(256), 0, ctx->stream, plan.active.p, plan.tentative.p,
+                       plan.nclusters);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }
@@ -584,7 +614,7 @@ int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int 
 }
 
 int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
-                 const double *d, int n, double damping, double tol, int count_steps)
+                 const double *d, int n, double damping, double tol, int count_steps, int lowp_step)
 {
     double *partials = nullptr;
     IPT_STCHK(context_scratch(ctx, SLOT_REDUCE,
@@ -649,7 +679,7 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
     }
     hipLaunchKernelGGL(k_block_finalize, dim3(plan.nclusters), dim3(256), 0, ctx->stream, partials,
                        plan.partial_stride, plan.clusters.p, plan.active.p, plan.fp.p, plan.iters.p, tol,
-                       count_steps, plan.nclusters, plan.flags.p);
+                       count_steps, plan.nclusters, plan.flags.p, plan.tentative.p, lowp_step);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

@@ -56,6 +56,8 @@ struct Context {
     int last_mlp_rejected = 0;
     long long last_davidson_matvecs = 0;
     double last_davidson_residual = 0.0;
+    // cluster ranges [first, last] (sorted row order) of the last solve
+    std::vector<int> last_ranges;
     size_t slot_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 

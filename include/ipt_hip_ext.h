@@ -94,6 +94,16 @@ int ipt_hip_last_solve_stats(void *handle, int *basis_cols, int *steps, long lon
 int ipt_hip_last_davidson_stats(void *handle, int *accepted, int *rejected, int *mlp_accepted, int *mlp_rejected,
                                 long long *matvecs, double *residual_after);
 
+/* Cluster ranges of the last block-cluster solve on `handle` (NULL: the
+ * library context of the current device): *count receives the number of
+ * clusters, and the first min(count, capacity) of them are written as
+ * (first, last) row pairs in the solver's sorted-diagonal order, singletons
+ * included, in basis-column order.  The reference computes the same ranges
+ * in ipt_target_degenerate_subspaces (IPT/src/ipt_preparation.cu:501-645)
+ * and ipt_block_cluster_prepare (IPT/src/ipt_cuda.cu:4551-4683) but does not
+ * expose them; callers: parity tests and tools that inspect a solve. */
+int ipt_hip_last_cluster_ranges(void *handle, int *first_last, int capacity, int *count);
+
 /* Learned Davidson preconditioner (per-component MLP 3 -> 64 -> 64 -> 1,
  * GELU; 4481 FP32 parameters: W1[64][3], b1[64], W2[64][64], b2[64],
  * W3[64], b3).  Correction t_i = (r_i / s_i) * y_i with y = MLP(x_i),

@@ -101,17 +101,27 @@ def cluster_ranges():
         ("rand_dense_n2048_k16_s0", 2048, 16, 0, 0.0, 10.0, 0.05),
         ("rand_dense_n2048_k64_s1", 2048, 64, 1, 0.0, 6.25, 0.1),
         ("eps_dense_n256_k256", 256, 256, 20260603 + 256, None, None, 0.05),
+        # ragged sizes (not multiples of the scan tiles) and the C2 recipe
+        ("rand_dense_n777_k20_s5", 777, 20, 5, 0.0, 10.0, 0.05),
+        ("rand_dense_n1000_k24_s7", 1000, 24, 7, 0.0, 12.0, 0.08),
+        ("rand_dense_n1531_k40_s2", 1531, 40, 2, 0.0, 30.0, 0.1),
+        ("rand_dense_n3001_k48_s9", 3001, 48, 9, 0.0, 40.0, 0.1),
+        ("rand_dense_n4096_k64_s4", 4096, 64, 4, 0.0, 50.0, 0.1),
+        ("rand_dense_n8192_k64_s0", 8192, 64, 0, 0.0, 50.0, 0.1),
+        ("rand_dense_n16384_k64_s0", 16384, 64, 0, 0.0, 50.0, 0.1),
     ]
     for name, n, k, seed, lo, hi, vs in specs:
         if lo is None:
             H = orc.eps_dense(n, vs, seed)
         else:
             H = orc.rand_dense(n, seed, lo, hi, vs)
+        # CSC of the nonzeros, column by column (rows ascending)
+        cols, rows = np.nonzero(H.T)
         cp = np.zeros(n + 1, dtype=np.int32)
-        nz = H != 0.0
-        cp[1:] = np.cumsum(nz.sum(axis=0))
-        ri = np.concatenate([np.nonzero(nz[:, c])[0] for c in range(n)]).astype(np.int32)
-        v = np.concatenate([H[nz[:, c], c] for c in range(n)])
+        cp[1:] = np.cumsum(np.bincount(cols, minlength=n))
+        ri = rows.astype(np.int32)
+        v = np.ascontiguousarray(H.T[cols, rows])
+        del cols, rows
         got = orc.ref_target_clusters(cp, ri, v, k)
         if got is None:
             raise SystemExit("oracle/_ref not built: run make -C oracle ref")

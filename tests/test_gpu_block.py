@@ -248,3 +248,35 @@ def test_adaptive_limit(block_env):
     gv, gl, info = api.ipt_cuda_sparse_csc_tol(*csc, 10, 1e-12, 200)
     _check_eigs(A, gv, gl, 10)
     assert info["adaptive_limit_hit"] == 1 and info["adaptive_added_indices"] == "none"
+
+
+def test_davidson_returns_best_state(block_env):
+    """Davidson with target tolerance 0 (corrections for every pair, also
+    the converged ones, whose noise directions crowd the restarted search
+    space): the largest residual does not fall monotonically, and steps 29
+    and 30 raise it above that of step 28.  The returned pairs are those of
+    the best state seen (IPT/src/ipt_cuda.cu:485-701, 4941-4983): bit for bit
+    the pairs of the same solve stopped at that step, with the recorded
+    largest residual."""
+    block_env.setenv("IPT_BLOCK_CLUSTER_MAXITER", "2")
+    block_env.setenv("IPT_DAVIDSON_ENRICH", "1")
+    block_env.setenv("IPT_DAVIDSON_STEPS", "30")
+    block_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "0")
+    n, k = 800, 16
+    H = orc.rand_dense(n, 3, 0.0, 10.0, 0.05)
+    gv, gl, info = api.ipt_cuda_tol(H, k, 1e-12, 100)
+    assert info["best_so_far_enabled"] == 1 and info["best_so_far_update_count"] >= 2
+    assert info["final_returned_from_best_so_far"] == 1, info
+    best = info["best_so_far_step"]
+    assert 1 <= best < info["accepted_steps"]
+    assert info["best_so_far_source"] == "davidson"
+    assert info["davidson_residual_after"] == info["best_so_far_max_residual"]
+    assert 0 <= info["best_so_far_max_residual_index"] < k
+    res = orc.residuals(gv, gl, dense=H)
+    assert abs(res.max() - info["best_so_far_max_residual"]) <= 1e-6 * res.max()
+    # the same solve stopped at the best step returns the same pairs
+    block_env.setenv("IPT_DAVIDSON_STEPS", str(best))
+    sv, sl_, sinfo = api.ipt_cuda_tol(H, k, 1e-12, 100)
+    assert sinfo["final_returned_from_best_so_far"] == 0 and sinfo["best_so_far_step"] == best
+    assert np.array_equal(sl_, gl) and np.array_equal(sv, gv)
+    assert sinfo["davidson_residual_after"] == info["best_so_far_max_residual"]
