@@ -242,6 +242,7 @@ SIGNATURES = {
     "ipt_hip_mlp_features": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int, _vp, _vp, _vp]),
     "ipt_hip_mlp_infer": (_c.c_int, [_vp, _vp, _vp, _c.c_longlong, _vp, _vp]),
     "ipt_hip_mlp_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_longlong, _vp, _vp, _vp]),
+    "ipt_hip_mlp_gradient_noalloc": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_longlong, _vp, _vp, _vp]),
     "ipt_hip_mlp_adam_step": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_float, _c.c_float, _c.c_int, _vp]),
     # include/ipt_hip_mixed.h
     "ipt_mixed_precision_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,

@@ -465,6 +465,14 @@ def mlp_gradient(d_params, d_feat, d_q, d_t, d_w, count, d_grad, d_loss, handle=
         raise IptError(st, "ipt_hip_mlp_gradient")
 
 
+def mlp_gradient_noalloc(d_params, d_feat, d_q, d_t, d_w, count, d_grad, d_loss, handle=None):
+    """ipt_hip_mlp_gradient through the entry point that allocates nothing."""
+    st = lib().ipt_hip_mlp_gradient_noalloc(_vpp(d_params), _vpp(d_feat), _vpp(d_q), _vpp(d_t), _vpp(d_w),
+                                            int(count), _vpp(d_grad), _vpp(d_loss), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_mlp_gradient_noalloc")
+
+
 def mlp_adam_step(d_params, d_m, d_v, d_grad, gscale, lr, step, handle=None):
     st = lib().ipt_hip_mlp_adam_step(_vpp(d_params), _vpp(d_m), _vpp(d_v), _vpp(d_grad), float(gscale), float(lr),
                                      int(step), handle)

@@ -14,6 +14,7 @@
 // Inference uses BF16 copies of W2 in MFMA operand order (FP32 accumulate);
 // training and the gradient run in FP32.
 #include <cmath>
+#include <mutex>
 
 #include "ipt_internal.h"
 
@@ -435,19 +436,77 @@ int mlp_correction(Context *ctx, const float *params, const double *R, long ldr,
     return IPT_CUDA_SUCCESS;
 }
 
+// Partial gradients: one slice of MLP_NPARAM floats per block (and one loss
+// float), at most kGradSlices blocks whatever the sample count (the blocks
+// stride over the 64-sample tiles).
+constexpr int kGradSlices = 1024;
+
+static int mlp_gradient_blocks(long count)
+{
+    const long tiles = (count + MLP_TILE - 1) / MLP_TILE;
+    return (int)std::max<long>(1, std::min<long>(tiles, kGradSlices));
+}
+
+// The two launches of the gradient; part holds nb * (MLP_NPARAM + 1) floats,
+// every one of them written by k_mlp_grad before k_mlp_reduce reads it.
+static int mlp_gradient_launch(hipStream_t stream, const float *params, const float *feat, const float *q,
+                               const float *t, const float *w, long count, float *grad, float *loss, int nb,
+                               float *part)
+{
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, stream, params, feat, q, t, w, count, part,
+                       part + (size_t)nb * MLP_NPARAM);
+    IPT_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, stream, part,
+                       part + (size_t)nb * MLP_NPARAM, nb, grad, loss);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
 static int mlp_gradient(Context *ctx, const float *params, const float *feat, const float *q, const float *t,
                         const float *w, long count, float *grad, float *loss)
 {
-    const long tiles = (count + MLP_TILE - 1) / MLP_TILE;
-    const int nb = (int)std::max<long>(1, std::min<long>(tiles, 1024));
+    const int nb = mlp_gradient_blocks(count);
     float *part = nullptr;
     IPT_STCHK(context_scratch(ctx, SLOT_MISC, ((size_t)nb * MLP_NPARAM + nb) * sizeof(float), (void **)&part));
-    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, ctx->stream, params, feat, q, t, w, count, part,
-                       part + (size_t)nb * MLP_NPARAM);
-    IPT_HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, ctx->stream, part,
-                       part + (size_t)nb * MLP_NPARAM, nb, grad, loss);
-    IPT_HIPCHK(hipGetLastError());
+    return mlp_gradient_launch(ctx->stream, params, feat, q, t, w, count, grad, loss, nb, part);
+}
+
+// ------------------------------------------------------------------ gradient without allocation
+
+// Partial gradients of ipt_hip_mlp_gradient_noalloc: a fixed array of
+// kGradSlices slices (18.4 MB), which covers every sample count.  Each GPU
+// holds its own copy until the process ends.  In host code `scratch` is a
+// host-side stand-in for the symbol: kernels get the device address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) float scratch[(size_t)kGradSlices * (MLP_NPARAM + 1)];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mutex;
+static float *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+static int scratch_address(float **out)
+{
+    int dev = -1;
+    IPT_HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) {
+        fprintf(stderr, "IPT: device %d beyond the %d devices the gradient scratch supports\n", dev,
+                kMaxScratchDevices);
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            fprintf(stderr, "IPT: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return IPT_CUDA_CUDA_ERROR;
+        }
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
     return IPT_CUDA_SUCCESS;
 }
 
@@ -485,6 +544,28 @@ extern "C" int ipt_hip_mlp_gradient(const float *d_params, const float *d_feat, 
     }
     IPT_STCHK(mlp_gradient(ctx, d_params, d_feat, d_q, d_t, d_w, (long)count, d_grad, d_loss));
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
+
+extern "C" int ipt_hip_mlp_gradient_noalloc(const float *d_params, const float *d_feat, const float *d_q,
+                                            const float *d_t, const float *d_w, long long count, float *d_grad,
+                                            float *d_loss, void *handle)
+{
+    if (d_params == nullptr || d_feat == nullptr || d_q == nullptr || d_t == nullptr || d_w == nullptr ||
+        d_grad == nullptr || d_loss == nullptr || count <= 0) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    const int nb = mlp_gradient_blocks((long)count);
+    if (nb > kGradSlices) {
+        fprintf(stderr, "IPT: %d partial gradients exceed the %d the fixed scratch holds\n", nb, kGradSlices);
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    float *part = nullptr;
+    IPT_STCHK(scratch_address(&part));
+    // a handle's stream, else the null stream: no library context is created
+    hipStream_t stream = handle != nullptr ? static_cast<Context *>(handle)->stream : nullptr;
+    IPT_STCHK(mlp_gradient_launch(stream, d_params, d_feat, d_q, d_t, d_w, (long)count, d_grad, d_loss, nb, part));
+    IPT_HIPCHK(hipStreamSynchronize(stream));
     return IPT_CUDA_SUCCESS;
 }
 

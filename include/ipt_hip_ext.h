@@ -131,6 +131,18 @@ int ipt_hip_mlp_infer(const float *d_params, const float *d_feat, const float *d
 int ipt_hip_mlp_gradient(const float *d_params, const float *d_feat, const float *d_q, const float *d_t,
                          const float *d_w, long long count, float *d_grad, float *d_loss, void *handle);
 
+/* ipt_hip_mlp_gradient without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight training or serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results (bit for bit) as
+ * ipt_hip_mlp_gradient.  Its partial gradients live in a fixed device array
+ * of 1024 x 4482 floats (18.4 MB per GPU, kept until the process ends),
+ * which covers every count; the entry point calls no hipMalloc/hipFree and
+ * creates no library context (handle NULL: the null stream).  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU race. */
+int ipt_hip_mlp_gradient_noalloc(const float *d_params, const float *d_feat, const float *d_q, const float *d_t,
+                                 const float *d_w, long long count, float *d_grad, float *d_loss, void *handle);
+
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with gradient
  * grad * gscale; m, v are the moment buffers (4481 floats each, zero at
  * step 1). */
