@@ -527,77 +527,150 @@ void csc_diagonal_sum(const HostCsc &a, std::vector<double> *d)
     }
 }
 
-// A <- S^-1 A S with S = I except S[first..last, first..last] = R (left =
-// R^-1 = R^T here), entries of each column summed by row, zeros dropped
-// (IPT/src/ipt_preparation.cu:1109-1220).
+// A <- S^-1 A S for S = I except on the block B = [first, last], where
+// S|B = R and S^-1|B = L (here L = R^T: R is orthogonal).  Only entries with
+// a row or a column in B change:
+//     rows outside B, columns in B:   A(o, B)  <- A(o, B) R
+//     rows in B, columns outside B:   A(B, o)  <- L A(B, o)
+//     the block itself:               A(B, B)  <- L A(B, B) R
+// The B columns and the entries of B rows are gathered into dense m-vectors
+// (duplicate entries summed), transformed by the small dense products
+// above, and every column is rebuilt once, rows ascending, duplicates
+// summed and exact zeros dropped.  The reference applies the same
+// similarity entry by entry (IPT/src/ipt_preparation.cu:1109-1220); sums are
+// formed in another order, so results agree to rounding.
 void local_similarity(HostCsc *a, int first, int last, const std::vector<double> &right,
                       const std::vector<double> &left)
 {
     const int n = a->n, m = last - first + 1;
-    std::vector<std::vector<std::pair<int, double>>> cols((size_t)n);
-    for (int col = 0; col < n; ++col) {
-        const int cl = col - first;
-        const bool cin = cl >= 0 && cl < m;
+    auto inb = [&](int i) { return i >= first && i <= last; };
+    // dense block A(B, B), column-major m x m
+    std::vector<double> blk((size_t)m * m, 0.0);
+    // A(o, B) by outside row: slot per row, m values each
+    std::vector<int> row_slot((size_t)n, -1), slot_row;
+    std::vector<double> rowv;
+    for (int j = 0; j < m; ++j) {
+        const int col = first + j;
         for (int p = a->col_ptr[(size_t)col]; p < a->col_ptr[(size_t)col + 1]; ++p) {
             const int row = a->row_ind[(size_t)p];
-            const int rl = row - first;
-            const bool rin = rl >= 0 && rl < m;
-            const double v = a->values[(size_t)p];
-            if (!rin && !cin) {
-                cols[(size_t)col].emplace_back(row, v);
-            } else if (!rin) {
-                for (int j = 0; j < m; ++j) {
-                    const double x = v * right[(size_t)cl + (size_t)j * m];
-                    if (x != 0.0) {
-                        cols[(size_t)(first + j)].emplace_back(row, x);
-                    }
-                }
-            } else if (!cin) {
-                for (int i = 0; i < m; ++i) {
-                    const double x = left[(size_t)i + (size_t)rl * m] * v;
-                    if (x != 0.0) {
-                        cols[(size_t)col].emplace_back(first + i, x);
-                    }
-                }
-            } else {
-                for (int j = 0; j < m; ++j) {
-                    const double rv = right[(size_t)cl + (size_t)j * m];
-                    if (rv == 0.0) {
-                        continue;
-                    }
-                    for (int i = 0; i < m; ++i) {
-                        const double x = left[(size_t)i + (size_t)rl * m] * v * rv;
-                        if (x != 0.0) {
-                            cols[(size_t)(first + j)].emplace_back(first + i, x);
-                        }
-                    }
-                }
+            if (inb(row)) {
+                blk[(size_t)(row - first) + (size_t)j * m] += a->values[(size_t)p];
+                continue;
+            }
+            if (row_slot[(size_t)row] < 0) {
+                row_slot[(size_t)row] = (int)slot_row.size();
+                slot_row.push_back(row);
+                rowv.resize(rowv.size() + (size_t)m, 0.0);
+            }
+            rowv[(size_t)row_slot[(size_t)row] * m + j] += a->values[(size_t)p];
+        }
+    }
+    // A(B, o) by outside column: slot per column, m values each
+    std::vector<int> col_slot((size_t)n, -1);
+    std::vector<double> colv;
+    int ncs = 0;
+    for (int col = 0; col < n; ++col) {
+        if (inb(col)) {
+            continue;
+        }
+        for (int p = a->col_ptr[(size_t)col]; p < a->col_ptr[(size_t)col + 1]; ++p) {
+            const int row = a->row_ind[(size_t)p];
+            if (!inb(row)) {
+                continue;
+            }
+            if (col_slot[(size_t)col] < 0) {
+                col_slot[(size_t)col] = ncs++;
+                colv.resize(colv.size() + (size_t)m, 0.0);
+            }
+            colv[(size_t)col_slot[(size_t)col] * m + (row - first)] += a->values[(size_t)p];
+        }
+    }
+    // the three dense products (left and right are column-major m x m)
+    auto mat = [m](const std::vector<double> &x, int i, int j) { return x[(size_t)i + (size_t)j * m]; };
+    std::vector<double> tmp((size_t)m * m, 0.0), nblk((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j) {  // tmp = A(B,B) R
+        for (int l = 0; l < m; ++l) {
+            const double r = mat(right, l, j);
+            for (int i = 0; i < m; ++i) {
+                tmp[(size_t)i + (size_t)j * m] += blk[(size_t)i + (size_t)l * m] * r;
             }
         }
     }
-    a->col_ptr.assign((size_t)n + 1, 0);
-    a->row_ind.clear();
-    a->values.clear();
+    for (int j = 0; j < m; ++j) {  // nblk = L tmp
+        for (int l = 0; l < m; ++l) {
+            const double t = tmp[(size_t)l + (size_t)j * m];
+            for (int i = 0; i < m; ++i) {
+                nblk[(size_t)i + (size_t)j * m] += mat(left, i, l) * t;
+            }
+        }
+    }
+    std::vector<double> nrow(rowv.size(), 0.0), ncol(colv.size(), 0.0);
+    for (size_t s = 0; s < slot_row.size(); ++s) {  // row vector times R
+        const double *x = rowv.data() + s * m;
+        for (int j = 0; j < m; ++j) {
+            double acc = 0.0;
+            for (int l = 0; l < m; ++l) {
+                acc += x[l] * mat(right, l, j);
+            }
+            nrow[s * m + (size_t)j] = acc;
+        }
+    }
+    for (int s = 0; s < ncs; ++s) {  // L times column vector
+        const double *u = colv.data() + (size_t)s * m;
+        for (int i = 0; i < m; ++i) {
+            double acc = 0.0;
+            for (int l = 0; l < m; ++l) {
+                acc += mat(left, i, l) * u[l];
+            }
+            ncol[(size_t)s * m + (size_t)i] = acc;
+        }
+    }
+    // rebuild every column once
+    HostCsc out;
+    out.n = n;
+    out.col_ptr.assign((size_t)n + 1, 0);
+    std::vector<std::pair<int, double>> e;
     for (int col = 0; col < n; ++col) {
-        auto &e = cols[(size_t)col];
-        std::stable_sort(e.begin(), e.end(), [](const std::pair<int, double> &x, const std::pair<int, double> &y) {
+        e.clear();
+        if (inb(col)) {
+            const int j = col - first;
+            for (size_t s = 0; s < slot_row.size(); ++s) {
+                e.emplace_back(slot_row[s], nrow[s * m + (size_t)j]);
+            }
+            for (int i = 0; i < m; ++i) {
+                e.emplace_back(first + i, nblk[(size_t)i + (size_t)j * m]);
+            }
+        } else {
+            for (int p = a->col_ptr[(size_t)col]; p < a->col_ptr[(size_t)col + 1]; ++p) {
+                if (!inb(a->row_ind[(size_t)p])) {
+                    e.emplace_back(a->row_ind[(size_t)p], a->values[(size_t)p]);
+                }
+            }
+            if (col_slot[(size_t)col] >= 0) {
+                const double *u = ncol.data() + (size_t)col_slot[(size_t)col] * m;
+                for (int i = 0; i < m; ++i) {
+                    e.emplace_back(first + i, u[i]);
+                }
+            }
+        }
+        std::sort(e.begin(), e.end(), [](const std::pair<int, double> &x, const std::pair<int, double> &y) {
             return x.first < y.first;
         });
-        a->col_ptr[(size_t)col] = (int)a->row_ind.size();
-        for (size_t i = 0; i < e.size(); ++i) {
-            const int row = e[i].first;
-            double v = e[i].second;
-            while (i + 1 < e.size() && e[i + 1].first == row) {
-                ++i;
-                v += e[i].second;
+        out.col_ptr[(size_t)col] = (int)out.row_ind.size();
+        for (size_t q = 0; q < e.size();) {
+            const int row = e[q].first;
+            double v = 0.0;
+            for (; q < e.size() && e[q].first == row; ++q) {
+                v += e[q].second;
             }
             if (v != 0.0) {
-                a->row_ind.push_back(row);
-                a->values.push_back(v);
+                out.row_ind.push_back(row);
+                out.values.push_back(v);
             }
         }
     }
-    a->col_ptr[(size_t)n] = (int)a->row_ind.size();
+    out.col_ptr[(size_t)n] = (int)out.row_ind.size();
+    *a = std::move(out);
 }
 
 }  // namespace

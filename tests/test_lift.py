@@ -73,7 +73,8 @@ needs_ref = pytest.mark.skipif(orc.ref_prepare_sparse_csc(*degenerate_csc(8, 0)[
 
 
 @needs_ref
-@pytest.mark.parametrize("n,k,seed,shuffle", [(40, 8, 1, False), (60, 20, 2, True), (120, 30, 3, True)])
+@pytest.mark.parametrize("n,k,seed,shuffle", [(40, 8, 1, False), (60, 20, 2, True), (120, 30, 3, True),
+                                              (300, 60, 4, True), (500, 90, 6, False)])
 def test_prepared_matrix_equals_reference(n, k, seed, shuffle):
     csc, A = degenerate_csc(n, seed, shuffle)
     st, D, perm, rot = ours(csc, k)
@@ -129,3 +130,28 @@ def test_no_degeneracy_is_a_pure_sort():
     assert st == 0 and rot == 0
     assert np.array_equal(np.diag(D), np.sort(d))
     assert np.array_equal(d[perm], np.sort(d))
+
+
+@needs_ref
+def test_duplicate_entries_summed_like_reference():
+    """A CSC input that stores some entries twice (split values): the
+    prepared matrix equals the reference's, which sums duplicates."""
+    (cp, ri, v), A = degenerate_csc(60, 8, shuffle=True)
+    rng = np.random.default_rng(8)
+    cols = [list(zip(ri[cp[c]:cp[c + 1]], v[cp[c]:cp[c + 1]])) for c in range(60)]
+    for c in range(60):
+        out = []
+        for r, x in cols[c]:
+            if r != c and rng.uniform() < 0.3:
+                f = rng.uniform(0.2, 0.8)
+                out += [(r, f * x), (r, (1 - f) * x)]
+            else:
+                out.append((r, x))
+        cols[c] = out
+    cp2 = np.array([0] + list(np.cumsum([len(e) for e in cols])), dtype=np.int32)
+    ri2 = np.array([r for e in cols for r, _ in e], dtype=np.int32)
+    v2 = np.array([x for e in cols for _, x in e])
+    st, D, perm, rot = ours((cp2, ri2, v2), 15)
+    rst, RD, rperm, rrot = orc.ref_prepare_sparse_csc(cp2, ri2, v2, 15)
+    assert st == rst == 0 and rot == rrot and np.array_equal(perm, rperm)
+    assert np.abs(D - RD).max() <= 1e-14 * np.abs(A).max()
