@@ -224,6 +224,7 @@ SIGNATURES = {
     "ipt_hip_generate_dense": (_c.c_int, [_vp, _c.c_int, _c.c_ulonglong, _c.c_double, _c.c_double,
                                           _c.c_double, _vp]),
     "ipt_hip_generate_eps": (_c.c_int, [_vp, _c.c_int, _c.c_double, _c.c_ulonglong, _vp]),
+    "ipt_hip_set_diagonal_gaps": (_c.c_int, [_vp, _c.c_int, _ip, _c.c_int, _c.c_double, _vp]),
     "ipt_hip_lowp_product": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _vp, _vp]),
     "ipt_hip_profile_enable": (None, [_c.c_int]),
     "ipt_hip_profile_enable_only": (_c.c_int, [_c.c_char_p]),
@@ -239,6 +240,7 @@ SIGNATURES = {
     "ipt_hip_cholqr_mixed": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp]),
     "ipt_hip_last_davidson_stats": (_c.c_int, [_vp, _ip, _ip, _ip, _ip, _c.POINTER(_c.c_longlong), _dp]),
     "ipt_hip_last_cluster_ranges": (_c.c_int, [_vp, _ip, _c.c_int, _ip]),
+    "ipt_hip_last_cluster_states": (_c.c_int, [_vp, _ip, _c.c_int, _ip]),
     "ipt_hip_mlp_features": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int, _vp, _vp, _vp]),
     "ipt_hip_mlp_infer": (_c.c_int, [_vp, _vp, _vp, _c.c_longlong, _vp, _vp]),
     "ipt_hip_mlp_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_longlong, _vp, _vp, _vp]),

@@ -64,7 +64,18 @@ struct BlockParams {
     int precond_mlp = 0;    // Davidson corrections from the learned MLP
     std::string mlp_model;  // its parameter file
     int verbose = 0;
+    int stall_steps = 0;    // consecutive growing steps that stop a cluster (0: never)
+    int max_basis = 0;      // Davidson search-space limit (0: the reference's k + max(16, (2k+2)/3))
+    int retain = -1;        // corrections kept over a restart (-1: the reference's max_basis - k - 1)
+    int max_block = 64;     // IPT_MAX_LIFT_BLOCK_SIZE
 };
+
+// Davidson search-space limit for k targets on n rows
+int davidson_max_basis(const BlockParams &p, int n, int k)
+{
+    const int kmax = p.max_basis > 0 ? std::max(p.max_basis, k + 1) : k + std::max(16, (2 * k + 2) / 3);
+    return std::min(n, kmax);
+}
 
 constexpr int kMlpParams = 4481;
 
@@ -336,11 +347,11 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
              const BlockParams &p, DavidsonStats *st)
 {
     const int steps = std::max(1, std::max(0, p.davidson_steps) + std::max(0, p.davidson_extra));
-    int kmax = std::min(n, k + std::max(16, (2 * k + 2) / 3));
+    int kmax = davidson_max_basis(p, n, k);
     if (kmax > cap) {
         kmax = cap;
     }
-    const int retain = std::max(0, kmax - k - 1);
+    const int retain = p.retain >= 0 ? std::min(p.retain, std::max(0, kmax - k - 1)) : std::max(0, kmax - k - 1);
     int cols = *cols_io;
     st->attempted = 1;
 
@@ -574,6 +585,14 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
         recent_count = kept;
         hres = tres;
         ++st->accepted_steps;
+        if (p.verbose) {
+            int conv = 0;
+            for (double r : hres) {
+                conv += r <= p.conv_tol;
+            }
+            fprintf(stderr, "IPT Davidson: step %d corrections %d kept %d basis %d, %d/%d pairs <= %.1e, largest %.3e\n",
+                    step, nt, kept, cols, conv, k, p.conv_tol, max_of(hres, nullptr));
+        }
         if (cols >= kmax) {
             IPT_STCHK(restart());
         }
@@ -613,12 +632,14 @@ struct BlockResult {
     long long fp64_products = 0;  // H * block products in FP64
     long long lowp_products = 0;  // low-precision correction products
     int widened = 0;              // final basis widened by unit vectors (rank below k)
+    int stalled = 0;              // clusters stopped by stall detection
     DavidsonStats dav;
 };
 
 void record_ranges(Context *ctx, const std::vector<std::pair<int, int>> &ranges)
 {
     ctx->last_ranges.clear();
+    ctx->last_cluster_state.clear();
     for (const auto &r : ranges) {
         ctx->last_ranges.push_back(r.first);
         ctx->last_ranges.push_back(r.second);
@@ -651,15 +672,18 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
                    BlockResult *out, bool sparse_products = false)
 {
     BlockPlan plan;
-    IPT_STCHK(block_plan_create(ctx, ranges, n, &plan));
+    // with stall detection on, clusters wider than the block limit are not
+    // an error: they are never iterated and Davidson refines their pairs
+    IPT_STCHK(block_plan_create(ctx, ranges, n, &plan, p.stall_steps > 0 ? p.max_block : 0));
     const int bc = plan.basis_cols;
     if (bc < k) {
         return IPT_CUDA_INVALID_ARGUMENT;
     }
     // room for k identity columns should the final basis be rank deficient
     int cap = std::min(bc + k, std::max(bc, n));
+    plan.stall_steps = p.stall_steps;
     if (p.davidson) {
-        const int kmax = std::min(n, k + std::max(16, (2 * k + 2) / 3));
+        const int kmax = davidson_max_basis(p, n, k);
         cap = std::max(bc, kmax) + k + 1;
         cap = std::min(std::max(cap, bc + k), std::max(bc, n));
     }
@@ -740,8 +764,22 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         hull_w = runs.back().first + runs.back().second - hull_lo;
         per_run = run_cost < kFixedCols + std::max(kFloorCols, hull_w);
     };
+    if (!plan.skipped.empty()) {
+        // skipped clusters are in no update launch: both iterate buffers
+        // hold their identity blocks
+        IPT_HIPCHK(hipMemcpyAsync(Bb.p, Ba.p, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        IPT_STCHK(block_plan_mark_skipped(ctx, &plan));
+        for (int c : plan.skipped) {
+            act[(size_t)c] = 0;
+        }
+        if (p.verbose) {
+            fprintf(stderr, "IPT block-cluster: %zu clusters wider than %d left to Rayleigh-Ritz and Davidson\n",
+                    plan.skipped.size(), p.max_block);
+        }
+    }
     rebuild_runs();
-    for (int it = 0; it < p.maxiter; ++it) {
+    for (int it = 0; it < p.maxiter && !runs.empty(); ++it) {
         const bool exact = !mixed || it == next_exact || it == p.maxiter - 1;
         const size_t ho = (size_t)hull_lo * n;
         if (mixed && exact) {
@@ -856,18 +894,39 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             break;
         }
     }
-    // per-cluster iterations and residuals
+    // per-cluster iterations and residuals; stalled clusters start the
+    // Rayleigh-Ritz basis again from their identity blocks
     {
-        std::vector<int> its((size_t)plan.nclusters);
+        std::vector<int> its((size_t)plan.nclusters), stl((size_t)plan.nclusters), live((size_t)plan.nclusters);
         std::vector<double> fps((size_t)plan.nclusters);
+        IPT_HIPCHK(hipMemcpyAsync(stl.data(), plan.stalled.p, stl.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        IPT_HIPCHK(hipMemcpyAsync(live.data(), plan.active.p, live.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream));
         IPT_STCHK(read_host(ctx, its.data(), plan.iters.p, its.size() * sizeof(int)));
         IPT_STCHK(read_host(ctx, fps.data(), plan.fp.p, fps.size() * sizeof(double)));
+        std::vector<int> reset;
+        ctx->last_cluster_state.assign((size_t)plan.nclusters, 0);
         for (int c = 0; c < plan.nclusters; ++c) {
             out->iterations = std::max(out->iterations, its[(size_t)c]);
+            if (stl[(size_t)c]) {
+                reset.push_back(c);
+                ctx->last_cluster_state[(size_t)c] = 2;
+                continue;
+            }
+            if (live[(size_t)c]) {
+                ctx->last_cluster_state[(size_t)c] = 1;
+            }
             if (std::isfinite(fps[(size_t)c]) && (!std::isfinite(out->fp_max) || fps[(size_t)c] > out->fp_max)) {
                 out->fp_max = fps[(size_t)c];
             }
         }
+        out->stalled = (int)reset.size();
+        if (p.verbose && !reset.empty()) {
+            fprintf(stderr, "IPT block-cluster: %zu clusters stalled; their bases restart from identity blocks\n",
+                    reset.size());
+        }
+        IPT_STCHK(block_identity_clusters(ctx, plan, reset, cur, n, n));
     }
     out->iteration_time = seconds_since(t0);
     // the basis lives in `cur`; Davidson needs it in the big buffer Ba
@@ -965,6 +1024,11 @@ BlockParams params_from_env(int maxiter, double tol, int use_tol)
         }
     }
     p.anchor_every = std::max(0, env_int("IPT_MIXED_ANCHOR_EVERY", 0));
+    // additions: stall detection and the Davidson search-space limits
+    p.stall_steps = std::max(0, env_int("IPT_BLOCK_CLUSTER_STALL", 0));
+    p.max_basis = std::max(0, env_int("IPT_DAVIDSON_MAX_BASIS", 0));
+    p.retain = env_int("IPT_DAVIDSON_RETAIN", -1);
+    p.max_block = degeneracy_options_from_env(0.0).max_block;
     p.verbose = env_flag("IPT_HIP_VERBOSE");
     // addition: IPT_DAVIDSON_PRECOND=mlp (model file IPT_DAVIDSON_MLP_MODEL)
     const char *pc = getenv("IPT_DAVIDSON_PRECOND");
@@ -1198,7 +1262,7 @@ int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vect
     bool too_large = false;
     *ranges = target_clusters(wc, sd, k, o, &too_large);
     record_ranges(ctx, *ranges);
-    if (too_large) {
+    if (too_large && env_int("IPT_BLOCK_CLUSTER_STALL", 0) <= 0) {
         int worst = 0;
         for (const auto &r : *ranges) {
             worst = std::max(worst, r.second - r.first + 1);
@@ -1555,7 +1619,7 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         const BlockTargets bt = block_targets_from_env(n, k);
         std::vector<std::pair<int, int>> ranges = target_clusters(cc, d, bt.target_k, o, &too_large);
         record_ranges(ctx, ranges);
-        if (too_large) {
+        if (too_large && env_int("IPT_BLOCK_CLUSTER_STALL", 0) <= 0) {
             int worst = 0;
             for (const auto &r : ranges) {
                 worst = std::max(worst, r.second - r.first + 1);
@@ -1715,6 +1779,27 @@ extern "C" int ipt_hip_last_cluster_ranges(void *handle, int *first_last, int ca
     for (int i = 0; i < std::min(total, capacity); ++i) {
         first_last[2 * i] = ctx->last_ranges[(size_t)2 * i];
         first_last[2 * i + 1] = ctx->last_ranges[(size_t)2 * i + 1];
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
+// State of each cluster of the last block-cluster solve, in the order of
+// ipt_hip_last_cluster_ranges: 0 converged, 1 still iterating when the
+// iteration limit was reached, 2 stopped by stall detection.
+extern "C" int ipt_hip_last_cluster_states(void *handle, int *states, int capacity, int *count)
+{
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    if (capacity < 0 || (capacity > 0 && states == nullptr)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    const int total = (int)ctx->last_cluster_state.size();
+    if (count) *count = total;
+    for (int i = 0; i < std::min(total, capacity); ++i) {
+        states[i] = ctx->last_cluster_state[(size_t)i];
     }
     return IPT_CUDA_SUCCESS;
 }

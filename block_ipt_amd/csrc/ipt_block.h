@@ -29,6 +29,11 @@ struct BlockPlan {
     DevBuf<int> iters;        // iterations done per cluster
     DevBuf<double> fp;        // last fixed-point residual per cluster
     DevBuf<int> tentative;    // stopped after a low-precision step, awaiting an exact update
+    DevBuf<double> fp_prev;   // fixed-point residual of the previous step (stall detection)
+    DevBuf<int> growth;       // consecutive steps with a growing residual
+    DevBuf<int> stalled;      // stopped by stall detection
+    int stall_steps = 0;      // growing steps that stall a cluster (0: no detection)
+    std::vector<int> skipped; // clusters never iterated (wider than the block limit)
     DevBuf<int> flags;        // [0] singular, [1] status bits, [2] active count
     int partial_stride = 0;   // max nblocks over clusters
     int w1_count = 0;         // clusters of width 1 (their own kernel)
@@ -42,7 +47,12 @@ struct BlockPlan {
 };
 
 // Build the plan for ranges [first,last] (sorted by first) on n rows.
-int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &ranges, int n, BlockPlan *plan);
+// skip_wider > 0: clusters wider than that are left out of every update
+// launch (they keep their identity blocks; block_plan_mark_skipped then
+// marks them stalled, for Rayleigh-Ritz and Davidson to take over).
+int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &ranges, int n, BlockPlan *plan,
+                      int skip_wider = 0);
+int block_plan_mark_skipped(Context *ctx, BlockPlan *plan);
 // Reset iteration state (all clusters active, counters zero).
 int block_plan_reset(Context *ctx, BlockPlan *plan);
 // B = identity blocks of every cluster (column-major, ld).
@@ -60,6 +70,9 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
 // Before an exact step: tentative clusters iterate once more (with the
 // exact product), after which the finalisation freezes them.
 int block_reactivate_tentative(Context *ctx, BlockPlan &plan);
+// B(:, columns of the clusters in list) = their identity blocks.
+int block_identity_clusters(Context *ctx, const BlockPlan &plan, const std::vector<int> &list, double *B, long ld,
+                            int n);
 
 // C (m x nc, ldc) = A^T B for A (nrows x m, lda), B (nrows x nc, ldb), all
 // column-major FP64; fixed-order two-stage reduction.

@@ -380,7 +380,9 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
                                                         int *__restrict__ active, double *__restrict__ fp_out,
                                                         int *__restrict__ iters, double tol, int count_steps,
                                                         int nclusters, int *__restrict__ flags,
-                                                        int *__restrict__ tentative, int lowp_step)
+                                                        int *__restrict__ tentative, int lowp_step,
+                                                        double *__restrict__ fp_prev, int *__restrict__ growth,
+                                                        int *__restrict__ stalled, int stall_steps)
 {
     const int c = blockIdx.x;
     __shared__ double sa[4], sb[4];
@@ -405,13 +407,27 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
         if (threadIdx.x == 0) {
             dn = sqrt(sa[0] + sa[1] + sa[2] + sa[3]);
             bn = sqrt(sb[0] + sb[1] + sb[2] + sb[3]);
-            if (!isfinite(dn) || !isfinite(bn) || bn <= 0.0) {
-                atomicOr(&flags[1], 1);
-            }
             const double fp = dn / fmax(1.0, bn);
             fp_out[c] = fp;
             iters[c] += count_steps;
-            if (lowp_step) {
+            bool stall = false;
+            if (stall_steps > 0) {
+                // stall detection: a cluster whose fixed-point residual grew
+                // over stall_steps consecutive steps (or is not finite) stops
+                // iterating; the caller resets its basis
+                const double prev = fp_prev[c];
+                fp_prev[c] = fp;
+                const bool finite = isfinite(dn) && isfinite(bn) && bn > 0.0;
+                growth[c] = (finite && isfinite(prev) && fp > prev) ? growth[c] + 1 : 0;
+                stall = !finite || growth[c] >= stall_steps;
+            } else if (!isfinite(dn) || !isfinite(bn) || bn <= 0.0) {
+                atomicOr(&flags[1], 1);
+            }
+            if (stall) {
+                active[c] = 0;
+                tentative[c] = 0;
+                stalled[c] = 1;
+            } else if (lowp_step) {
                 if (fp <= tol) {
                     active[c] = 0;
                     tentative[c] = 1;
@@ -452,13 +468,17 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
 }
 
 __global__ void k_block_reset(int *__restrict__ active, int *__restrict__ iters, double *__restrict__ fp,
-                              int nclusters, int *__restrict__ flags, int *__restrict__ tentative)
+                              int nclusters, int *__restrict__ flags, int *__restrict__ tentative,
+                              double *__restrict__ fp_prev, int *__restrict__ growth, int *__restrict__ stalled)
 {
     for (int c = threadIdx.x; c < nclusters; c += blockDim.x) {
         active[c] = 1;
         iters[c] = 0;
         fp[c] = NAN;
         tentative[c] = 0;
+        fp_prev[c] = NAN;
+        growth[c] = 0;
+        stalled[c] = 0;
     }
     if (threadIdx.x == 0) {
         flags[0] = 0;
@@ -486,11 +506,43 @@ __global__ void k_block_identity(double *__restrict__ B, long ld, int n, const B
     }
 }
 
-int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &ranges, int n, BlockPlan *plan)
+// B(:, columns of cluster c) = its identity block, for the clusters in list
+__global__ void k_block_identity_some(double *__restrict__ B, long ld, int n,
+                                      const BlockCluster *__restrict__ clusters, const int *__restrict__ list)
+{
+    const BlockCluster cl = clusters[list[blockIdx.y]];
+    const long total = (long)n * cl.m;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        const long r = idx % n;
+        const int j = (int)(idx / n);
+        B[r + (long)(cl.off + j) * ld] = (r == cl.first + j) ? 1.0 : 0.0;
+    }
+}
+
+int block_identity_clusters(Context *ctx, const BlockPlan &plan, const std::vector<int> &list, double *B, long ld,
+                            int n)
+{
+    if (list.empty()) {
+        return IPT_CUDA_SUCCESS;
+    }
+    DevBuf<int> dl;
+    IPT_STCHK(dl.alloc(list.size()));
+    IPT_HIPCHK(hipMemcpyAsync(dl.p, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_block_identity_some, dim3(256, (unsigned)list.size()), dim3(256), 0, ctx->stream, B, ld, n,
+                       plan.clusters.p, dl.p);
+    IPT_HIPCHK(hipGetLastError());
+    // the list must outlive the launch
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
+
+int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &ranges, int n, BlockPlan *plan,
+                      int skip_wider)
 {
     plan->nclusters = (int)ranges.size();
     plan->host.clear();
     plan->wide_host.clear();
+    plan->skipped.clear();
     std::vector<std::vector<int>> lists(7);
     int off = 0;
     plan->partial_stride = 1;
@@ -501,6 +553,15 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
         cl.m = cl.last - cl.first + 1;
         cl.off = off;
         const int cls = lane_class(cl.m);
+        if (skip_wider > 0 && cl.m > skip_wider) {
+            // never iterated: its columns keep their identity block
+            cl.nblocks = 1;
+            cl.pad = 0;
+            off += cl.m;
+            plan->skipped.push_back(c);
+            plan->host.push_back(cl);
+            continue;
+        }
         if (cl.m > kMaxWideCluster) {
             fprintf(stderr, "IPT block-cluster: a cluster of width %d exceeds the widest supported update (%d)\n",
                     cl.m, kMaxWideCluster);
@@ -533,6 +594,9 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
     IPT_STCHK(plan->iters.alloc((size_t)nc));
     IPT_STCHK(plan->fp.alloc((size_t)nc));
     IPT_STCHK(plan->tentative.alloc((size_t)nc));
+    IPT_STCHK(plan->fp_prev.alloc((size_t)nc));
+    IPT_STCHK(plan->growth.alloc((size_t)nc));
+    IPT_STCHK(plan->stalled.alloc((size_t)nc));
     IPT_STCHK(plan->flags.alloc(4));
     // width-1 clusters: their own kernel
     plan->w1_count = (int)lists[0].size();
@@ -580,10 +644,36 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
     return block_plan_reset(ctx, plan);
 }
 
+__global__ void k_block_mark_stalled(int *__restrict__ active, int *__restrict__ stalled, const int *__restrict__ list,
+                                     int count)
+{
+    for (int q = threadIdx.x; q < count; q += blockDim.x) {
+        active[list[q]] = 0;
+        stalled[list[q]] = 1;
+    }
+}
+
+int block_plan_mark_skipped(Context *ctx, BlockPlan *plan)
+{
+    if (plan->skipped.empty()) {
+        return IPT_CUDA_SUCCESS;
+    }
+    DevBuf<int> dl;
+    IPT_STCHK(dl.alloc(plan->skipped.size()));
+    IPT_HIPCHK(hipMemcpyAsync(dl.p, plan->skipped.data(), plan->skipped.size() * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
+    hipLaunchKernelGGL(k_block_mark_stalled, dim3(1), dim3(256), 0, ctx->stream, plan->active.p, plan->stalled.p,
+                       dl.p, (int)plan->skipped.size());
+    IPT_HIPCHK(hipGetLastError());
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}
+
 int block_plan_reset(Context *ctx, BlockPlan *plan)
 {
     hipLaunchKernelGGL(k_block_reset, dim3(1), dim3(256), 0, ctx->stream, plan->active.p, plan->iters.p,
-                       plan->fp.p, plan->nclusters, plan->flags.p, plan->tentative.p);
+                       plan->fp.p, plan->nclusters, plan->flags.p, plan->tentative.p, plan->fp_prev.p,
+                       plan->growth.p, plan->stalled.p);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }
@@ -679,7 +769,8 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
     }
     hipLaunchKernelGGL(k_block_finalize, dim3(plan.nclusters), dim3(256), 0, ctx->stream, partials,
                        plan.partial_stride, plan.clusters.p, plan.active.p, plan.fp.p, plan.iters.p, tol,
-                       count_steps, plan.nclusters, plan.flags.p, plan.tentative.p, lowp_step);
+                       count_steps, plan.nclusters, plan.flags.p, plan.tentative.p, lowp_step, plan.fp_prev.p,
+                       plan.growth.p, plan.stalled.p, plan.stall_steps);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

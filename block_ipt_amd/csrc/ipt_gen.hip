@@ -118,3 +118,44 @@ extern "C" int ipt_hip_generate_dense(double *d_H, int n, unsigned long long see
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     return IPT_CUDA_SUCCESS;
 }
+
+// Near-degenerate variant of a generated matrix: for each position i the
+// diagonal entry i+1 becomes H(i,i) + gap.  The diagonal must stay sorted:
+// a position whose next diagonal entry would not stay above the new value
+// (H(i+2,i+2) <= H(i,i) + gap), or out of range, is rejected before any
+// entry is written.
+extern "C" int ipt_hip_set_diagonal_gaps(double *d_H, int n, const int *positions, int count, double gap,
+                                         void *handle)
+{
+    if (d_H == nullptr || n <= 1 || count < 0 || (count > 0 && positions == nullptr) || !(gap > 0.0)) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    auto at = [&](int i) { return d_H + (size_t)i * n + i; };
+    std::vector<double> nv((size_t)count);
+    for (int q = 0; q < count; ++q) {
+        const int i = positions[q];
+        if (i < 0 || i + 1 >= n) {
+            fprintf(stderr, "IPT: diagonal gap position %d out of range for n=%d\n", i, n);
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        double di = 0.0, dnext = INFINITY;
+        IPT_HIPCHK(hipMemcpy(&di, at(i), sizeof(double), hipMemcpyDeviceToHost));
+        if (i + 2 < n) {
+            IPT_HIPCHK(hipMemcpy(&dnext, at(i + 2), sizeof(double), hipMemcpyDeviceToHost));
+        }
+        nv[(size_t)q] = di + gap;
+        if (!(nv[(size_t)q] < dnext)) {
+            fprintf(stderr, "IPT: a gap of %g after diagonal entry %d would unsort the diagonal\n", gap, i);
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+    }
+    for (int q = 0; q < count; ++q) {
+        IPT_HIPCHK(hipMemcpy(at(positions[q] + 1), &nv[(size_t)q], sizeof(double), hipMemcpyHostToDevice));
+    }
+    return IPT_CUDA_SUCCESS;
+}

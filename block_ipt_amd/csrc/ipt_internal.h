@@ -58,6 +58,8 @@ struct Context {
     double last_davidson_residual = 0.0;
     // cluster ranges [first, last] (sorted row order) of the last solve
     std::vector<int> last_ranges;
+    // per cluster: 0 converged, 1 iterating at the limit, 2 stalled
+    std::vector<int> last_cluster_state;
     size_t slot_bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 

@@ -21,6 +21,13 @@ extern "C" {
 int ipt_hip_generate_dense(double *d_H, int n, unsigned long long seed, double lo, double hi, double vscale,
                            void *handle);
 
+/* Near-degenerate variant of a dense matrix with sorted diagonal (the
+ * dense N=65536, k=256 workload's variant: positions 10, 50, 120, 200, gap
+ * 1e-4): H(i+1,i+1) = H(i,i) + gap for each position i.  Status 1 when a
+ * position is out of range or the new entry would not stay below
+ * H(i+2,i+2) (nothing is written then).  No reference counterpart. */
+int ipt_hip_set_diagonal_gaps(double *d_H, int n, const int *positions, int count, double gap, void *handle);
+
 /* The reference's epsilon-sweep matrix on the device (column-major n x n):
  * H_ii = i + 1 + eps U(seed, n, i, i), H_ij = H_ji = eps (U(seed,n,i,j) +
  * U(seed,n,j,i)) / 2, U as above; the generator of
@@ -103,6 +110,12 @@ int ipt_hip_last_davidson_stats(void *handle, int *accepted, int *rejected, int 
  * and ipt_block_cluster_prepare (IPT/src/ipt_cuda.cu:4551-4683) but does not
  * expose them; callers: parity tests and tools that inspect a solve. */
 int ipt_hip_last_cluster_ranges(void *handle, int *first_last, int capacity, int *count);
+/* State of each cluster of the last block-cluster solve, in the order of
+ * ipt_hip_last_cluster_ranges: 0 converged to the block tolerance, 1 still
+ * iterating at the iteration limit, 2 stopped by stall detection
+ * (IPT_BLOCK_CLUSTER_STALL); no reference counterpart (the reference prints
+ * per-cluster iteration lines).  Callers: tests and tools. */
+int ipt_hip_last_cluster_states(void *handle, int *states, int capacity, int *count);
 
 /* Learned Davidson preconditioner (per-component MLP 3 -> 64 -> 64 -> 1,
  * GELU; 4481 FP32 parameters: W1[64][3], b1[64], W2[64][64], b2[64],

@@ -140,6 +140,16 @@ def rand_dense(n, seed, lo, hi, vscale):
     return H
 
 
+def set_diagonal_gaps(H, positions=(10, 50, 120, 200), gap=1e-4):
+    """Near-degenerate variant (in place): H[i+1, i+1] = H[i, i] + gap for
+    each position i; the diagonal must stay sorted."""
+    for i in positions:
+        if not H[i, i] + gap < (H[i + 2, i + 2] if i + 2 < H.shape[0] else np.inf):
+            raise ValueError("gap at %d would unsort the diagonal" % i)
+        H[i + 1, i + 1] = H[i, i] + gap
+    return H
+
+
 def orbital_csr(n, m, seed, norb=40, nsub=8, elo=-20.0, ehi=5.0, sigma=0.02):
     """CSR (= CSC) of the orbital-sum sparse model: (row_ptr, col_ind, values)."""
     cap = n * (2 * m + 1)

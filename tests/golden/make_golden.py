@@ -193,6 +193,33 @@ def c2_lowest():
                               "values": [float(x) for x in w]})
 
 
+def c5_lowest():
+    """The 256 lowest eigenvalues (LAPACK dsyevr) of the dense k=256
+    workload's recipe at N=8192: d ~ U[0,50] sorted, V ~ N(0,1) vscale/sqrt(N),
+    seed 0, with vscale 0.1 (the recipe as written) and 0.1 sqrt(8) (the
+    coupling-to-spacing ratio of N=65536), the latter also with the
+    near-degenerate gaps (1e-4 after diagonal positions 10, 50, 120, 200).
+    About a minute on 8 cores."""
+    import numpy as np
+    import scipy.linalg as sl
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    n, k = 8192, 256
+    cases = []
+    for name, vs, gaps in [("recipe", 0.1, False), ("regime", 0.1 * np.sqrt(8.0), False),
+                           ("regime_near_degenerate", 0.1 * np.sqrt(8.0), True)]:
+        H = orc.rand_dense(n, 0, 0.0, 50.0, vs)
+        if gaps:
+            orc.set_diagonal_gaps(H)
+        w = sl.eigh(H, eigvals_only=True, subset_by_index=[0, k], driver="evr", overwrite_a=True,
+                    check_finite=False)
+        cases.append({"name": name, "vscale": float(vs), "gaps": gaps, "values": [float(x) for x in w]})
+    _write("c5_lowest.json", {"n": n, "k": k, "seed": 0, "lo": 0.0, "hi": 50.0,
+                              "gap_positions": [10, 50, 120, 200], "gap": 1e-4,
+                              "source": "scipy.linalg.eigh (LAPACK dsyevr) of oracle.rand_dense(8192, 0, 0, 50, vscale)",
+                              "cases": cases})
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1:
         for name in sys.argv[1:]:
@@ -206,3 +233,4 @@ if __name__ == "__main__":
     acx_sweep()
     acx_mixed_sweep()
     c2_lowest()
+    c5_lowest()

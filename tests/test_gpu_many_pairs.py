@@ -1,0 +1,96 @@
+"""k = 256 pairs of a strongly coupled dense matrix: the FP8 block path
+whose non-contracting clusters are handed to Rayleigh-Ritz and Davidson
+(api.MANY_PAIRS_OPTIONS).  At N=8192 against LAPACK (tests/golden/
+c5_lowest.json: the recipe d ~ U[0,50], V ~ N(0,1) vscale/sqrt(N); the
+coupling-to-spacing ratio of N=65536; its near-degenerate variant); at
+N=65536 by properties: every residual <= 1e-10, orthonormal vectors, the
+values of the FP64 path."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+from block_ipt_amd import api  # noqa: E402
+
+G5 = load_golden("c5_lowest.json")
+
+
+@pytest.fixture
+def many_env(monkeypatch):
+    for v in ("IPT_PRECISION", "IPT_MAX_LIFT_BLOCK_SIZE", "IPT_BLOCK_CLUSTER_TOL", "IPT_BLOCK_CLUSTER_OVERSAMPLE",
+              "IPT_BLOCK_CLUSTER_ADAPTIVE", "IPT_DAVIDSON_PRECOND", "IPT_DAVIDSON_EXTRA_STEPS"):
+        monkeypatch.delenv(v, raising=False)
+    for key, val in api.MANY_PAIRS_OPTIONS.items():
+        monkeypatch.setenv(key, val)
+    return monkeypatch
+
+
+def _scaled_residuals(H, X, lam):
+    normH = np.abs(H).sum(axis=0).max()
+    R = H @ X - X * lam
+    return np.linalg.norm(R, axis=0) / (max(1.0, normH) * np.maximum(1.0, np.linalg.norm(X, axis=0)))
+
+
+@pytest.mark.parametrize("ratio", ["0.5", "0.2"])
+@pytest.mark.parametrize("case", G5["cases"], ids=lambda c: c["name"])
+def test_many_pairs_match_lapack(many_env, case, ratio):
+    """ratio 0.2 (the reference's default) joins the lowest targets into
+    clusters wider than the block limit: with stall detection on they are
+    not iterated and Davidson refines their pairs."""
+    n, k = G5["n"], G5["k"]
+    many_env.setenv("IPT_COUPLING_GAP_RATIO_TOL", ratio)
+    H = orc.rand_dense(n, G5["seed"], G5["lo"], G5["hi"], case["vscale"])
+    if case["gaps"]:
+        orc.set_diagonal_gaps(H, G5["gap_positions"], G5["gap"])
+    print("many pairs: %s ratio %s" % (case["name"], ratio), flush=True)
+    X, lam, info = api.ipt_cuda_tol(H, k, 1e-9, 40)
+    w = np.asarray(case["values"][:k])
+    order = np.argsort(lam)
+    assert np.abs(lam[order] - w).max() <= 1e-10 * max(1.0, np.abs(w).max())
+    assert _scaled_residuals(H, X, lam).max() <= 1e-10
+    assert np.abs(X.T @ X - np.eye(k)).max() <= 1e-12
+    states = api.last_cluster_states()
+    assert len(states) == len(api.last_cluster_ranges())
+    if ratio == "0.2" and case["name"] != "recipe":
+        # the widest clusters exceed the block limit: never iterated
+        widths = [b - a + 1 for a, b in api.last_cluster_ranges()]
+        assert max(widths) > 64 and all(s == 2 for s, w in zip(states, widths) if w > 64)
+
+
+def test_many_pairs_full_size(many_env):
+    """N=65536, k=256 (FP8 copy of E with per-128x128-tile scales): all 256
+    pairs converged, orthonormal, equal to the FP64 path's; the same for the
+    near-degenerate variant.  Residuals with rocBLAS products."""
+    from devblas import scaled_residuals_device
+    hip = api.hip()
+    n, k = 65536, 256
+    dH = hip.malloc(n * n * 8)
+    try:
+        api.generate_dense(dH, n, 0, 0.0, 50.0, 0.1)
+        out = {}
+        for variant in ("fp8", "fp64", "near_degenerate_fp8"):
+            if variant == "near_degenerate_fp8":
+                api.set_diagonal_gaps(dH, n)
+            if variant == "fp64":
+                many_env.delenv("IPT_PRECISION", raising=False)
+            else:
+                many_env.setenv("IPT_PRECISION", "fp8")
+            print("many pairs full size: %s" % variant, flush=True)
+            dv, dl, it, fp = api.ipt_cuda_device_tol(dH, n, k, 1e-9, 40)
+            try:
+                res, lam, orth = scaled_residuals_device(dH, n, dv, dl, k)
+            finally:
+                api.free_device_result(dv, dl)
+            print("  max residual %.2e, orthogonality %.1e" % (res.max(), orth), flush=True)
+            assert res.max() <= 1e-10 and orth <= 1e-12
+            out[variant] = np.sort(lam)
+        assert np.abs(out["fp8"] - out["fp64"]).max() <= 1e-10 * max(1.0, np.abs(out["fp64"]).max())
+        # the gaps move four diagonal entries down by less than 2e-3
+        assert np.abs(out["near_degenerate_fp8"] - out["fp8"]).max() <= 5e-3
+    finally:
+        hip.free(dH)
