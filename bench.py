@@ -50,11 +50,18 @@ def parse():
     ap.add_argument("--block-tol", type=float, default=1e-8)
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed solves of the plain FP64 path (0: skip)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-sample-iters", type=int, default=2)
-    ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products", "acx"],
+    ap.add_argument("--cpu-sample-iters", type=int, default=2,
+                    help="CPU baseline: iterations of the one-thread sample")
+    ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products", "acx", "many"],
                     help="dense: the headline solve; sparse: the N=1e6 orbital-sum CSR solve and its SpMM; "
                          "products: N=65536 k=256 correction products by precision; "
-                         "acx: ACX vs plain IPT on the epsilon-sweep matrix (k = N, --n, --eps)")
+                         "acx: ACX vs plain IPT on the epsilon-sweep matrix (k = N, --n, --eps); "
+                         "many: the k=256 lowest pairs of the N=65536 matrix, FP8 vs FP64")
+    ap.add_argument("--extra", type=int, default=1,
+                    help="dense workload: also report the sparse N=1e6 solves and the N=65536 k=256 solves "
+                         "(one timed solve each) under the keys 'sparse' and 'many_pairs'")
+    ap.add_argument("--cpu-sample-k", type=int, default=12,
+                    help="CPU baseline: pairs of the sample solved completely by the oracle")
     ap.add_argument("--eps", type=float, default=0.12, help="acx workload: coupling scale epsilon")
     ap.add_argument("--sparse-precision", default="fp64", choices=["fp64", "fp32"],
                     help="sparse workload: fp32 = FP32 correction products anchored by FP64 ones "
@@ -117,25 +124,57 @@ def residuals_host(H, X, theta, normH):
     return np.linalg.norm(R, axis=0) / (max(1.0, normH) * np.maximum(1.0, np.linalg.norm(X, axis=0)))
 
 
-def cpu_baseline(n, k, seed, sample_iters, full_iters):
-    """The oracle's block pipeline (restatement of the reference's CPU path)
-    on the same matrix, `sample_iters` iterations per cluster, scaled to a
-    full solve of `full_iters` iterations."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(n, seed, sample_k, one_core_iters):
+    """The oracle (restatement of the reference's CPU path: block-cluster IPT
+    with per-cluster products, Cholesky Rayleigh-Ritz) on the same matrix,
+    measured: a complete solve of the `sample_k` lowest pairs (the first
+    cluster of the workload) on all host threads, converged pairs counted
+    with the FP64 residual; and on one thread, `one_core_iters` iterations of
+    the same solve timed and scaled to the iteration count of the complete
+    one (the per-iteration cost is constant: the same cluster, product and
+    update each iteration)."""
     from oracle import oracle as orc
 
     H = orc.rand_dense(n, seed, 0.0, 50.0, 0.1)
+    normH = float(np.abs(H).sum(axis=0).max())
+    threads = orc.num_threads()
     t0 = time.perf_counter()
-    val, vec, info = orc.block_pipeline(k, dense=H, maxiter=sample_iters, tol=1e-9)
-    t = time.perf_counter() - t0
-    est = t / sample_iters * full_iters
+    val, vec, info = orc.block_pipeline(sample_k, dense=H, maxiter=400, tol=1e-12)
+    t_all = time.perf_counter() - t0
+    res = residuals_host(H, vec, val, normH)
+    conv = int(np.sum(res <= 1e-10))
+    iters = info["max_iterations"]
+    orc.set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        orc.block_pipeline(sample_k, dense=H, maxiter=one_core_iters, tol=1e-12)
+        t_one = (time.perf_counter() - t0) / one_core_iters * iters
+    finally:
+        orc.set_num_threads(threads)
     return {
-        "value": k / est,
+        "value": conv / t_all,
         "unit": "eigenpairs/s",
-        "cores": orc.num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": "oracle block-cluster IPT + Cholesky RR on the same N=%d k=%d matrix, %d iterations per "
-                  "cluster in %.2f s, scaled to the %d iterations the FP64 GPU solve needed (RR counted once; "
-                  "clusters that stop early are charged the full count)" % (n, k, sample_iters, t, full_iters),
+        "sample": "oracle block-cluster IPT + Cholesky RR, the %d lowest pairs (one cluster) of the same N=%d "
+                  "matrix solved completely on %d threads: %d iterations, %.2f s, %d pairs <= 1e-10"
+                  % (sample_k, n, threads, iters, t_all, conv),
+        "one_core": {"value": conv / t_one, "seconds": t_one,
+                     "sample": "%d iterations on 1 thread timed, scaled to the %d of the complete solve"
+                               % (one_core_iters, iters)},
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
     }
 
 
@@ -239,6 +278,144 @@ def sparse_workload(a):
         out["fp64_path"] = fp64_leg
         out["mixed_vs_fp64"] = fp64_leg["ms_per_solve"] / ms
     print(json.dumps(out), flush=True)
+
+
+def many_pairs_solves(api, hip, dH, n, k, variants):
+    """One warm-up and one timed solve per variant ("fp8", "fp64",
+    "near_degenerate_fp8", applied in that order: the last one writes the
+    diagonal gaps into dH); residuals with rocBLAS products."""
+    from block_ipt_amd.devcheck import scaled_residuals_device
+
+    saved = {key: os.environ.get(key) for key in api.MANY_PAIRS_OPTIONS}
+    os.environ.update(api.MANY_PAIRS_OPTIONS)
+    for v in ("IPT_MAX_LIFT_BLOCK_SIZE", "IPT_BLOCK_CLUSTER_TOL", "IPT_DAVIDSON_PRECOND"):
+        os.environ.pop(v, None)
+    out = {}
+    try:
+        for variant in variants:
+            if variant.startswith("near_degenerate"):
+                api.set_diagonal_gaps(dH, n)
+            if variant.endswith("fp64"):
+                os.environ.pop("IPT_PRECISION", None)
+            else:
+                os.environ["IPT_PRECISION"] = "fp8"
+            dv, dl, _, _ = api.ipt_cuda_device_tol(dH, n, k, 1e-9, 40)
+            api.free_device_result(dv, dl)
+            hip.synchronize()
+            t0 = time.perf_counter()
+            dv, dl, _, _ = api.ipt_cuda_device_tol(dH, n, k, 1e-9, 40)
+            hip.to_host(dl, (k,))
+            t = time.perf_counter() - t0
+            stats, dav = api.last_solve_stats(), api.last_davidson_stats()
+            try:
+                res, lam, orth = scaled_residuals_device(dH, n, dv, dl, k)
+            finally:
+                api.free_device_result(dv, dl)
+            conv = int(np.sum(res <= 1e-10))
+            out[variant] = {"pairs_per_sec": conv / t, "seconds_per_solve": t, "converged_pairs": conv,
+                            "max_residual": float(res.max()), "orthogonality": float(orth),
+                            "ipt_steps": stats["steps"], "fp64_products": stats["fp64_products"],
+                            "lowp_products": stats["lowp_products"], "davidson_steps": dav["accepted"],
+                            "davidson_columns": dav["matvecs"], "values": np.sort(lam)}
+    finally:
+        for key, val in saved.items():
+            if val is None:
+                os.environ.pop(key, None)
+            else:
+                os.environ[key] = val
+    if "fp8" in out and "fp64" in out:
+        out["fp8_vs_fp64"] = out["fp8"]["pairs_per_sec"] / max(out["fp64"]["pairs_per_sec"], 1e-30)
+        out["max_value_diff_fp8_fp64"] = float(np.abs(out["fp8"]["values"] - out["fp64"]["values"]).max())
+    for v in variants:
+        if v in out:
+            out[v].pop("values")
+    return out
+
+
+def many_pairs_workload(a):
+    """k=256 lowest pairs of the dense N=65536 matrix (d ~ U[0,50] sorted,
+    V ~ N(0,1) 0.1/sqrt(N), seed 0): FP8 correction products (one scale per
+    128x128 tile of E), FP64 anchors, Rayleigh-Ritz and Davidson to the
+    1e-10 residual; the FP64 path and the near-degenerate variant (diagonal
+    gaps of 1e-4 after positions 10, 50, 120, 200) beside it."""
+    from block_ipt_amd import api
+
+    hip = api.hip()
+    n, k = 65536, 256
+    dH = hip.malloc(n * n * 8)
+    try:
+        api.generate_dense(dH, n, a.seed, 0.0, 50.0, 0.1)
+        res = many_pairs_solves(api, hip, dH, n, k, ("fp8", "fp64", "near_degenerate_fp8"))
+    finally:
+        hip.free(dH)
+    fp8 = res["fp8"]
+    out = {
+        "metric": "converged eigenpairs/sec at residual<=1e-10, mixed vs FP64 on 1 MI355X",
+        "value": fp8["pairs_per_sec"], "unit": "eigenpairs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * fp8["seconds_per_solve"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp8+f64", "data": "synthetic",
+        "config": {"workload": "dense symmetric N=%d, k=%d lowest, d~U[0,50] sorted, V~N(0,1)*0.1/sqrt(N), "
+                               "seed %d, FP8 e4m3 E per 128x128 tile" % (n, k, a.seed)},
+        "many_pairs": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def sparse_summary(api, hip, seed):
+    """Both sparse solves (FP64 block path, FP32-anchored) of the N=1e6
+    orbital-sum model, one warm-up and one timed solve each, converged pairs
+    by an independent residual (the library's CSR product, norms on the
+    host)."""
+    n, m, k = 1000000, 200, 32
+    cap = n * (2 * m + 1)
+    drp, dci, dv = hip.malloc((n + 1) * 4), hip.malloc(cap * 4), hip.malloc(cap * 8)
+    saved = {key: os.environ.get(key) for key in ("IPT_BLOCK_CLUSTER", "IPT_DAVIDSON_ENRICH", "IPT_DAVIDSON_STEPS",
+                                                  "IPT_DAVIDSON_CONVERGED_TOL", "IPT_PRECISION",
+                                                  "IPT_BLOCK_CLUSTER_TOL", "IPT_DAVIDSON_PRECOND",
+                                                  "IPT_MIXED_ANCHOR_EVERY")}
+    out = {"workload": "sparse orbital-sum model N=%d, 200 draws per row, k=%d lowest, seed %d" % (n, k, seed)}
+    try:
+        nnz = api.generate_orbital_csr(n, m, seed, drp, dci, dv, cap)
+        out["nnz"] = nnz
+        os.environ.update({"IPT_BLOCK_CLUSTER": "1", "IPT_DAVIDSON_ENRICH": "1", "IPT_DAVIDSON_STEPS": "10",
+                           "IPT_DAVIDSON_CONVERGED_TOL": "1e-10"})
+        for v in ("IPT_PRECISION", "IPT_BLOCK_CLUSTER_TOL", "IPT_DAVIDSON_PRECOND", "IPT_MIXED_ANCHOR_EVERY"):
+            os.environ.pop(v, None)
+        vals_abs = np.abs(hip.to_host(dv, (nnz,)))
+        rp = hip.to_host(drp, (n + 1,), dtype=np.int32)
+        normH = float(np.add.reduceat(vals_abs, rp[:-1]).max())
+        del vals_abs
+        for prec in ("fp64", "fp32"):
+            if prec == "fp32":
+                os.environ["IPT_PRECISION"] = "fp32"
+            dvec, dval, _, _ = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, nnz, 1e-12, 200)
+            api.free_device_result(dvec, dval)
+            hip.synchronize()
+            t0 = time.perf_counter()
+            dvec, dval, _, _ = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, nnz, 1e-12, 200)
+            vals = hip.to_host(dval, (k,))
+            t = time.perf_counter() - t0
+            X = hip.to_host(dvec, (n, k))
+            api.free_device_result(dvec, dval)
+            dX, dY = hip.to_device(np.ascontiguousarray(X)), hip.malloc(n * k * 8)
+            api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
+            HX = hip.to_host(dY, (n, k), order="C")
+            hip.free(dX)
+            hip.free(dY)
+            res = np.linalg.norm(HX - X * vals, axis=0) / (max(1.0, normH) * np.maximum(1.0, np.linalg.norm(X, axis=0)))
+            conv = int(np.sum(res <= 1e-10))
+            out[prec] = {"pairs_per_sec": conv / t, "ms_per_solve": 1e3 * t, "converged_pairs": conv,
+                         "max_residual": float(res.max()), "stats": api.last_solve_stats()}
+        out["fp32_vs_fp64"] = out["fp32"]["pairs_per_sec"] / max(out["fp64"]["pairs_per_sec"], 1e-30)
+    finally:
+        for p in (drp, dci, dv):
+            hip.free(p)
+        for key, val in saved.items():
+            if val is None:
+                os.environ.pop(key, None)
+            else:
+                os.environ[key] = val
+    return out
 
 
 def products_workload(a):
@@ -349,6 +526,9 @@ def main():
     if a.workload == "acx":
         acx_workload(a)
         return
+    if a.workload == "many":
+        many_pairs_workload(a)
+        return
     from block_ipt_amd import api
 
     hip = api.hip()
@@ -457,9 +637,28 @@ def main():
         out["fp64_path"] = {"pairs_per_sec": fp64_value, "ms_per_solve": fp64["ms_per_solve"],
                             "converged_pairs": converged64, "stats": fp64["stats"]}
         out["mixed_vs_fp64"] = value / fp64_value
+    if a.extra:
+        # the other two solver workloads, one timed solve each (keys only;
+        # `value` stays the dense N=16384 measurement above)
+        try:
+            out["sparse"] = sparse_summary(api, hip, a.seed)
+        except Exception as e:
+            out["sparse"] = {"error": repr(e)}
+        try:
+            nn, kk = 65536, 256
+            dH = hip.malloc(nn * nn * 8)
+            try:
+                api.generate_dense(dH, nn, a.seed, 0.0, 50.0, 0.1)
+                out["many_pairs"] = many_pairs_solves(api, hip, dH, nn, kk, ("fp8", "fp64", "near_degenerate_fp8"))
+                out["many_pairs"]["workload"] = ("dense N=65536, k=256 lowest, d~U[0,50] sorted, V~N(0,1)*0.1/sqrt(N), "
+                                                 "seed %d; FP8 e4m3 E per 128x128 tile" % a.seed)
+            finally:
+                hip.free(dH)
+        except Exception as e:
+            out["many_pairs"] = {"error": repr(e)}
     if a.cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(n, k, a.seed, a.cpu_sample_iters, fp64_iters or stats["steps"])
+            out["cpu_baseline"] = cpu_baseline(n, a.seed, a.cpu_sample_k, a.cpu_sample_iters)
         except Exception as e:  # reported, never fatal for the GPU measurement
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
     print(json.dumps(out), flush=True)

@@ -1,12 +1,13 @@
-"""Independent device-side checks for the GPU tests: rocBLAS (the vendor
-library, not this repository's kernels) through ctypes, on device pointers
-of the same HIP runtime the library uses.  Used where a matrix is too large
-to check on the host (dense N=65536)."""
+"""Independent device-side checks for the GPU tests and the benchmark:
+rocBLAS (the vendor library, not this repository's kernels) through ctypes,
+on device pointers of the same HIP runtime the library uses.  Used where a
+matrix is too large to check on the host (dense N=65536).  Not part of the
+solver."""
 import ctypes
 
 import numpy as np
 
-from block_ipt_amd import api
+from . import api
 
 _OP_N, _OP_T = 111, 112
 

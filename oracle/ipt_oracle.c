@@ -1542,6 +1542,17 @@ double orc_mlp_grad(const double *P, const double *feat, const double *q, const 
     return l;
 }
 
+void orc_set_num_threads(int t)
+{
+#ifdef _OPENMP
+    if (t > 0) {
+        omp_set_num_threads(t);
+    }
+#else
+    (void)t;
+#endif
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

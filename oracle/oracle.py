@@ -397,6 +397,10 @@ def num_threads():
     return lib().orc_num_threads()
 
 
+def set_num_threads(t):
+    lib().orc_set_num_threads(int(t))
+
+
 # ------------------------------------------------------------------ learned preconditioner
 
 MLP_NPARAM = 4481

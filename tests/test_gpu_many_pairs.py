@@ -66,7 +66,7 @@ def test_many_pairs_full_size(many_env):
     """N=65536, k=256 (FP8 copy of E with per-128x128-tile scales): all 256
     pairs converged, orthonormal, equal to the FP64 path's; the same for the
     near-degenerate variant.  Residuals with rocBLAS products."""
-    from devblas import scaled_residuals_device
+    from block_ipt_amd.devcheck import scaled_residuals_device
     hip = api.hip()
     n, k = 65536, 256
     dH = hip.malloc(n * n * 8)
