@@ -188,6 +188,15 @@ class ACXRayleighMixedTF32CudaResult(_c.Structure):
         ("d_vectors", _c.c_void_p), ("d_values", _c.c_void_p)]
 
 
+class IPTFciCsc(_c.Structure):
+    """include/ipt_hip_fci.h"""
+    _fields_ = [
+        ("n", _c.c_int), ("nnz", _c.c_int), ("col_ptr", _c.POINTER(_c.c_int)), ("row_ind", _c.POINTER(_c.c_int)),
+        ("values", _c.POINTER(_c.c_double)), ("diagonal", _c.POINTER(_c.c_double)), ("perm", _c.POINTER(_c.c_int)),
+        ("matrix_inf_norm", _c.c_double), ("norb", _c.c_int), ("nelec", _c.c_int), ("neleca", _c.c_int),
+        ("nelecb", _c.c_int), ("na", _c.c_int), ("nb", _c.c_int)]
+
+
 _dp = _c.POINTER(_c.c_double)
 _ip = _c.POINTER(_c.c_int)
 _dpp = _c.POINTER(_dp)
@@ -246,6 +255,15 @@ SIGNATURES = {
     "ipt_hip_mlp_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_longlong, _vp, _vp, _vp]),
     "ipt_hip_mlp_gradient_noalloc": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_longlong, _vp, _vp, _vp]),
     "ipt_hip_mlp_adam_step": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_float, _c.c_float, _c.c_int, _vp]),
+    # include/ipt_hip_fci.h
+    "ipt_hip_fci_build_csc": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _dp, _dp, _c.c_double, _c.c_int,
+                                         _c.POINTER(IPTFciCsc)]),
+    "ipt_hip_fci_free_csc": (None, [_c.POINTER(IPTFciCsc)]),
+    "ipt_hip_fcidump_read_header": (_c.c_int, [_c.c_char_p, _ip, _ip, _ip]),
+    "ipt_hip_fcidump_read": (_c.c_int, [_c.c_char_p, _c.c_int, _dp, _dp, _dp]),
+    "ipt_hip_fcidump_write": (_c.c_int, [_c.c_char_p, _c.c_int, _c.c_int, _c.c_int, _dp, _dp, _c.c_double]),
+    "ipt_hip_csc_cache_write": (_c.c_int, [_c.c_char_p, _c.POINTER(IPTFciCsc)]),
+    "ipt_hip_csc_cache_read": (_c.c_int, [_c.c_char_p, _c.POINTER(IPTFciCsc)]),
     # include/ipt_hip_mixed.h
     "ipt_mixed_precision_cuda_device_tol": (_c.c_int, [_vp, _c.c_int, _c.c_int, _c.c_double, _c.c_int,
                                                        _c.c_double, _c.c_int, _vp,

@@ -7,6 +7,7 @@ fields.  Device-memory entry points take raw device pointers; the small
 allocates and copies device memory for callers without another framework.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -552,3 +553,91 @@ def load_mlp_model(path):
             raise ValueError("not an IPT MLP model file")
         cnt = int(np.fromfile(f, dtype=np.int32, count=1)[0])
         return np.fromfile(f, dtype=np.float32, count=cnt)
+
+
+# ------------------------------------------------------------------ FCI inputs (host only)
+
+def _fci_to_dict(m, with_perm=True):
+    n, nnz = m.n, m.nnz
+    out = {
+        "n": n, "nnz": nnz,
+        "col_ptr": np.ctypeslib.as_array(m.col_ptr, shape=(n + 1,)).copy(),
+        "row_ind": np.ctypeslib.as_array(m.row_ind, shape=(max(nnz, 1),))[:nnz].copy(),
+        "values": np.ctypeslib.as_array(m.values, shape=(max(nnz, 1),))[:nnz].copy(),
+        "diagonal": np.ctypeslib.as_array(m.diagonal, shape=(n,)).copy(),
+        "matrix_inf_norm": m.matrix_inf_norm, "norb": m.norb, "nelec": m.nelec, "neleca": m.neleca,
+        "nelecb": m.nelecb, "na": m.na, "nb": m.nb,
+    }
+    out["perm"] = np.ctypeslib.as_array(m.perm, shape=(n,)).copy() if (with_perm and m.perm) else None
+    return out
+
+
+def fci_build_csc(norb, neleca, nelecb, h1, eri, ecore, sort_by_diagonal=True):
+    """Full-CI Hamiltonian (CSC) of the active space; see ipt_hip_fci.h."""
+    h1 = np.ascontiguousarray(h1, dtype=np.float64).reshape(norb * norb)
+    eri = np.ascontiguousarray(eri, dtype=np.float64).reshape(norb ** 4)
+    m = abi.IPTFciCsc()
+    st = lib().ipt_hip_fci_build_csc(int(norb), int(neleca), int(nelecb), _d(h1), _d(eri), float(ecore),
+                                     1 if sort_by_diagonal else 0, ctypes.byref(m))
+    if st != 0:
+        raise IptError(st, "ipt_hip_fci_build_csc")
+    try:
+        return _fci_to_dict(m)
+    finally:
+        lib().ipt_hip_fci_free_csc(ctypes.byref(m))
+
+
+def fcidump_read(path):
+    """(norb, nelec, ms2, h1 (norb x norb), eri (norb^4 as 4-d), ecore)."""
+    norb, nelec, ms2 = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    st = lib().ipt_hip_fcidump_read_header(os.fsencode(path), ctypes.byref(norb), ctypes.byref(nelec),
+                                           ctypes.byref(ms2))
+    if st != 0:
+        raise IptError(st, "ipt_hip_fcidump_read_header")
+    no = norb.value
+    h1 = np.zeros(no * no)
+    eri = np.zeros(no ** 4)
+    ecore = ctypes.c_double(0.0)
+    st = lib().ipt_hip_fcidump_read(os.fsencode(path), no, _d(h1), _d(eri), ctypes.byref(ecore))
+    if st != 0:
+        raise IptError(st, "ipt_hip_fcidump_read")
+    return no, nelec.value, ms2.value, h1.reshape(no, no), eri.reshape(no, no, no, no), ecore.value
+
+
+def fcidump_write(path, norb, nelec, ms2, h1, eri, ecore):
+    h1 = np.ascontiguousarray(h1, dtype=np.float64).reshape(norb * norb)
+    eri = np.ascontiguousarray(eri, dtype=np.float64).reshape(norb ** 4)
+    st = lib().ipt_hip_fcidump_write(os.fsencode(path), int(norb), int(nelec), int(ms2), _d(h1), _d(eri),
+                                     float(ecore))
+    if st != 0:
+        raise IptError(st, "ipt_hip_fcidump_write")
+
+
+def csc_cache_write(path, csc):
+    """Write a CSC dict (keys of fci_build_csc) in the reference's binary
+    cache format."""
+    cp = np.ascontiguousarray(csc["col_ptr"], dtype=np.int32)
+    ri = np.ascontiguousarray(csc["row_ind"], dtype=np.int32)
+    v = np.ascontiguousarray(csc["values"], dtype=np.float64)
+    dg = np.ascontiguousarray(csc["diagonal"], dtype=np.float64)
+    m = abi.IPTFciCsc()
+    m.n, m.nnz = len(cp) - 1, len(ri)
+    m.col_ptr, m.row_ind, m.values, m.diagonal = _i(cp), _i(ri), _d(v), _d(dg)
+    m.perm = None
+    m.matrix_inf_norm = float(csc["matrix_inf_norm"])
+    for key in ("norb", "nelec", "neleca", "nelecb", "na", "nb"):
+        setattr(m, key, int(csc.get(key, 0)))
+    st = lib().ipt_hip_csc_cache_write(os.fsencode(path), ctypes.byref(m))
+    if st != 0:
+        raise IptError(st, "ipt_hip_csc_cache_write")
+
+
+def csc_cache_read(path):
+    m = abi.IPTFciCsc()
+    st = lib().ipt_hip_csc_cache_read(os.fsencode(path), ctypes.byref(m))
+    if st != 0:
+        raise IptError(st, "ipt_hip_csc_cache_read")
+    try:
+        return _fci_to_dict(m, with_perm=False)
+    finally:
+        lib().ipt_hip_fci_free_csc(ctypes.byref(m))

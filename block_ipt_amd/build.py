@@ -37,6 +37,7 @@ SOURCES = [
     "ipt_acx.hip",
     "ipt_mlp.hip",
     "ipt_profile.cpp",
+    "ipt_fci.cpp",
 ]
 
 # Packed-FP32 VALU ops are disabled for all device code.
