@@ -194,3 +194,19 @@ def test_builder_rejects_invalid_sizes():
         st = abi.lib().ipt_hip_fci_build_csc(norb, na, nb, h1.ctypes.data_as(abi._dp), eri.ctypes.data_as(abi._dp),
                                              0.0, 1, ctypes.byref(m))
         assert st == 1 and m.n == 0 and not m.col_ptr
+
+
+def test_comparator_report_on_host(tmp_path):
+    """The comparator harness (block_ipt_amd/compare.py) with the host
+    method only: ARPACK's pairs of a built Hamiltonian equal LAPACK's and the
+    report has the reference's key=value layout."""
+    from block_ipt_amd import compare
+    m = api.fci_build_csc(6, 3, 2, *ofci.random_integrals(6, 9), -1.0)
+    res = compare.compare(m, 6, 1e-10, methods=("arpack",))
+    w = np.linalg.eigvalsh(_dense(m))[:6]
+    assert res[0]["status"] == "success" and np.abs(np.asarray(res[0]["eigenvalues"]) - w).max() <= 1e-10
+    path = compare.write_report(res, str(tmp_path), "m", 6, 1e-10, "test")
+    text = open(path).read()
+    assert "convergence_metric=max_relative_eigen_residual" in text
+    assert "method=arpack status=success returned_k=6" in text
+    assert os.path.exists(str(tmp_path / "m_compare.csv"))

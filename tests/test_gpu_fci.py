@@ -41,3 +41,20 @@ def test_fci_hamiltonian_solved_on_gpu(monkeypatch, tmp_path, norb, na, nb, k):
     order = np.argsort(val)
     assert np.abs(val[order] - w).max() <= 1e-10 * max(1.0, np.abs(w).max())
     assert orc.residuals(vec, val, dense=H).max() <= 1e-10
+
+
+def test_comparator_all_methods(tmp_path):
+    """The comparator harness on one built Hamiltonian: block IPT, Davidson
+    alone and ARPACK reach the same pairs with residuals under the
+    threshold, and the report names each method's time."""
+    from block_ipt_amd import compare
+    m = api.fci_build_csc(8, 3, 3, *ofci.random_integrals(8, 21), -2.0)
+    res = compare.compare(m, 8, 1e-10)
+    assert [r["method"] for r in res] == ["ipt", "davidson", "arpack"]
+    for r in res:
+        assert r["status"] == "success", r
+    w0 = np.asarray(res[0]["eigenvalues"])
+    for r in res[1:]:
+        assert np.abs(np.asarray(r["eigenvalues"]) - w0).max() <= 1e-9
+    text = open(compare.write_report(res, str(tmp_path), "fci", 8, 1e-10, "test")).read()
+    assert "davidson_speedup_vs_ipt=" in text and "arpack_speedup_vs_ipt=" in text
