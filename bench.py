@@ -52,11 +52,13 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=2,
                     help="CPU baseline: iterations of the one-thread sample")
-    ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products", "acx", "many"],
+    ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products", "acx", "many", "spmm"],
                     help="dense: the headline solve; sparse: the N=1e6 orbital-sum CSR solve and its SpMM; "
                          "products: N=65536 k=256 correction products by precision; "
                          "acx: ACX vs plain IPT on the epsilon-sweep matrix (k = N, --n, --eps); "
-                         "many: the k=256 lowest pairs of the N=65536 matrix, FP8 vs FP64")
+                         "many: the k=256 lowest pairs of the N=65536 matrix, FP8 vs FP64; "
+                         "spmm: the N=1e6 CSR product at --widths")
+    ap.add_argument("--widths", default="1,2,4,8,16,32,64", help="spmm workload: product widths")
     ap.add_argument("--extra", type=int, default=1,
                     help="dense workload: also report the sparse N=1e6 solves and the N=65536 k=256 solves "
                          "(one timed solve each) under the keys 'sparse' and 'many_pairs'")
@@ -418,6 +420,44 @@ def sparse_summary(api, hip, seed):
     return out
 
 
+def spmm_workload(a):
+    """CSR products of the N=1e6 orbital-sum matrix (200 draws per row,
+    seed 0) at the widths of --widths, after a 1 GiB device copy (the
+    reference point of the HBM counters): per width, the average of --steps
+    launches timed with HIP events and the compulsory bytes."""
+    import ctypes
+    from block_ipt_amd import api
+
+    hip = api.hip()
+    src, dst = hip.malloc(1 << 30), hip.malloc(1 << 30)
+    hip.check(hip.h.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), 1 << 30, 3), "copy")
+    hip.synchronize()
+    hip.free(src)
+    hip.free(dst)
+    n, m = 1000000, 200
+    cap = n * (2 * m + 1)
+    drp, dci, dv = hip.malloc((n + 1) * 4), hip.malloc(cap * 4), hip.malloc(cap * 8)
+    nnz = api.generate_orbital_csr(n, m, a.seed, drp, dci, dv, cap)
+    widths = [int(w) for w in a.widths.split(",")]
+    kmax = max(widths)
+    dX = hip.to_device(np.random.default_rng(0).standard_normal(n * kmax))
+    dY = hip.malloc(n * kmax * 8)
+    out = {"workload": "CSR product, orbital-sum model N=%d, nnz=%d, seed %d" % (n, nnz, a.seed)}
+    for k in widths:
+        api.profile_reset()
+        api.profile_enable(True, only=["spmm"])
+        for _ in range(max(1, a.steps)):
+            api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
+        api.profile_enable(False)
+        ms, cnt = api.profile_get("spmm")
+        comp = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n * k
+        out["k%d" % k] = {"avg_ms": ms / cnt, "launches": cnt, "compulsory_bytes": comp,
+                          "compulsory_GB/s": comp / (ms / cnt) / 1e6}
+    for p in (drp, dci, dv, dX, dY):
+        hip.free(p)
+    print(json.dumps(out), flush=True)
+
+
 def products_workload(a):
     """Correction products at N=65536, k=256 (C2 recipe): the anchored
     low-precision product E (B - B_a) per precision (one launch per 128
@@ -528,6 +568,9 @@ def main():
         return
     if a.workload == "many":
         many_pairs_workload(a)
+        return
+    if a.workload == "spmm":
+        spmm_workload(a)
         return
     from block_ipt_amd import api
 
