@@ -466,6 +466,23 @@ def last_cluster_ranges(handle=None):
     return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(cnt.value)]
 
 
+def sym_eig(A, handle=None):
+    """Eigenvalues and eigenvectors of the symmetric host matrix A through
+    the library's device eigensolver."""
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    m = A.shape[0]
+    h = hip()
+    dA, dw, dV = h.to_device(A), h.malloc(m * 8), h.malloc(m * m * 8)
+    try:
+        st = lib().ipt_hip_sym_eig(ctypes.c_void_p(dA), m, m, ctypes.c_void_p(dw), ctypes.c_void_p(dV), m, handle)
+        if st != 0:
+            raise IptError(st, "ipt_hip_sym_eig")
+        return h.to_host(dw, (m,)), h.to_host(dV, (m, m))
+    finally:
+        for p in (dA, dw, dV):
+            h.free(p)
+
+
 def last_cluster_states(handle=None):
     """Per cluster of the last block solve: 0 converged, 1 iterating at the
     limit, 2 stalled."""

@@ -38,6 +38,7 @@ SOURCES = [
     "ipt_mlp.hip",
     "ipt_profile.cpp",
     "ipt_fci.cpp",
+    "ipt_eig.hip",
 ]
 
 # Packed-FP32 VALU ops are disabled for all device code.

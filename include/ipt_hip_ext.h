@@ -101,6 +101,15 @@ int ipt_hip_last_solve_stats(void *handle, int *basis_cols, int *steps, long lon
 int ipt_hip_last_davidson_stats(void *handle, int *accepted, int *rejected, int *mlp_accepted, int *mlp_rejected,
                                 long long *matvecs, double *residual_after);
 
+/* Eigenvalues (ascending, d_w: m) and orthonormal eigenvectors (d_V, m x m,
+ * column-major, leading dimension ldv) of the symmetric d_A (m x m, lda; read
+ * only), on the device by two-sided block Jacobi (64 x 64 subproblems
+ * diagonalised in LDS, block rotations applied by tiled kernels).  The
+ * projected problems of Rayleigh-Ritz and Davidson use it in place of the
+ * reference's cusolverDnDsygvd / Dsyevd (IPT/src/ipt_cuda.cu:3213-3226).
+ * Status 6 when it does not converge in 40 sweeps. */
+int ipt_hip_sym_eig(const double *d_A, int lda, int m, double *d_w, double *d_V, int ldv, void *handle);
+
 /* Cluster ranges of the last block-cluster solve on `handle` (NULL: the
  * library context of the current device): *count receives the number of
  * clusters, and the first min(count, capacity) of them are written as
