@@ -18,9 +18,6 @@
 //    Gram-Schmidt passes against the basis, then a Cholesky QR of the block
 //    with rejection of numerically dependent directions) instead of one
 //    vector at a time.
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -102,97 +99,166 @@ int load_mlp_model(const std::string &path, DevBuf<float> *out)
     return IPT_CUDA_SUCCESS;
 }
 
-// Projected problems up to this order are solved on the host.
-constexpr int kHostEigMax = 128;
-// Internal status of rayleigh_ritz: the basis has numerical rank below k.
-constexpr int kRankDeficient = 100;
-
-// Rayleigh-Ritz over V (n x cols) with HV = H V: generalized problem
-// (V^T H V) c = theta (V^T V) c, Cholesky-based like the reference's
-// cusolverDnDsygvd (host Householder + QL for cols <= kHostEigMax,
-// rocsolver dsygvd above); X = V C(:, :k), HX = HV C(:, :k), columns of X
-// normalised (HX scaled alike).  theta (device) gets all `cols` values.
-int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int cols, int k, double *theta,
-                  double *X, double *HX)
-{
-    double *G = nullptr;
-    IPT_STCHK(context_scratch(ctx, SLOT_RR, (size_t)(2 * cols * cols + cols + 64) * sizeof(double), (void **)&G));
-    double *P = G + (size_t)cols * cols;
-    double *E = P + (size_t)cols * cols;
-    int *info = reinterpret_cast<int *>(E + cols);
-    IPT_STCHK(gemm_tn_tall(ctx, V, n, V, n, G, cols, n, cols, cols));
-    IPT_STCHK(gemm_tn_tall(ctx, V, n, HV, n, P, cols, n, cols, cols));
-    // small projected problems are solved on the host (one round trip of
-    // 2 cols^2 doubles instead of a few hundred tiny solver launches); a
-    // Gram matrix that is not positive definite (nearly dependent basis)
-    // falls back to canonical orthogonalisation on the host at any size
-    bool host = cols <= kHostEigMax;
-    if (!host) {
-        void *blas = nullptr;
-        IPT_STCHK(context_blas(ctx, &blas));
-        // rocsolver overwrites G and P: keep copies for the fallback
-        DevBuf<double> g0, p0;
-        IPT_STCHK(g0.alloc((size_t)cols * cols));
-        IPT_STCHK(p0.alloc((size_t)cols * cols));
-        IPT_HIPCHK(hipMemcpyAsync(g0.p, G, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
-                                  ctx->stream));
-        IPT_HIPCHK(hipMemcpyAsync(p0.p, P, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
-                                  ctx->stream));
-        rocblas_status rs = rocsolver_dsygvd((rocblas_handle)blas, rocblas_eform_ax, rocblas_evect_original,
-                                             rocblas_fill_lower, cols, P, cols, G, cols, theta, E, info);
-        if (rs != rocblas_status_success) {
-            fprintf(stderr, "IPT Rayleigh-Ritz: rocsolver_dsygvd status %d\n", (int)rs);
-            return IPT_CUDA_CUSOLVER_ERROR;
-        }
-        int hinfo = 0;
-        IPT_HIPCHK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (hinfo > cols) {
-            // leading minor of G not positive definite
-            IPT_HIPCHK(hipMemcpyAsync(G, g0.p, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
-                                      ctx->stream));
-            IPT_HIPCHK(hipMemcpyAsync(P, p0.p, (size_t)cols * cols * sizeof(double), hipMemcpyDeviceToDevice,
-                                      ctx->stream));
-            host = true;
-        } else if (hinfo != 0) {
-            fprintf(stderr, "IPT Rayleigh-Ritz failed: info=%d\n", hinfo);
-            return IPT_CUDA_CUSOLVER_ERROR;
-        }
-    }
-    if (host) {
-        std::vector<double> hg((size_t)cols * cols), hp((size_t)cols * cols), hc((size_t)cols * cols),
-            ht((size_t)cols);
-        IPT_HIPCHK(hipMemcpyAsync(hg.data(), G, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        IPT_HIPCHK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-        const int st = sym_generalized_eig_robust(cols, hg.data(), hp.data(), ht.data(), hc.data(), k);
-        if (st == 1) {
-            // the caller may widen the basis and retry
-            return kRankDeficient;
-        }
-        if (st != 0) {
-            fprintf(stderr, "IPT Rayleigh-Ritz failed: eigenvalue iteration did not converge\n");
-            return IPT_CUDA_CUSOLVER_ERROR;
-        }
-        IPT_HIPCHK(hipMemcpyAsync(P, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        IPT_HIPCHK(hipMemcpyAsync(theta, ht.data(), ht.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        // the host vectors must stay alive until the copies have run
-        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    }
-    IPT_STCHK(gemm_nn_small(ctx, V, n, P, cols, X, n, n, cols, k, 1.0, 0.0));
-    IPT_STCHK(gemm_nn_small(ctx, HV, n, P, cols, HX, n, n, cols, k, 1.0, 0.0));
-    double *sq = E;  // reuse (cols >= k)
-    IPT_STCHK(column_sqnorms(ctx, X, n, n, k, sq));
-    IPT_STCHK(scale_columns_inv_sqrt(ctx, X, HX, n, n, k, sq));
-    return IPT_CUDA_SUCCESS;
-}
-
 int read_host(Context *ctx, void *dst, const void *src, size_t bytes)
 {
     IPT_HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     return IPT_CUDA_SUCCESS;
 }
+
+// Internal status of rayleigh_ritz: the basis has numerical rank below k.
+constexpr int kRankDeficient = 100;
+
+// Options of one Rayleigh-Ritz projection (IPT/src/ipt_cuda.cu:3039-3316):
+// use_qr -- IPT_BLOCK_CLUSTER_QR: the projection is formed on an explicitly
+// orthonormalised basis Q and its images HQ (the reference's geqrf/orgqr,
+// then Q^T H Q and syevd) instead of the Gram/projected pair; stats --
+// IPT_RR_HOST_ORTHOGONALITY_STATS: ||B^T B - I|| of the basis used and
+// ||X^T X - I|| of the Ritz vectors (:2489-2524, :3168-3206).
+struct RROptions {
+    bool use_qr = false;
+    bool stats = false;
+};
+struct RRInfo {
+    int used_qr = 0;
+    double basis_fro = NAN, basis_max = NAN;
+    int basis_bad = 0;
+    double ritz_fro = NAN, ritz_max = NAN;
+    int ritz_bad = 0;
+};
+
+RROptions rr_options_from_env()
+{
+    RROptions o;
+    o.use_qr = env_flag("IPT_BLOCK_CLUSTER_QR") != 0;
+    o.stats = env_flag("IPT_RR_HOST_ORTHOGONALITY_STATS") != 0;
+    return o;
+}
+
+// ||G - I||_F and max |G - I| of a symmetric m x m device matrix (NaN-aware:
+// *bad = 1 when an entry is not finite)
+int identity_deviation(Context *ctx, const double *G, int m, double *fro, double *mx, int *bad)
+{
+    std::vector<double> h((size_t)m * m);
+    IPT_HIPCHK(hipMemcpyAsync(h.data(), G, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    double f = 0.0, x = 0.0;
+    int b = 0;
+    for (int j = 0; j < m; ++j) {
+        for (int i = 0; i < m; ++i) {
+            const double v = h[(size_t)i + (size_t)j * m];
+            if (!std::isfinite(v)) {
+                b = 1;
+                continue;
+            }
+            const double e = v - (i == j ? 1.0 : 0.0);
+            f += e * e;
+            x = std::max(x, std::fabs(e));
+        }
+    }
+    *bad = b;
+    *fro = b ? NAN : std::sqrt(f);
+    *mx = b ? NAN : x;
+    return IPT_CUDA_SUCCESS;
+}
+
+// Rayleigh-Ritz over V (n x cols) with HV = H V, on the device:
+//   G = V^T V = W diag(g) W^T (the device eigensolver), S = the columns of
+//   W with g > 1e-13 max(g) scaled by g^-1/2 (canonical orthogonalisation:
+//   V S is an orthonormal basis of span(V), dependent directions dropped),
+//   A = S^T (V^T H V) S = Z diag(theta) Z^T, C = S Z,
+//   X = V C(:, :k), HX = HV C(:, :k), columns of X normalised (HX alike).
+// The same pairs as the Cholesky-based cusolverDnDsygvd of the reference
+// when G is positive definite; a nearly dependent basis (rank >= k) is
+// handled instead of failing.  theta (device) receives all r values of the
+// kept directions, +inf after them up to cols.  kRankDeficient when fewer
+// than k directions remain.
+int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int cols, int k, double *theta,
+                  double *X, double *HX, const RROptions &opt = RROptions(), RRInfo *info = nullptr)
+{
+    RRInfo dummy;
+    RRInfo &inf = info != nullptr ? *info : dummy;
+    inf = RRInfo();
+    const size_t cc = (size_t)cols * cols;
+    DevBuf<double> G, P, W, gw, S, T, A, Z, C;
+    IPT_STCHK(G.alloc(cc));
+    IPT_STCHK(P.alloc(cc));
+    IPT_STCHK(W.alloc(cc));
+    IPT_STCHK(gw.alloc((size_t)cols));
+    IPT_STCHK(gemm_tn_tall(ctx, V, n, V, n, G.p, cols, n, cols, cols));
+    IPT_STCHK(symmetrize(ctx, G.p, cols, cols));
+    if (opt.stats && !opt.use_qr) {
+        IPT_STCHK(identity_deviation(ctx, G.p, cols, &inf.basis_fro, &inf.basis_max, &inf.basis_bad));
+    }
+    IPT_STCHK(sym_eig_device(ctx, G.p, cols, cols, gw.p, W.p, cols, nullptr));
+    std::vector<double> hg((size_t)cols);
+    IPT_STCHK(read_host(ctx, hg.data(), gw.p, hg.size() * sizeof(double)));
+    const double gmax = hg.back();
+    int first = cols;
+    while (first > 0 && hg[(size_t)first - 1] > 1e-13 * gmax) {
+        --first;
+    }
+    const int r = cols - first;
+    if (!(gmax > 0.0) || !std::isfinite(gmax) || r < k) {
+        return kRankDeficient;
+    }
+    // S (cols x r): the kept eigenvectors (the last r) over sqrt(g)
+    IPT_STCHK(S.alloc((size_t)cols * r));
+    IPT_HIPCHK(hipMemcpyAsync(S.p, W.p + (size_t)first * cols, (size_t)cols * r * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    IPT_STCHK(scale_columns_inv_sqrt(ctx, S.p, nullptr, cols, cols, r, gw.p + first));
+    IPT_STCHK(A.alloc((size_t)r * r));
+    DevBuf<double> Q, HQ;
+    if (opt.use_qr) {
+        // explicit orthonormal basis Q = V S and its images HQ = HV S, then
+        // the projection Q^T (H Q)
+        IPT_STCHK(Q.alloc((size_t)n * r));
+        IPT_STCHK(HQ.alloc((size_t)n * r));
+        IPT_STCHK(gemm_nn_small(ctx, V, n, S.p, cols, Q.p, n, n, cols, r, 1.0, 0.0));
+        IPT_STCHK(gemm_nn_small(ctx, HV, n, S.p, cols, HQ.p, n, n, cols, r, 1.0, 0.0));
+        IPT_STCHK(gemm_tn_tall(ctx, Q.p, n, HQ.p, n, A.p, r, n, r, r));
+        if (opt.stats) {
+            IPT_STCHK(T.alloc((size_t)r * r));
+            IPT_STCHK(gemm_tn_tall(ctx, Q.p, n, Q.p, n, T.p, r, n, r, r));
+            IPT_STCHK(identity_deviation(ctx, T.p, r, &inf.basis_fro, &inf.basis_max, &inf.basis_bad));
+        }
+        inf.used_qr = 1;
+    } else {
+        IPT_STCHK(gemm_tn_tall(ctx, V, n, HV, n, P.p, cols, n, cols, cols));
+        IPT_STCHK(T.alloc((size_t)r * cols));
+        IPT_STCHK(gemm_tn_tall(ctx, S.p, cols, P.p, cols, T.p, r, cols, r, cols));
+        IPT_STCHK(gemm_nn_small(ctx, T.p, r, S.p, cols, A.p, r, r, cols, r, 1.0, 0.0));
+    }
+    IPT_STCHK(symmetrize(ctx, A.p, r, r));
+    IPT_STCHK(Z.alloc((size_t)r * r));
+    IPT_STCHK(sym_eig_device(ctx, A.p, r, r, theta, Z.p, r, nullptr));
+    if (r < cols) {
+        std::vector<double> inf_fill((size_t)(cols - r), INFINITY);
+        IPT_HIPCHK(hipMemcpyAsync(theta + r, inf_fill.data(), inf_fill.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if (opt.use_qr) {
+        IPT_STCHK(gemm_nn_small(ctx, Q.p, n, Z.p, r, X, n, n, r, k, 1.0, 0.0));
+        IPT_STCHK(gemm_nn_small(ctx, HQ.p, n, Z.p, r, HX, n, n, r, k, 1.0, 0.0));
+    } else {
+        IPT_STCHK(C.alloc((size_t)cols * k));
+        IPT_STCHK(gemm_nn_small(ctx, S.p, cols, Z.p, r, C.p, cols, cols, r, k, 1.0, 0.0));
+        IPT_STCHK(gemm_nn_small(ctx, V, n, C.p, cols, X, n, n, cols, k, 1.0, 0.0));
+        IPT_STCHK(gemm_nn_small(ctx, HV, n, C.p, cols, HX, n, n, cols, k, 1.0, 0.0));
+    }
+    double *sq = gw.p;  // reuse (cols >= k)
+    IPT_STCHK(column_sqnorms(ctx, X, n, n, k, sq));
+    IPT_STCHK(scale_columns_inv_sqrt(ctx, X, HX, n, n, k, sq));
+    if (opt.stats) {
+        DevBuf<double> XX;
+        IPT_STCHK(XX.alloc((size_t)k * k));
+        IPT_STCHK(gemm_tn_tall(ctx, X, n, X, n, XX.p, k, n, k, k));
+        IPT_STCHK(identity_deviation(ctx, XX.p, k, &inf.ritz_fro, &inf.ritz_max, &inf.ritz_bad));
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
 
 // Project the block T (n x t) against the first `cols` columns of V (two
 // classical Gram-Schmidt passes) and orthonormalise it with a Cholesky QR;
@@ -344,8 +410,9 @@ double max_of(const std::vector<double> &v, int *idx)
 // V/HV: n x cap buffers holding the search space (first *cols columns).
 int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *rowabs, const float *mlp, int n, int k,
              double normH, double *V, double *HV, int *cols_io, int cap, double *theta, double *X, double *HX,
-             const BlockParams &p, DavidsonStats *st)
+             const BlockParams &p, DavidsonStats *st, const RROptions &rro, RRInfo *rr_info)
 {
+    RRInfo best_rr = *rr_info, trial_rr;
     const int steps = std::max(1, std::max(0, p.davidson_steps) + std::max(0, p.davidson_extra));
     int kmax = davidson_max_basis(p, n, k);
     if (kmax > cap) {
@@ -499,7 +566,7 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
                                       hipMemcpyDeviceToDevice, ctx->stream));
             IPT_HIPCHK(hipMemcpyAsync(HV + (size_t)cols * n, HT.p, (size_t)n * kept * sizeof(double),
                                       hipMemcpyDeviceToDevice, ctx->stream));
-            int rs = rayleigh_ritz(ctx, V, HV, n, cols + kept, k, ttheta.p, tX.p, tHX.p);
+            int rs = rayleigh_ritz(ctx, V, HV, n, cols + kept, k, ttheta.p, tX.p, tHX.p, rro, &trial_rr);
             if (rs == kRankDeficient) {
                 fprintf(stderr, "IPT Davidson: search space of rank below k\n");
                 rs = IPT_CUDA_CUSOLVER_ERROR;
@@ -574,7 +641,9 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
             }
             if (note_best(tres, "davidson", step, tcols)) {
                 best_current = true;
+                best_rr = trial_rr;
             }
+            *rr_info = trial_rr;
         }
         IPT_HIPCHK(hipMemcpyAsync(X, tX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         IPT_HIPCHK(hipMemcpyAsync(HX, tHX.p, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -611,6 +680,7 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
         st->returned_best = 1;
         st->residual_after = st->best_max;
         st->basis_cols = st->best_basis_cols;
+        *rr_info = best_rr;
     }
     *cols_io = cols;
     return IPT_CUDA_SUCCESS;
@@ -633,6 +703,7 @@ struct BlockResult {
     long long lowp_products = 0;  // low-precision correction products
     int widened = 0;              // final basis widened by unit vectors (rank below k)
     int stalled = 0;              // clusters stopped by stall detection
+    RRInfo rr;                    // the Rayleigh-Ritz of the returned pairs
     DavidsonStats dav;
 };
 
@@ -942,7 +1013,8 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     IPT_STCHK(out->X.alloc((size_t)n * k));
     IPT_STCHK(out->HX.alloc((size_t)n * k));
     int rr_cols = bc;
-    int rs = rayleigh_ritz(ctx, Ba.p, Y.p, n, bc, k, out->theta.p, out->X.p, out->HX.p);
+    const RROptions rro = rr_options_from_env();
+    int rs = rayleigh_ritz(ctx, Ba.p, Y.p, n, bc, k, out->theta.p, out->X.p, out->HX.p, rro, &out->rr);
     if (rs == kRankDeficient && bc + k <= cap) {
         // Clusters that drifted onto overlapping subspaces leave a basis of
         // rank below k: the unit vectors of the k target rows widen it, so
@@ -956,7 +1028,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         out->fp64_products += 1;
         out->widened = 1;
         rr_cols = bc + k;
-        rs = rayleigh_ritz(ctx, Ba.p, Y.p, n, rr_cols, k, out->theta.p, out->X.p, out->HX.p);
+        rs = rayleigh_ritz(ctx, Ba.p, Y.p, n, rr_cols, k, out->theta.p, out->X.p, out->HX.p, rro, &out->rr);
     }
     if (rs == kRankDeficient) {
         fprintf(stderr, "IPT Rayleigh-Ritz failed: basis of rank below k\n");
@@ -983,7 +1055,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             IPT_STCHK(load_mlp_model(p.mlp_model, &mlp));
         }
         IPT_STCHK(davidson(ctx, apply, d, rowabs, mlp.p, n, k, normH, Ba.p, Y.p, &cols, cap, out->theta.p, out->X.p,
-                           out->HX.p, p, &out->dav));
+                           out->HX.p, p, &out->dav, rro, &out->rr));
         out->matvecs += out->dav.matvecs;
         out->basis_cols = out->dav.basis_cols;
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1064,6 +1136,14 @@ void fill_result(IPTCudaResult *r, const BlockResult &b, int n, int k)
     r->davidson_restart_count = b.dav.restarts;
     r->accepted_steps = b.dav.accepted_steps;
     r->rejected_steps = b.dav.rejected_steps;
+    // IPT/src/ipt_cuda.cu:3168-3206 (NaN unless IPT_RR_HOST_ORTHOGONALITY_STATS)
+    r->rayleigh_ritz_used_qr = b.rr.used_qr;
+    r->basis_orthogonality_frobenius_error = b.rr.basis_fro;
+    r->basis_orthogonality_max_abs_error = b.rr.basis_max;
+    r->basis_has_nan_or_inf = b.rr.basis_bad;
+    r->ritz_vectors_orthogonality_frobenius_error = b.rr.ritz_fro;
+    r->ritz_vectors_orthogonality_max_abs_error = b.rr.ritz_max;
+    r->ritz_vectors_has_nan_or_inf = b.rr.ritz_bad;
     // IPT/src/ipt_cuda.cu:678-701
     const DavidsonStats &d = b.dav;
     const bool has = d.best_update_count > 0;

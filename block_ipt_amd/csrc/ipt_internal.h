@@ -206,6 +206,11 @@ int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, cons
 // (IPT/src/mixed_precision/ipt_rayleigh_mixed_cuda.cu:84-106).
 int symmetrize(Context *ctx, double *A, long ld, int n);
 
+// Symmetric eigendecomposition on the device (ipt_eig.hip): eigenvalues
+// ascending in w (m), orthonormal eigenvectors in V (m x m, ldv); A (m x m,
+// lda) is read only.  *sweeps_out (may be NULL): block-Jacobi sweeps used.
+int sym_eig_device(Context *ctx, const double *A, long lda, int m, double *w, double *V, long ldv, int *sweeps_out);
+
 // Layout copies between column-major (ld) and row-major (ld) n x k.
 int colmajor_to_rowmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k);
 int rowmajor_to_colmajor(Context *ctx, const double *A, long lda, double *B, long ldb, int n, int k);

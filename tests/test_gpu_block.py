@@ -280,3 +280,30 @@ def test_davidson_returns_best_state(block_env):
     assert sinfo["final_returned_from_best_so_far"] == 0 and sinfo["best_so_far_step"] == best
     assert np.array_equal(sl_, gl) and np.array_equal(sv, gv)
     assert sinfo["davidson_residual_after"] == info["best_so_far_max_residual"]
+
+
+def test_rayleigh_ritz_qr_option_and_orthogonality_stats(block_env):
+    """IPT_BLOCK_CLUSTER_QR (IPT/src/ipt_cuda.cu:3071, 3138-3221): the
+    projection on an explicitly orthonormalised basis gives the same pairs
+    and sets rayleigh_ritz_used_qr; IPT_RR_HOST_ORTHOGONALITY_STATS
+    (:3072-3073, 2489-2524) reports ||B^T B - I|| of the basis used (large for
+    the raw cluster bases, rounding-level for the orthonormalised one) and
+    ||X^T X - I|| of the Ritz vectors."""
+    H = orc.rand_dense(900, 5, 0.0, 10.0, 0.06)
+    block_env.setenv("IPT_RR_HOST_ORTHOGONALITY_STATS", "1")
+    gv, gl, info = api.ipt_cuda_tol(H, 20, 1e-12, 200)
+    _check_eigs(H, gv, gl, 20)
+    assert info["rayleigh_ritz_used_qr"] == 0
+    assert info["basis_orthogonality_max_abs_error"] > 1e-6  # cluster bases are not orthonormal
+    assert info["ritz_vectors_orthogonality_max_abs_error"] <= 1e-13
+    assert info["basis_has_nan_or_inf"] == 0 and info["ritz_vectors_has_nan_or_inf"] == 0
+    block_env.setenv("IPT_BLOCK_CLUSTER_QR", "1")
+    qv, ql, qinfo = api.ipt_cuda_tol(H, 20, 1e-12, 200)
+    _check_eigs(H, qv, ql, 20)
+    assert qinfo["rayleigh_ritz_used_qr"] == 1
+    assert qinfo["basis_orthogonality_max_abs_error"] <= 1e-13
+    assert qinfo["ritz_vectors_orthogonality_max_abs_error"] <= 1e-13
+    assert np.abs(ql - gl).max() <= 1e-11 * max(1.0, np.abs(gl).max())
+    block_env.delenv("IPT_RR_HOST_ORTHOGONALITY_STATS")
+    _, _, ninfo = api.ipt_cuda_tol(H, 20, 1e-12, 200)
+    assert np.isnan(ninfo["basis_orthogonality_frobenius_error"])
