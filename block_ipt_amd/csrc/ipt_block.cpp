@@ -163,9 +163,11 @@ int identity_deviation(Context *ctx, const double *G, int m, double *fro, double
 }
 
 // Rayleigh-Ritz over V (n x cols) with HV = H V, on the device:
-//   G = V^T V = W diag(g) W^T (the device eigensolver), S = the columns of
-//   W with g > 1e-13 max(g) scaled by g^-1/2 (canonical orthogonalisation:
-//   V S is an orthonormal basis of span(V), dependent directions dropped),
+//   G = V^T V; S = G^-1/2 by Newton-Schulz when the column-scaled G is
+//   within 0.25 of the identity, else G = W diag(g) W^T (the device
+//   eigensolver) and S = the columns of W with g > 1e-13 max(g) scaled by
+//   g^-1/2 (canonical orthogonalisation: V S is an orthonormal basis of
+//   span(V), dependent directions dropped);
 //   A = S^T (V^T H V) S = Z diag(theta) Z^T, C = S Z,
 //   X = V C(:, :k), HX = HV C(:, :k), columns of X normalised (HX alike).
 // The same pairs as the Cholesky-based cusolverDnDsygvd of the reference
@@ -190,23 +192,35 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
     if (opt.stats && !opt.use_qr) {
         IPT_STCHK(identity_deviation(ctx, G.p, cols, &inf.basis_fro, &inf.basis_max, &inf.basis_bad));
     }
-    IPT_STCHK(sym_eig_device(ctx, G.p, cols, cols, gw.p, W.p, cols, nullptr));
-    std::vector<double> hg((size_t)cols);
-    IPT_STCHK(read_host(ctx, hg.data(), gw.p, hg.size() * sizeof(double)));
-    const double gmax = hg.back();
-    int first = cols;
-    while (first > 0 && hg[(size_t)first - 1] > 1e-13 * gmax) {
-        --first;
+    // A basis within 0.25 (Frobenius) of orthonormal after column scaling
+    // (always so for Davidson's orthonormalised basis) is full rank with
+    // condition number below 5/3: S = G^-1/2 by a few products, no
+    // eigensolve.  Otherwise the canonical orthogonalisation below.
+    int r = cols;
+    {
+        IPT_STCHK(S.alloc(cc));
+        double dev = NAN;
+        int applied = 0;
+        IPT_STCHK(gram_inverse_sqrt(ctx, G.p, cols, 0.25, S.p, &dev, &applied));
+        if (!applied) {
+            IPT_STCHK(sym_eig_device(ctx, G.p, cols, cols, gw.p, W.p, cols, nullptr));
+            std::vector<double> hg((size_t)cols);
+            IPT_STCHK(read_host(ctx, hg.data(), gw.p, hg.size() * sizeof(double)));
+            const double gmax = hg.back();
+            int first = cols;
+            while (first > 0 && hg[(size_t)first - 1] > 1e-13 * gmax) {
+                --first;
+            }
+            r = cols - first;
+            if (!(gmax > 0.0) || !std::isfinite(gmax) || r < k) {
+                return kRankDeficient;
+            }
+            // S (cols x r): the kept eigenvectors (the last r) over sqrt(g)
+            IPT_HIPCHK(hipMemcpyAsync(S.p, W.p + (size_t)first * cols, (size_t)cols * r * sizeof(double),
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+            IPT_STCHK(scale_columns_inv_sqrt(ctx, S.p, nullptr, cols, cols, r, gw.p + first));
+        }
     }
-    const int r = cols - first;
-    if (!(gmax > 0.0) || !std::isfinite(gmax) || r < k) {
-        return kRankDeficient;
-    }
-    // S (cols x r): the kept eigenvectors (the last r) over sqrt(g)
-    IPT_STCHK(S.alloc((size_t)cols * r));
-    IPT_HIPCHK(hipMemcpyAsync(S.p, W.p + (size_t)first * cols, (size_t)cols * r * sizeof(double),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-    IPT_STCHK(scale_columns_inv_sqrt(ctx, S.p, nullptr, cols, cols, r, gw.p + first));
     IPT_STCHK(A.alloc((size_t)r * r));
     DevBuf<double> Q, HQ;
     if (opt.use_qr) {

@@ -87,6 +87,12 @@ int column_sqnorms(Context *ctx, const double *X, long ld, long nrows, int nc, d
 // X[:, j] *= 1/sqrt(sq[j]) (and the same for HX when non-NULL)
 int scale_columns_inv_sqrt(Context *ctx, double *X, double *HX, long ld, long nrows, int nc,
                            const double *sq);
+// For a symmetric positive Gram matrix G (m x m, ld m): *dev_out =
+// ||D^-1/2 G D^-1/2 - I||_F with D = diag(G).  When that is at most max_dev
+// (< 1), S (m x m, ld m) = D^-1/2 (D^-1/2 G D^-1/2)^-1/2, so that
+// S^T G S = I, and *applied = 1; otherwise S is untouched and *applied = 0.
+int gram_inverse_sqrt(Context *ctx, const double *G, int m, double max_dev, double *S, double *dev_out,
+                      int *applied);
 // res[j] = ||HX_j - theta_j X_j|| / (max(1,normH) max(1,||X_j||)); also
 // writes R = HX - X theta when R != NULL.
 int residual_norms(Context *ctx, const double *X, const double *HX, long ld, long nrows, int nc,

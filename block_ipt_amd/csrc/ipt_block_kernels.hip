@@ -992,6 +992,139 @@ int scale_columns_inv_sqrt(Context *ctx, double *X, double *HX, long ld, long nr
     return IPT_CUDA_SUCCESS;
 }
 
+// ------------------------------------------------------------------ Gram inverse square root
+
+// dinv[i] = G_ii^-1/2; E = D^-1/2 G D^-1/2 - I (m x m, ld m); part[b] = sum
+// of E^2 over the grid-stride share of block b (256 blocks).
+__global__ __launch_bounds__(256) void k_gram_deviation(const double *__restrict__ G, int m,
+                                                        double *__restrict__ E, double *__restrict__ part)
+{
+    double acc = 0.0;
+    const long total = (long)m * m;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % m), j = (int)(idx / m);
+        const double gi = G[i + (long)i * m], gj = G[j + (long)j * m];
+        const double e = G[idx] / (sqrt(gi) * sqrt(gj)) - (i == j ? 1.0 : 0.0);
+        E[idx] = e;
+        acc += e * e;
+    }
+    __shared__ double sh[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        acc += __shfl_down(acc, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    }
+}
+
+// out[0] = sum of the 256 partials (fixed order); out[1..m] = G_ii^-1/2
+__global__ __launch_bounds__(64) void k_gram_deviation_final(const double *__restrict__ part,
+                                                             const double *__restrict__ G, int m,
+                                                             double *__restrict__ out)
+{
+    double s = 0.0;
+    for (int b = threadIdx.x; b < 256; b += 64) {
+        s += part[b];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s;
+    }
+    for (int i = threadIdx.x; i < m; i += 64) {
+        out[1 + i] = 1.0 / sqrt(G[i + (long)i * m]);
+    }
+}
+
+// T = a I + b W (m x m)
+__global__ void k_shifted_scale(const double *__restrict__ W, int m, double a, double b, double *__restrict__ T)
+{
+    const long total = (long)m * m;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % m), j = (int)(idx / m);
+        T[idx] = b * W[idx] + (i == j ? a : 0.0);
+    }
+}
+
+// S[i + j m] = dinv[i] Z[i + j m]
+__global__ void k_scale_rows(const double *__restrict__ Z, int m, const double *__restrict__ dinv,
+                             double *__restrict__ S)
+{
+    const long total = (long)m * m;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long)gridDim.x * blockDim.x) {
+        S[idx] = dinv[idx % m] * Z[idx];
+    }
+}
+
+int gram_inverse_sqrt(Context *ctx, const double *G, int m, double max_dev, double *S, double *dev_out,
+                      int *applied)
+{
+    *applied = 0;
+    *dev_out = NAN;
+    if (m <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    const size_t mm = (size_t)m * m;
+    DevBuf<double> E, Y, Z, W, T, aux;
+    IPT_STCHK(E.alloc(mm));
+    IPT_STCHK(aux.alloc(256 + 1 + (size_t)m));
+    double *part = aux.p, *out = aux.p + 256;
+    const int blocks = blocks_for((long)mm, 256, 4096);
+    hipLaunchKernelGGL(k_gram_deviation, dim3(256), dim3(256), 0, ctx->stream, G, m, E.p, part);
+    hipLaunchKernelGGL(k_gram_deviation_final, dim3(1), dim3(64), 0, ctx->stream, part, G, m, out);
+    IPT_HIPCHK(hipGetLastError());
+    double fro2 = 0.0;
+    IPT_HIPCHK(hipMemcpyAsync(&fro2, out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    const double dev = std::sqrt(fro2);
+    *dev_out = dev;
+    if (!(dev <= max_dev) || !(max_dev < 1.0)) {
+        return IPT_CUDA_SUCCESS;
+    }
+    const double *dinv = out + 1;
+    IPT_STCHK(Z.alloc(mm));
+    // ||E||_2 <= dev.  Below 1e-8 the series (I + E)^-1/2 = I - E/2 + 3E^2/8
+    // - ... is exact to rounding after its first-order term.  Otherwise the
+    // coupled Newton-Schulz iteration Y <- Y T, Z <- T Z, T = (3I - Z Y)/2
+    // from Y = I + E, Z = I (Higham, Functions of Matrices, 2008, 6.3)
+    // converges to Z = (I + E)^-1/2; an eigenvalue 1 - delta of Z Y maps to
+    // 1 - (3/4) delta^2 - (1/4) delta^3, so the step count follows from dev.
+    if (dev <= 1e-8) {
+        hipLaunchKernelGGL(k_shifted_scale, dim3(blocks), dim3(256), 0, ctx->stream, E.p, m, 1.0, -0.5, Z.p);
+        IPT_HIPCHK(hipGetLastError());
+    } else {
+        IPT_STCHK(Y.alloc(mm));
+        IPT_STCHK(W.alloc(mm));
+        IPT_STCHK(T.alloc(mm));
+        hipLaunchKernelGGL(k_shifted_scale, dim3(blocks), dim3(256), 0, ctx->stream, E.p, m, 1.0, 1.0, Y.p);
+        hipLaunchKernelGGL(k_shifted_scale, dim3(blocks), dim3(256), 0, ctx->stream, E.p, m, 1.0, 0.0, Z.p);
+        IPT_HIPCHK(hipGetLastError());
+        double delta = dev;
+        while (delta > 1e-17) {
+            IPT_STCHK(gemm_f64(ctx, Z.p, m, Y.p, m, W.p, m, m, m));
+            hipLaunchKernelGGL(k_shifted_scale, dim3(blocks), dim3(256), 0, ctx->stream, W.p, m, 1.5, -0.5, T.p);
+            IPT_HIPCHK(hipGetLastError());
+            IPT_STCHK(gemm_f64(ctx, Y.p, m, T.p, m, W.p, m, m, m));
+            std::swap(Y.p, W.p);
+            IPT_STCHK(gemm_f64(ctx, T.p, m, Z.p, m, W.p, m, m, m));
+            std::swap(Z.p, W.p);
+            delta = 0.75 * delta * delta + 0.25 * delta * delta * delta;
+        }
+    }
+    hipLaunchKernelGGL(k_scale_rows, dim3(blocks), dim3(256), 0, ctx->stream, Z.p, m, dinv, S);
+    IPT_HIPCHK(hipGetLastError());
+    *applied = 1;
+    return IPT_CUDA_SUCCESS;
+}
+
 // IPT/src/ipt_cuda.cu:2399-2447, 2684-2697: R = HX - X theta and the scaled
 // residual norm per column.  One block per column.
 __global__ void k_residual_norms(const double *__restrict__ X, const double *__restrict__ HX, long ld, long nrows,

@@ -1,18 +1,26 @@
 // Symmetric eigendecomposition on the device for the projected problems of
 // Rayleigh-Ritz (orders up to a few thousand), by two-sided block Jacobi.
 //
-// The matrix is padded to M = 32 NBK rows (NBK even; padding rows and
+// The matrix is padded to M = JB NBK rows (NBK even; padding rows and
 // columns zero, padding diagonal far above the spectrum, so padding never
 // rotates with real indices and its eigenvalues sort last).  A sweep visits
-// every pair (p, q) of 32-row blocks once, in round-robin order: the NBK/2
-// pairs of a round are disjoint.  Per round
-//   k_jac_pair : one workgroup per pair diagonalises its 64 x 64 block
-//                [[A_pp, A_pq], [A_qp, A_qq]] by cyclic Jacobi in LDS
-//                (Golub & Van Loan, Matrix Computations, 4th ed., 8.5,
-//                parallel ordering of section 8.5.5) and writes its 64 x 64
-//                orthogonal factor Q;
-//   k_jac_rows : [A_p; A_q] <- Q^T [A_p; A_q]   (64-column tiles)
-//   k_jac_cols : [A_.p A_.q] <- [A_.p A_.q] Q, the same for V (64-row tiles)
+// every pair (p, q) of JB-row blocks once, in round-robin order: the NBK/2
+// pairs of a round are disjoint and cover every block.  Per round
+//   k_jac_pair  : one workgroup per pair runs cyclic Jacobi sweeps (Golub &
+//                 Van Loan, Matrix Computations, 4th ed., 8.5, parallel
+//                 ordering of section 8.5.5) on its 2JB x 2JB block
+//                 [[A_pp, A_pq], [A_qp, A_qq]] in LDS and writes the
+//                 accumulated orthogonal factor Q (or a flag that it did not
+//                 rotate).  Each inner round updates the block by its JB
+//                 disjoint rotations at once: every 2 x 2 sub-block (pair i,
+//                 pair j) is rotated from both sides by one thread.  A
+//                 bounded number of inner sweeps (one by default) suffices:
+//                 the outer iteration converges in as many sweeps as with
+//                 fully diagonalised subproblems, at a fraction of the
+//                 serial depth;
+//   k_jac_apply : every tile (pair P, pair R) of A becomes Q_P^T A_PR Q_R
+//                 (the upper tiles, mirrored), and every row tile of V
+//                 becomes V_TR Q_R; pairs that did not rotate are skipped.
 // The off-diagonal Frobenius norm is checked after each sweep (one read of
 // two scalars); converged when it falls below 1e-15 of the matrix norm, or
 // below 1e-12 of it and no longer shrinking (the rounding floor).
@@ -29,9 +37,12 @@ namespace ipt {
 
 namespace {
 
-constexpr int JB = 32;       // block rows
+constexpr int JB = 16;       // block rows
 constexpr int JS = 2 * JB;   // subproblem order
-constexpr int JT = 256;      // threads per workgroup
+constexpr int NR = JB;       // rotations per inner round
+constexpr int PT = 256;      // threads of the subproblem workgroup
+constexpr int AT = 256;      // threads of the apply workgroup
+constexpr int NPARTS = 256;  // partial sums of the norm pass
 
 // pair i of round r among nb (even) players by the circle method: player 0
 // fixed, the other nb - 1 on a circle rotated by one place per round; pair
@@ -52,10 +63,25 @@ __host__ __device__ inline void round_pair(int nb, int r, int i, int *p, int *q)
     *q = a < b ? b : a;
 }
 
-// pair i of inner round r among JS indices: the same tournament
-__device__ inline void inner_pair(int r, int i, int *p, int *q)
+// global row of local index l of the pair (bp, bq)
+__device__ inline int pair_row(int l, int bp, int bq)
 {
-    round_pair(JS, r, i, p, q);
+    return l < JB ? JB * bp + l : JB * bq + l - JB;
+}
+
+// (i, j), i <= j < n, of the b-th entry of an upper triangle stored by
+// columns: j = floor((sqrt(8b + 1) - 1) / 2), corrected for rounding
+__device__ inline void tri_index(int b, int *i, int *j)
+{
+    int c = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+    while (c * (c + 1) / 2 > b) {
+        --c;
+    }
+    while ((c + 1) * (c + 2) / 2 <= b) {
+        ++c;
+    }
+    *j = c;
+    *i = b - c * (c + 1) / 2;
 }
 
 __global__ void k_jac_init(const double *__restrict__ A, long lda, int m, double *__restrict__ Ap, double *__restrict__ V,
@@ -75,126 +101,148 @@ __global__ void k_jac_init(const double *__restrict__ A, long lda, int m, double
     }
 }
 
-// Diagonalise the 64 x 64 block of pair (round, blockIdx.x) in LDS; Q out.
-__global__ __launch_bounds__(JT) void k_jac_pair(const double *__restrict__ Ap, int M, int nb, int round,
-                                                 double *__restrict__ Qout, int inner_sweeps)
+// Cyclic Jacobi on the 2JB x 2JB block of pair (round, blockIdx.x), at most
+// `inner_sweeps` sweeps, stopping early when its off-diagonal sum of squares
+// is at most abs_thr2 or a sweep rotated nothing.  Writes qflag (1 when any
+// rotation was applied) and, if so, the factor Q (column-major JS x JS).
+// Thread t < JB(JB+1)/2 owns the 2 x 2 sub-block (pair i, pair j), i <= j,
+// of its triangle index in every inner round, and two (row, pair) entries
+// of Q; the pair indices of a round are recomputed in registers.
+__global__ __launch_bounds__(PT) void k_jac_pair(const double *__restrict__ Ap, int M, int nb, int round,
+                                                 double *__restrict__ Qout, int *__restrict__ qflag,
+                                                 int inner_sweeps, double abs_thr2)
 {
     __shared__ double S[JS][JS + 1];
     __shared__ double Q[JS][JS + 1];
-    __shared__ double rc[JS / 2], rs[JS / 2];
-    __shared__ double red[JT / 64];
-    __shared__ int done, rotated;
-    static_assert(JT % 64 == 0, "whole wavefronts");
+    __shared__ double rc[NR], rs[NR], rt[NR];
+    __shared__ double red[PT / 64];
+    __shared__ int sweep_rot;
+    static_assert(PT % 64 == 0 && PT >= NR * (NR + 1) / 2 && (JS * NR) % PT == 0, "thread mapping");
+    constexpr int QPT = JS * NR / PT;  // Q entries per thread
     int bp, bq;
     round_pair(nb, round, blockIdx.x, &bp, &bq);
     const int t = threadIdx.x;
-    // local index l < 32 -> global row 32 bp + l, l >= 32 -> 32 bq + l - 32
-    for (int e = t; e < JS * JS; e += JT) {
+    for (int e = t; e < JS * JS; e += PT) {
         const int i = e % JS, j = e / JS;
-        const int gi = i < JB ? JB * bp + i : JB * bq + i - JB;
-        const int gj = j < JB ? JB * bp + j : JB * bq + j - JB;
-        S[i][j] = Ap[gi + (long)gj * M];
+        S[i][j] = Ap[pair_row(i, bp, bq) + (long)pair_row(j, bp, bq) * M];
         Q[i][j] = i == j ? 1.0 : 0.0;
     }
-    __syncthreads();
-    // the Frobenius norm is invariant under the rotations
-    if (t == 0) {
-        rotated = 0;
+    const bool owns_block = t < NR * (NR + 1) / 2;
+    int bi = 0, bj = 0;
+    if (owns_block) {
+        tri_index(t, &bi, &bj);
     }
-    double tot = 0.0;
-    for (int e = t; e < JS * JS; e += JT) {
-        tot += S[e % JS][e / JS] * S[e % JS][e / JS];
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        tot += __shfl_down(tot, o, 64);
-    }
-    if ((t & 63) == 0) {
-        red[t >> 6] = tot;
-    }
-    __syncthreads();
-    tot = 0.0;
-    for (int w = 0; w < JT / 64; ++w) {
-        tot += red[w];
-    }
+    int any_rot = 0;
     __syncthreads();
     for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
         double off = 0.0;
-        for (int e = t; e < JS * JS; e += JT) {
+        for (int e = t; e < JS * JS; e += PT) {
             const int i = e % JS, j = e / JS;
             if (i != j) {
                 off += S[i][j] * S[i][j];
             }
         }
         for (int o = 32; o > 0; o >>= 1) {
-            off += __shfl_down(off, o, 64);
+            off += __shfl_xor(off, o, 64);
         }
         if ((t & 63) == 0) {
             red[t >> 6] = off;
         }
-        __syncthreads();
         if (t == 0) {
-            double so = 0.0;
-            for (int w = 0; w < JT / 64; ++w) {
-                so += red[w];
-            }
-            // off-diagonal below 1e-14 of the norm, or the last sweep found
-            // nothing above the rotation threshold
-            done = !(so > 1e-28 * tot) || (sweep > 0 && !rotated);
-            rotated = 0;
+            sweep_rot = 0;
         }
         __syncthreads();
-        if (done) {
+        off = 0.0;
+        for (int w = 0; w < PT / 64; ++w) {
+            off += red[w];
+        }
+        if (!(off > abs_thr2)) {
             break;
         }
         for (int r = 0; r < JS - 1; ++r) {
-            if (t < JS / 2) {
+            if (t < NR) {
                 int p, q;
-                inner_pair(r, t, &p, &q);
+                round_pair(JS, r, t, &p, &q);
                 const double a = S[p][p], b = S[q][q], c = S[p][q];
-                double cs = 1.0, sn = 0.0;
+                double cs = 1.0, sn = 0.0, tn = 0.0;
                 if (c != 0.0 && fabs(c) > 1e-17 * sqrt(fabs(a) * fabs(b))) {
-                    rotated = 1;
                     const double tau = (b - a) / (2.0 * c);
-                    const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    cs = 1.0 / sqrt(1.0 + tt * tt);
-                    sn = tt * cs;
+                    tn = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    cs = 1.0 / sqrt(1.0 + tn * tn);
+                    sn = tn * cs;
+                    sweep_rot = 1;
                 }
                 rc[t] = cs;
                 rs[t] = sn;
+                rt[t] = tn;
             }
             __syncthreads();
-            // rows: S <- J^T S
-            for (int e = t; e < (JS / 2) * JS; e += JT) {
-                const int i = e % (JS / 2), j = e / (JS / 2);
-                int p, q;
-                inner_pair(r, i, &p, &q);
-                const double cs = rc[i], sn = rs[i];
-                const double sp = S[p][j], sq = S[q][j];
-                S[p][j] = cs * sp - sn * sq;
-                S[q][j] = sn * sp + cs * sq;
+            // S <- J^T S J by 2 x 2 sub-blocks, the lower ones mirrored;
+            // J = [[c, s], [-s, c]] per pair
+            if (owns_block) {
+                int pi, qi;
+                round_pair(JS, r, bi, &pi, &qi);
+                if (bi == bj) {
+                    const double tn = rt[bi];
+                    if (tn != 0.0) {
+                        const double x = S[pi][qi];
+                        S[pi][pi] -= tn * x;
+                        S[qi][qi] += tn * x;
+                        S[pi][qi] = 0.0;
+                        S[qi][pi] = 0.0;
+                    }
+                } else {
+                    int pj, qj;
+                    round_pair(JS, r, bj, &pj, &qj);
+                    const double ci = rc[bi], si = rs[bi], cj = rc[bj], sj = rs[bj];
+                    const double x00 = S[pi][pj], x01 = S[pi][qj], x10 = S[qi][pj], x11 = S[qi][qj];
+                    const double y00 = ci * x00 - si * x10, y01 = ci * x01 - si * x11;
+                    const double y10 = si * x00 + ci * x10, y11 = si * x01 + ci * x11;
+                    const double z00 = cj * y00 - sj * y01, z01 = sj * y00 + cj * y01;
+                    const double z10 = cj * y10 - sj * y11, z11 = sj * y10 + cj * y11;
+                    S[pi][pj] = z00;
+                    S[pi][qj] = z01;
+                    S[qi][pj] = z10;
+                    S[qi][qj] = z11;
+                    S[pj][pi] = z00;
+                    S[qj][pi] = z01;
+                    S[pj][qi] = z10;
+                    S[qj][qi] = z11;
+                }
             }
-            __syncthreads();
-            // columns: S <- S J, Q <- Q J
-            for (int e = t; e < (JS / 2) * JS; e += JT) {
-                const int i = e % (JS / 2), j = e / (JS / 2);
-                int p, q;
-                inner_pair(r, i, &p, &q);
-                const double cs = rc[i], sn = rs[i];
-                const double sp = S[j][p], sq = S[j][q];
-                S[j][p] = cs * sp - sn * sq;
-                S[j][q] = sn * sp + cs * sq;
-                const double qp = Q[j][p], qq = Q[j][q];
-                Q[j][p] = cs * qp - sn * qq;
-                Q[j][q] = sn * qp + cs * qq;
+            // Q <- Q J
+#pragma unroll
+            for (int u = 0; u < QPT; ++u) {
+                const int e = t + u * PT;
+                const int row = e % JS, j = e / JS;
+                int pj, qj;
+                round_pair(JS, r, j, &pj, &qj);
+                const double cj = rc[j], sj = rs[j];
+                const double a = Q[row][pj], b = Q[row][qj];
+                Q[row][pj] = cj * a - sj * b;
+                Q[row][qj] = sj * a + cj * b;
             }
             __syncthreads();
         }
+        const int rotated = sweep_rot;
+        __syncthreads();  // every wavefront has read the flag before it is reset
+        any_rot |= rotated;
+        if (!rotated) {
+            break;
+        }
     }
-    // Q is a product of up to a few hundred rotations per column; one
-    // Newton-Schulz step Q (3I - Q^T Q) / 2 (quadratic in the departure
-    // from orthogonality) keeps the outer similarity orthogonal to rounding,
-    // so the spectrum does not drift over the outer rounds.  S is free now.
-    for (int e = t; e < JS * JS; e += JT) {
+    if (t == 0) {
+        qflag[blockIdx.x] = any_rot;
+    }
+    if (!any_rot) {
+        return;
+    }
+    // Q is a product of a few hundred rotations per column; one Newton-Schulz
+    // step Q (3I - Q^T Q) / 2 (quadratic in the departure from orthogonality)
+    // keeps the outer similarity orthogonal to rounding, so the spectrum does
+    // not drift over the outer rounds.  S is free now.
+    __syncthreads();
+    for (int e = t; e < JS * JS; e += PT) {
         const int i = e % JS, j = e / JS;
         double acc = 0.0;
         for (int l = 0; l < JS; ++l) {
@@ -204,7 +252,7 @@ __global__ __launch_bounds__(JT) void k_jac_pair(const double *__restrict__ Ap, 
     }
     __syncthreads();
     double *Qo = Qout + (long)blockIdx.x * JS * JS;
-    for (int e = t; e < JS * JS; e += JT) {
+    for (int e = t; e < JS * JS; e += PT) {
         const int i = e % JS, j = e / JS;
         double acc = 0.0;
         for (int l = 0; l < JS; ++l) {
@@ -214,84 +262,161 @@ __global__ __launch_bounds__(JT) void k_jac_pair(const double *__restrict__ Ap, 
     }
 }
 
-// [A_p; A_q](:, tile) <- Q^T [A_p; A_q](:, tile): one workgroup per (pair,
-// 64-column tile)
-__global__ __launch_bounds__(JT) void k_jac_rows(double *__restrict__ Ap, int M, int nb, int round,
-                                                 const double *__restrict__ Qall)
+// Y = op(L) X for JS x JS LDS tiles, 256 threads with 2 x 2 outputs each
+// (rows ti, ti + 16; columns tj, tj + 16); transpose_left: L^T.
+template <bool transpose_left>
+__device__ inline void tile_mul(const double (*L)[JS + 1], const double (*X)[JS + 1], double acc[2][2])
 {
-    __shared__ double Qs[JS][JS + 1];
-    __shared__ double X[JS][JS + 1];
-    int bp, bq;
-    round_pair(nb, round, blockIdx.x, &bp, &bq);
-    const double *Q = Qall + (long)blockIdx.x * JS * JS;
-    const int c0 = blockIdx.y * JS;
-    const int t = threadIdx.x;
-    for (int e = t; e < JS * JS; e += JT) {
-        const int i = e % JS, j = e / JS;
-        Qs[i][j] = Q[e];
-        const int gi = i < JB ? JB * bp + i : JB * bq + i - JB;
-        const int gj = c0 + j;
-        X[i][j] = gj < M ? Ap[gi + (long)gj * M] : 0.0;
-    }
-    __syncthreads();
-    for (int e = t; e < JS * JS; e += JT) {
-        const int i = e % JS, j = e / JS;
-        const int gj = c0 + j;
-        double acc = 0.0;
+    const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
+    acc[0][0] = acc[0][1] = acc[1][0] = acc[1][1] = 0.0;
 #pragma unroll 8
-        for (int l = 0; l < JS; ++l) {
-            acc = fma(Qs[l][i], X[l][j], acc);
-        }
-        const int gi = i < JB ? JB * bp + i : JB * bq + i - JB;
-        if (gj < M) {
-            Ap[gi + (long)gj * M] = acc;
-        }
+    for (int l = 0; l < JS; ++l) {
+        const double a0 = transpose_left ? L[l][ti] : L[ti][l];
+        const double a1 = transpose_left ? L[l][ti + 16] : L[ti + 16][l];
+        const double b0 = X[l][tj], b1 = X[l][tj + 16];
+        acc[0][0] = fma(a0, b0, acc[0][0]);
+        acc[0][1] = fma(a0, b1, acc[0][1]);
+        acc[1][0] = fma(a1, b0, acc[1][0]);
+        acc[1][1] = fma(a1, b1, acc[1][1]);
     }
 }
 
-// [A_.p A_.q](tile, :) <- [A_.p A_.q](tile, :) Q and the same for V: one
-// workgroup per (pair, 64-row tile)
-__global__ __launch_bounds__(JT) void k_jac_cols(double *__restrict__ Ap, double *__restrict__ V, int M, int nb,
-                                                 int round, const double *__restrict__ Qall)
+// Workgroups [0, ntile_a): the upper tiles (P <= R) of A by pairs,
+// A_PR <- Q_P^T A_PR Q_R and A_RP <- its transpose (a diagonal tile is
+// symmetrised); the rest: row tile T (JS rows) of V and pair R,
+// V_TR <- V_TR Q_R.  Identity factors (qflag 0) are skipped.
+__global__ __launch_bounds__(AT) void k_jac_apply(double *__restrict__ Ap, double *__restrict__ V, int M, int nb,
+                                                  int round, const double *__restrict__ Qall,
+                                                  const int *__restrict__ qflag, int ntile_a, int v_rows)
 {
-    __shared__ double Qs[JS][JS + 1];
+    static_assert(AT == 256 && JS == 32, "2 x 2 outputs per thread cover a 32 x 32 tile");
     __shared__ double X[JS][JS + 1];
-    int bp, bq;
-    round_pair(nb, round, blockIdx.x, &bp, &bq);
-    const double *Q = Qall + (long)blockIdx.x * JS * JS;
-    const int r0 = blockIdx.y * JS;
+    __shared__ double Y[JS][JS + 1];
+    __shared__ double QL[JS][JS + 1];
+    __shared__ double QR[JS][JS + 1];
+    const int np = nb / 2;
     const int t = threadIdx.x;
-    for (int e = t; e < JS * JS; e += JT) {
-        Qs[e % JS][e / JS] = Q[e];
-    }
-    for (int which = 0; which < 2; ++which) {
-        double *Mx = which == 0 ? Ap : V;
-        __syncthreads();
-        for (int e = t; e < JS * JS; e += JT) {
+    const int ti = t & 15, tj = t >> 4;
+    int id = blockIdx.x;
+    if (id < ntile_a) {
+        int P, R;
+        tri_index(id, &P, &R);
+        const int fP = qflag[P], fR = qflag[R];
+        if (!fP && !fR) {
+            return;
+        }
+        int pp, pq, rp_, rq_;
+        round_pair(nb, round, P, &pp, &pq);
+        round_pair(nb, round, R, &rp_, &rq_);
+        const double *Qp = Qall + (long)P * JS * JS;
+        const double *Qr = Qall + (long)R * JS * JS;
+        for (int e = t; e < JS * JS; e += AT) {
             const int i = e % JS, j = e / JS;
-            const int gi = r0 + i;
-            const int gj = j < JB ? JB * bp + j : JB * bq + j - JB;
-            X[i][j] = gi < M ? Mx[gi + (long)gj * M] : 0.0;
+            X[i][j] = Ap[pair_row(i, pp, pq) + (long)pair_row(j, rp_, rq_) * M];
+            if (fP) {
+                QL[i][j] = Qp[e];
+            }
+            if (fR) {
+                QR[i][j] = Qr[e];
+            }
         }
         __syncthreads();
-        for (int e = t; e < JS * JS; e += JT) {
-            const int i = e % JS, j = e / JS;
-            const int gi = r0 + i;
-            double acc = 0.0;
+        double acc[2][2];
+        if (fP) {
+            tile_mul<true>(QL, X, acc);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    Y[ti + 16 * a][tj + 16 * b] = acc[a][b];
+                }
+            }
+        } else {
+            for (int e = t; e < JS * JS; e += AT) {
+                Y[e % JS][e / JS] = X[e % JS][e / JS];
+            }
+        }
+        __syncthreads();
+        // X <- Y Q_R: (Q_R^T Y^T)^T, with Y read transposed
+        if (fR) {
+            const int ri = t & 15, rj = t >> 4;
+            double o[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll 8
             for (int l = 0; l < JS; ++l) {
-                acc = fma(X[i][l], Qs[l][j], acc);
+                const double a0 = Y[ri][l], a1 = Y[ri + 16][l];
+                const double b0 = QR[l][rj], b1 = QR[l][rj + 16];
+                o[0][0] = fma(a0, b0, o[0][0]);
+                o[0][1] = fma(a0, b1, o[0][1]);
+                o[1][0] = fma(a1, b0, o[1][0]);
+                o[1][1] = fma(a1, b1, o[1][1]);
             }
-            const int gj = j < JB ? JB * bp + j : JB * bq + j - JB;
-            if (gi < M) {
-                Mx[gi + (long)gj * M] = acc;
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    X[ri + 16 * a][rj + 16 * b] = o[a][b];
+                }
+            }
+        } else {
+            for (int e = t; e < JS * JS; e += AT) {
+                X[e % JS][e / JS] = Y[e % JS][e / JS];
+            }
+        }
+        __syncthreads();
+        for (int e = t; e < JS * JS; e += AT) {
+            const int i = e % JS, j = e / JS;
+            double v = X[i][j];
+            if (P == R) {
+                v = 0.5 * (v + X[j][i]);
+            }
+            Ap[pair_row(i, pp, pq) + (long)pair_row(j, rp_, rq_) * M] = v;
+            if (P != R) {
+                Ap[pair_row(i, rp_, rq_) + (long)pair_row(j, pp, pq) * M] = X[j][i];
+            }
+        }
+        return;
+    }
+    id -= ntile_a;
+    const int R = id % np, T = id / np;
+    if (!qflag[R]) {
+        return;
+    }
+    int rp_, rq_;
+    round_pair(nb, round, R, &rp_, &rq_);
+    const double *Qr = Qall + (long)R * JS * JS;
+    const int r0 = T * JS;
+    for (int e = t; e < JS * JS; e += AT) {
+        const int i = e % JS, j = e / JS;
+        const int gi = r0 + i;
+        X[i][j] = gi < v_rows ? V[gi + (long)pair_row(j, rp_, rq_) * M] : 0.0;
+        QR[i][j] = Qr[e];
+    }
+    __syncthreads();
+    double o[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll 8
+    for (int l = 0; l < JS; ++l) {
+        const double a0 = X[ti][l], a1 = X[ti + 16][l];
+        const double b0 = QR[l][tj], b1 = QR[l][tj + 16];
+        o[0][0] = fma(a0, b0, o[0][0]);
+        o[0][1] = fma(a0, b1, o[0][1]);
+        o[1][0] = fma(a1, b0, o[1][0]);
+        o[1][1] = fma(a1, b1, o[1][1]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int gi = r0 + ti + 16 * a;
+            if (gi < v_rows) {
+                V[gi + (long)pair_row(tj + 16 * b, rp_, rq_) * M] = o[a][b];
             }
         }
     }
 }
 
-// out[0] = sum of squared off-diagonal entries, out[1] = sum of squares
-// (fixed-order two-level reduction: per-block partials, then one block)
+// part[2b] = sum of squared off-diagonal entries, part[2b + 1] = sum of
+// squares, over the entries of grid-stride block b
 __global__ __launch_bounds__(256) void k_jac_norms(const double *__restrict__ Ap, int M, double *__restrict__ part)
 {
     double off = 0.0, all = 0.0;
@@ -319,14 +444,20 @@ __global__ __launch_bounds__(256) void k_jac_norms(const double *__restrict__ Ap
     }
 }
 
-__global__ void k_jac_norms_final(const double *__restrict__ part, int nparts, double *__restrict__ out)
+// out = sums of the NPARTS partial pairs (fixed order: one wavefront, a
+// strided sum per lane, then a butterfly)
+__global__ __launch_bounds__(64) void k_jac_norms_final(const double *__restrict__ part, double *__restrict__ out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double off = 0.0, all = 0.0;
-        for (int b = 0; b < nparts; ++b) {
-            off += part[2 * b];
-            all += part[2 * b + 1];
-        }
+    double off = 0.0, all = 0.0;
+    for (int b = threadIdx.x; b < NPARTS; b += 64) {
+        off += part[2 * b];
+        all += part[2 * b + 1];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        off += __shfl_xor(off, o, 64);
+        all += __shfl_xor(all, o, 64);
+    }
+    if (threadIdx.x == 0) {
         out[0] = off;
         out[1] = all;
     }
@@ -373,24 +504,29 @@ int sym_eig_device(Context *ctx, const double *A, long lda, int m, double *w, do
     }
     const int M = nb * JB;
     const int npairs = nb / 2;
-    const int nparts = 256;
+    const int ntile_a = npairs * (npairs + 1) / 2;
+    // row tiles of V that hold real rows (padding rows of V stay zero in
+    // the real columns and never mix with them)
+    const int vtiles = (m + JS - 1) / JS;
+    const int inner = std::max(1, env_int("IPT_EIG_INNER_SWEEPS", 1));
     DevBuf<double> Ap, Vp, Q, part, red, dg;
-    DevBuf<int> dperm;
+    DevBuf<int> dperm, qflag;
     IPT_STCHK(Ap.alloc((size_t)M * M));
     IPT_STCHK(Vp.alloc((size_t)M * M));
     IPT_STCHK(Q.alloc((size_t)npairs * JS * JS));
-    IPT_STCHK(part.alloc(2 * (size_t)nparts));
+    IPT_STCHK(qflag.alloc((size_t)npairs));
+    IPT_STCHK(part.alloc(2 * (size_t)NPARTS));
     IPT_STCHK(red.alloc(2));
     IPT_STCHK(dg.alloc((size_t)M));
     IPT_STCHK(dperm.alloc((size_t)m));
+    const int init_blocks = std::min(4096, (M * M + 255) / 256);
     // padding above the spectrum: |lambda| <= ||A||_F; the norms pass gives it
-    hipLaunchKernelGGL(k_jac_init, dim3(std::min(4096, (M * M + 255) / 256)), dim3(256), 0, ctx->stream, A, lda, m,
-                       Ap.p, Vp.p, M, 0.0);
+    hipLaunchKernelGGL(k_jac_init, dim3(init_blocks), dim3(256), 0, ctx->stream, A, lda, m, Ap.p, Vp.p, M, 0.0);
     IPT_HIPCHK(hipGetLastError());
     double h[2] = {0.0, 0.0};
     auto norms = [&]() -> int {
-        hipLaunchKernelGGL(k_jac_norms, dim3(nparts), dim3(256), 0, ctx->stream, Ap.p, M, part.p);
-        hipLaunchKernelGGL(k_jac_norms_final, dim3(1), dim3(64), 0, ctx->stream, part.p, nparts, red.p);
+        hipLaunchKernelGGL(k_jac_norms, dim3(NPARTS), dim3(256), 0, ctx->stream, Ap.p, M, part.p);
+        hipLaunchKernelGGL(k_jac_norms_final, dim3(1), dim3(64), 0, ctx->stream, part.p, red.p);
         IPT_HIPCHK(hipGetLastError());
         IPT_HIPCHK(hipMemcpyAsync(h, red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -402,8 +538,7 @@ int sym_eig_device(Context *ctx, const double *A, long lda, int m, double *w, do
     }
     if (M > m) {
         const double big = 2.0 * std::sqrt(h[1]) + 1.0;
-        hipLaunchKernelGGL(k_jac_init, dim3(std::min(4096, (M * M + 255) / 256)), dim3(256), 0, ctx->stream, A, lda,
-                           m, Ap.p, Vp.p, M, big);
+        hipLaunchKernelGGL(k_jac_init, dim3(init_blocks), dim3(256), 0, ctx->stream, A, lda, m, Ap.p, Vp.p, M, big);
         IPT_HIPCHK(hipGetLastError());
         IPT_STCHK(norms());
     }
@@ -411,19 +546,23 @@ int sym_eig_device(Context *ctx, const double *A, long lda, int m, double *w, do
     // matrix's, or below 1e-12 of it and no longer halving (the rounding
     // floor of an order-M matrix is about eps M of its norm)
     const double total = h[1];
+    const double abs_thr2 = 1e-30 * total;
     int sweep = 0;
-    bool converged = !(h[0] > 1e-30 * total);
+    bool converged = !(h[0] > abs_thr2);
     double prev = h[0];
-    const dim3 tiles(npairs, M / JS);
+    // one pair: its subproblem is the whole matrix, solved in one round
+    const int sweeps_per_round = nb == 2 ? 30 : inner;
+    const int apply_blocks = ntile_a + vtiles * npairs;
     for (sweep = 0; sweep < 40 && !converged; ++sweep) {
         for (int r = 0; r < nb - 1; ++r) {
-            hipLaunchKernelGGL(k_jac_pair, dim3(npairs), dim3(JT), 0, ctx->stream, Ap.p, M, nb, r, Q.p, 12);
-            hipLaunchKernelGGL(k_jac_rows, tiles, dim3(JT), 0, ctx->stream, Ap.p, M, nb, r, Q.p);
-            hipLaunchKernelGGL(k_jac_cols, tiles, dim3(JT), 0, ctx->stream, Ap.p, Vp.p, M, nb, r, Q.p);
+            hipLaunchKernelGGL(k_jac_pair, dim3(npairs), dim3(PT), 0, ctx->stream, Ap.p, M, nb, r, Q.p, qflag.p,
+                               sweeps_per_round, abs_thr2);
+            hipLaunchKernelGGL(k_jac_apply, dim3(apply_blocks), dim3(AT), 0, ctx->stream, Ap.p, Vp.p, M, nb, r, Q.p,
+                               qflag.p, ntile_a, m);
             IPT_HIPCHK(hipGetLastError());
         }
         IPT_STCHK(norms());
-        converged = !(h[0] > 1e-30 * total) || (h[0] <= 1e-24 * total && h[0] >= 0.25 * prev);
+        converged = !(h[0] > abs_thr2) || (h[0] <= 1e-24 * total && h[0] >= 0.25 * prev);
         prev = h[0];
     }
     if (sweeps_out) {
