@@ -84,6 +84,46 @@ __device__ inline void tri_index(int b, int *i, int *j)
     *i = b - c * (c + 1) / 2;
 }
 
+// 1/x and 1/sqrt(x) from the hardware estimates (relative error near 2^-24)
+// and two Newton steps each (2^-48, then rounding level)
+__device__ inline double rcp_refined(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+__device__ inline double rsq_refined(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-(x * y), y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// The rotation J = [[c, s], [-s, c]] with J^T [[a, x], [x, b]] J diagonal,
+// |angle| <= pi/4 (Golub & Van Loan, Algorithm 8.5.1): t = tan(angle) =
+// sign(d) 2x / (|d| + sqrt(d^2 + 4x^2)), d = b - a, evaluated on operands
+// scaled by a power of two (exact) so that the squares neither overflow nor
+// underflow; c = (1 + t^2)^-1/2, s = t c.  Requires x != 0.
+__device__ inline void jacobi_rotation(double a, double b, double x, double *cs, double *sn, double *tn)
+{
+    const double d = b - a;
+    int ex;
+    (void)frexp(fmax(fabs(d), 2.0 * fabs(x)), &ex);
+    const double dn = ldexp(d, -ex), xn = ldexp(x, -ex);
+    const double h2 = fma(dn, dn, 4.0 * xn * xn);  // in [1/4, 2)
+    const double h = h2 * rsq_refined(h2);
+    const double t = (d >= 0.0 ? 2.0 : -2.0) * xn * rcp_refined(fabs(dn) + h);
+    const double c = rsq_refined(fma(t, t, 1.0));
+    *cs = c;
+    *sn = t * c;
+    *tn = t;
+}
+
 __global__ void k_jac_init(const double *__restrict__ A, long lda, int m, double *__restrict__ Ap, double *__restrict__ V,
                            int M, double big)
 {
@@ -165,11 +205,9 @@ __global__ __launch_bounds__(PT) void k_jac_pair(const double *__restrict__ Ap, 
                 round_pair(JS, r, t, &p, &q);
                 const double a = S[p][p], b = S[q][q], c = S[p][q];
                 double cs = 1.0, sn = 0.0, tn = 0.0;
-                if (c != 0.0 && fabs(c) > 1e-17 * sqrt(fabs(a) * fabs(b))) {
-                    const double tau = (b - a) / (2.0 * c);
-                    tn = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    cs = 1.0 / sqrt(1.0 + tn * tn);
-                    sn = tn * cs;
+                // rotate unless |c| <= 1e-17 sqrt(|a b|)
+                if (c * c > (1e-17 * fabs(a)) * (1e-17 * fabs(b))) {
+                    jacobi_rotation(a, b, c, &cs, &sn, &tn);
                     sweep_rot = 1;
                 }
                 rc[t] = cs;
