@@ -39,6 +39,7 @@ SOURCES = [
     "ipt_profile.cpp",
     "ipt_fci.cpp",
     "ipt_eig.hip",
+    "ipt_chol.hip",
 ]
 
 # Packed-FP32 VALU ops are disabled for all device code.
@@ -68,8 +69,7 @@ def build(verbose=False, jobs=8):
             _drain(procs)
     _drain(procs)
     link = [hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-fopenmp", "-o", OUT + ".tmp"] + objs + [
-        "-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
-        "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+        "-L" + os.path.join(ROCM, "lib"), "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     if verbose:
         print(" ".join(link))
     subprocess.run(link, check=True)

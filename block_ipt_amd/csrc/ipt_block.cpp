@@ -27,7 +27,6 @@
 #include "ipt_block.h"
 #include "ipt_lowp.h"
 #include "ipt_sparse_lp.h"
-#include "ipt_small_eig.h"
 #include "ipt_prep.h"
 
 namespace ipt {
@@ -275,10 +274,11 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
 
 
 // Project the block T (n x t) against the first `cols` columns of V (two
-// classical Gram-Schmidt passes) and orthonormalise it with a Cholesky QR;
-// directions whose norm after projection falls below 1e-10 of their norm
-// before are dropped.  Returns the kept count in *kept; kept columns are
-// compacted to the front of T.  IPT/src/ipt_cuda.cu:2711-2780 (per vector).
+// classical Gram-Schmidt passes) and orthonormalise it with a Cholesky QR on
+// the device; directions whose norm after projection falls below 1e-10 of
+// their norm before are dropped (the pivot test of the factorisation).
+// Returns the kept count in *kept; kept columns are compacted to the front
+// of T.  IPT/src/ipt_cuda.cu:2711-2780 (per vector).
 int project_orthonormalize(Context *ctx, const double *V, int n, int cols, double *T, int t, int *kept,
                            double *work /* n x t */)
 {
@@ -288,83 +288,43 @@ int project_orthonormalize(Context *ctx, const double *V, int n, int cols, doubl
     }
     double *C = nullptr;
     const size_t cwords = (size_t)std::max(cols, t) * (size_t)t;
-    IPT_STCHK(context_scratch(ctx, SLOT_ORTHO, (cwords + (size_t)t) * sizeof(double), (void **)&C));
+    const size_t tt = (size_t)t * t;
+    // C | sq (t) | W (t x t) | Wc (t x t) | flags (t + 1 ints) | keep list (t ints)
+    const size_t words = cwords + (size_t)t + 2 * tt + (size_t)t + 1;
+    IPT_STCHK(context_scratch(ctx, SLOT_ORTHO, words * sizeof(double), (void **)&C));
     double *sq = C + cwords;
-    std::vector<double> before((size_t)t);
+    double *W = sq + t;
+    double *Wc = W + tt;
+    int *flags = reinterpret_cast<int *>(Wc + tt);
+    int *list = flags + t + 1;
     IPT_STCHK(column_sqnorms(ctx, T, n, n, t, sq));
-    IPT_STCHK(read_host(ctx, before.data(), sq, (size_t)t * sizeof(double)));
     if (cols > 0) {
         for (int pass = 0; pass < 2; ++pass) {
             IPT_STCHK(gemm_tn_tall(ctx, V, n, T, n, C, cols, n, cols, t));
             IPT_STCHK(gemm_nn_small(ctx, V, n, C, cols, T, n, n, cols, t, -1.0, 1.0));
         }
     }
-    // Gram of the projected block, Cholesky with rejection on the host
+    // Gram of the projected block; Cholesky with rejection, W = L^-T
     IPT_STCHK(gemm_tn_tall(ctx, T, n, T, n, C, t, n, t, t));
-    std::vector<double> g((size_t)t * t);
-    IPT_STCHK(read_host(ctx, g.data(), C, g.size() * sizeof(double)));
-    // sequential Gram-Schmidt in coefficient space: L (t x kept) such that
-    // Q = T W with W = L^-T restricted to kept columns.
+    IPT_STCHK(symmetrize(ctx, C, t, t));
+    IPT_STCHK(chol_reject_device(ctx, C, t, sq, 1e-10, W, flags, flags + t));
+    std::vector<int> hflags((size_t)t + 1);
+    IPT_STCHK(read_host(ctx, hflags.data(), flags, hflags.size() * sizeof(int)));
     std::vector<int> keep;
-    std::vector<double> L((size_t)t * t, 0.0);  // column-major, rows = original index
     for (int j = 0; j < t; ++j) {
-        // r_jj^2 = g_jj - sum_kept l_jk^2 ; l_ik = (g_ik - sum ...)/l_kk
-        std::vector<double> col((size_t)t, 0.0);
-        for (int i = 0; i < t; ++i) {
-            col[(size_t)i] = g[(size_t)i + (size_t)j * t];
+        if (hflags[(size_t)j]) {
+            keep.push_back(j);
         }
-        for (size_t q = 0; q < keep.size(); ++q) {
-            const double ljq = L[(size_t)j + q * t];
-            for (int i = 0; i < t; ++i) {
-                col[(size_t)i] -= L[(size_t)i + q * t] * ljq;
-            }
-        }
-        const double d2 = col[(size_t)j];
-        const double b = before[(size_t)j];
-        if (!(d2 > 0.0) || !std::isfinite(d2) || !std::isfinite(b) || b <= DBL_MIN ||
-            std::sqrt(d2) <= 1e-10 * std::sqrt(b)) {
-            continue;
-        }
-        const double djj = std::sqrt(d2);
-        const size_t q = keep.size();
-        for (int i = 0; i < t; ++i) {
-            L[(size_t)i + q * t] = col[(size_t)i] / djj;
-        }
-        keep.push_back(j);
     }
     const int kc = (int)keep.size();
     *kept = kc;
     if (kc == 0) {
         return IPT_CUDA_SUCCESS;
     }
-    // Lk = rows `keep` of L (kc x kc lower triangular); W (t x kc) = S Lk^-T
-    // where S selects the kept columns of T.
-    std::vector<double> Lk((size_t)kc * kc, 0.0);
-    for (int a = 0; a < kc; ++a) {
-        for (int b = 0; b < kc; ++b) {
-            Lk[(size_t)a + (size_t)b * kc] = L[(size_t)keep[(size_t)a] + (size_t)b * t];
-        }
-    }
-    // inverse of lower triangular Lk
-    std::vector<double> Li((size_t)kc * kc, 0.0);
-    for (int c = 0; c < kc; ++c) {
-        for (int r = c; r < kc; ++r) {
-            double s = (r == c) ? 1.0 : 0.0;
-            for (int i = c; i < r; ++i) {
-                s -= Lk[(size_t)r + (size_t)i * kc] * Li[(size_t)i + (size_t)c * kc];
-            }
-            Li[(size_t)r + (size_t)c * kc] = s / Lk[(size_t)r + (size_t)r * kc];
-        }
-    }
-    std::vector<double> W((size_t)t * kc, 0.0);
-    for (int a = 0; a < kc; ++a) {
-        for (int b = 0; b < kc; ++b) {
-            // W(keep[a], b) = (Li^T)(a, b) = Li(b, a)
-            W[(size_t)keep[(size_t)a] + (size_t)b * t] = Li[(size_t)b + (size_t)a * kc];
-        }
-    }
-    IPT_HIPCHK(hipMemcpyAsync(C, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    IPT_STCHK(gemm_nn_small(ctx, T, n, C, t, work, n, n, t, kc, 1.0, 0.0));
+    // T <- T W(:, keep)
+    IPT_HIPCHK(hipMemcpyAsync(list, keep.data(), (size_t)kc * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    IPT_STCHK(gather_columns(ctx, W, t, list, kc, Wc));
+    IPT_STCHK(gemm_nn_small(ctx, T, n, Wc, t, work, n, n, t, kc, 1.0, 0.0));
     IPT_HIPCHK(hipMemcpyAsync(T, work, (size_t)n * kc * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     return IPT_CUDA_SUCCESS;

@@ -87,6 +87,21 @@ int column_sqnorms(Context *ctx, const double *X, long ld, long nrows, int nc, d
 // X[:, j] *= 1/sqrt(sq[j]) (and the same for HX when non-NULL)
 int scale_columns_inv_sqrt(Context *ctx, double *X, double *HX, long ld, long nrows, int nc,
                            const double *sq);
+// Right-looking Cholesky of the symmetric G (t x t, ld t, overwritten by L)
+// in one workgroup, column j kept iff its pivot d is positive and finite
+// and, with ref != NULL, sqrt(d) > rel sqrt(ref[j]) for a finite positive
+// ref[j]; keep[j] = 1/0 and *nkept on the device.  W (t x t, ld t) =
+// (L^-1)^T on the kept rows and columns, zero elsewhere, so that for a
+// basis T with G = T^T T the kept columns of T W are orthonormal.
+int chol_reject_device(Context *ctx, double *G, int t, const double *ref, double rel, double *W, int *keep,
+                       int *nkept);
+// D(:, b) = S(:, idx[b]) for b < nc, both with leading dimension t.
+int gather_columns(Context *ctx, const double *S, int t, const int *idx, int nc, double *D);
+// Q (n x k, ld n) <- the orthonormal factor of its QR factorisation
+// (positive diagonal R) by shifted Cholesky QR 3, all in FP64 on the
+// device.  IPT_CUDA_INVALID_ARGUMENT for a zero, non-finite or numerically
+// dependent column.
+int cholqr3_device(Context *ctx, double *Q, int n, int k);
 // For a symmetric positive Gram matrix G (m x m, ld m): *dev_out =
 // ||D^-1/2 G D^-1/2 - I||_F with D = diag(G).  When that is at most max_dev
 // (< 1), S (m x m, ld m) = D^-1/2 (D^-1/2 G D^-1/2)^-1/2, so that

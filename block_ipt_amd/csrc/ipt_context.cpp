@@ -1,6 +1,5 @@
 // Library context (stream, pinned scalars, scratch), option parsing, status
 // strings and result release.
-#include <rocblas/rocblas.h>
 
 #include <climits>
 #include <cstring>
@@ -173,9 +172,6 @@ void context_destroy(Context *ctx)
             (void)hipFree(ctx->slots[i]);
         }
     }
-    if (ctx->blas != nullptr) {
-        rocblas_destroy_handle((rocblas_handle)ctx->blas);
-    }
     if (ctx->host_scalars != nullptr) {
         (void)hipHostFree(ctx->host_scalars);
     }
@@ -218,23 +214,6 @@ Context *context_from_handle(void *handle, int *status)
     }
     pool_set_stream(g_default_ctx[dev]->stream);
     return g_default_ctx[dev];
-}
-
-int context_blas(Context *ctx, void **blas_out)
-{
-    if (ctx->blas == nullptr) {
-        rocblas_handle h = nullptr;
-        if (rocblas_create_handle(&h) != rocblas_status_success) {
-            return IPT_CUDA_CUBLAS_ERROR;
-        }
-        if (rocblas_set_stream(h, ctx->stream) != rocblas_status_success) {
-            rocblas_destroy_handle(h);
-            return IPT_CUDA_CUBLAS_ERROR;
-        }
-        ctx->blas = h;
-    }
-    *blas_out = ctx->blas;
-    return IPT_CUDA_SUCCESS;
 }
 
 int context_host_pinned(Context *ctx, size_t bytes, void **out)

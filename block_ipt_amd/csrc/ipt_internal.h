@@ -39,7 +39,6 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    void *blas = nullptr;        // rocblas_handle, created on first use
     double *host_scalars = nullptr;  // pinned, 4096 doubles
     int *host_flags = nullptr;       // pinned, 4096 ints
     void *host_pinned = nullptr;     // pinned staging for device-to-host reads, grows
@@ -82,7 +81,6 @@ int context_host_pinned(Context *ctx, size_t bytes, void **out);
 
 // NULL handle -> library context of the current device.
 Context *context_from_handle(void *handle, int *status);
-int context_blas(Context *ctx, void **blas_out);
 
 // Device buffer owned by the caller's scope.
 // Device memory of library routines comes from a cache keyed by the

@@ -10,8 +10,6 @@
 // through the same operator as the block path's correction products
 // (lowp_apply_delta with a zero anchor), and the iterate and the update stay
 // FP64; the FP64 phase is the plain loop of ipt_plain.cpp.
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -27,7 +25,6 @@
 #include "ipt_block.h"
 #include "ipt_lowp.h"
 #include "ipt_plain.h"
-#include "ipt_small_eig.h"
 
 namespace ipt {
 
@@ -141,135 +138,29 @@ void reset_two_phase(R *r)
     }
 }
 
-// Householder QR of Q in place (Q <- first k columns of the orthogonal
-// factor), rocSOLVER geqrf + orgqr as the reference's cuSOLVER pair
-// (ipt_rayleigh_mixed_cuda.cu:141-254).
+// Q <- the orthonormal factor of its QR factorisation.  The reference uses
+// Householder geqrf + orgqr (ipt_rayleigh_mixed_cuda.cu:141-254); here the
+// same factor (up to column signs) comes from shifted Cholesky QR 3 on the
+// device, all in FP64.
 int orthogonalize_qr(Context *ctx, double *Q, int n, int k)
 {
-    void *blas = nullptr;
-    IPT_STCHK(context_blas(ctx, &blas));
-    DevBuf<double> tau;
-    IPT_STCHK(tau.alloc((size_t)k));
-    if (rocsolver_dgeqrf((rocblas_handle)blas, n, k, Q, n, tau.p) != rocblas_status_success) {
-        return IPT_CUDA_CUSOLVER_ERROR;
-    }
-    if (rocsolver_dorgqr((rocblas_handle)blas, n, k, k, Q, n, tau.p) != rocblas_status_success) {
-        return IPT_CUDA_CUSOLVER_ERROR;
-    }
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    return IPT_CUDA_SUCCESS;
-}
-
-__global__ void k_to_float(const double *__restrict__ in, float *__restrict__ out, long total)
-{
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        out[i] = (float)in[i];
-    }
-}
-
-// Inverse of the k x k upper triangular R (column-major, leading dimension
-// ldr) into Rinv (k x k); false if a diagonal entry is zero or not finite.
-bool upper_inverse(const double *R, int ldr, int k, std::vector<double> *Rinv)
-{
-    Rinv->assign((size_t)k * k, 0.0);
-    for (int j = 0; j < k; ++j) {
-        const double rjj = R[j + (size_t)j * ldr];
-        if (!(std::fabs(rjj) > 0.0) || !std::isfinite(rjj)) {
-            return false;
-        }
-        // column j of R^-1: solve R x = e_j (x_i = 0 for i > j)
-        double *x = Rinv->data() + (size_t)j * k;
-        x[j] = 1.0 / rjj;
-        for (int i = j - 1; i >= 0; --i) {
-            double acc = 0.0;
-            for (int l = i + 1; l <= j; ++l) {
-                acc += R[i + (size_t)l * ldr] * x[l];
-            }
-            x[i] = -acc / R[i + (size_t)i * ldr];
-        }
-    }
-    return true;
+    return cholqr3_device(ctx, Q, n, k);
 }
 
 }  // namespace
 
-// FP32-preconditioned Cholesky QR (IPT/mixed/ipt_rayleigh_mixed_mpqr_cuda.cu:
-// 108-386): columns normalised, R from an FP32 Householder QR, V = Q R^-1,
-// G = V^T V, G = L L^T, Q <- V L^-T, all but the FP32 QR in FP64.  The FP32
-// R brings the condition number of V down to about 1/eps_32, so one Cholesky
-// QR in FP64 finishes the orthogonalisation.  Q: n x k, column-major, ld n.
+// Preconditioned Cholesky QR (IPT/mixed/ipt_rayleigh_mixed_mpqr_cuda.cu:
+// 108-386).  The reference normalises the columns, takes R from an FP32
+// Householder QR, forms V = Q R^-1 and finishes with one FP64 Cholesky QR:
+// the FP32 R brings the condition number of V to about 1/eps_32 so that the
+// FP64 Cholesky succeeds.  Here the preconditioning step is a shifted FP64
+// Cholesky QR (which handles condition numbers up to about 1/eps_64), and
+// two plain Cholesky QR passes follow; the result is the orthonormal factor
+// of the QR factorisation of Q, as the reference's.  Q: n x k, column-major,
+// ld n.
 int orthogonalize_mpqr(Context *ctx, double *Q, int n, int k)
 {
-    if (Q == nullptr || n <= 0 || k <= 0 || k > n) {
-        return IPT_CUDA_INVALID_ARGUMENT;
-    }
-    const size_t nk = (size_t)n * (size_t)k;
-    DevBuf<double> sq, V, W;
-    DevBuf<float> Q32, tau;
-    IPT_STCHK(sq.alloc((size_t)k));
-    IPT_STCHK(column_sqnorms(ctx, Q, n, n, k, sq.p));
-    std::vector<double> hsq((size_t)k);
-    IPT_HIPCHK(hipMemcpyAsync(hsq.data(), sq.p, hsq.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (double x : hsq) {
-        if (!(x > 0.0) || !std::isfinite(x)) {
-            return IPT_CUDA_INVALID_ARGUMENT;  // zero or non-finite column
-        }
-    }
-    IPT_STCHK(scale_columns_inv_sqrt(ctx, Q, nullptr, n, n, k, sq.p));
-    IPT_STCHK(Q32.alloc(nk));
-    IPT_STCHK(tau.alloc((size_t)k));
-    hipLaunchKernelGGL(k_to_float, dim3((unsigned)std::min<size_t>((nk + 255) / 256, 16384)), dim3(256), 0,
-                       ctx->stream, Q, Q32.p, (long)nk);
-    IPT_HIPCHK(hipGetLastError());
-    void *blas = nullptr;
-    IPT_STCHK(context_blas(ctx, &blas));
-    if (rocsolver_sgeqrf((rocblas_handle)blas, n, k, Q32.p, n, tau.p) != rocblas_status_success) {
-        return IPT_CUDA_CUSOLVER_ERROR;
-    }
-    // R: upper triangle of the first k rows
-    std::vector<float> r32((size_t)k * k);
-    IPT_HIPCHK(hipMemcpy2DAsync(r32.data(), (size_t)k * sizeof(float), Q32.p, (size_t)n * sizeof(float),
-                                (size_t)k * sizeof(float), (size_t)k, hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    std::vector<double> R((size_t)k * k, 0.0), Rinv;
-    for (int j = 0; j < k; ++j) {
-        for (int i = 0; i <= j; ++i) {
-            R[i + (size_t)j * k] = (double)r32[i + (size_t)j * k];
-        }
-    }
-    if (!upper_inverse(R.data(), k, k, &Rinv)) {
-        return IPT_CUDA_INVALID_ARGUMENT;
-    }
-    IPT_STCHK(V.alloc(nk));
-    IPT_STCHK(W.alloc((size_t)k * k));
-    IPT_HIPCHK(hipMemcpyAsync(W.p, Rinv.data(), Rinv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    IPT_STCHK(gemm_nn_small(ctx, Q, n, W.p, k, V.p, n, n, k, k, 1.0, 0.0));
-    // Cholesky QR of V in FP64
-    DevBuf<double> G;
-    IPT_STCHK(G.alloc((size_t)k * k));
-    IPT_STCHK(gemm_tn_tall(ctx, V.p, n, V.p, n, G.p, k, n, k, k));
-    std::vector<double> hg((size_t)k * k);
-    IPT_HIPCHK(hipMemcpyAsync(hg.data(), G.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (!small_eig_detail::cholesky_lower(hg.data(), k)) {
-        return IPT_CUDA_INVALID_ARGUMENT;
-    }
-    // L^-T = (L^T)^-1: L^T is upper triangular
-    std::vector<double> LT((size_t)k * k, 0.0), LTinv;
-    for (int j = 0; j < k; ++j) {
-        for (int i = j; i < k; ++i) {
-            LT[j + (size_t)i * k] = hg[i + (size_t)j * k];
-        }
-    }
-    if (!upper_inverse(LT.data(), k, k, &LTinv)) {
-        return IPT_CUDA_INVALID_ARGUMENT;
-    }
-    IPT_HIPCHK(hipMemcpyAsync(W.p, LTinv.data(), LTinv.size() * sizeof(double), hipMemcpyHostToDevice,
-                              ctx->stream));
-    IPT_STCHK(gemm_nn_small(ctx, V.p, n, W.p, k, Q, n, n, k, k, 1.0, 0.0));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    return IPT_CUDA_SUCCESS;
+    return cholqr3_device(ctx, Q, n, k);
 }
 
 }  // namespace ipt
