@@ -253,20 +253,25 @@ def test_adaptive_limit(block_env):
 def test_davidson_returns_best_state(block_env):
     """Davidson with target tolerance 0 (corrections for every pair, also
     the converged ones, whose noise directions crowd the restarted search
-    space): the largest residual does not fall monotonically, and steps 29
-    and 30 raise it above that of step 28.  The returned pairs are those of
-    the best state seen (IPT/src/ipt_cuda.cu:485-701, 4941-4983): bit for bit
-    the pairs of the same solve stopped at that step, with the recorded
-    largest residual."""
+    space): the largest residual does not fall monotonically, so among step
+    limits 16..30 there are runs whose last accepted step raised it above
+    an earlier one.  For the first such limit, the returned pairs are those
+    of the best state seen (IPT/src/ipt_cuda.cu:485-701, 4941-4983): bit
+    for bit the pairs of the same solve stopped at that step, with the
+    recorded largest residual."""
     block_env.setenv("IPT_BLOCK_CLUSTER_MAXITER", "2")
     block_env.setenv("IPT_DAVIDSON_ENRICH", "1")
-    block_env.setenv("IPT_DAVIDSON_STEPS", "30")
     block_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "0")
     n, k = 800, 16
     H = orc.rand_dense(n, 3, 0.0, 10.0, 0.05)
-    gv, gl, info = api.ipt_cuda_tol(H, k, 1e-12, 100)
+    for steps in range(30, 15, -1):
+        block_env.setenv("IPT_DAVIDSON_STEPS", str(steps))
+        gv, gl, info = api.ipt_cuda_tol(H, k, 1e-12, 100)
+        if info["final_returned_from_best_so_far"] == 1:
+            break
+    else:
+        pytest.fail("no step limit in 16..30 ended above the best state")
     assert info["best_so_far_enabled"] == 1 and info["best_so_far_update_count"] >= 2
-    assert info["final_returned_from_best_so_far"] == 1, info
     best = info["best_so_far_step"]
     assert 1 <= best < info["accepted_steps"]
     assert info["best_so_far_source"] == "davidson"
