@@ -5,9 +5,13 @@ Data: for each training seed, a matrix of the class (dense, default: the
 N=16384 class d ~ U[0,50] sorted with symmetric N(0,1) * 0.1/sqrt(N)
 off-diagonal; or, with --matrix sparse, the orbital-sum CSR model), built by
 the library's device generators, its exact lowest pairs from the library's
-block solver, and the states of a Davidson run from the identity start
-(the library's restart rule and residual scaling, re-expressed with torch
-on the device).  At every state and for every unconverged target j the
+block solver, and the Ritz states of the library's own solver: the same
+solve (one block IPT step, then Davidson) stopped after s = 0, 1, 2, ...
+Davidson steps, each returning its pairs, until the tolerance or the step
+cap (--states library, the default).  --states torch instead re-expresses
+the Davidson iteration with torch on the device (the library's restart rule
+and residual scaling) and records every step of one run.  At every state
+and for every unconverged target j the
 pairs are (features of component i, q_i = r_i / s_i) -> t_i, the component
 of the Jacobi-Davidson correction u*_j / (u*_j . x_j) - x_j after projection
 off the current Ritz vectors; the loss weights each target column by
@@ -309,6 +313,49 @@ def davidson_states(pb, k, net, conv_tol, max_steps):
     return states, steps, trials, float(res.max())
 
 
+def library_states(args, pb, model, conv_tol, max_steps):
+    """Ritz states of the library's solver: one block IPT step, then
+    Davidson stopped after s = 0, 1, ... steps (diagonal corrections, or the
+    guarded learned ones with `model`); the pairs each run returns (its best
+    state so far) are recorded with their residuals until every target
+    reaches conv_tol or s reaches max_steps.  Returns (states, steps of the
+    last run, final largest residual)."""
+    hip = api.hip()
+    n, k = pb.n, pb.k
+    scale = max(1.0, pb.normH)
+    env = {"IPT_BLOCK_CLUSTER": "1", "IPT_BLOCK_CLUSTER_MAXITER": str(args.ipt_steps), "IPT_DAVIDSON_ENRICH": "1",
+           "IPT_DAVIDSON_CONVERGED_TOL": repr(conv_tol)}
+    for kk in ("IPT_PRECISION", "IPT_DAVIDSON_PRECOND", "IPT_DAVIDSON_MLP_MODEL"):
+        os.environ.pop(kk, None)
+    os.environ.update(env)
+    if model is not None:
+        os.environ["IPT_DAVIDSON_PRECOND"] = "mlp"
+        os.environ["IPT_DAVIDSON_MLP_MODEL"] = model
+    states, last, steps = [], None, 0
+    try:
+        for s_lim in range(0, max_steps + 1):
+            os.environ["IPT_DAVIDSON_STEPS"] = str(s_lim)
+            dv, dl = pb.mat.solve(k, args.ipt_steps)
+            X = torch.tensor(hip.to_host(dv, (n, k)), device="cuda")
+            th = torch.tensor(hip.to_host(dl, (k,)), device="cuda")
+            api.free_device_result(dv, dl)
+            steps = api.last_davidson_stats()["accepted"]
+            R = pb.apply(X) - X * th
+            res = torch.linalg.norm(R, dim=0) / (scale * torch.clamp(torch.linalg.norm(X, dim=0), min=1.0))
+            last = float(res.max())
+            cand = torch.nonzero(res > conv_tol).flatten()
+            if cand.numel() == 0:
+                break
+            if steps < s_lim:
+                break  # the run stopped before its limit (no further progress)
+            states.append((R[:, cand].clone(), th[cand].clone(), X[:, cand].clone(), X.clone()))
+    finally:
+        for kk in ("IPT_DAVIDSON_PRECOND", "IPT_DAVIDSON_MLP_MODEL", "IPT_DAVIDSON_STEPS",
+                   "IPT_BLOCK_CLUSTER_MAXITER", "IPT_DAVIDSON_CONVERGED_TOL", "IPT_DAVIDSON_ENRICH"):
+            os.environ.pop(kk, None)
+    return states, steps, last
+
+
 def samples_from_states(pb, states, net_for_features, near=64, rate=32, seed=0):
     """Stratified (features, q, target, weight) samples of the states."""
     gen = torch.Generator(device="cuda")
@@ -400,6 +447,8 @@ def main(argv=None):
     ap.add_argument("--ipt-steps", type=int, default=1)
     ap.add_argument("--report", default=None, help="JSON file for the measurement")
     ap.add_argument("--measure-only", action="store_true")
+    ap.add_argument("--states", default="library", choices=["library", "torch"],
+                    help="training states: the library solver's (default) or a torch re-expression of Davidson")
     args = ap.parse_args(argv)
 
     def seeds(s):
@@ -422,8 +471,16 @@ def main(argv=None):
                 if not pb.ok:
                     log("  round %d seed %d: reference pairs not converged, seed skipped" % (rnd, sd))
                     continue
-                states, steps, trials, r = davidson_states(pb, args.k, net if rnd > 0 else None, args.conv_tol,
-                                                           args.max_steps)
+                if args.states == "library":
+                    model = None
+                    if rnd > 0:
+                        model = args.out + ".round%d" % rnd
+                        api.save_mlp_model(model, net.p.cpu().numpy())
+                    states, steps, r = library_states(args, pb, model, args.conv_tol, args.max_steps)
+                    trials = steps
+                else:
+                    states, steps, trials, r = davidson_states(pb, args.k, net if rnd > 0 else None, args.conv_tol,
+                                                               args.max_steps)
                 parts.append(samples_from_states(pb, states, net, seed=sd * 7 + rnd))
                 log("  round %d seed %d: %d states, %d steps, %d trials, final residual %.2e"
                     % (rnd, sd, len(states), steps, trials, r))

@@ -16,6 +16,7 @@ from block_ipt_amd import api  # noqa: E402
 
 P = api.MLP_NPARAM
 MODEL = os.path.join(os.path.dirname(api.__file__), "davidson_mlp.bin")
+MODEL_SPARSE = os.path.join(os.path.dirname(api.__file__), "davidson_mlp_sparse.bin")
 
 
 def _params(seed):
@@ -231,11 +232,13 @@ def test_trained_model_solves_to_tolerance(dav_env):
 
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="trained model file not present")
 def test_learned_preconditioner_needs_fewer_steps(dav_env):
-    """The N=16384, k=64 dense class (d ~ U[0,50], V scale 0.1/sqrt(N)), test
-    seeds outside the training seeds: one block IPT step, then Davidson to
-    the 1e-10 scaled residual.  The learned corrections (guarded) need at
-    least 30% fewer Davidson steps and fewer block products than the
-    diagonal ones, and every learned run reaches the tolerance."""
+    """The N=16384, k=64 dense class (d ~ U[0,50], V scale 0.1/sqrt(N)), the
+    ten held-out seeds 100-109 (training used 1-20): one block IPT step,
+    then Davidson to the 1e-10 scaled residual with at most 120 steps.
+    Every learned (guarded) run reaches the tolerance.  On the seeds where
+    the diagonal run also converges, the learned corrections need at least
+    30% fewer Davidson steps and fewer block products; where the diagonal
+    run stops at the cap, the learned run needs fewer steps than the cap."""
     n, k = 16384, 64
     hip = api.hip()
     dav_env.setenv("IPT_BLOCK_CLUSTER_MAXITER", "1")
@@ -243,9 +246,11 @@ def test_learned_preconditioner_needs_fewer_steps(dav_env):
     dav_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "1e-10")
     dH = hip.malloc(n * n * 8)
     tot = {"diag": [0, 0], "mlp": [0, 0]}
+    capped = []
     try:
-        for seed in (100, 101, 104):
+        for seed in range(100, 110):
             api.generate_dense(dH, n, seed, 0.0, 50.0, 0.1)
+            run = {}
             for mode in ("diag", "mlp"):
                 if mode == "mlp":
                     dav_env.setenv("IPT_DAVIDSON_PRECOND", "mlp")
@@ -254,12 +259,58 @@ def test_learned_preconditioner_needs_fewer_steps(dav_env):
                     dav_env.delenv("IPT_DAVIDSON_PRECOND", raising=False)
                 dv, dl, _, _ = api.ipt_cuda_device_tol(dH, n, k, 1e-12, 1)
                 api.free_device_result(dv, dl)
-                st = api.last_davidson_stats()
-                tot[mode][0] += st["accepted"]
-                tot[mode][1] += st["accepted"] + st["mlp_rejected"]
-                if mode == "mlp":
-                    assert st["residual_after"] <= 1e-10
+                run[mode] = api.last_davidson_stats()
+            assert run["mlp"]["residual_after"] <= 1e-10, (seed, run["mlp"])
+            if run["diag"]["residual_after"] > 1e-10:
+                capped.append(seed)
+                assert run["mlp"]["accepted"] < 120
+                continue
+            for mode in ("diag", "mlp"):
+                tot[mode][0] += run[mode]["accepted"]
+                tot[mode][1] += run[mode]["accepted"] + run[mode]["mlp_rejected"]
     finally:
         hip.free(dH)
+    assert len(capped) <= 6, capped
     assert tot["mlp"][0] <= 0.7 * tot["diag"][0], tot
+    assert tot["mlp"][1] < tot["diag"][1], tot
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL_SPARSE), reason="trained sparse model file not present")
+def test_learned_preconditioner_sparse_n1e6(dav_env):
+    """The sparse orbital-sum class at N=10^6, k=32 (40 orbital energies
+    U[-20,5], diagonal = sums of 8, 200 drawn N(0, 0.02^2) entries per row,
+    mirrored), held-out seeds 100-104: one block IPT step, then Davidson to
+    the 1e-10 scaled residual.  Every learned (guarded) run reaches the
+    tolerance, and the learned corrections need fewer Davidson steps and
+    fewer single-column products than the diagonal ones in total (the
+    measured reduction is recorded with the model's profile)."""
+    n, m, k = 1000000, 200, 32
+    hip = api.hip()
+    dav_env.setenv("IPT_BLOCK_CLUSTER_MAXITER", "1")
+    dav_env.setenv("IPT_DAVIDSON_STEPS", "120")
+    dav_env.setenv("IPT_DAVIDSON_CONVERGED_TOL", "1e-10")
+    cap = n * (2 * m + 1)
+    drp = hip.malloc((n + 1) * 4)
+    dci = hip.malloc(cap * 4)
+    dv_ = hip.malloc(cap * 8)
+    tot = {"diag": [0, 0], "mlp": [0, 0]}
+    try:
+        for seed in range(100, 105):
+            nnz = api.generate_orbital_csr(n, m, seed, drp, dci, dv_, cap)
+            for mode in ("diag", "mlp"):
+                if mode == "mlp":
+                    dav_env.setenv("IPT_DAVIDSON_PRECOND", "mlp")
+                    dav_env.setenv("IPT_DAVIDSON_MLP_MODEL", MODEL_SPARSE)
+                else:
+                    dav_env.delenv("IPT_DAVIDSON_PRECOND", raising=False)
+                dv, dl, _, _ = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv_, n, k, nnz, 1e-12, 1)
+                api.free_device_result(dv, dl)
+                st = api.last_davidson_stats()
+                assert st["residual_after"] <= 1e-10, (seed, mode, st)
+                tot[mode][0] += st["accepted"]
+                tot[mode][1] += st["matvecs"]
+    finally:
+        for p_ in (drp, dci, dv_):
+            hip.free(p_)
+    assert tot["mlp"][0] < tot["diag"][0], tot
     assert tot["mlp"][1] < tot["diag"][1], tot
