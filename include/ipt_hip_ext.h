@@ -56,12 +56,14 @@ int ipt_hip_generate_orbital_csr(int n, int m, unsigned long long seed, int norb
 int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n, const double *d_X,
                      int ldx, double *d_Y, int ldy, int k, void *handle);
 
-/* In place: Q (n x k, column-major, leading dimension n, k <= n) <- an
- * orthonormal basis of its column space by FP32-preconditioned Cholesky QR
- * (IPT/mixed/ipt_rayleigh_mixed_mpqr_cuda.cu:169-386: columns normalised,
- * R of an FP32 Householder QR, V = Q R^-1, V^T V = L L^T, Q = V L^-T, in
- * FP64 except the FP32 QR).  Status 1 for a zero column or a basis whose
- * Gram matrix is not numerically positive definite after preconditioning.
+/* In place: Q (n x k, column-major, leading dimension n, k <= n) <- the
+ * orthonormal factor of its QR factorisation (positive diagonal R), by
+ * shifted Cholesky QR 3 in FP64 on the device: columns normalised, one
+ * Cholesky QR pass on the shifted Gram, then two plain passes.  Replaces the
+ * reference's FP32-preconditioned Cholesky QR
+ * (IPT/mixed/ipt_rayleigh_mixed_mpqr_cuda.cu:169-386: FP32 Householder R,
+ * V = Q R^-1, one FP64 Cholesky QR), which gives the same factor.  Status 1
+ * for a zero or non-finite column or a numerically dependent basis.
  * ipt_rayleigh_mixed_tf32_cuda_device_tol uses it with IPT_RAYLEIGH_QR=mpqr. */
 int ipt_hip_cholqr_mixed(double *d_Q, int n, int k, void *handle);
 
@@ -103,8 +105,9 @@ int ipt_hip_last_davidson_stats(void *handle, int *accepted, int *rejected, int 
 
 /* Eigenvalues (ascending, d_w: m) and orthonormal eigenvectors (d_V, m x m,
  * column-major, leading dimension ldv) of the symmetric d_A (m x m, lda; read
- * only), on the device by two-sided block Jacobi (64 x 64 subproblems
- * diagonalised in LDS, block rotations applied by tiled kernels).  The
+ * only), on the device by two-sided block Jacobi (32 x 32 subproblems, one
+ * cyclic sweep each per round in LDS, block rotations applied by a tiled
+ * kernel).  The
  * projected problems of Rayleigh-Ritz and Davidson use it in place of the
  * reference's cusolverDnDsygvd / Dsyevd (IPT/src/ipt_cuda.cu:3213-3226).
  * Status 6 when it does not converge in 40 sweeps. */
