@@ -25,6 +25,7 @@
 #include <functional>
 
 #include "ipt_block.h"
+#include "ipt_emit.h"
 #include "ipt_lowp.h"
 #include "ipt_sparse_lp.h"
 #include "ipt_prep.h"
@@ -64,6 +65,7 @@ struct BlockParams {
     int max_basis = 0;      // Davidson search-space limit (0: the reference's k + max(16, (2k+2)/3))
     int retain = -1;        // corrections kept over a restart (-1: the reference's max_basis - k - 1)
     int max_block = 64;     // IPT_MAX_LIFT_BLOCK_SIZE
+    int emit = 1;           // dense FP16/BF16/BF16x3: the update writes the next product's operand
 };
 
 // Davidson search-space limit for k targets on n rows
@@ -714,7 +716,7 @@ void record_stats(Context *ctx, const BlockResult &br)
 int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_product, const DeltaFn *lowp,
                    const double *d, const double *rowabs, int n, int k,
                    const std::vector<std::pair<int, int>> &ranges, double normH, const BlockParams &p,
-                   BlockResult *out, bool sparse_products = false)
+                   BlockResult *out, bool sparse_products = false, const LowpOperator *dense_op = nullptr)
 {
     BlockPlan plan;
     // with stall detection on, clusters wider than the block limit are not
@@ -750,6 +752,34 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             out->fp64_products += 1;
         }
     }
+    // Operand emission (dense FP16 / BF16 / BF16x3 correction products):
+    // every update writes the operand of the next low-precision product
+    // (ipt_emit.h), so a low-precision step is the product, its epilogue
+    // and the update.  The wide-cluster kernel does not emit.
+    const bool fused = mixed && p.emit && dense_op != nullptr && plan.wide_host.empty() &&
+                       (dense_op->mode == LOWP_FP16 || dense_op->mode == LOWP_BF16 ||
+                        dense_op->mode == LOWP_BF16X3);
+    const int nb_all = (bc + 15) / 16;
+    DevBuf<unsigned short> zhi, zlo;
+    DevBuf<double> colscale;
+    if (fused) {
+        const size_t zel = (size_t)dense_op->nkb * nb_all * 512;  // 16-bit elements per plane
+        IPT_STCHK(zhi.alloc(zel));
+        IPT_HIPCHK(hipMemsetAsync(zhi.p, 0, zel * sizeof(unsigned short), ctx->stream));
+        if (dense_op->mode == LOWP_BF16X3) {
+            IPT_STCHK(zlo.alloc(zel));
+            IPT_HIPCHK(hipMemsetAsync(zlo.p, 0, zel * sizeof(unsigned short), ctx->stream));
+        }
+        IPT_STCHK(colscale.alloc((size_t)nb_all * 16));
+        const std::vector<double> ones((size_t)nb_all * 16, 1.0);
+        IPT_HIPCHK(hipMemcpyAsync(colscale.p, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        IPT_STCHK(plan.cmax.alloc((size_t)2 * plan.nclusters));
+        IPT_HIPCHK(hipMemsetAsync(plan.cmax.p, 0, (size_t)2 * plan.nclusters * sizeof(double), ctx->stream));
+        // the host vector above must outlive the copy
+        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    plan.frozen.assign((size_t)plan.nclusters, 0);
     double *cur = Ba.p, *nxt = Bb.p;
     auto t0 = std::chrono::steady_clock::now();
     // Exact steps are scheduled: at most anchor_every low-precision steps
@@ -840,6 +870,9 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             }
         } else if (exact) {
             IPT_STCHK(apply(cur + ho, Y.p + ho, hull_w));
+        } else if (fused) {
+            IPT_STCHK(lowp_apply_emitted(ctx, *dense_op, zhi.p, zlo.p, nb_all, hull_lo, hull_w, colscale.p, cur,
+                                         Banc.p, Yanc.p, n, d, bc, Y.p));
         } else if (lowp_hull || !per_run) {
             IPT_STCHK((*lowp)(cur + ho, Banc.p + ho, Yanc.p + ho, hull_w, Y.p + ho));
         } else {
@@ -856,7 +889,20 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         }
         // a low-precision step stops clusters that reach tol there; the next
         // exact step confirms them (block_update, k_block_finalize)
-        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, p.tol, 1, exact ? 0 : 1));
+        LowpEmit em;
+        if (fused) {
+            em.zhi = zhi.p;
+            em.zlo = zlo.p;
+            em.mode = dense_op->mode;
+            em.nb_all = nb_all;
+            // an exact step's pair becomes the anchor of the next product
+            em.anc = exact ? nullptr : Banc.p;
+            em.cmax = plan.cmax.p;
+            em.fp = plan.fp.p;
+            em.colscale = colscale.p;
+        }
+        IPT_STCHK(block_update(ctx, plan, cur, Y.p, nxt, n, d, n, p.damping, p.tol, 1, exact ? 0 : 1,
+                               fused ? &em : nullptr));
         if (mixed && exact) {
             // the exact pair (B, H B) becomes the anchor of the next steps
             IPT_HIPCHK(hipMemcpyAsync(Banc.p, cur, (size_t)n * bc * sizeof(double), hipMemcpyDeviceToDevice,
@@ -876,6 +922,17 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
                                       hipMemcpyDeviceToHost, ctx->stream));
         }
         IPT_STCHK(read_host(ctx, flags, plan.flags.p, 3 * sizeof(int)));
+        // After an exact step an inactive cluster is frozen for good: its
+        // columns go to the other iterate buffer once, and launches whose
+        // clusters are all frozen are skipped from here on.
+        for (int c = 0; c < plan.nclusters; ++c) {
+            if (!act[(size_t)c] && !plan.frozen[(size_t)c]) {
+                const size_t o = (size_t)plan.host[(size_t)c].off * n;
+                IPT_HIPCHK(hipMemcpyAsync(nxt + o, cur + o, (size_t)n * plan.host[(size_t)c].m * sizeof(double),
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+                plan.frozen[(size_t)c] = 1;
+            }
+        }
         rebuild_runs();
         if (mixed) {
             int steps = anchor_every + 1;
@@ -1070,6 +1127,7 @@ BlockParams params_from_env(int maxiter, double tol, int use_tol)
         }
     }
     p.anchor_every = std::max(0, env_int("IPT_MIXED_ANCHOR_EVERY", 0));
+    p.emit = env_int("IPT_MIXED_EMIT", 1) != 0;
     // additions: stall detection and the Davidson search-space limits
     p.stall_steps = std::max(0, env_int("IPT_BLOCK_CLUSTER_STALL", 0));
     p.max_basis = std::max(0, env_int("IPT_DAVIDSON_MAX_BASIS", 0));
@@ -1447,7 +1505,7 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     };
     BlockResult br;
     IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp != nullptr ? &delta : nullptr, pr.d.p, pr.rowabs.p, n,
-                             k, pr.ranges, pr.normH, p, &br));
+                             k, pr.ranges, pr.normH, p, &br, false, lp));
     record_stats(ctx, br);
     if (p.verbose) {
         fprintf(stderr,

@@ -44,7 +44,13 @@ struct BlockPlan {
     DevBuf<int> width_lists;     // cluster ids grouped by width
     std::vector<int> wide_host;  // clusters of width 65..142 (one launch each)
     DevBuf<int> wide_list;       // their ids
+    // host view of clusters frozen for good (as of the last exact step):
+    // launches whose clusters are all frozen are skipped; empty: none known
+    std::vector<char> frozen;
+    DevBuf<double> cmax;  // operand emission: per cluster step and anchor maxima of the last update
 };
+
+struct LowpEmit;
 
 // Build the plan for ranges [first,last] (sorted by first) on n rows.
 // skip_wider > 0: clusters wider than that are left out of every update
@@ -65,8 +71,12 @@ int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int 
 // finalised steps, so the active set is the same over those steps).
 // lowp_step = 1: Y came from a low-precision product; clusters reaching
 // fp <= tol stop and are marked tentative (see block_reactivate_tentative).
+// emit != NULL: the update also writes the operand of the next
+// low-precision product (ipt_emit.h); the caller keeps the planes,
+// colscale and plan.cmax; em.mpart is set here.
 int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
-                 const double *d, int n, double damping, double tol, int count_steps = 1, int lowp_step = 0);
+                 const double *d, int n, double damping, double tol, int count_steps = 1, int lowp_step = 0,
+                 const LowpEmit *emit = nullptr);
 // Before an exact step: tentative clusters iterate once more (with the
 // exact product), after which the finalisation freezes them.
 int block_reactivate_tentative(Context *ctx, BlockPlan &plan);

@@ -17,6 +17,7 @@
 
 #include "ipt_block.h"
 #include "ipt_block_width.h"
+#include "ipt_emit.h"
 
 namespace ipt {
 
@@ -46,7 +47,7 @@ __device__ __forceinline__ void block_update_rows(
     int cid, int lblk, const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn,
     long ld, const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
     const int *__restrict__ active, double damping, double *__restrict__ partials, int pstride,
-    int *__restrict__ flags)
+    int *__restrict__ flags, const LowpEmit &em)
 {
     constexpr int GROUPS = 256 / P;
     const BlockCluster cl = clusters[cid];
@@ -56,6 +57,12 @@ __device__ __forceinline__ void block_update_rows(
     const long row = (long)lblk * GROUPS + grp;
     const bool live = active[cid] != 0;
     double sdn = 0.0, sbn = 0.0;
+    const bool emit = em.zhi != nullptr;
+    const double escale = emit ? emit_scale(em, cid) : 1.0;
+    double mstep = 0.0, manc = 0.0;
+    if (emit && live && lblk == 0 && grp == 0 && t < m) {
+        em.colscale[cl.off + t] = escale;
+    }
     // Off-diagonal absolute row sums of the cluster matrix h, shared by
     // every row of the block: a row whose system matrix (d_r I - h^T) is
     // column diagonally dominant by a factor of 2 keeps its pivots on the
@@ -95,6 +102,12 @@ __device__ __forceinline__ void block_update_rows(
                 Bnc[row + (long)t * ld] = nv;
                 sdn += (nv - ov) * (nv - ov);
                 sbn += ov * ov;
+                if (emit) {
+                    const double av = em.anc != nullptr ? em.anc[row + (long)(cl.off + t) * ld] : ov;
+                    emit_store(em, em.mode, row, cl.off + t, nv, av, escale);
+                    mstep = fmax(mstep, fabs(nv - ov));
+                    manc = fmax(manc, fabs(nv - av));
+                }
             }
         } else {
             const double dr = d[row];
@@ -182,8 +195,17 @@ __device__ __forceinline__ void block_update_rows(
                 Bnc[row + (long)mypc * ld] = v;
                 sdn += (v - ov) * (v - ov);
                 sbn += ov * ov;
+                if (emit) {
+                    const double av = em.anc != nullptr ? em.anc[row + (long)(cl.off + mypc) * ld] : ov;
+                    emit_store(em, em.mode, row, cl.off + mypc, v, av, escale);
+                    mstep = fmax(mstep, fabs(v - ov));
+                    manc = fmax(manc, fabs(v - av));
+                }
             }
         }
+    }
+    if (emit) {
+        emit_block_max(em, (long)cid * pstride + lblk, mstep, manc);
     }
     __shared__ double sa[4], sb[4];
     for (int o = 32; o > 0; o >>= 1) {
@@ -208,10 +230,10 @@ __global__ __launch_bounds__(256) void k_block_update(
     const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
     const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
     const int *__restrict__ class_list, const int *__restrict__ active, double damping,
-    double *__restrict__ partials, int pstride, int *__restrict__ flags)
+    double *__restrict__ partials, int pstride, int *__restrict__ flags, LowpEmit em)
 {
     block_update_rows<P>(class_list[blockIdx.y], blockIdx.x, B, Y, Bn, ld, d, n, clusters, active, damping,
-                         partials, pstride, flags);
+                         partials, pstride, flags, em);
 }
 
 // Widths 2..32: one kernel per width (ipt_block_width.h).
@@ -298,9 +320,10 @@ __global__ __launch_bounds__(256) void k_block_update_w1(
     const double *__restrict__ B, const double *__restrict__ Y, double *__restrict__ Bn, long ld,
     const double *__restrict__ d, int n, const BlockCluster *__restrict__ clusters,
     const int *__restrict__ w1_list, int nw1, const int *__restrict__ active, double damping,
-    double *__restrict__ partials, int pstride, int *__restrict__ flags)
+    double *__restrict__ partials, int pstride, int *__restrict__ flags, LowpEmit em)
 {
     const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool emit = em.zhi != nullptr;
     const int chunk = (int)(row >> 6);
     const bool in = row < n;
     const double dr = in ? d[row] : 0.0;
@@ -315,14 +338,23 @@ __global__ __launch_bounds__(256) void k_block_update_w1(
         const bool live = active[cid] != 0;
         const long col = (long)cl.off * ld;
         double sdn = 0.0, sbn = 0.0;
+        double mstep = 0.0, manc = 0.0;
+        const double escale = emit ? emit_scale(em, cid) : 1.0;
+        if (emit && live && row == 0) {
+            em.colscale[cl.off] = escale;
+        }
         if (in) {
             const double ov = B[row + col];
+            double nv = ov;
+            bool wrote = false;
             if (!live) {
                 Bn[row + col] = ov;
             } else if (row == cl.first) {
                 Bn[row + col] = 1.0;
                 sdn = (1.0 - ov) * (1.0 - ov);
                 sbn = ov * ov;
+                nv = 1.0;
+                wrote = true;
             } else {
                 const double a = dr - Y[cl.first + col];
                 const double rhs = dr * ov - Y[row + col];
@@ -340,7 +372,15 @@ __global__ __launch_bounds__(256) void k_block_update_w1(
                     Bn[row + col] = v;
                     sdn = (v - ov) * (v - ov);
                     sbn = ov * ov;
+                    nv = v;
+                    wrote = true;
                 }
+            }
+            if (emit && wrote) {
+                const double av = em.anc != nullptr ? em.anc[row + col] : ov;
+                emit_store(em, em.mode, row, cl.off, nv, av, escale);
+                mstep = fabs(nv - ov);
+                manc = fabs(nv - av);
             }
         }
         // a wavefront entirely past the last row has no partial slot
@@ -348,11 +388,19 @@ __global__ __launch_bounds__(256) void k_block_update_w1(
             for (int o = 32; o > 0; o >>= 1) {
                 sdn += __shfl_down(sdn, o, 64);
                 sbn += __shfl_down(sbn, o, 64);
+                if (emit) {
+                    mstep = fmax(mstep, __shfl_down(mstep, o, 64));
+                    manc = fmax(manc, __shfl_down(manc, o, 64));
+                }
             }
             if ((threadIdx.x & 63) == 0) {
                 const long slot = (long)cid * pstride + chunk;
                 partials[2 * slot] = sdn;
                 partials[2 * slot + 1] = sbn;
+                if (emit) {
+                    em.mpart[2 * slot] = mstep;
+                    em.mpart[2 * slot + 1] = manc;
+                }
             }
         }
     }
@@ -382,28 +430,42 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
                                                         int nclusters, int *__restrict__ flags,
                                                         int *__restrict__ tentative, int lowp_step,
                                                         double *__restrict__ fp_prev, int *__restrict__ growth,
-                                                        int *__restrict__ stalled, int stall_steps)
+                                                        int *__restrict__ stalled, int stall_steps,
+                                                        const double *__restrict__ mpart, double *__restrict__ cmax)
 {
     const int c = blockIdx.x;
-    __shared__ double sa[4], sb[4];
+    __shared__ double sa[4], sb[4], sm[4], sn[4];
     __shared__ int wl[4];
     __shared__ int last;
     if (active[c]) {
         const int nb = clusters[c].nblocks;
-        double dn = 0.0, bn = 0.0;
+        double dn = 0.0, bn = 0.0, ms = 0.0, ma = 0.0;
         for (int b = threadIdx.x; b < nb; b += blockDim.x) {
             dn += partials[2 * ((long)c * pstride + b)];
             bn += partials[2 * ((long)c * pstride + b) + 1];
+            if (mpart != nullptr) {
+                ms = fmax(ms, mpart[2 * ((long)c * pstride + b)]);
+                ma = fmax(ma, mpart[2 * ((long)c * pstride + b) + 1]);
+            }
         }
         for (int o = 32; o > 0; o >>= 1) {
             dn += __shfl_down(dn, o, 64);
             bn += __shfl_down(bn, o, 64);
+            ms = fmax(ms, __shfl_down(ms, o, 64));
+            ma = fmax(ma, __shfl_down(ma, o, 64));
         }
         if ((threadIdx.x & 63) == 0) {
             sa[threadIdx.x >> 6] = dn;
             sb[threadIdx.x >> 6] = bn;
+            sm[threadIdx.x >> 6] = ms;
+            sn[threadIdx.x >> 6] = ma;
         }
         __syncthreads();
+        if (threadIdx.x == 0 && mpart != nullptr) {
+            // the update's operand maxima of this step scale the next one
+            cmax[2 * c] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+            cmax[2 * c + 1] = fmax(fmax(sn[0], sn[1]), fmax(sn[2], sn[3]));
+        }
         if (threadIdx.x == 0) {
             dn = sqrt(sa[0] + sa[1] + sa[2] + sa[3]);
             bn = sqrt(sb[0] + sb[1] + sb[2] + sb[3]);
@@ -704,23 +766,44 @@ int block_identity(Context *ctx, const BlockPlan &plan, double *B, long ld, int 
 }
 
 int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y, double *Bn, long ld,
-                 const double *d, int n, double damping, double tol, int count_steps, int lowp_step)
+                 const double *d, int n, double damping, double tol, int count_steps, int lowp_step,
+                 const LowpEmit *emit)
 {
+    // norm partials (2 doubles per slot), then the operand maxima partials
+    const size_t slots = (size_t)std::max(plan.nclusters, 1) * plan.partial_stride;
     double *partials = nullptr;
-    IPT_STCHK(context_scratch(ctx, SLOT_REDUCE,
-                              (size_t)std::max(plan.nclusters, 1) * plan.partial_stride * 2 * sizeof(double),
-                              (void **)&partials));
+    IPT_STCHK(context_scratch(ctx, SLOT_REDUCE, slots * 4 * sizeof(double), (void **)&partials));
+    LowpEmit em;
+    if (emit != nullptr) {
+        em = *emit;
+        em.mpart = partials + 2 * slots;
+    }
+    // A launch whose clusters are all frozen (host view of the last exact
+    // step) is skipped: their columns are the same in both iterate buffers.
+    auto all_frozen = [&](int lo, int hi) {
+        if (plan.frozen.empty()) {
+            return false;
+        }
+        for (int c = 0; c < plan.nclusters; ++c) {
+            const int m = plan.host[(size_t)c].m;
+            if (m >= lo && m <= hi && !plan.frozen[(size_t)c]) {
+                return false;
+            }
+        }
+        return true;
+    };
     const ProfileToken prof = profile_begin(ctx, PROF_BLOCK_UPDATE);
-    if (plan.w1_count > 0) {
+    if (plan.w1_count > 0 && !all_frozen(1, 1)) {
         dim3 grid((n + 255) / 256, (plan.w1_count + kClustersPerBlock - 1) / kClustersPerBlock);
         hipLaunchKernelGGL(k_block_update_w1, grid, dim3(256), 0, ctx->stream, B, Y, Bn, ld, d, n, plan.clusters.p,
                            plan.w1_list.p, plan.w1_count, plan.active.p, damping, partials, plan.partial_stride,
-                           plan.flags.p);
+                           plan.flags.p, em);
         IPT_HIPCHK(hipGetLastError());
     }
     if (plan.wide_count > 0) {
-        const BlockUpdateArgs g{B, Y, Bn, ld, d, n, plan.clusters.p, plan.active.p, damping, partials,
-                                plan.partial_stride, plan.flags.p};
+        BlockUpdateArgs g{B, Y, Bn, ld, d, n, plan.clusters.p, plan.active.p, damping, partials,
+                          plan.partial_stride, plan.flags.p};
+        g.em = em;
         // widest first: their blocks take longest.  Widths 3..16 that share
         // a lane class go out as one launch when more than one is present.
         const bool merge = env_int("IPT_BLOCK_CLASS_LAUNCH", 1) != 0;
@@ -733,34 +816,41 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
                     count += plan.width_count[q];
                 }
                 if (present > 1) {
-                    IPT_STCHK(block_update_class(lo, plan.width_lists.p + plan.width_offset[lo], count, g,
-                                                 ctx->stream));
+                    if (!all_frozen(lo, m)) {
+                        IPT_STCHK(block_update_class(lo, plan.width_lists.p + plan.width_offset[lo], count, g,
+                                                     ctx->stream));
+                    }
                     m = lo;  // the loop's decrement moves on to lo - 1
                     continue;
                 }
             }
-            if (plan.width_count[m] > 0) {
+            if (plan.width_count[m] > 0 && !all_frozen(m, m)) {
                 IPT_STCHK(block_update_width(m, plan.width_lists.p + plan.width_offset[m], plan.width_count[m], g,
                                              ctx->stream));
             }
         }
     }
-    if (plan.class_count[6] > 0) {
+    if (plan.class_count[6] > 0 && !all_frozen(33, 64)) {
         // widths 33..64 keep their own launch (their registers would lower
         // the occupancy of every fused block)
         const int rows = 256 >> 6;
         dim3 grid((n + rows - 1) / rows, plan.class_count[6]);
         hipLaunchKernelGGL(k_block_update<64>, grid, dim3(256), 0, ctx->stream, B, Y, Bn, ld, d, n,
                            plan.clusters.p, plan.class_lists.p + plan.class_offset[6], plan.active.p, damping,
-                           partials, plan.partial_stride, plan.flags.p);
+                           partials, plan.partial_stride, plan.flags.p, em);
         IPT_HIPCHK(hipGetLastError());
     }
     if (!plan.wide_host.empty()) {
+        // the wide kernel writes no operand (the caller keeps emission off
+        // when wide clusters are present)
         const BlockUpdateArgs g{B, Y, Bn, ld, d, n, plan.clusters.p, plan.active.p, damping, partials,
                                 plan.partial_stride, plan.flags.p};
         for (size_t i = 0; i < plan.wide_host.size(); ++i) {
-            const int m = plan.host[(size_t)plan.wide_host[i]].m;
-            IPT_STCHK(launch_block_wide(m, plan.wide_list.p + i, 1, g, ctx->stream));
+            const int c = plan.wide_host[i];
+            if (!plan.frozen.empty() && plan.frozen[(size_t)c]) {
+                continue;
+            }
+            IPT_STCHK(launch_block_wide(plan.host[(size_t)c].m, plan.wide_list.p + i, 1, g, ctx->stream));
         }
     }
     profile_end(ctx, prof);
@@ -770,7 +860,8 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
     hipLaunchKernelGGL(k_block_finalize, dim3(plan.nclusters), dim3(256), 0, ctx->stream, partials,
                        plan.partial_stride, plan.clusters.p, plan.active.p, plan.fp.p, plan.iters.p, tol,
                        count_steps, plan.nclusters, plan.flags.p, plan.tentative.p, lowp_step, plan.fp_prev.p,
-                       plan.growth.p, plan.stalled.p, plan.stall_steps);
+                       plan.growth.p, plan.stalled.p, plan.stall_steps, emit != nullptr ? em.mpart : nullptr,
+                       emit != nullptr ? plan.cmax.p : nullptr);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

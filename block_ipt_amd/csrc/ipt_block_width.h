@@ -18,6 +18,7 @@
 #include <cfloat>
 
 #include "ipt_block.h"
+#include "ipt_emit.h"
 
 namespace ipt {
 
@@ -34,6 +35,7 @@ struct BlockUpdateArgs {
     double *partials;
     int pstride;
     int *flags;
+    LowpEmit em;  // operand of the next low-precision product (em.zhi NULL: none)
 };
 
 // Broadcast of lane L of each P-lane group by DPP: row_newbcast (lane L of
@@ -109,6 +111,12 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
     const long row = (long)lblk * GROUPS + grp;
     const bool live = g.active[cid] != 0;
     double sdn = 0.0, sbn = 0.0;
+    const bool emit = g.em.zhi != nullptr;
+    const double escale = emit ? emit_scale(g.em, cid) : 1.0;
+    double mstep = 0.0, manc = 0.0;
+    if (emit && live && lblk == 0 && grp == 0 && t < M) {
+        g.em.colscale[cl.off + t] = escale;
+    }
     // h staged in LDS once per block (coalesced: column u of h is M
     // consecutive rows of Y), hs[t][u] = h(u, t), so that the equations of
     // every row of the block read it from there
@@ -152,6 +160,12 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
                 Bnc[row + (long)t * ld] = nv;
                 sdn += (nv - ov) * (nv - ov);
                 sbn += ov * ov;
+                if (emit) {
+                    const double av = g.em.anc != nullptr ? g.em.anc[row + (long)(cl.off + t) * ld] : ov;
+                    emit_store(g.em, g.em.mode, row, cl.off + t, nv, av, escale);
+                    mstep = fmax(mstep, fabs(nv - ov));
+                    manc = fmax(manc, fabs(nv - av));
+                }
             }
         } else {
             const double dr = g.d[row];
@@ -229,8 +243,17 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
                 Bnc[row + (long)mypc * ld] = v;
                 sdn += (v - ov) * (v - ov);
                 sbn += ov * ov;
+                if (emit) {
+                    const double av = g.em.anc != nullptr ? g.em.anc[row + (long)(cl.off + mypc) * ld] : ov;
+                    emit_store(g.em, g.em.mode, row, cl.off + mypc, v, av, escale);
+                    mstep = fmax(mstep, fabs(v - ov));
+                    manc = fmax(manc, fabs(v - av));
+                }
             }
         }
+    }
+    if (emit) {
+        emit_block_max(g.em, (long)cid * g.pstride + lblk, mstep, manc);
     }
     __shared__ double sa[4], sb[4];
     for (int o = 32; o > 0; o >>= 1) {

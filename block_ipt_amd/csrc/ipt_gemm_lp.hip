@@ -459,25 +459,64 @@ constexpr int lowp_kw(int mode, int nb)
 
 template <int MODE, int NB>
 static void launch_lowp_nb(dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
-                           const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw)
+                           const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw, int zstride,
+                           int zcb0)
 {
     hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, lowp_kw(MODE, NB)>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, ts, P,
-                       nkb, nrb, rbw);
+                       nkb, nrb, rbw, zstride, zcb0);
 }
 
 template <int MODE>
 static void launch_lowp(int NB, dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
-                        const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw)
+                        const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw, int zstride,
+                        int zcb0)
 {
     switch (NB) {
-    case 1: launch_lowp_nb<MODE, 1>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
-    case 2: launch_lowp_nb<MODE, 2>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
-    case 3: launch_lowp_nb<MODE, 3>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
-    case 4: launch_lowp_nb<MODE, 4>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
-    case 5: launch_lowp_nb<MODE, 5>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
-    case 6: launch_lowp_nb<MODE, 6>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
-    case 7: launch_lowp_nb<MODE, 7>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
-    default: launch_lowp_nb<MODE, 8>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw); break;
+    case 1: launch_lowp_nb<MODE, 1>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 2: launch_lowp_nb<MODE, 2>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 3: launch_lowp_nb<MODE, 3>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 4: launch_lowp_nb<MODE, 4>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 5: launch_lowp_nb<MODE, 5>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 6: launch_lowp_nb<MODE, 6>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 7: launch_lowp_nb<MODE, 7>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    default: launch_lowp_nb<MODE, 8>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    }
+}
+
+// Grid of one product: splits of 8 wavefronts' k blocks, row groups so that
+// about one workgroup per CU runs.
+struct LowpGrid {
+    int splits, rbw, gx;
+};
+
+static LowpGrid lowp_grid(const LowpOperator &op, int NB)
+{
+    const int kw = lowp_kw(op.mode, NB);
+    const int kspan = 8 * kw;
+    LowpGrid g;
+    g.splits = op.nkb / kspan;  // nkb is a multiple of 64 >= kspan
+    const int groups = std::max(1, std::min(op.nrb, (256 + g.splits - 1) / g.splits));
+    g.rbw = (op.nrb + groups - 1) / groups;
+    g.gx = (op.nrb + g.rbw - 1) / g.rbw;
+    return g;
+}
+
+static void launch_lowp_mode(const LowpOperator &op, int NB, dim3 grid, hipStream_t st, const uint4 *zhi,
+                             const uint4 *zlo, float *P, int rbw, int zstride, int zcb0)
+{
+    const uint4 *Ehi = reinterpret_cast<const uint4 *>(op.hi.p);
+    const uint4 *Elo = reinterpret_cast<const uint4 *>(op.lo.p);
+    const float *ts = op.tscale.p;
+    if (op.mode == LOWP_BF16X3) {
+        launch_lowp<LOWP_BF16X3>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+    } else if (op.mode == LOWP_FP32) {
+        launch_lowp<LOWP_FP32>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+    } else if (op.mode == LOWP_FP8) {
+        launch_lowp<LOWP_FP8>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+    } else if (op.mode == LOWP_FP16) {
+        launch_lowp<LOWP_FP16>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+    } else {
+        launch_lowp<LOWP_BF16>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
     }
 }
 
@@ -488,14 +527,10 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
     for (int c0 = 0; c0 < nc; c0 += 128) {
         const int w = std::min(128, nc - c0);
         const int NB = (w + 15) / 16;
-        const int kw = lowp_kw(op.mode, NB);
         // one split = the k blocks of 8 wavefronts; about one workgroup of
         // 8 wavefronts per CU over 256 CUs
-        const int kspan = 8 * kw;
-        const int splits = op.nkb / kspan;  // nkb is a multiple of 64 >= kspan
-        const int groups = std::max(1, std::min(op.nrb, (256 + splits - 1) / splits));
-        const int rbw = (op.nrb + groups - 1) / groups;
-        const int gx = (op.nrb + rbw - 1) / rbw;
+        const LowpGrid lg = lowp_grid(op, NB);
+        const int splits = lg.splits, rbw = lg.rbw, gx = lg.gx;
         // scratch: colmax partials (nparts x w) | colscale (w) | Z hi/lo
         // (fragment-major) | partial sums
         const int nparts = std::min(8, (n + 255) / 256);
@@ -522,27 +557,42 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
                            Bac, ld, n, w, op.nkb, NB, op.mode, colmax, nparts, colscale, zhi, zlo);
         IPT_HIPCHK(hipGetLastError());
         dim3 grid(gx, splits);
-        const uint4 *Ehi = reinterpret_cast<const uint4 *>(op.hi.p);
-        const uint4 *Elo = reinterpret_cast<const uint4 *>(op.lo.p);
         const ProfileToken prof = profile_begin(ctx, PROF_LOWP_GEMM);
-        const float *ts = op.tscale.p;
-        if (op.mode == LOWP_BF16X3) {
-            launch_lowp<LOWP_BF16X3>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
-        } else if (op.mode == LOWP_FP32) {
-            launch_lowp<LOWP_FP32>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
-        } else if (op.mode == LOWP_FP8) {
-            launch_lowp<LOWP_FP8>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
-        } else if (op.mode == LOWP_FP16) {
-            launch_lowp<LOWP_FP16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
-        } else {
-            launch_lowp<LOWP_BF16>(NB, grid, ctx->stream, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw);
-        }
+        launch_lowp_mode(op, NB, grid, ctx->stream, zhi, zlo, P, rbw, NB, 0);
         profile_end(ctx, prof, w);
         IPT_HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_lowp_epilogue, dim3(blocks_for((long)op.nrb * NB * 64, 256, 16384)), dim3(256), 0,
                            ctx->stream, reinterpret_cast<const f4_t *>(P), splits, n, w, op.nrb, NB,
                            Yac, Bc, Bac, ld, d, colscale, 1.0 / (double)op.scale,
                            Y + (size_t)c0 * ld);
+        IPT_HIPCHK(hipGetLastError());
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
+int lowp_apply_emitted(Context *ctx, const LowpOperator &op, const unsigned short *zhi, const unsigned short *zlo,
+                       int nb_all, int c_lo, int c_w, const double *colscale, const double *B, const double *Ba,
+                       const double *Ya, long ld, const double *d, int bc, double *Y)
+{
+    const int n = op.n;
+    const int cb_end = (c_lo + c_w + 15) / 16;
+    for (int cb = c_lo / 16; cb < cb_end; cb += 8) {
+        const int NB = std::min(8, cb_end - cb);
+        const LowpGrid lg = lowp_grid(op, NB);
+        float *P = nullptr;
+        IPT_STCHK(context_scratch(ctx, SLOT_LOWP, (size_t)lg.splits * op.nrb * NB * 1024 + 256, (void **)&P));
+        const ProfileToken prof = profile_begin(ctx, PROF_LOWP_GEMM);
+        launch_lowp_mode(op, NB, dim3(lg.gx, lg.splits), ctx->stream, reinterpret_cast<const uint4 *>(zhi),
+                         reinterpret_cast<const uint4 *>(zlo), P, lg.rbw, nb_all, cb);
+        profile_end(ctx, prof, std::min(16 * NB, bc - 16 * cb));
+        IPT_HIPCHK(hipGetLastError());
+        // columns past the basis (the last block's padding) are not written
+        const int c0 = 16 * cb;
+        const int w = std::min(16 * NB, bc - c0);
+        hipLaunchKernelGGL(k_lowp_epilogue, dim3(blocks_for((long)op.nrb * NB * 64, 256, 16384)), dim3(256), 0,
+                           ctx->stream, reinterpret_cast<const f4_t *>(P), lg.splits, n, w, op.nrb, NB,
+                           Ya + (size_t)c0 * ld, B + (size_t)c0 * ld, Ba + (size_t)c0 * ld, ld, d, colscale + c0,
+                           1.0 / (double)op.scale, Y + (size_t)c0 * ld);
         IPT_HIPCHK(hipGetLastError());
     }
     return IPT_CUDA_SUCCESS;

@@ -46,6 +46,8 @@ __device__ inline uint4 load_nt(const uint4 *p)
 // deterministic.
 //
 // tscale: FP8 only (else unused).
+// The operand holds zstride column blocks per k block; the product takes
+// the NB blocks from zcb0 on.
 // P layout: tile (z, rb, cb) of 1 KiB at ((z * nrb + rb) * NB + cb) * 256
 // floats, lane l, element v = MFMA D layout (row 4(l/16) + v, column l%16).
 // Row blocks past nrb are handled by clamped loads whose results are never
@@ -58,7 +60,7 @@ template <int NB, int MODE, int KW>
 __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi, const uint4 *__restrict__ Elo,
                                                    const uint4 *__restrict__ Zhi, const uint4 *__restrict__ Zlo,
                                                    const float *__restrict__ tscale, float *__restrict__ P, int nkb,
-                                                   int nrb, int rb_per_wg)
+                                                   int nrb, int rb_per_wg, int zstride, int zcb0)
 {
     constexpr int W = 8;  // wavefronts per workgroup
     constexpr bool X3 = MODE == LOWP_BF16X3;
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(512) void k_lowp_gemm(const uint4 *__restrict__ Ehi
     for (int kk = 0; kk < LK; ++kk) {
 #pragma unroll
         for (int cb = 0; cb < NB; ++cb) {
-            const long i = ((long)(ke0 + kk) * NB + cb) * 64 + lane;
+            const long i = ((long)(ke0 + kk) * zstride + zcb0 + cb) * 64 + lane;
             bz[0][kk][cb] = Zhi[i];
             if (TWO) {
                 bz[NZ - 1][kk][cb] = Zlo[i];

@@ -39,4 +39,14 @@ int lowp_operator_create(Context *ctx, const double *H, long ld, int n, int mode
 int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, const double *Ba, const double *Ya,
                      long ld, const double *d, int nc, double *Y);
 
+// The same product from an operand the block update already wrote
+// (ipt_emit.h): planes over nb_all column blocks of 16, scale colscale[c]
+// per basis column; the columns [c_lo, c_lo + c_w) are covered, rounded out
+// to whole column blocks (at most 16 * nb_all = the basis padded, bc the
+// basis width: no column past it is written).  Ya, B, Ba, Y: the full basis
+// (column 0 first).
+int lowp_apply_emitted(Context *ctx, const LowpOperator &op, const unsigned short *zhi, const unsigned short *zlo,
+                       int nb_all, int c_lo, int c_w, const double *colscale, const double *B, const double *Ba,
+                       const double *Ya, long ld, const double *d, int bc, double *Y);
+
 }  // namespace ipt
