@@ -1297,7 +1297,11 @@ __global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H
                                                     double *__restrict__ colsum_part,
                                                     double *__restrict__ colmax_part, int *__restrict__ asym)
 {
-    __shared__ double tt[64][65];
+    // ta = T = H[bi rows, bj cols], tb = T' = H[bj rows, bi cols], both
+    // [row][col] with a padded stride (conflict-free row and column walks)
+    __shared__ double ta[64][65];
+    __shared__ double tb[64][65];
+    __shared__ double rs[2][4][64], rm[2][4][64];
     // blockIdx.x -> (bi, bj), bi <= bj, row-major over the upper triangle of
     // tile pairs: row bi starts at bi*nb - bi*(bi-1)/2
     const long idx = blockIdx.x;
@@ -1311,88 +1315,75 @@ __global__ __launch_bounds__(256) void k_dense_scan(const double *__restrict__ H
         ++bi;
     }
     const int bj = bi + (int)(idx - ((long)bi * nb - (long)bi * (bi - 1) / 2));
-    // thread -> column c of a tile, rows 16q .. 16q+15 (one 128-byte segment)
-    const int c = threadIdx.x >> 2;
-    const int q = threadIdx.x & 3;
+    const bool diag = bi == bj;
     const long R0 = (long)bi * 64, C0 = (long)bj * 64;
-    int bad = 0;
-    // T = H[bi rows, bj cols] is loaded first, into registers, so its loads
-    // are in flight together with those of T' below
-    double tv[16];
-    {
-        const long gc = C0 + c;
+    // loads: lane = row of the tile, wavefront w = columns w, w+4, ..., so
+    // every load instruction reads 512 contiguous bytes of one column; both
+    // tiles' 32 loads per thread are issued before the first is used
+    const int lr = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double tv[16], uv[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const long gr = R0 + 16 * q + i;
-            tv[i] = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+    for (int j = 0; j < 16; ++j) {
+        const long gr = R0 + lr, gc = C0 + w + 4 * j;
+        tv[j] = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
+    }
+    if (!diag) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const long gr = C0 + lr, gc = R0 + w + 4 * j;
+            uv[j] = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
         }
     }
-    if (bi != bj) {
-        // T' = H[bj rows, bi cols]: stats of columns R0 + c, kept in LDS
-        const long gc = R0 + c;
-        double s = 0.0, mx = 0.0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = 16 * q + i;
-            const long gr = C0 + r;
-            const double v = (gr < n && gc < n) ? H[gr + gc * ld] : 0.0;
-            tt[r][c] = v;
-            s += fabs(v);
-            mx = fmax(mx, fabs(v));  // rows of block bj never meet column gc's diagonal
-        }
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        mx = fmax(mx, __shfl_xor(mx, 1, 64));
-        mx = fmax(mx, __shfl_xor(mx, 2, 64));
-        if (q == 0 && gc < n) {
-            colsum_part[(long)bj * n + gc] = s;
-            colmax_part[(long)bj * n + gc] = mx;
+    for (int j = 0; j < 16; ++j) {
+        ta[lr][w + 4 * j] = tv[j];
+        if (!diag) {
+            tb[lr][w + 4 * j] = uv[j];
         }
     }
     __syncthreads();
-    // T = H[bi rows, bj cols]: compare with T'^T, stats of columns C0 + c
-    {
-        const long gc = C0 + c;
-        double s = 0.0, mx = 0.0;
+    // thread (c = lane, rows 16w .. 16w+15): column c of T (global column
+    // C0 + c) against row c of T' (T' transposed), and column c of T'
+    // (global column R0 + c)
+    const int c = lr;
+    int bad = 0;
+    double sa = 0.0, ma = 0.0, sb = 0.0, mb = 0.0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = 16 * q + i;
-            const long gr = R0 + r;
-            const double v = tv[i];
-            if (bi != bj) {
-                const double w = tt[c][r];
-                const double scale = fmax(1.0, fmax(fabs(v), fabs(w)));
-                if (fabs(v - w) > 1e-12 * scale) {
-                    bad = 1;
-                }
-            }
-            s += fabs(v);
-            if (gr != gc) {
-                mx = fmax(mx, fabs(v));
-            }
+    for (int i = 0; i < 16; ++i) {
+        const int r = 16 * w + i;
+        const double v = ta[r][c];
+        const double u = diag ? ta[c][r] : tb[c][r];  // T(r, c) pairs with H(C0 + c, R0 + r)
+        const double scale = fmax(1.0, fmax(fabs(v), fabs(u)));
+        if (fabs(v - u) > 1e-12 * scale) {
+            bad = 1;
         }
-        if (bi == bj) {
-            // diagonal tile: symmetry within the tile through LDS
-            for (int i = 0; i < 16; ++i) {
-                tt[16 * q + i][c] = tv[i];
-            }
-            __syncthreads();
-            for (int i = 0; i < 16; ++i) {
-                const int r = 16 * q + i;
-                const double v = tt[r][c], w = tt[c][r];
-                const double scale = fmax(1.0, fmax(fabs(v), fabs(w)));
-                if (fabs(v - w) > 1e-12 * scale) {
-                    bad = 1;
-                }
-            }
+        sa += fabs(v);
+        if (!(diag && r == c)) {
+            ma = fmax(ma, fabs(v));
         }
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        mx = fmax(mx, __shfl_xor(mx, 1, 64));
-        mx = fmax(mx, __shfl_xor(mx, 2, 64));
-        if (q == 0 && gc < n) {
-            colsum_part[(long)bi * n + gc] = s;
-            colmax_part[(long)bi * n + gc] = mx;
+        if (!diag) {
+            // rows of block bj never meet the diagonal of columns R0 + c
+            const double x = tb[r][c];
+            sb += fabs(x);
+            mb = fmax(mb, fabs(x));
+        }
+    }
+    rs[0][w][c] = sa;
+    rm[0][w][c] = ma;
+    rs[1][w][c] = sb;
+    rm[1][w][c] = mb;
+    __syncthreads();
+    if (threadIdx.x < 128) {
+        const int h = threadIdx.x >> 6;  // 0: columns of T (C0 + c), 1: of T' (R0 + c)
+        if (h == 0 || !diag) {
+            const double s = ((rs[h][0][c] + rs[h][1][c]) + rs[h][2][c]) + rs[h][3][c];
+            const double m = fmax(fmax(rm[h][0][c], rm[h][1][c]), fmax(rm[h][2][c], rm[h][3][c]));
+            const long gc = (h == 0 ? C0 : R0) + c;
+            const long slot = h == 0 ? bi : bj;
+            if (gc < n) {
+                colsum_part[slot * n + gc] = s;
+                colmax_part[slot * n + gc] = m;
+            }
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
