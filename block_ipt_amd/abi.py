@@ -235,6 +235,7 @@ SIGNATURES = {
     "ipt_hip_generate_eps": (_c.c_int, [_vp, _c.c_int, _c.c_double, _c.c_ulonglong, _vp]),
     "ipt_hip_set_diagonal_gaps": (_c.c_int, [_vp, _c.c_int, _ip, _c.c_int, _c.c_double, _vp]),
     "ipt_hip_lowp_product": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _c.c_int, _vp, _vp]),
+    "ipt_hip_dense_product": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_int, _vp, _c.c_int, _vp]),
     "ipt_hip_profile_enable": (None, [_c.c_int]),
     "ipt_hip_profile_enable_only": (_c.c_int, [_c.c_char_p]),
     "ipt_hip_profile_reset": (None, []),

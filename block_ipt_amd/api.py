@@ -408,6 +408,14 @@ def lowp_product(d_H, n, mode, d_B, d_Ba, d_Ya, d_diag, nc, d_Y, handle=None):
         raise IptError(st, "ipt_hip_lowp_product")
 
 
+def dense_product(d_H, n, d_Z, nc, d_Y, packed=False, handle=None):
+    """Y = H Z in FP64 on the block path's MFMA kernel (device pointers)."""
+    st = lib().ipt_hip_dense_product(ctypes.c_void_p(d_H), int(n), ctypes.c_void_p(d_Z), int(nc),
+                                     ctypes.c_void_p(d_Y), 1 if packed else 0, handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_dense_product")
+
+
 def profile_enable(on=True, only=None):
     """Event timing of the library's kernel families; `only` (a list of
     family names) times just those."""

@@ -1487,7 +1487,19 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     IPT_STCHK(dense_prepare(ctx, dH, n, k, &pr, pack));
     const double prep = seconds_since(t0);
     const double *H = pr.H;
-    ApplyFn apply = [&](const double *X, double *Y, int nc) { return gemm_f64(ctx, H, n, X, n, Y, n, n, nc); };
+    // When every product is FP64, a copy of H in MFMA operand order makes
+    // each of them stream contiguously; one pass over H, repaid within a
+    // few products.
+    F64Packed packed;
+    if (p.lowp_mode < 0 && n >= 4096 && env_int("IPT_F64_PACKED", 1) != 0) {
+        IPT_STCHK(f64_packed_create(ctx, H, n, n, &packed));
+    }
+    ApplyFn apply = [&](const double *X, double *Y, int nc) {
+        if (packed.p.p != nullptr) {
+            return gemm_f64_packed(ctx, packed, X, n, Y, n, nc);
+        }
+        return gemm_f64(ctx, H, n, X, n, Y, n, n, nc);
+    };
     // H * (identity block of cluster [first,last]) = columns first..last of H
     const std::vector<std::pair<int, int>> &rg = pr.ranges;
     ApplyFn identity_product = [&](const double *, double *Y, int) -> int {

@@ -164,6 +164,17 @@ int env_flag_default(const char *name, int default_value);
 // Y column-major (ldy).  FP64 MFMA.
 int gemm_f64(Context *ctx, const double *H, long ldh, const double *Z, long ldz, double *Y,
              long ldy, int n, int ncols);
+// FP64 H packed in MFMA A-operand order (ipt_gemm_f64.hip): one extra copy
+// of H, streamed contiguously by gemm_f64_packed; worth its one pass over H
+// when a solve runs many FP64 products.
+struct F64Packed {
+    int n = 0;
+    int nkp = 0;  // k pairs of 8
+    DevBuf<double2> p;
+};
+int f64_packed_create(Context *ctx, const double *H, long ld, int n, F64Packed *op);
+// Y (n x ncols, ldy) = H Z (Z n x ncols, ldz) from the packed copy
+int gemm_f64_packed(Context *ctx, const F64Packed &op, const double *Z, long ldz, double *Y, long ldy, int ncols);
 
 // d[i] = H[i + i*ldh]
 int dense_diagonal(Context *ctx, const double *H, long ldh, int n, double *d);

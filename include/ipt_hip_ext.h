@@ -74,6 +74,13 @@ int ipt_hip_cholqr_mixed(double *d_Q, int n, int k, void *handle);
 int ipt_hip_lowp_product(const double *d_H, int n, int mode, const double *d_B, const double *d_Ba,
                          const double *d_Ya, const double *d_diag, int nc, double *d_Y, void *handle);
 
+/* Y = H Z in FP64 on the MFMA kernel of the block path: packed = 0 reads
+ * column-major H directly, packed = 1 first copies H into MFMA operand
+ * order (the layout the all-FP64 dense block solve uses for n >= 4096).
+ * All device, column-major: H n x n, Z and Y n x nc (leading dimension n). */
+int ipt_hip_dense_product(const double *d_H, int n, const double *d_Z, int nc, double *d_Y, int packed,
+                          void *handle);
+
 /* Kernel timing with HIP events on the library stream.  enable(1) starts
  * recording; get() waits for the recorded events and returns the summed
  * duration (ms) and launch count of one kernel family: "gemm_f64",

@@ -285,3 +285,28 @@ def test_mixed_strongly_coupled_no_rank_loss(mixed_env, spacing):
     w = sl.eigh(H, eigvals_only=True, subset_by_index=[0, k - 1])
     assert np.abs(np.sort(ld) - w).max() <= 1e-10 * max(1.0, np.abs(w).max())
     assert _scaled_residuals(H, vd, ld).max() <= 1e-10
+
+
+@pytest.mark.parametrize("packed", [False, True])
+@pytest.mark.parametrize("n,nc", [(17, 3), (1000, 70), (4097, 129), (8192, 16)])
+def test_dense_fp64_product(packed, n, nc):
+    """The FP64 MFMA product of the block path, from column-major H and from
+    the copy in MFMA operand order (k padded to 16, row blocks clamped),
+    against numpy at ragged sizes, to a rounding bound of the summation
+    (64 eps sqrt(n) |H||Z| elementwise)."""
+    hip = api.hip()
+    rng = np.random.default_rng(n + 7 * nc)
+    H = np.asfortranarray(rng.standard_normal((n, n)))
+    Z = np.asfortranarray(rng.standard_normal((n, nc)))
+    dH = hip.to_device(H)
+    dZ = hip.to_device(Z)
+    dY = hip.malloc(n * nc * 8)
+    try:
+        api.dense_product(dH, n, dZ, nc, dY, packed=packed)
+        Y = hip.to_host(dY, (n, nc))
+    finally:
+        for p in (dH, dZ, dY):
+            hip.free(p)
+    ref = H @ Z
+    scale = np.abs(H) @ np.abs(Z)
+    assert np.all(np.abs(Y - ref) <= 64 * np.finfo(float).eps * np.sqrt(n) * scale + 1e-300)
