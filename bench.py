@@ -444,15 +444,23 @@ def spmm_workload(a):
     dY = hip.malloc(n * kmax * 8)
     out = {"workload": "CSR product, orbital-sum model N=%d, nnz=%d, seed %d" % (n, nnz, a.seed)}
     for k in widths:
-        api.profile_reset()
-        api.profile_enable(True, only=["spmm"])
-        for _ in range(max(1, a.steps)):
-            api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
-        api.profile_enable(False)
-        ms, cnt = api.profile_get("spmm")
         comp = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n * k
-        out["k%d" % k] = {"avg_ms": ms / cnt, "launches": cnt, "compulsory_bytes": comp,
-                          "compulsory_GB/s": comp / (ms / cnt) / 1e6}
+        # one pass over the whole matrix, then column-sliced passes (the
+        # automatic count of the block path, and fixed counts for k <= 8)
+        counts = [None, 0] + ([2, 4, 8, 16] if k <= 8 else [])
+        for sl in counts:
+            api.profile_reset()
+            api.profile_enable(True, only=["spmm"])
+            for _ in range(max(1, a.steps)):
+                if sl is None:
+                    api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
+                else:
+                    api.csr_spmm_sliced(drp, dci, dv, n, dX, k, dY, k, k, sl)
+            api.profile_enable(False)
+            ms, cnt = api.profile_get("spmm")
+            key = "k%d" % k if sl is None else "k%d_slices%s" % (k, "auto" if sl == 0 else sl)
+            out[key] = {"avg_ms": ms / cnt, "launches": cnt, "compulsory_bytes": comp,
+                        "compulsory_GB/s": comp / (ms / cnt) / 1e6}
     for p in (drp, dci, dv, dX, dY):
         hip.free(p)
     print(json.dumps(out), flush=True)

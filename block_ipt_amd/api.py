@@ -394,6 +394,14 @@ def csr_spmm(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, handle=No
         raise IptError(st, "ipt_hip_csr_spmm")
 
 
+def csr_spmm_sliced(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, slices=0, handle=None):
+    """The CSR product in column-sliced passes (columns ascending per row)."""
+    st = lib().ipt_hip_csr_spmm_sliced(_vpp(d_row_ptr), _vpp(d_col_ind), _vpp(d_values), int(n), _vpp(d_X),
+                                       int(ldx), _vpp(d_Y), int(ldy), int(k), int(slices), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_csr_spmm_sliced")
+
+
 def cholqr_mixed(d_Q, n, k, handle=None):
     st = lib().ipt_hip_cholqr_mixed(_vpp(d_Q), int(n), int(k), handle)
     if st != 0:

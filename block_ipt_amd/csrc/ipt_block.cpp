@@ -915,13 +915,18 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             continue;
         }
         int flags[3];
-        IPT_HIPCHK(hipMemcpyAsync(act.data(), plan.active.p, act.size() * sizeof(int), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-        if (mixed) {
-            IPT_HIPCHK(hipMemcpyAsync(fp_now.data(), plan.fp.p, fp_now.size() * sizeof(double),
-                                      hipMemcpyDeviceToHost, ctx->stream));
+        {
+            // one read-back per exact step (the finalisation's status block)
+            std::vector<double> stv((size_t)2 + 2 * plan.nclusters);
+            IPT_STCHK(read_host(ctx, stv.data(), plan.status.p, stv.size() * sizeof(double)));
+            flags[0] = 0;
+            flags[1] = (int)stv[0];
+            flags[2] = (int)stv[1];
+            for (int c = 0; c < plan.nclusters; ++c) {
+                act[(size_t)c] = stv[(size_t)2 + c] != 0.0 ? 1 : 0;
+                fp_now[(size_t)c] = stv[(size_t)2 + plan.nclusters + c];
+            }
         }
-        IPT_STCHK(read_host(ctx, flags, plan.flags.p, 3 * sizeof(int)));
         // After an exact step an inactive cluster is frozen for good: its
         // columns go to the other iterate buffer once, and launches whose
         // clusters are all frozen are skipped from here on.
@@ -1625,11 +1630,17 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
     DevBuf<double> xr, yr;
     IPT_STCHK(xr.alloc((size_t)n * chunk));
     IPT_STCHK(yr.alloc((size_t)n * chunk));
+    // column slices for the narrow products (rows ascend on this path)
+    CsrSlices slices;
+    if (env_int("IPT_CSR_SLICES", 1) != 0) {
+        IPT_STCHK(csr_slices_create(ctx, d_rp, d_ci, n, &slices));
+    }
     ApplyFn apply = [&](const double *X, double *Y, int nc) -> int {
         for (int c0 = 0; c0 < nc; c0 += chunk) {
             const int w = std::min(chunk, nc - c0);
             IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));
-            IPT_STCHK(spmm_csr_rowmajor(ctx, d_rp, d_ci, d_v, n, xr.p, w, yr.p, w, w));
+            IPT_STCHK(spmm_csr_rowmajor(ctx, d_rp, d_ci, d_v, n, xr.p, w, yr.p, w, w,
+                                        slices.nslices > 0 ? &slices : nullptr));
             IPT_STCHK(rowmajor_to_colmajor(ctx, yr.p, w, Y + (size_t)c0 * n, n, n, w));
         }
         return IPT_CUDA_SUCCESS;
@@ -1806,11 +1817,16 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         maxcols = std::max(maxcols, std::min(n, k + std::max(16, (2 * k + 2) / 3)) + k + 1);
         IPT_STCHK(xr.alloc((size_t)n * std::min(maxcols, 128)));
         IPT_STCHK(yr.alloc((size_t)n * std::min(maxcols, 128)));
+        CsrSlices slices;
+        if (env_int("IPT_CSR_SLICES", 1) != 0) {
+            IPT_STCHK(csr_slices_create(ctx, rp.p, ci.p, n, &slices));
+        }
         ApplyFn apply = [&](const double *X, double *Y, int nc) -> int {
             for (int c0 = 0; c0 < nc; c0 += 128) {
                 const int w = std::min(128, nc - c0);
                 IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));
-                IPT_STCHK(spmm_csr_rowmajor(ctx, rp.p, ci.p, vv.p, n, xr.p, w, yr.p, w, w));
+                IPT_STCHK(spmm_csr_rowmajor(ctx, rp.p, ci.p, vv.p, n, xr.p, w, yr.p, w, w,
+                                            slices.nslices > 0 ? &slices : nullptr));
                 IPT_STCHK(rowmajor_to_colmajor(ctx, yr.p, w, Y + (size_t)c0 * n, n, n, w));
             }
             return IPT_CUDA_SUCCESS;

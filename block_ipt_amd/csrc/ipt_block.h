@@ -35,6 +35,9 @@ struct BlockPlan {
     int stall_steps = 0;      // growing steps that stall a cluster (0: no detection)
     std::vector<int> skipped; // clusters never iterated (wider than the block limit)
     DevBuf<int> flags;        // [0] singular, [1] status bits, [2] active count
+    // written by each finalisation for one read-back: [0] status bits,
+    // [1] active count, [2 + c] cluster c active, [2 + nclusters + c] its fp
+    DevBuf<double> status;
     int partial_stride = 0;   // max nblocks over clusters
     int w1_count = 0;         // clusters of width 1 (their own kernel)
     DevBuf<int> w1_list;      // their ids

@@ -431,7 +431,8 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
                                                         int *__restrict__ tentative, int lowp_step,
                                                         double *__restrict__ fp_prev, int *__restrict__ growth,
                                                         int *__restrict__ stalled, int stall_steps,
-                                                        const double *__restrict__ mpart, double *__restrict__ cmax)
+                                                        const double *__restrict__ mpart, double *__restrict__ cmax,
+                                                        double *__restrict__ status)
 {
     const int c = blockIdx.x;
     __shared__ double sa[4], sb[4], sm[4], sn[4];
@@ -518,6 +519,14 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
             wl[threadIdx.x >> 6] = live;
         }
         __syncthreads();
+        // one status block for the host: status bits, active count, then
+        // per cluster active flag and fixed-point residual
+        for (int q = threadIdx.x; q < nclusters; q += blockDim.x) {
+            status[2 + q] = __atomic_load_n(&active[q], __ATOMIC_RELAXED) != 0 ? 1.0 : 0.0;
+            const unsigned long long fb =
+                __atomic_load_n(reinterpret_cast<unsigned long long *>(&fp_out[q]), __ATOMIC_RELAXED);
+            status[2 + nclusters + q] = __longlong_as_double((long long)fb);
+        }
         if (threadIdx.x == 0) {
             if (__atomic_load_n(&flags[0], __ATOMIC_RELAXED)) {
                 atomicOr(&flags[1], 2);
@@ -525,6 +534,8 @@ __global__ __launch_bounds__(256) void k_block_finalize(const double *__restrict
             flags[0] = 0;
             flags[2] = wl[0] + wl[1] + wl[2] + wl[3];
             flags[3] = 0;
+            status[0] = (double)__atomic_load_n(&flags[1], __ATOMIC_RELAXED);
+            status[1] = (double)flags[2];
         }
     }
 }
@@ -660,6 +671,7 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
     IPT_STCHK(plan->growth.alloc((size_t)nc));
     IPT_STCHK(plan->stalled.alloc((size_t)nc));
     IPT_STCHK(plan->flags.alloc(4));
+    IPT_STCHK(plan->status.alloc((size_t)2 + 2 * (size_t)nc));
     // width-1 clusters: their own kernel
     plan->w1_count = (int)lists[0].size();
     IPT_STCHK(plan->w1_list.alloc(std::max<size_t>(lists[0].size(), 1)));
@@ -861,7 +873,7 @@ int block_update(Context *ctx, BlockPlan &plan, const double *B, const double *Y
                        plan.partial_stride, plan.clusters.p, plan.active.p, plan.fp.p, plan.iters.p, tol,
                        count_steps, plan.nclusters, plan.flags.p, plan.tentative.p, lowp_step, plan.fp_prev.p,
                        plan.growth.p, plan.stalled.p, plan.stall_steps, emit != nullptr ? em.mpart : nullptr,
-                       emit != nullptr ? plan.cmax.p : nullptr);
+                       emit != nullptr ? plan.cmax.p : nullptr, plan.status.p);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
 }

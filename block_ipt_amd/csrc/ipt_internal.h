@@ -208,8 +208,28 @@ int csr_gather_pairs(Context *ctx, const int *rp, const int *ci, const double *v
                      double *out);
 
 // CSR SpMM: Y(n x k, row-major ld ldy) = A(CSR) * X(n x k, row-major ld ldx).
+// Column slices of a CSR whose columns ascend within each row: for slice
+// b of kCsrMaxSlices (columns [b w, (b+1) w), w = ceil(n / kCsrMaxSlices)),
+// row r's entries are [split[b n + r], split[(b+1) n + r]).  Bounds only;
+// the entries stay where they are.
+constexpr int kCsrMaxSlices = 16;
+struct CsrSlices {
+    int n = 0;
+    int nslices = 0;
+    DevBuf<int> split;  // (nslices + 1) x n
+};
+int csr_slices_create(Context *ctx, const int *row_ptr, const int *col_ind, int n, CsrSlices *sl);
+// passes the product of width k takes (1: the whole matrix at once)
+int csr_slices_for(const CsrSlices &sl, int k);
+// Y = A X, X and Y row-major (ld >= k); with slices, narrow products run
+// as one pass per column slice (gathers confined to a slice of X).
 int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,
-                      int n, const double *X, long ldx, double *Y, long ldy, int k);
+                      int n, const double *X, long ldx, double *Y, long ldy, int k,
+                      const CsrSlices *slices = nullptr);
+// the same product in exactly ns passes (a power of two dividing
+// slices.nslices; 1: the whole matrix in one pass)
+int spmm_csr_sliced_passes(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals, int n,
+                           const double *X, long ldx, double *Y, long ldy, int k, const CsrSlices &slices, int ns);
 
 // A <- (A + A^T)/2 for a square column-major n x n (ld) matrix
 // (IPT/src/mixed_precision/ipt_rayleigh_mixed_cuda.cu:84-106).

@@ -228,12 +228,16 @@ int gather_diagonal(Context *ctx, const double *Y, int k, long rs, long cs, doub
 // contiguous rows of X, and the unrolled steps keep several gathers in
 // flight per lane -- the gathers, not the compulsory bytes, bound this
 // kernel when X (n x k FP64) is larger than the L2.
+//
+// Row r's entries are [rs[r], re[r]): rs = row_ptr, re = row_ptr + 1 for
+// the whole matrix, or the bounds of one column slice (CsrSlices); with
+// accumulate the products are added to Y.
 template <int S>
-__global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ rp,
+__global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ rs, const int *__restrict__ re,
                                                        const int *__restrict__ ci,
                                                        const double *__restrict__ vals, int n,
                                                        const double *__restrict__ X, long ldx,
-                                                       double *__restrict__ Y, long ldy, int k)
+                                                       double *__restrict__ Y, long ldy, int k, int accumulate)
 {
     constexpr int G = 64 / S;
     constexpr int STEPS = 64 / G;
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ r
     const long wave_global = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long waves = ((long)gridDim.x * blockDim.x) >> 6;
     for (long row = wave_global; row < n; row += waves) {
-        const int pb = rp[row], pe = rp[row + 1];
+        const int pb = rs[row], pe = re[row];
         for (int c0 = 0; c0 < k; c0 += S) {
             const int col = c0 + s;
             const bool active = col < k;
@@ -281,36 +285,123 @@ __global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ r
                 acc += __shfl_xor(acc, off, 64);
             }
             if (g == 0 && active) {
-                Y[row * ldy + col] = acc;
+                if (accumulate) {
+                    Y[row * ldy + col] += acc;
+                } else {
+                    Y[row * ldy + col] = acc;
+                }
             }
         }
     }
 }
 
-int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,
-                      int n, const double *X, long ldx, double *Y, long ldy, int k)
+static void launch_spmm(hipStream_t st, const int *rs, const int *re, const int *col_ind, const double *vals,
+                        int n, const double *X, long ldx, double *Y, long ldy, int k, int accumulate)
 {
     const int blocks = blocks_for((long)n * 64, 256, 65536);
     dim3 g(blocks), b(256);
-    const ProfileToken prof = profile_begin(ctx, PROF_SPMM);
     if (k <= 1) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<1>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+        hipLaunchKernelGGL(k_spmm_rowmajor<1>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
     } else if (k <= 2) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<2>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+        hipLaunchKernelGGL(k_spmm_rowmajor<2>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
     } else if (k <= 4) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<4>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+        hipLaunchKernelGGL(k_spmm_rowmajor<4>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
     } else if (k <= 8) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<8>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+        hipLaunchKernelGGL(k_spmm_rowmajor<8>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
     } else if (k <= 16) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<16>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+        hipLaunchKernelGGL(k_spmm_rowmajor<16>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
     } else if (k <= 32) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<32>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+        hipLaunchKernelGGL(k_spmm_rowmajor<32>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
     } else {
-        hipLaunchKernelGGL(k_spmm_rowmajor<64>, g, b, 0, ctx->stream, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k);
+        hipLaunchKernelGGL(k_spmm_rowmajor<64>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+    }
+}
+
+int spmm_csr_sliced_passes(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals, int n,
+                           const double *X, long ldx, double *Y, long ldy, int k, const CsrSlices &slices, int ns)
+{
+    const ProfileToken prof = profile_begin(ctx, PROF_SPMM);
+    if (ns > 1) {
+        // column slices of the same CSR, one pass each: a pass gathers
+        // rows of X from one slice (an L2-sized part of X), the later
+        // passes add to Y
+        const int *sp = slices.split.p;
+        const long stride = (long)slices.n;
+        const int step = slices.nslices / ns;
+        for (int b = 0; b < ns; ++b) {
+            launch_spmm(ctx->stream, sp + (long)(b * step) * stride, sp + (long)((b + 1) * step) * stride, col_ind,
+                        vals, n, X, ldx, Y, ldy, k, b > 0);
+        }
+    } else {
+        launch_spmm(ctx->stream, row_ptr, row_ptr + 1, col_ind, vals, n, X, ldx, Y, ldy, k, 0);
     }
     profile_end(ctx, prof, k);
     IPT_HIPCHK(hipGetLastError());
     return IPT_CUDA_SUCCESS;
+}
+
+int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,
+                      int n, const double *X, long ldx, double *Y, long ldy, int k, const CsrSlices *slices)
+{
+    if (slices == nullptr) {
+        const CsrSlices none;
+        return spmm_csr_sliced_passes(ctx, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k, none, 1);
+    }
+    return spmm_csr_sliced_passes(ctx, row_ptr, col_ind, vals, n, X, ldx, Y, ldy, k, *slices,
+                                  csr_slices_for(*slices, k));
+}
+
+// split[b n + r] = first entry of row r whose column is >= b * width
+// (width = ceil(n / nslices)), split[0] = row_ptr, split[nslices] =
+// row_ptr + 1; columns ascend within each row.  One thread per row.
+__global__ __launch_bounds__(256) void k_csr_split(const int *__restrict__ rp, const int *__restrict__ ci, int n,
+                                                   int nslices, int width, int *__restrict__ split)
+{
+    for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        const int pb = rp[r], pe = rp[r + 1];
+        split[r] = pb;
+        split[(long)nslices * n + r] = pe;
+        int lo = pb;
+        for (int b = 1; b < nslices; ++b) {
+            const int c = b * width;
+            int hi = pe;
+            while (lo < hi) {  // first index in [lo, pe) with ci >= c
+                const int mid = (lo + hi) >> 1;
+                if (ci[mid] < c) {
+                    lo = mid + 1;
+                } else {
+                    hi = mid;
+                }
+            }
+            split[(long)b * n + r] = lo;
+        }
+    }
+}
+
+int csr_slices_create(Context *ctx, const int *row_ptr, const int *col_ind, int n, CsrSlices *sl)
+{
+    sl->n = n;
+    sl->nslices = kCsrMaxSlices;
+    IPT_STCHK(sl->split.alloc((size_t)(kCsrMaxSlices + 1) * n));
+    const int width = (n + kCsrMaxSlices - 1) / kCsrMaxSlices;
+    hipLaunchKernelGGL(k_csr_split, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, ctx->stream, row_ptr, col_ind, n,
+                       kCsrMaxSlices, width, sl->split.p);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+int csr_slices_for(const CsrSlices &sl, int k)
+{
+    // k <= 8 and X beyond 2 MiB: 4 or 8 passes, at most about 4 MiB of X
+    // per pass (an XCD's L2) where 8 passes allow it.  Measured on the
+    // N = 1e6 orbital model (200 draws per row): the best counts were 4 at
+    // k = 1 and 2, 8 at k = 4 and 8 (16 passes were slower at every width:
+    // each pass re-reads Y and the row bounds)
+    const long xbytes = (long)sl.n * k * 8;
+    if (k > 8 || xbytes <= (2L << 20) || sl.nslices < 8) {
+        return 1;
+    }
+    return xbytes > (16L << 20) ? 8 : 4;
 }
 
 // ------------------------------------------------------------------ layouts

@@ -233,3 +233,29 @@ extern "C" int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, cons
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     return IPT_CUDA_SUCCESS;
 }
+
+extern "C" int ipt_hip_csr_spmm_sliced(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n,
+                                       const double *d_X, int ldx, double *d_Y, int ldy, int k, int slices,
+                                       void *handle)
+{
+    if (d_row_ptr == nullptr || d_col_ind == nullptr || d_values == nullptr || d_X == nullptr || d_Y == nullptr ||
+        n <= 0 || k <= 0 || ldx < k || ldy < k || slices < 0 || slices > kCsrMaxSlices ||
+        (slices & (slices - 1)) != 0) {
+        return IPT_CUDA_INVALID_ARGUMENT;
+    }
+    int st = IPT_CUDA_SUCCESS;
+    Context *ctx = context_from_handle(handle, &st);
+    if (ctx == nullptr) {
+        return st;
+    }
+    CsrSlices sl;
+    IPT_STCHK(csr_slices_create(ctx, d_row_ptr, d_col_ind, n, &sl));
+    if (slices > 0) {
+        // a fixed pass count: the slice table coarsened to `slices` passes
+        IPT_STCHK(spmm_csr_sliced_passes(ctx, d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, sl, slices));
+    } else {
+        IPT_STCHK(spmm_csr_rowmajor(ctx, d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, &sl));
+    }
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
+}

@@ -56,6 +56,15 @@ int ipt_hip_generate_orbital_csr(int n, int m, unsigned long long seed, int norb
 int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n, const double *d_X,
                      int ldx, double *d_Y, int ldy, int k, void *handle);
 
+/* The same product for a CSR whose column indices ascend within each row,
+ * run as column-sliced passes: pass b multiplies the entries with columns
+ * in one slice of [0, n) and adds to Y, so its gathers stay within that
+ * slice of X.  slices = 0: the pass count the block path uses for width k
+ * (1 for k > 8 or X of at most 2 MiB, else 4 passes up to 16 MiB of X
+ * and 8 beyond); 1, 2, 4, 8, 16: that many passes. */
+int ipt_hip_csr_spmm_sliced(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n,
+                            const double *d_X, int ldx, double *d_Y, int ldy, int k, int slices, void *handle);
+
 /* In place: Q (n x k, column-major, leading dimension n, k <= n) <- the
  * orthonormal factor of its QR factorisation (positive diagonal R), by
  * shifted Cholesky QR 3 in FP64 on the device: columns normalised, one
