@@ -1653,6 +1653,7 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
         int nnz = 0;
         IPT_HIPCHK(hipMemcpy(&nnz, d_rp + n, sizeof(int), hipMemcpyDeviceToHost));
         IPT_STCHK(sparse_lowp_create(ctx, d_rp, d_ci, d_v, n, nnz, &slp));
+        slp.slices = slices.nslices > 0 ? &slices : nullptr;
         delta = [&](const double *B, const double *Ba, const double *Ya, int nc, double *Y) {
             return sparse_lowp_apply_delta(ctx, slp, B, Ba, Ya, n, d.p, nc, Y);
         };
@@ -1835,6 +1836,7 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         DeltaFn delta;
         if (p.lowp_mode >= 0) {
             IPT_STCHK(sparse_lowp_create(ctx, rp.p, ci.p, vv.p, n, pn, &slp));
+            slp.slices = slices.nslices > 0 ? &slices : nullptr;
             delta = [&](const double *B, const double *Ba, const double *Ya, int nc, double *Y) {
                 return sparse_lowp_apply_delta(ctx, slp, B, Ba, Ya, n, dd.p, nc, Y);
             };

@@ -11,6 +11,7 @@ struct SparseLowp {
     const int *ci = nullptr;
     int n = 0;
     DevBuf<float> vals;       // FP32 copy of the values
+    const CsrSlices *slices = nullptr;  // column slices of the same CSR (narrow products), or none
 };
 
 int sparse_lowp_create(Context *ctx, const int *rp, const int *ci, const double *vals, int n, long long nnz,

@@ -52,11 +52,15 @@ __global__ __launch_bounds__(256) void k_delta_rows_f32(const double *__restrict
 
 // Z (row-major FP32, ld w) = E D: a wavefront per row, S lanes per column
 // group, the row's (column, value) pairs read 64 at a time and handed out
-// by shuffles (as k_spmm_rowmajor); the diagonal entry is skipped.
+// by shuffles (as k_spmm_rowmajor); the diagonal entry is skipped.  Row r's
+// entries are [rs[r], re[r]) (the whole row, or one column slice; with
+// accumulate the slice's products are added to Z).
 template <int S>
-__global__ __launch_bounds__(256) void k_spmm_offdiag_f32(const int *__restrict__ rp, const int *__restrict__ ci,
+__global__ __launch_bounds__(256) void k_spmm_offdiag_f32(const int *__restrict__ rs, const int *__restrict__ re,
+                                                          const int *__restrict__ ci,
                                                           const float *__restrict__ vals, int n,
-                                                          const float *__restrict__ X, int w, float *__restrict__ Z)
+                                                          const float *__restrict__ X, int w, float *__restrict__ Z,
+                                                          int accumulate)
 {
     constexpr int G = 64 / S;
     constexpr int STEPS = 64 / G;
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(256) void k_spmm_offdiag_f32(const int *__restrict_
     const long wave_global = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long waves = ((long)gridDim.x * blockDim.x) >> 6;
     for (long row = wave_global; row < n; row += waves) {
-        const int pb = rp[row], pe = rp[row + 1];
+        const int pb = rs[row], pe = re[row];
         for (int c0 = 0; c0 < w; c0 += S) {
             const int col = c0 + s;
             const bool active = col < w;
@@ -106,7 +110,11 @@ __global__ __launch_bounds__(256) void k_spmm_offdiag_f32(const int *__restrict_
                 acc += __shfl_xor(acc, off, 64);
             }
             if (g == 0 && active) {
-                Z[row * w + col] = acc;
+                if (accumulate) {
+                    Z[row * w + col] += acc;
+                } else {
+                    Z[row * w + col] = acc;
+                }
             }
         }
     }
@@ -136,10 +144,30 @@ __global__ __launch_bounds__(256) void k_delta_epilogue(const float *__restrict_
 }
 
 template <int S>
-void launch_spmm_f32(dim3 g, hipStream_t st, const int *rp, const int *ci, const float *v, int n, const float *X,
-                     int w, float *Z)
+void launch_spmm_f32(dim3 g, hipStream_t st, const int *rs, const int *re, const int *ci, const float *v, int n,
+                     const float *X, int w, float *Z, int accumulate)
 {
-    hipLaunchKernelGGL(k_spmm_offdiag_f32<S>, g, dim3(256), 0, st, rp, ci, v, n, X, w, Z);
+    hipLaunchKernelGGL(k_spmm_offdiag_f32<S>, g, dim3(256), 0, st, rs, re, ci, v, n, X, w, Z, accumulate);
+}
+
+void spmm_f32_pass(dim3 g, hipStream_t st, const int *rs, const int *re, const int *ci, const float *v, int n,
+                   const float *X, int w, float *Z, int accumulate)
+{
+    if (w <= 1) {
+        launch_spmm_f32<1>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    } else if (w <= 2) {
+        launch_spmm_f32<2>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    } else if (w <= 4) {
+        launch_spmm_f32<4>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    } else if (w <= 8) {
+        launch_spmm_f32<8>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    } else if (w <= 16) {
+        launch_spmm_f32<16>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    } else if (w <= 32) {
+        launch_spmm_f32<32>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    } else {
+        launch_spmm_f32<64>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    }
 }
 
 }  // namespace
@@ -175,14 +203,18 @@ int sparse_lowp_apply_delta(Context *ctx, const SparseLowp &op, const double *B,
         hipLaunchKernelGGL(k_delta_rows_f32, tiles, dim3(256), 0, ctx->stream, B + o, Ba + o, ld, n, w, D);
         IPT_HIPCHK(hipGetLastError());
         const ProfileToken prof = profile_begin(ctx, PROF_SPMM_F32);
-        if (w <= 8) {
-            launch_spmm_f32<8>(g, ctx->stream, op.rp, op.ci, op.vals.p, n, D, w, Z);
-        } else if (w <= 16) {
-            launch_spmm_f32<16>(g, ctx->stream, op.rp, op.ci, op.vals.p, n, D, w, Z);
-        } else if (w <= 32) {
-            launch_spmm_f32<32>(g, ctx->stream, op.rp, op.ci, op.vals.p, n, D, w, Z);
+        // FP32 rows of X: the pass count of an FP64 product half as wide
+        const int ns = op.slices != nullptr ? csr_slices_for(*op.slices, (w + 1) / 2) : 1;
+        if (ns > 1) {
+            const int *sp = op.slices->split.p;
+            const long stride = (long)op.slices->n;
+            const int step = op.slices->nslices / ns;
+            for (int b = 0; b < ns; ++b) {
+                spmm_f32_pass(g, ctx->stream, sp + (long)(b * step) * stride, sp + (long)((b + 1) * step) * stride,
+                              op.ci, op.vals.p, n, D, w, Z, b > 0);
+            }
         } else {
-            launch_spmm_f32<64>(g, ctx->stream, op.rp, op.ci, op.vals.p, n, D, w, Z);
+            spmm_f32_pass(g, ctx->stream, op.rp, op.rp + 1, op.ci, op.vals.p, n, D, w, Z, 0);
         }
         profile_end(ctx, prof, w);
         IPT_HIPCHK(hipGetLastError());

@@ -174,11 +174,13 @@ def test_device_sparse_block_nonsymmetric_rejected(block_env, dev):
         _solve_device(dev, rp, ci, v, n, 4)
 
 
-@pytest.mark.parametrize("n,m,k,seed", [(3000, 20, 8, 1), (20000, 40, 32, 2)])
+@pytest.mark.parametrize("n,m,k,seed", [(3000, 20, 8, 1), (20000, 40, 32, 2), (300000, 16, 8, 3)])
 def test_sparse_fp32_anchored(block_env, dev, n, m, k, seed):
     """IPT_PRECISION on CSR input: FP32 correction products anchored by FP64
     ones (the dense path's scheme); the pairs equal the FP64 block solve's
-    and the solve used low-precision products."""
+    and the solve used low-precision products.  At n = 300000 both the FP64
+    and the FP32 products of at most 8 columns run in column-sliced
+    passes."""
     rp, ci, v = orc.orbital_csr(n, m, seed)
     vec64, val64, _, _ = _solve_device(dev, rp, ci, v, n, k)
     st64 = api.last_solve_stats()
