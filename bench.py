@@ -26,6 +26,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 MFMA
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per k_lowp_gemm launch measured with rocprofv3 FETCH_SIZE and
@@ -50,8 +51,8 @@ def parse():
     ap.add_argument("--block-tol", type=float, default=1e-8)
     ap.add_argument("--fp64-steps", type=int, default=2, help="timed solves of the plain FP64 path (0: skip)")
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-sample-iters", type=int, default=2,
-                    help="CPU baseline: iterations of the one-thread sample")
+    ap.add_argument("--cpu-sample-iters", type=int, default=400,
+                    help="CPU baseline: iteration cap of the one-thread complete solve")
     ap.add_argument("--workload", default="dense", choices=["dense", "sparse", "products", "acx", "many", "spmm"],
                     help="dense: the headline solve; sparse: the N=1e6 orbital-sum CSR solve and its SpMM; "
                          "products: N=65536 k=256 correction products by precision; "
@@ -140,12 +141,13 @@ def cpu_model():
 def cpu_baseline(n, seed, sample_k, one_core_iters):
     """The oracle (restatement of the reference's CPU path: block-cluster IPT
     with per-cluster products, Cholesky Rayleigh-Ritz) on the same matrix,
-    measured: a complete solve of the `sample_k` lowest pairs (the first
-    cluster of the workload) on all host threads, converged pairs counted
-    with the FP64 residual; and on one thread, `one_core_iters` iterations of
-    the same solve timed and scaled to the iteration count of the complete
-    one (the per-iteration cost is constant: the same cluster, product and
-    update each iteration)."""
+    measured, not extrapolated: a complete solve of the `sample_k` lowest
+    pairs (the first cluster of the workload) on all host threads, and on
+    one thread a complete solve of the lowest target pair that forms a
+    cluster of its own (block IPT of that cluster to the same tolerance,
+    then its Rayleigh-Ritz value); converged pairs counted with the FP64
+    residual.  `one_core_iters` caps the one-thread iterations (a time
+    bound: a run stopped by it counts no pair it did not converge)."""
     from oracle import oracle as orc
 
     H = orc.rand_dense(n, seed, 0.0, 50.0, 0.1)
@@ -157,13 +159,23 @@ def cpu_baseline(n, seed, sample_k, one_core_iters):
     res = residuals_host(H, vec, val, normH)
     conv = int(np.sum(res <= 1e-10))
     iters = info["max_iterations"]
-    orc.set_num_threads(1)
-    try:
-        t0 = time.perf_counter()
-        orc.block_pipeline(sample_k, dense=H, maxiter=one_core_iters, tol=1e-12)
-        t_one = (time.perf_counter() - t0) / one_core_iters * iters
-    finally:
-        orc.set_num_threads(threads)
+    ranges, _ = orc.target_clusters(64, dense=H)
+    singles = [r for r in ranges if r[0] == r[1]]
+    one = {"value": None, "sample": "no target pair forms a cluster of its own"}
+    if singles:
+        f = singles[0][0]
+        orc.set_num_threads(1)
+        try:
+            t0 = time.perf_counter()
+            B, it1, _ = orc.block_cluster_solve(f, f, max(1, one_core_iters), 1e-12, dense=H)
+            v1, x1 = orc.rayleigh_ritz(B, 1, dense=H)
+            t_one = time.perf_counter() - t0
+        finally:
+            orc.set_num_threads(threads)
+        conv1 = int(np.sum(residuals_host(H, x1, v1, normH) <= 1e-10))
+        one = {"value": conv1 / t_one, "seconds": t_one, "converged_pairs": conv1,
+               "sample": "the target pair of sorted row %d (a cluster of its own) solved completely on 1 thread: "
+                         "%d iterations, %.2f s, %d pair <= 1e-10" % (f, it1, t_one, conv1)}
     return {
         "value": conv / t_all,
         "unit": "eigenpairs/s",
@@ -172,9 +184,7 @@ def cpu_baseline(n, seed, sample_k, one_core_iters):
         "sample": "oracle block-cluster IPT + Cholesky RR, the %d lowest pairs (one cluster) of the same N=%d "
                   "matrix solved completely on %d threads: %d iterations, %.2f s, %d pairs <= 1e-10"
                   % (sample_k, n, threads, iters, t_all, conv),
-        "one_core": {"value": conv / t_one, "seconds": t_one,
-                     "sample": "%d iterations on 1 thread timed, scaled to the %d of the complete solve"
-                               % (one_core_iters, iters)},
+        "one_core": one,
         "nproc": os.cpu_count(),
         "cpu_model": cpu_model(),
     }
@@ -595,12 +605,24 @@ def main():
         solver_env("fp64", a.anchor_every, a.block_tol)
         _, dv_w, _ = run_solves(api, hip, dH, n, k, 1, maxiter)  # warm-up
         api.free_device_result(dv_w, 0)
+        # events around the products, as in the timed mixed solves below
+        api.profile_reset()
+        api.profile_enable(True, only=["gemm_f64"])
         t, dv, vals64 = run_solves(api, hip, dH, n, k, a.fp64_steps, maxiter)
+        api.profile_enable(False)
+        g_ms, g_launches = api.profile_get("gemm_f64")
+        g_cols = api.profile_columns("gemm_f64") / max(1, g_launches)
         X64 = hip.to_host(dv, (n, k))
         api.free_device_result(dv, 0)
         st64 = api.last_solve_stats()
         fp64_iters = st64["steps"]
         fp64 = {"ms_per_solve": 1e3 * t / a.fp64_steps, "stats": st64, "X": X64, "vals": vals64}
+        if g_launches > 0:
+            avg = g_ms / g_launches
+            tf = 2.0 * n * n * g_cols / (avg * 1e-3) / 1e12
+            fp64["gemm"] = {"kernel": "k_gemm_f64_packed" if n >= 4096 else "k_gemm_f64", "avg_ms": avg,
+                            "launches": g_launches, "avg_columns": g_cols, "achieved_tflops": tf,
+                            "peak_tflops": FP64_PEAK_TFS, "frac": tf / FP64_PEAK_TFS}
 
     # ---------------- mixed path (timed)
     solver_env(a.precision, a.anchor_every, a.block_tol)
@@ -619,6 +641,18 @@ def main():
     f64_cols = api.profile_columns("gemm_f64") / max(1, f64_launches)
     stats = api.last_solve_stats()
     ms_per_step = 1e3 * t / a.steps
+
+    # ---------------- the C2 configuration's other precision, BF16x3 (a key of its own)
+    x3 = None
+    if a.extra and a.precision != "bf16x3":
+        solver_env("bf16x3", a.anchor_every, a.block_tol)
+        _, dvw, _ = run_solves(api, hip, dH, n, k, 1, maxiter)
+        api.free_device_result(dvw, 0)
+        t3, dv3, vals3 = run_solves(api, hip, dH, n, k, max(1, min(a.steps, 3)), maxiter)
+        x3 = {"ms_per_solve": 1e3 * t3 / max(1, min(a.steps, 3)), "X": hip.to_host(dv3, (n, k)), "vals": vals3,
+              "stats": api.last_solve_stats()}
+        api.free_device_result(dv3, 0)
+        solver_env(a.precision, a.anchor_every, a.block_tol)
 
     # ---------------- independent FP64 verification (outside the timed region)
     X = hip.to_host(dv, (n, k))
@@ -640,6 +674,12 @@ def main():
     if fp64 is not None:
         res64 = residuals_host(H, fp64.pop("X"), fp64.pop("vals"), normH)
         converged64 = int(np.sum(res64 <= 1e-10))
+    if x3 is not None:
+        res3 = residuals_host(H, x3.pop("X"), x3["vals"], normH)
+        x3["converged_pairs"] = int(np.sum(res3 <= 1e-10))
+        x3["max_residual"] = float(res3.max())
+        x3["max_abs_diff_vs_fp16_values"] = float(np.abs(np.sort(x3.pop("vals")) - np.sort(vals)).max())
+        x3["pairs_per_sec"] = x3["converged_pairs"] / (x3["ms_per_solve"] / 1e3)
     del H
     value = converged / (ms_per_step / 1e3)
 
@@ -686,8 +726,12 @@ def main():
     if fp64 is not None:
         fp64_value = converged64 / (fp64["ms_per_solve"] / 1e3)
         out["fp64_path"] = {"pairs_per_sec": fp64_value, "ms_per_solve": fp64["ms_per_solve"],
-                            "converged_pairs": converged64, "stats": fp64["stats"]}
+                            "converged_pairs": converged64, "stats": fp64["stats"], "gemm": fp64.get("gemm")}
         out["mixed_vs_fp64"] = value / fp64_value
+    if x3 is not None:
+        if fp64 is not None:
+            x3["vs_fp64"] = x3["pairs_per_sec"] / out["fp64_path"]["pairs_per_sec"]
+        out["bf16x3"] = x3
     if a.extra:
         # the other two solver workloads, one timed solve each (keys only;
         # `value` stays the dense N=16384 measurement above)

@@ -716,7 +716,8 @@ void record_stats(Context *ctx, const BlockResult &br)
 int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_product, const DeltaFn *lowp,
                    const double *d, const double *rowabs, int n, int k,
                    const std::vector<std::pair<int, int>> &ranges, double normH, const BlockParams &p,
-                   BlockResult *out, bool sparse_products = false, const LowpOperator *dense_op = nullptr)
+                   BlockResult *out, bool sparse_products = false, const LowpOperator *dense_op = nullptr,
+                   const std::function<int()> &before_davidson = std::function<int()>())
 {
     BlockPlan plan;
     // with stall detection on, clusters wider than the block limit are not
@@ -1090,7 +1091,17 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         if (p.precond_mlp && rowabs != nullptr) {
             IPT_STCHK(load_mlp_model(p.mlp_model, &mlp));
         }
-        IPT_STCHK(davidson(ctx, apply, d, rowabs, mlp.p, n, k, normH, Ba.p, Y.p, &cols, cap, out->theta.p, out->X.p,
+        // before_davidson runs once, at Davidson's first product (a solve
+        // whose Rayleigh-Ritz pairs have converged makes none)
+        bool dav_started = false;
+        const ApplyFn dav_apply = [&](const double *X, double *Y, int nc) -> int {
+            if (!dav_started && before_davidson) {
+                dav_started = true;
+                IPT_STCHK(before_davidson());
+            }
+            return apply(X, Y, nc);
+        };
+        IPT_STCHK(davidson(ctx, dav_apply, d, rowabs, mlp.p, n, k, normH, Ba.p, Y.p, &cols, cap, out->theta.p, out->X.p,
                            out->HX.p, p, &out->dav, rro, &out->rr));
         out->matvecs += out->dav.matvecs;
         out->basis_cols = out->dav.basis_cols;
@@ -1496,9 +1507,18 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     // each of them stream contiguously; one pass over H, repaid within a
     // few products.
     F64Packed packed;
-    if (p.lowp_mode < 0 && n >= 4096 && env_int("IPT_F64_PACKED", 1) != 0) {
+    const bool pack_ok = n >= 4096 && env_int("IPT_F64_PACKED", 1) != 0;
+    if (p.lowp_mode < 0 && pack_ok) {
         IPT_STCHK(f64_packed_create(ctx, H, n, n, &packed));
     }
+    // the mixed path's FP64 products are few until Davidson, whose
+    // corrections are all FP64 products: the copy is made when it starts
+    const std::function<int()> before_davidson = [&]() -> int {
+        if (packed.p.p == nullptr && pack_ok) {
+            return f64_packed_create(ctx, H, n, n, &packed);
+        }
+        return IPT_CUDA_SUCCESS;
+    };
     ApplyFn apply = [&](const double *X, double *Y, int nc) {
         if (packed.p.p != nullptr) {
             return gemm_f64_packed(ctx, packed, X, n, Y, n, nc);
@@ -1522,7 +1542,7 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     };
     BlockResult br;
     IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp != nullptr ? &delta : nullptr, pr.d.p, pr.rowabs.p, n,
-                             k, pr.ranges, pr.normH, p, &br, false, lp));
+                             k, pr.ranges, pr.normH, p, &br, false, lp, before_davidson));
     record_stats(ctx, br);
     if (p.verbose) {
         fprintf(stderr,
