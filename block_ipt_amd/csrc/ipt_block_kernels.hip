@@ -1441,8 +1441,12 @@ __global__ __launch_bounds__(256) void k_colsum_reduce(const double *__restrict_
 int dense_scan(Context *ctx, const double *H, long ld, int n, double *colsum, double *colmax, int *asym)
 {
     const int nb = (n + 63) / 64;
-    double *part = nullptr;
-    IPT_STCHK(context_scratch(ctx, SLOT_SOLVER, (size_t)2 * nb * n * sizeof(double), (void **)&part));
+    // 2 nb n doubles (n^2 / 4 bytes: 1 GiB at n = 65536), needed only here:
+    // a pooled buffer, returned at exit (and trimmed under memory pressure)
+    // instead of a scratch slot held for the context's lifetime
+    DevBuf<double> partbuf;
+    IPT_STCHK(partbuf.alloc((size_t)2 * nb * n));
+    double *part = partbuf.p;
     IPT_HIPCHK(hipMemsetAsync(asym, 0, sizeof(int), ctx->stream));
     const long pairs = (long)nb * (nb + 1) / 2;
     hipLaunchKernelGGL(k_dense_scan, dim3((unsigned)pairs), dim3(256), 0, ctx->stream, H, ld, n, nb, part,
