@@ -43,6 +43,7 @@ double seconds_since(const std::chrono::steady_clock::time_point &t0)
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+
 // Y(:, 0:nc) = H X(:, 0:nc), both column-major with leading dimension n.
 using ApplyFn = std::function<int(const double *X, double *Y, int nc)>;
 // Y = Ya + diag(d) (B - Ba) + E (B - Ba) with E (B - Ba) in low precision
@@ -100,11 +101,44 @@ int load_mlp_model(const std::string &path, DevBuf<float> *out)
     return IPT_CUDA_SUCCESS;
 }
 
+// Synchronous device-to-host reads of several arrays with one stream
+// synchronisation, staged through pinned memory (a copy into pageable
+// memory is staged by the runtime and waits on its own): the context's
+// pinned scalars up to 32 KiB in all, else its pinned staging buffer (which
+// must not hold data of a caller then).
+struct HostRead {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+
+int read_host_many(Context *ctx, const std::vector<HostRead> &reads)
+{
+    size_t total = 0;
+    for (const auto &r : reads) {
+        total += (r.bytes + 7) / 8 * 8;
+    }
+    char *stage = reinterpret_cast<char *>(ctx->host_scalars);
+    if (total > 4096 * sizeof(double)) {
+        IPT_STCHK(context_host_pinned(ctx, total, (void **)&stage));
+    }
+    size_t o = 0;
+    for (const auto &r : reads) {
+        IPT_HIPCHK(hipMemcpyAsync(stage + o, r.src, r.bytes, hipMemcpyDeviceToHost, ctx->stream));
+        o += (r.bytes + 7) / 8 * 8;
+    }
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    o = 0;
+    for (const auto &r : reads) {
+        memcpy(r.dst, stage + o, r.bytes);
+        o += (r.bytes + 7) / 8 * 8;
+    }
+    return IPT_CUDA_SUCCESS;
+}
+
 int read_host(Context *ctx, void *dst, const void *src, size_t bytes)
 {
-    IPT_HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    return IPT_CUDA_SUCCESS;
+    return read_host_many(ctx, {HostRead{dst, src, bytes}});
 }
 
 // Internal status of rayleigh_ritz: the basis has numerical rank below k.
@@ -203,6 +237,7 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
         double dev = NAN;
         int applied = 0;
         IPT_STCHK(gram_inverse_sqrt(ctx, G.p, cols, 0.25, S.p, &dev, &applied));
+        trace_mark("Gram inverse square root enqueued");
         if (!applied) {
             IPT_STCHK(sym_eig_device(ctx, G.p, cols, cols, gw.p, W.p, cols, nullptr));
             std::vector<double> hg((size_t)cols);
@@ -247,6 +282,7 @@ int rayleigh_ritz(Context *ctx, const double *V, const double *HV, int n, int co
     IPT_STCHK(symmetrize(ctx, A.p, r, r));
     IPT_STCHK(Z.alloc((size_t)r * r));
     IPT_STCHK(sym_eig_device(ctx, A.p, r, r, theta, Z.p, r, nullptr));
+    trace_mark("projected eigensolve enqueued");
     if (r < cols) {
         std::vector<double> inf_fill((size_t)(cols - r), INFINITY);
         IPT_HIPCHK(hipMemcpyAsync(theta + r, inf_fill.data(), inf_fill.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -415,7 +451,9 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
     int diag_hold = 0;  // steps left on the diagonal correction after a rejected learned step
 
     std::vector<double> htheta((size_t)k), hres((size_t)k);
-    IPT_STCHK(read_host(ctx, htheta.data(), theta, (size_t)k * sizeof(double)));
+    IPT_STCHK(residual_norms(ctx, X, HX, n, n, k, theta, normH, res.p, R.p));
+    IPT_STCHK(read_host_many(ctx, {HostRead{htheta.data(), theta, (size_t)k * sizeof(double)},
+                                   HostRead{hres.data(), res.p, (size_t)k * sizeof(double)}}));
     {
         double scale = std::max(1.0, std::fabs(normH));
         for (double th : htheta) {
@@ -428,8 +466,6 @@ int davidson(Context *ctx, const ApplyFn &apply, const double *d, const double *
             st->clip = std::pow(DBL_EPSILON, 2.0 / 3.0);
         }
     }
-    IPT_STCHK(residual_norms(ctx, X, HX, n, n, k, theta, normH, res.p, R.p));
-    IPT_STCHK(read_host(ctx, hres.data(), res.p, (size_t)k * sizeof(double)));
     st->residual_before = max_of(hres, &st->target_index);
 
     // best-so-far bookkeeping: the best state is either the current one
@@ -722,7 +758,9 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     BlockPlan plan;
     // with stall detection on, clusters wider than the block limit are not
     // an error: they are never iterated and Davidson refines their pairs
+    trace_mark("pipeline start");
     IPT_STCHK(block_plan_create(ctx, ranges, n, &plan, p.stall_steps > 0 ? p.max_block : 0));
+    trace_mark("plan created");
     const int bc = plan.basis_cols;
     if (bc < k) {
         return IPT_CUDA_INVALID_ARGUMENT;
@@ -772,15 +810,12 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             IPT_HIPCHK(hipMemsetAsync(zlo.p, 0, zel * sizeof(unsigned short), ctx->stream));
         }
         IPT_STCHK(colscale.alloc((size_t)nb_all * 16));
-        const std::vector<double> ones((size_t)nb_all * 16, 1.0);
-        IPT_HIPCHK(hipMemcpyAsync(colscale.p, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice,
-                                  ctx->stream));
+        IPT_STCHK(fill_device(ctx, colscale.p, (long)nb_all * 16, 1.0));
         IPT_STCHK(plan.cmax.alloc((size_t)2 * plan.nclusters));
         IPT_HIPCHK(hipMemsetAsync(plan.cmax.p, 0, (size_t)2 * plan.nclusters * sizeof(double), ctx->stream));
-        // the host vector above must outlive the copy
-        IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     plan.frozen.assign((size_t)plan.nclusters, 0);
+    trace_mark("iteration set up");
     double *cur = Ba.p, *nxt = Bb.p;
     auto t0 = std::chrono::steady_clock::now();
     // Exact steps are scheduled: at most anchor_every low-precision steps
@@ -920,6 +955,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
             // one read-back per exact step (the finalisation's status block)
             std::vector<double> stv((size_t)2 + 2 * plan.nclusters);
             IPT_STCHK(read_host(ctx, stv.data(), plan.status.p, stv.size() * sizeof(double)));
+            trace_mark("exact step status on the host");
             flags[0] = 0;
             flags[1] = (int)stv[0];
             flags[2] = (int)stv[1];
@@ -1007,12 +1043,10 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     {
         std::vector<int> its((size_t)plan.nclusters), stl((size_t)plan.nclusters), live((size_t)plan.nclusters);
         std::vector<double> fps((size_t)plan.nclusters);
-        IPT_HIPCHK(hipMemcpyAsync(stl.data(), plan.stalled.p, stl.size() * sizeof(int), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-        IPT_HIPCHK(hipMemcpyAsync(live.data(), plan.active.p, live.size() * sizeof(int), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-        IPT_STCHK(read_host(ctx, its.data(), plan.iters.p, its.size() * sizeof(int)));
-        IPT_STCHK(read_host(ctx, fps.data(), plan.fp.p, fps.size() * sizeof(double)));
+        IPT_STCHK(read_host_many(ctx, {HostRead{stl.data(), plan.stalled.p, stl.size() * sizeof(int)},
+                                       HostRead{live.data(), plan.active.p, live.size() * sizeof(int)},
+                                       HostRead{its.data(), plan.iters.p, its.size() * sizeof(int)},
+                                       HostRead{fps.data(), plan.fp.p, fps.size() * sizeof(double)}}));
         std::vector<int> reset;
         ctx->last_cluster_state.assign((size_t)plan.nclusters, 0);
         for (int c = 0; c < plan.nclusters; ++c) {
@@ -1036,6 +1070,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         }
         IPT_STCHK(block_identity_clusters(ctx, plan, reset, cur, n, n));
     }
+    trace_mark("iteration done");
     out->iteration_time = seconds_since(t0);
     // the basis lives in `cur`; Davidson needs it in the big buffer Ba
     if (cur != Ba.p) {
@@ -1044,6 +1079,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
     Bb.release();
     auto t1 = std::chrono::steady_clock::now();
     IPT_STCHK(apply(Ba.p, Y.p, bc));
+    trace_mark("final product enqueued");
     out->matvecs += bc;
     out->fp64_products += 1;
     IPT_STCHK(out->theta.alloc((size_t)cap));
@@ -1072,7 +1108,9 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         rs = IPT_CUDA_CUSOLVER_ERROR;
     }
     IPT_STCHK(rs);
+    trace_mark("Rayleigh-Ritz enqueued");
     IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    trace_mark("Rayleigh-Ritz done");
     out->rr_time = seconds_since(t1);
     out->basis_cols = rr_cols;
     if (p.davidson) {
@@ -1106,6 +1144,7 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         out->matvecs += out->dav.matvecs;
         out->basis_cols = out->dav.basis_cols;
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+        trace_mark("Davidson done");
         out->davidson_time = seconds_since(t2);
     }
     return IPT_CUDA_SUCCESS;
@@ -1340,6 +1379,7 @@ int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vect
             }
         }
     }
+    trace_mark("detection pairs listed");
     WindowCoupling wc;
     wc.nn = n;
     wc.dg = &sd;
@@ -1366,6 +1406,7 @@ int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vect
             return IPT_CUDA_CUDA_ERROR;
         }
     }
+    trace_mark("pair gather enqueued");
     if (overlap) {
         const int st = overlap();
         if (st != IPT_CUDA_SUCCESS) {
@@ -1376,6 +1417,7 @@ int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vect
             return st;
         }
     }
+    trace_mark("overlapped work enqueued");
     if (npairs > 0) {
         const hipError_t es = hipEventSynchronize(ready);
         (void)hipEventDestroy(ready);
@@ -1387,9 +1429,11 @@ int detect_clusters(Context *ctx, const std::vector<double> &sd, const std::vect
         }
         std::sort(wc.lut.begin(), wc.lut.end());
     }
+    trace_mark("pair values on the host");
     bool too_large = false;
     *ranges = target_clusters(wc, sd, k, o, &too_large);
     record_ranges(ctx, *ranges);
+    trace_mark("clusters detected");
     if (too_large && env_int("IPT_BLOCK_CLUSTER_STALL", 0) <= 0) {
         int worst = 0;
         for (const auto &r : *ranges) {
@@ -1414,14 +1458,18 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
     IPT_STCHK(colsum.alloc((size_t)n));
     IPT_STCHK(colmax.alloc((size_t)n));
     IPT_STCHK(asym.alloc(1));
+    trace_mark("preparation buffers");
     IPT_STCHK(dense_diagonal(ctx, dH, n, n, pr->d.p));
+    trace_mark("diagonal enqueued");
     IPT_STCHK(dense_scan(ctx, dH, n, n, colsum.p, colmax.p, asym.p));
     std::vector<double> hd((size_t)n), hsum((size_t)n), hmax((size_t)n);
     int hasym = 0;
-    IPT_HIPCHK(hipMemcpyAsync(hd.data(), pr->d.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipMemcpyAsync(hsum.data(), colsum.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipMemcpyAsync(hmax.data(), colmax.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    IPT_STCHK(read_host(ctx, &hasym, asym.p, sizeof(int)));  // one synchronisation for all four
+    trace_mark("scan enqueued");
+    IPT_STCHK(read_host_many(ctx, {HostRead{hd.data(), pr->d.p, (size_t)n * sizeof(double)},
+                                   HostRead{hsum.data(), colsum.p, (size_t)n * sizeof(double)},
+                                   HostRead{hmax.data(), colmax.p, (size_t)n * sizeof(double)},
+                                   HostRead{&hasym, asym.p, sizeof(int)}}));
+    trace_mark("scan results on the host");
     if (hasym) {
         fprintf(stderr, "IPT block-cluster path currently requires a real symmetric matrix\n");
         return IPT_CUDA_INVALID_ARGUMENT;
@@ -1440,19 +1488,21 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
             break;
         }
     }
-    IPT_STCHK(pr->dperm.alloc((size_t)n));
-    IPT_HIPCHK(hipMemcpyAsync(pr->dperm.p, pr->perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
-                              ctx->stream));
-    std::vector<double> sd((size_t)n), soff((size_t)n), sabs((size_t)n);
+    if (pr->permuted) {
+        IPT_STCHK(pr->dperm.alloc((size_t)n));
+        IPT_HIPCHK(hipMemcpyAsync(pr->dperm.p, pr->perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    }
+    std::vector<double> sd((size_t)n), soff((size_t)n);
     for (int s = 0; s < n; ++s) {
         sd[(size_t)s] = hd[(size_t)pr->perm[(size_t)s]];
         soff[(size_t)s] = hmax[(size_t)pr->perm[(size_t)s]];
-        // column sums of |H| (= row sums, symmetric) minus the diagonal
-        sabs[(size_t)s] = std::max(0.0, hsum[(size_t)pr->perm[(size_t)s]] - std::fabs(sd[(size_t)s]));
     }
+    // column sums of |H| (= row sums, symmetric) minus the diagonal, in
+    // sorted order, formed on the device from the scan's column sums and the
+    // unsorted diagonal (pr->d is overwritten by the sorted one after it)
     IPT_STCHK(pr->rowabs.alloc((size_t)n));
-    IPT_HIPCHK(hipMemcpyAsync(pr->rowabs.p, sabs.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice,
-                              ctx->stream));
+    IPT_STCHK(offdiag_row_sums(ctx, colsum.p, pr->d.p, pr->permuted ? pr->dperm.p : nullptr, n, pr->rowabs.p));
     if (pr->permuted) {
         IPT_STCHK(pr->Hs.alloc((size_t)n * n));
         IPT_STCHK(dense_permute(ctx, dH, n, pr->Hs.p, n, n, pr->dperm.p));
@@ -1462,6 +1512,7 @@ int dense_prepare(Context *ctx, const double *dH, int n, int k, DensePrep *pr,
     } else {
         pr->H = dH;
     }
+    trace_mark("diagonal order and row sums sent");
     pr->hd = sd;
     pr->max_offdiag = 0.0;
     for (double x : soff) {
@@ -1488,7 +1539,9 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
                        double **d_vec_out, double **d_val_out, IPTCudaResult *info)
 {
     auto t0 = std::chrono::steady_clock::now();
+    trace_begin();
     const BlockParams p = params_from_env(maxiter, tol, use_tol);
+    trace_mark("options read");
     DensePrep pr;
     LowpOperator lowp;
     const LowpOperator *lp = nullptr;
@@ -1501,6 +1554,7 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
         return IPT_CUDA_SUCCESS;
     };
     IPT_STCHK(dense_prepare(ctx, dH, n, k, &pr, pack));
+    trace_mark("dense preparation returned");
     const double prep = seconds_since(t0);
     const double *H = pr.H;
     // When every product is FP64, a copy of H in MFMA operand order makes
@@ -1544,6 +1598,7 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     IPT_STCHK(block_pipeline(ctx, apply, identity_product, lp != nullptr ? &delta : nullptr, pr.d.p, pr.rowabs.p, n,
                              k, pr.ranges, pr.normH, p, &br, false, lp, before_davidson));
     record_stats(ctx, br);
+    trace_mark("block pipeline returned");
     if (p.verbose) {
         fprintf(stderr,
                 "IPT block-cluster dense: n=%d k=%d clusters=%zu basis_cols=%d precision=%d steps=%d "

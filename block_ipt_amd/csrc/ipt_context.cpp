@@ -4,6 +4,7 @@
 #include <climits>
 #include <cstring>
 #include <map>
+#include <chrono>
 #include <mutex>
 #include <unordered_map>
 
@@ -58,6 +59,27 @@ void pool_set_stream(hipStream_t s)
     t_pool_stream = s;
 }
 
+namespace {
+bool g_trace_on = false;
+std::chrono::steady_clock::time_point g_trace_t0;
+}  // namespace
+
+void trace_begin()
+{
+    g_trace_on = env_flag("IPT_HIP_HOST_TRACE") != 0;
+    g_trace_t0 = std::chrono::steady_clock::now();
+}
+
+bool trace_on() { return g_trace_on; }
+
+void trace_mark(const char *what)
+{
+    if (g_trace_on) {
+        const double us = 1e6 * std::chrono::duration<double>(std::chrono::steady_clock::now() - g_trace_t0).count();
+        fprintf(stderr, "IPT host trace %9.1f us  %s\n", us, what);
+    }
+}
+
 int pool_alloc(void **p, size_t bytes)
 {
     const size_t rb = pool_round(bytes);
@@ -73,6 +95,11 @@ int pool_alloc(void **p, size_t bytes)
         }
     }
     void *q = nullptr;
+    if (g_trace_on) {
+        char msg[96];
+        snprintf(msg, sizeof(msg), "hipMalloc of %zu bytes (none cached)", rb);
+        trace_mark(msg);
+    }
     if (hipMalloc(&q, rb) != hipSuccess) {
         (void)hipGetLastError();
         pool_trim_all();  // cached blocks of other sizes may be what is missing
@@ -105,6 +132,9 @@ void pool_free(void *p)
                 return;
             }
         }
+    }
+    if (g_trace_on) {
+        trace_mark("hipFree of a block not cached");
     }
     (void)hipFree(p);  // not cached (foreign, detached, or over the cap)
 }

@@ -524,12 +524,43 @@ __global__ void k_jac_diag(const double *__restrict__ Ap, int M, double *__restr
     }
 }
 
+// Padding diagonal entries i = m..M-1 set to big + (i - m), big = 2 ||Ap||_F + 1
+// from red[1] (the sum of squares after k_jac_init with big = 0): distinct
+// values above the spectrum, computed without a host round trip.
+__global__ void k_jac_pad(double *__restrict__ Ap, int M, int m, const double *__restrict__ red)
+{
+    const int i = m + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) {
+        Ap[i + (long)i * M] = (2.0 * sqrt(red[1]) + 1.0) + (double)(i - m);
+    }
+}
+
+// perm[r] = the index of the diagonal entry of rank r (ascending, ties by
+// index: a stable sort) for r < m, by counting; d has M entries.
+__global__ void k_jac_order(const double *__restrict__ d, int M, int m, int *__restrict__ perm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) {
+        return;
+    }
+    const double di = d[i];
+    int r = 0;
+    for (int j = 0; j < M; ++j) {
+        const double dj = d[j];
+        r += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    if (r < m) {
+        perm[r] = i;
+    }
+}
+
 }  // namespace
 
 // Eigenvalues (ascending) w[0..m) and eigenvectors V (m x m, ldv) of the
 // symmetric A (m x m, lda; read only).  *sweeps_out (may be NULL): outer
 // sweeps used.  Returns IPT_CUDA_CUSOLVER_ERROR when 40 sweeps do not
-// converge.
+// converge.  The results are complete in stream order on ctx->stream (the
+// host does not wait for them).
 int sym_eig_device(Context *ctx, const double *A, long lda, int m, double *w, double *V, long ldv, int *sweeps_out)
 {
     if (m <= 0) {
@@ -570,15 +601,16 @@ int sym_eig_device(Context *ctx, const double *A, long lda, int m, double *w, do
         IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
         return IPT_CUDA_SUCCESS;
     };
+    if (M > m) {
+        // the padding values follow from the norm of the matrix on the device
+        hipLaunchKernelGGL(k_jac_norms, dim3(NPARTS), dim3(256), 0, ctx->stream, Ap.p, M, part.p);
+        hipLaunchKernelGGL(k_jac_norms_final, dim3(1), dim3(64), 0, ctx->stream, part.p, red.p);
+        hipLaunchKernelGGL(k_jac_pad, dim3((M - m + 255) / 256), dim3(256), 0, ctx->stream, Ap.p, M, m, red.p);
+        IPT_HIPCHK(hipGetLastError());
+    }
     IPT_STCHK(norms());
     if (!std::isfinite(h[1])) {
         return IPT_CUDA_CUSOLVER_ERROR;
-    }
-    if (M > m) {
-        const double big = 2.0 * std::sqrt(h[1]) + 1.0;
-        hipLaunchKernelGGL(k_jac_init, dim3(init_blocks), dim3(256), 0, ctx->stream, A, lda, m, Ap.p, Vp.p, M, big);
-        IPT_HIPCHK(hipGetLastError());
-        IPT_STCHK(norms());
     }
     // converged when the off-diagonal Frobenius norm is below 1e-15 of the
     // matrix's, or below 1e-12 of it and no longer halving (the rounding
@@ -611,21 +643,13 @@ int sym_eig_device(Context *ctx, const double *A, long lda, int m, double *w, do
                 std::sqrt(h[0]), std::sqrt(total));
         return IPT_CUDA_CUSOLVER_ERROR;
     }
-    // ascending order (the padding values are above every real one)
-    std::vector<double> hd((size_t)M);
+    // ascending order (the padding values are above every real one), sorted
+    // on the device; the results are ordered on the stream behind it
     hipLaunchKernelGGL(k_jac_diag, dim3((M + 255) / 256), dim3(256), 0, ctx->stream, Ap.p, M, dg.p);
-    IPT_HIPCHK(hipGetLastError());
-    IPT_HIPCHK(hipMemcpyAsync(hd.data(), dg.p, (size_t)M * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
-    std::vector<int> all((size_t)M);
-    std::iota(all.begin(), all.end(), 0);
-    std::stable_sort(all.begin(), all.end(), [&](int a, int b) { return hd[(size_t)a] < hd[(size_t)b]; });
-    const std::vector<int> perm(all.begin(), all.begin() + m);
-    IPT_HIPCHK(hipMemcpyAsync(dperm.p, perm.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_jac_order, dim3((M + 255) / 256), dim3(256), 0, ctx->stream, dg.p, M, m, dperm.p);
     hipLaunchKernelGGL(k_jac_gather, dim3(std::min(4096, (m * m + 255) / 256)), dim3(256), 0, ctx->stream, Ap.p, Vp.p,
                        M, m, dperm.p, w, V, ldv);
     IPT_HIPCHK(hipGetLastError());
-    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
     return IPT_CUDA_SUCCESS;
 }
 
@@ -643,5 +667,7 @@ extern "C" int ipt_hip_sym_eig(const double *d_A, int lda, int m, double *d_w, d
     if (ctx == nullptr) {
         return st;
     }
-    return sym_eig_device(ctx, d_A, lda, m, d_w, d_V, ldv, nullptr);
+    IPT_STCHK(sym_eig_device(ctx, d_A, lda, m, d_w, d_V, ldv, nullptr));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    return IPT_CUDA_SUCCESS;
 }

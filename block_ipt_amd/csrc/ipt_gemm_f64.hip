@@ -143,12 +143,12 @@ __global__ void k_splitk_reduce(const double *__restrict__ P, long stride, int s
 }
 
 template <int NB>
-static void launch_gemm(dim3 grid, hipStream_t st, const double *H, long ldh, const double *Z,
-                        long ldz, double *Y, long ldy, long split_stride, int n, int ncols,
+static void launch_gemm(const ProfileToken &prof, dim3 grid, hipStream_t st, const double *H, long ldh,
+                        const double *Z, long ldz, double *Y, long ldy, long split_stride, int n, int ncols,
                         long kchunk)
 {
-    hipLaunchKernelGGL(k_gemm_f64<NB>, grid, dim3(256), 0, st, H, ldh, Z, ldz, Y, ldy,
-                       split_stride, n, ncols, kchunk);
+    launch_timed(prof, true, true, k_gemm_f64<NB>, grid, dim3(256), 0, st, H, ldh, Z, ldz, Y, ldy, split_stride, n,
+                 ncols, kchunk);
 }
 
 static int gemm_f64_pass(Context *ctx, const double *H, long ldh, const double *Z, long ldz,
@@ -180,16 +180,16 @@ static int gemm_f64_pass(Context *ctx, const double *H, long ldh, const double *
         out_ld = n;
         stride = (long)n * ncols;
     }
-    const ProfileToken prof = profile_begin(ctx, PROF_GEMM_F64);
+    const ProfileToken prof = profile_begin_launches(PROF_GEMM_F64);
     switch (NB) {
-    case 1: launch_gemm<1>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
-    case 2: launch_gemm<2>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
-    case 3: launch_gemm<3>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
-    case 4: launch_gemm<4>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
-    case 5: launch_gemm<5>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
-    case 6: launch_gemm<6>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
-    case 7: launch_gemm<7>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
-    default: launch_gemm<8>(grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    case 1: launch_gemm<1>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    case 2: launch_gemm<2>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    case 3: launch_gemm<3>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    case 4: launch_gemm<4>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    case 5: launch_gemm<5>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    case 6: launch_gemm<6>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    case 7: launch_gemm<7>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
+    default: launch_gemm<8>(prof, grid, ctx->stream, H, ldh, Z, ldz, out, out_ld, stride, n, ncols, kchunk); break;
     }
     profile_end(ctx, prof, ncols);
     IPT_HIPCHK(hipGetLastError());
@@ -383,11 +383,11 @@ int f64_packed_create(Context *ctx, const double *H, long ld, int n, F64Packed *
 }
 
 template <int NB>
-static void launch_packed(dim3 grid, hipStream_t st, const F64Packed &op, const double *Z, long ldz, double *Y,
-                          long ldy, long split_stride, int ncols, long kchunk)
+static void launch_packed(const ProfileToken &prof, dim3 grid, hipStream_t st, const F64Packed &op, const double *Z,
+                          long ldz, double *Y, long ldy, long split_stride, int ncols, long kchunk)
 {
-    hipLaunchKernelGGL(k_gemm_f64_packed<NB>, grid, dim3(256), 0, st, op.p.p, op.nkp, Z, ldz, Y, ldy, split_stride,
-                       op.n, ncols, kchunk);
+    launch_timed(prof, true, true, k_gemm_f64_packed<NB>, grid, dim3(256), 0, st, (const double2 *)op.p.p, op.nkp, Z,
+                 ldz, Y, ldy, split_stride, op.n, ncols, kchunk);
 }
 
 static int gemm_packed_pass(Context *ctx, const F64Packed &op, const double *Z, long ldz, double *Y, long ldy,
@@ -412,16 +412,16 @@ static int gemm_packed_pass(Context *ctx, const F64Packed &op, const double *Z, 
         out_ld = n;
         stride = (long)n * ncols;
     }
-    const ProfileToken prof = profile_begin(ctx, PROF_GEMM_F64);
+    const ProfileToken prof = profile_begin_launches(PROF_GEMM_F64);
     switch (NB) {
-    case 1: launch_packed<1>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
-    case 2: launch_packed<2>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
-    case 3: launch_packed<3>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
-    case 4: launch_packed<4>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
-    case 5: launch_packed<5>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
-    case 6: launch_packed<6>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
-    case 7: launch_packed<7>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
-    default: launch_packed<8>(grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    case 1: launch_packed<1>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    case 2: launch_packed<2>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    case 3: launch_packed<3>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    case 4: launch_packed<4>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    case 5: launch_packed<5>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    case 6: launch_packed<6>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    case 7: launch_packed<7>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
+    default: launch_packed<8>(prof, grid, ctx->stream, op, Z, ldz, out, out_ld, stride, ncols, kchunk); break;
     }
     profile_end(ctx, prof, ncols);
     IPT_HIPCHK(hipGetLastError());

@@ -458,28 +458,28 @@ constexpr int lowp_kw(int mode, int nb)
 }
 
 template <int MODE, int NB>
-static void launch_lowp_nb(dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
-                           const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw, int zstride,
-                           int zcb0)
+static void launch_lowp_nb(const ProfileToken &prof, dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo,
+                           const uint4 *Zhi, const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw,
+                           int zstride, int zcb0)
 {
-    hipLaunchKernelGGL((k_lowp_gemm<NB, MODE, lowp_kw(MODE, NB)>), grid, dim3(512), 0, st, Ehi, Elo, Zhi, Zlo, ts, P,
-                       nkb, nrb, rbw, zstride, zcb0);
+    launch_timed(prof, true, true, k_lowp_gemm<NB, MODE, lowp_kw(MODE, NB)>, grid, dim3(512), 0, st, Ehi, Elo, Zhi,
+                 Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0);
 }
 
 template <int MODE>
-static void launch_lowp(int NB, dim3 grid, hipStream_t st, const uint4 *Ehi, const uint4 *Elo, const uint4 *Zhi,
-                        const uint4 *Zlo, const float *ts, float *P, int nkb, int nrb, int rbw, int zstride,
-                        int zcb0)
+static void launch_lowp(const ProfileToken &prof, int NB, dim3 grid, hipStream_t st, const uint4 *Ehi,
+                        const uint4 *Elo, const uint4 *Zhi, const uint4 *Zlo, const float *ts, float *P, int nkb,
+                        int nrb, int rbw, int zstride, int zcb0)
 {
     switch (NB) {
-    case 1: launch_lowp_nb<MODE, 1>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
-    case 2: launch_lowp_nb<MODE, 2>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
-    case 3: launch_lowp_nb<MODE, 3>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
-    case 4: launch_lowp_nb<MODE, 4>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
-    case 5: launch_lowp_nb<MODE, 5>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
-    case 6: launch_lowp_nb<MODE, 6>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
-    case 7: launch_lowp_nb<MODE, 7>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
-    default: launch_lowp_nb<MODE, 8>(grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 1: launch_lowp_nb<MODE, 1>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 2: launch_lowp_nb<MODE, 2>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 3: launch_lowp_nb<MODE, 3>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 4: launch_lowp_nb<MODE, 4>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 5: launch_lowp_nb<MODE, 5>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 6: launch_lowp_nb<MODE, 6>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    case 7: launch_lowp_nb<MODE, 7>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
+    default: launch_lowp_nb<MODE, 8>(prof, grid, st, Ehi, Elo, Zhi, Zlo, ts, P, nkb, nrb, rbw, zstride, zcb0); break;
     }
 }
 
@@ -501,22 +501,22 @@ static LowpGrid lowp_grid(const LowpOperator &op, int NB)
     return g;
 }
 
-static void launch_lowp_mode(const LowpOperator &op, int NB, dim3 grid, hipStream_t st, const uint4 *zhi,
-                             const uint4 *zlo, float *P, int rbw, int zstride, int zcb0)
+static void launch_lowp_mode(const ProfileToken &prof, const LowpOperator &op, int NB, dim3 grid, hipStream_t st,
+                             const uint4 *zhi, const uint4 *zlo, float *P, int rbw, int zstride, int zcb0)
 {
     const uint4 *Ehi = reinterpret_cast<const uint4 *>(op.hi.p);
     const uint4 *Elo = reinterpret_cast<const uint4 *>(op.lo.p);
     const float *ts = op.tscale.p;
     if (op.mode == LOWP_BF16X3) {
-        launch_lowp<LOWP_BF16X3>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+        launch_lowp<LOWP_BF16X3>(prof, NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
     } else if (op.mode == LOWP_FP32) {
-        launch_lowp<LOWP_FP32>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+        launch_lowp<LOWP_FP32>(prof, NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
     } else if (op.mode == LOWP_FP8) {
-        launch_lowp<LOWP_FP8>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+        launch_lowp<LOWP_FP8>(prof, NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
     } else if (op.mode == LOWP_FP16) {
-        launch_lowp<LOWP_FP16>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+        launch_lowp<LOWP_FP16>(prof, NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
     } else {
-        launch_lowp<LOWP_BF16>(NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
+        launch_lowp<LOWP_BF16>(prof, NB, grid, st, Ehi, Elo, zhi, zlo, ts, P, op.nkb, op.nrb, rbw, zstride, zcb0);
     }
 }
 
@@ -557,8 +557,8 @@ int lowp_apply_delta(Context *ctx, const LowpOperator &op, const double *B, cons
                            Bac, ld, n, w, op.nkb, NB, op.mode, colmax, nparts, colscale, zhi, zlo);
         IPT_HIPCHK(hipGetLastError());
         dim3 grid(gx, splits);
-        const ProfileToken prof = profile_begin(ctx, PROF_LOWP_GEMM);
-        launch_lowp_mode(op, NB, grid, ctx->stream, zhi, zlo, P, rbw, NB, 0);
+        const ProfileToken prof = profile_begin_launches(PROF_LOWP_GEMM);
+        launch_lowp_mode(prof, op, NB, grid, ctx->stream, zhi, zlo, P, rbw, NB, 0);
         profile_end(ctx, prof, w);
         IPT_HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(k_lowp_epilogue, dim3(blocks_for((long)op.nrb * NB * 64, 256, 16384)), dim3(256), 0,
@@ -581,8 +581,8 @@ int lowp_apply_emitted(Context *ctx, const LowpOperator &op, const unsigned shor
         const LowpGrid lg = lowp_grid(op, NB);
         float *P = nullptr;
         IPT_STCHK(context_scratch(ctx, SLOT_LOWP, (size_t)lg.splits * op.nrb * NB * 1024 + 256, (void **)&P));
-        const ProfileToken prof = profile_begin(ctx, PROF_LOWP_GEMM);
-        launch_lowp_mode(op, NB, dim3(lg.gx, lg.splits), ctx->stream, reinterpret_cast<const uint4 *>(zhi),
+        const ProfileToken prof = profile_begin_launches(PROF_LOWP_GEMM);
+        launch_lowp_mode(prof, op, NB, dim3(lg.gx, lg.splits), ctx->stream, reinterpret_cast<const uint4 *>(zhi),
                          reinterpret_cast<const uint4 *>(zlo), P, lg.rbw, nb_all, cb);
         profile_end(ctx, prof, std::min(16 * NB, bc - 16 * cb));
         IPT_HIPCHK(hipGetLastError());

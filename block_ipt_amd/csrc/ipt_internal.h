@@ -1,6 +1,7 @@
 // Internal declarations shared by the translation units of libipt_hip.so.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -144,11 +145,40 @@ struct ProfileToken {
     int id = -1;
     hipEvent_t start = nullptr;
     hipEvent_t stop = nullptr;
+    bool markers = true;  // events recorded as stream markers (else by the launches)
 };
 bool profile_on();
+// A span of launches timed by events recorded on the stream before and after
+// it (marker packets: each costs a few microseconds between the kernels).
 ProfileToken profile_begin(Context *ctx, int id);
+// A span whose first and last launches go through launch_timed: the events
+// take those dispatches' own start and end timestamps, so timing adds no
+// packets to the stream and no gaps between the kernels.
+ProfileToken profile_begin_launches(int id);
 // cols: the columns the timed launch processed (summed per family)
 void profile_end(Context *ctx, const ProfileToken &t, long long cols = 0);
+
+// One launch of a span opened by profile_begin_launches: the first launch
+// carries the start event, the last the stop event (hipExtLaunchKernelGGL);
+// untimed launches and spans go out as ordinary launches.
+template <typename... Args, typename F = void (*)(Args...)>
+inline void launch_timed(const ProfileToken &t, bool first, bool last, F kernel, dim3 grid, dim3 block,
+                         unsigned shmem, hipStream_t st, Args... args)
+{
+    if (t.id >= 0 && !t.markers && (first || last)) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, first ? t.start : nullptr, last ? t.stop : nullptr, 0,
+                              args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+    }
+}
+
+// IPT_HIP_HOST_TRACE=1: the host's time at the stages of a solve
+// (microseconds since trace_begin, on stderr), to place the host work that
+// leaves the GPU idle; device allocations and releases are marked too.
+void trace_begin();
+void trace_mark(const char *what);
+bool trace_on();
 
 // ------------------------------------------------------------------ options
 
@@ -178,6 +208,11 @@ int gemm_f64_packed(Context *ctx, const F64Packed &op, const double *Z, long ldz
 
 // d[i] = H[i + i*ldh]
 int dense_diagonal(Context *ctx, const double *H, long ldh, int n, double *d);
+// x[0..count) = v on the device (no host buffer, no synchronisation)
+int fill_device(Context *ctx, double *x, long count, double v);
+// out[s] = max(0, colsum[p] - |d[p]|), p = perm[s] (perm NULL: identity):
+// off-diagonal row sums of |H| (symmetric H, column sums of |H| in colsum)
+int offdiag_row_sums(Context *ctx, const double *colsum, const double *d, const int *perm, int n, double *out);
 // CSC diagonal (first match per column), IPT/src/ipt_cuda.cu:1469-1489.
 int csc_diagonal(Context *ctx, const int *col_ptr, const int *row_ind, const double *values, int n,
                  double *d);

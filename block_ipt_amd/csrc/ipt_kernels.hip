@@ -31,6 +31,42 @@ int dense_diagonal(Context *ctx, const double *H, long ldh, int n, double *d)
     return IPT_CUDA_SUCCESS;
 }
 
+__global__ void k_fill(double *__restrict__ x, long count, double v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x) {
+        x[i] = v;
+    }
+}
+
+int fill_device(Context *ctx, double *x, long count, double v)
+{
+    if (count <= 0) {
+        return IPT_CUDA_SUCCESS;
+    }
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(count, 256, 4096)), dim3(256), 0, ctx->stream, x, count, v);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
+// out[s] = max(0, colsum[p] - |d[p]|), p = perm[s] (perm NULL: p = s): the
+// off-diagonal row sums of |H| of a symmetric matrix in sorted row order
+__global__ void k_offdiag_row_sums(const double *__restrict__ colsum, const double *__restrict__ d,
+                                   const int *__restrict__ perm, int n, double *__restrict__ out)
+{
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (long)gridDim.x * blockDim.x) {
+        const long p = perm != nullptr ? perm[s] : s;
+        out[s] = fmax(0.0, colsum[p] - fabs(d[p]));
+    }
+}
+
+int offdiag_row_sums(Context *ctx, const double *colsum, const double *d, const int *perm, int n, double *out)
+{
+    hipLaunchKernelGGL(k_offdiag_row_sums, dim3(blocks_for(n, 256, 4096)), dim3(256), 0, ctx->stream, colsum, d,
+                       perm, n, out);
+    IPT_HIPCHK(hipGetLastError());
+    return IPT_CUDA_SUCCESS;
+}
+
 // first stored entry with row == col, IPT/src/ipt_cuda.cu:1469-1489
 __global__ void k_csc_diag(const int *__restrict__ cp, const int *__restrict__ ri,
                            const double *__restrict__ v, int n, double *__restrict__ d)
@@ -295,32 +331,41 @@ __global__ __launch_bounds__(256) void k_spmm_rowmajor(const int *__restrict__ r
     }
 }
 
-static void launch_spmm(hipStream_t st, const int *rs, const int *re, const int *col_ind, const double *vals,
-                        int n, const double *X, long ldx, double *Y, long ldy, int k, int accumulate)
+// one pass of the product; first / last: the pass opens / closes the timed span
+static void launch_spmm(const ProfileToken &prof, bool first, bool last, hipStream_t st, const int *rs,
+                        const int *re, const int *col_ind, const double *vals, int n, const double *X, long ldx,
+                        double *Y, long ldy, int k, int accumulate)
 {
     const int blocks = blocks_for((long)n * 64, 256, 65536);
     dim3 g(blocks), b(256);
     if (k <= 1) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<1>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+        launch_timed(prof, first, last, k_spmm_rowmajor<1>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy,
+                     k, accumulate);
     } else if (k <= 2) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<2>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+        launch_timed(prof, first, last, k_spmm_rowmajor<2>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy,
+                     k, accumulate);
     } else if (k <= 4) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<4>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+        launch_timed(prof, first, last, k_spmm_rowmajor<4>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy,
+                     k, accumulate);
     } else if (k <= 8) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<8>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+        launch_timed(prof, first, last, k_spmm_rowmajor<8>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy,
+                     k, accumulate);
     } else if (k <= 16) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<16>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+        launch_timed(prof, first, last, k_spmm_rowmajor<16>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy,
+                     k, accumulate);
     } else if (k <= 32) {
-        hipLaunchKernelGGL(k_spmm_rowmajor<32>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+        launch_timed(prof, first, last, k_spmm_rowmajor<32>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy,
+                     k, accumulate);
     } else {
-        hipLaunchKernelGGL(k_spmm_rowmajor<64>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy, k, accumulate);
+        launch_timed(prof, first, last, k_spmm_rowmajor<64>, g, b, 0, st, rs, re, col_ind, vals, n, X, ldx, Y, ldy,
+                     k, accumulate);
     }
 }
 
 int spmm_csr_sliced_passes(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals, int n,
                            const double *X, long ldx, double *Y, long ldy, int k, const CsrSlices &slices, int ns)
 {
-    const ProfileToken prof = profile_begin(ctx, PROF_SPMM);
+    const ProfileToken prof = profile_begin_launches(PROF_SPMM);
     if (ns > 1) {
         // column slices of the same CSR, one pass each: a pass gathers
         // rows of X from one slice (an L2-sized part of X), the later
@@ -329,11 +374,11 @@ int spmm_csr_sliced_passes(Context *ctx, const int *row_ptr, const int *col_ind,
         const long stride = (long)slices.n;
         const int step = slices.nslices / ns;
         for (int b = 0; b < ns; ++b) {
-            launch_spmm(ctx->stream, sp + (long)(b * step) * stride, sp + (long)((b + 1) * step) * stride, col_ind,
-                        vals, n, X, ldx, Y, ldy, k, b > 0);
+            launch_spmm(prof, b == 0, b == ns - 1, ctx->stream, sp + (long)(b * step) * stride,
+                        sp + (long)((b + 1) * step) * stride, col_ind, vals, n, X, ldx, Y, ldy, k, b > 0);
         }
     } else {
-        launch_spmm(ctx->stream, row_ptr, row_ptr + 1, col_ind, vals, n, X, ldx, Y, ldy, k, 0);
+        launch_spmm(prof, true, true, ctx->stream, row_ptr, row_ptr + 1, col_ind, vals, n, X, ldx, Y, ldy, k, 0);
     }
     profile_end(ctx, prof, k);
     IPT_HIPCHK(hipGetLastError());

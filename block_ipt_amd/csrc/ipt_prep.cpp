@@ -329,12 +329,15 @@ void for_joined_pairs(const CouplingSource &m, const std::vector<double> &diag, 
 
     std::vector<int> order((size_t)n);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        if (diag[(size_t)a] != diag[(size_t)b]) {
-            return diag[(size_t)a] < diag[(size_t)b];
-        }
-        return a < b;
-    });
+    // a sorted diagonal (the block path sorts it first) is its own order
+    if (!std::is_sorted(diag.begin(), diag.end())) {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (diag[(size_t)a] != diag[(size_t)b]) {
+                return diag[(size_t)a] < diag[(size_t)b];
+            }
+            return a < b;
+        });
+    }
     std::vector<double> sd((size_t)n);
     for (int i = 0; i < n; ++i) {
         sd[(size_t)i] = diag[(size_t)order[(size_t)i]];

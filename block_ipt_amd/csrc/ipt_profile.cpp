@@ -1,5 +1,7 @@
-// Kernel timing instrumentation: HIP events recorded around the launches
-// of the main kernels on the stream they run on; summed on request.
+// Kernel timing instrumentation: HIP events on the stream the main kernels
+// run on -- either the dispatches' own start and end timestamps
+// (hipExtLaunchKernelGGL, no extra packets) or markers recorded around a
+// span of launches -- summed on request.
 #include <cstring>
 #include <mutex>
 
@@ -55,12 +57,32 @@ ProfileToken profile_begin(Context *ctx, int id)
     return t;
 }
 
+ProfileToken profile_begin_launches(int id)
+{
+    ProfileToken t;
+    t.id = -1;
+    t.markers = false;
+    if (id < 0 || id >= PROF_COUNT || !((g_mask >> id) & 1u)) {
+        return t;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    t.start = take_event();
+    t.stop = take_event();
+    if (t.start == nullptr || t.stop == nullptr) {
+        return t;
+    }
+    t.id = id;
+    return t;
+}
+
 void profile_end(Context *ctx, const ProfileToken &t, long long cols)
 {
     if (t.id < 0) {
         return;
     }
-    (void)hipEventRecord(t.stop, ctx->stream);
+    if (t.markers) {
+        (void)hipEventRecord(t.stop, ctx->stream);
+    }
     std::lock_guard<std::mutex> lock(g_mu);
     g_ev[t.id].push_back(Timed{t.start, t.stop, cols});
 }

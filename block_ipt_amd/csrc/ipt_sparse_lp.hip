@@ -144,29 +144,33 @@ __global__ __launch_bounds__(256) void k_delta_epilogue(const float *__restrict_
 }
 
 template <int S>
-void launch_spmm_f32(dim3 g, hipStream_t st, const int *rs, const int *re, const int *ci, const float *v, int n,
-                     const float *X, int w, float *Z, int accumulate)
+void launch_spmm_f32(const ProfileToken &prof, bool first, bool last, dim3 g, hipStream_t st, const int *rs,
+                     const int *re, const int *ci, const float *v, int n, const float *X, int w, float *Z,
+                     int accumulate)
 {
-    hipLaunchKernelGGL(k_spmm_offdiag_f32<S>, g, dim3(256), 0, st, rs, re, ci, v, n, X, w, Z, accumulate);
+    launch_timed(prof, first, last, k_spmm_offdiag_f32<S>, g, dim3(256), 0, st, rs, re, ci, v, n, X, w, Z,
+                 accumulate);
 }
 
-void spmm_f32_pass(dim3 g, hipStream_t st, const int *rs, const int *re, const int *ci, const float *v, int n,
-                   const float *X, int w, float *Z, int accumulate)
+// one pass of the product; first / last: the pass opens / closes the timed span
+void spmm_f32_pass(const ProfileToken &prof, bool first, bool last, dim3 g, hipStream_t st, const int *rs,
+                   const int *re, const int *ci, const float *v, int n, const float *X, int w, float *Z,
+                   int accumulate)
 {
     if (w <= 1) {
-        launch_spmm_f32<1>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<1>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
     } else if (w <= 2) {
-        launch_spmm_f32<2>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<2>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
     } else if (w <= 4) {
-        launch_spmm_f32<4>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<4>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
     } else if (w <= 8) {
-        launch_spmm_f32<8>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<8>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
     } else if (w <= 16) {
-        launch_spmm_f32<16>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<16>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
     } else if (w <= 32) {
-        launch_spmm_f32<32>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<32>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
     } else {
-        launch_spmm_f32<64>(g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<64>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
     }
 }
 
@@ -202,7 +206,7 @@ int sparse_lowp_apply_delta(Context *ctx, const SparseLowp &op, const double *B,
         const size_t o = (size_t)c0 * ld;
         hipLaunchKernelGGL(k_delta_rows_f32, tiles, dim3(256), 0, ctx->stream, B + o, Ba + o, ld, n, w, D);
         IPT_HIPCHK(hipGetLastError());
-        const ProfileToken prof = profile_begin(ctx, PROF_SPMM_F32);
+        const ProfileToken prof = profile_begin_launches(PROF_SPMM_F32);
         // FP32 rows of X: the pass count of an FP64 product half as wide
         const int ns = op.slices != nullptr ? csr_slices_for(*op.slices, (w + 1) / 2) : 1;
         if (ns > 1) {
@@ -210,11 +214,11 @@ int sparse_lowp_apply_delta(Context *ctx, const SparseLowp &op, const double *B,
             const long stride = (long)op.slices->n;
             const int step = op.slices->nslices / ns;
             for (int b = 0; b < ns; ++b) {
-                spmm_f32_pass(g, ctx->stream, sp + (long)(b * step) * stride, sp + (long)((b + 1) * step) * stride,
-                              op.ci, op.vals.p, n, D, w, Z, b > 0);
+                spmm_f32_pass(prof, b == 0, b == ns - 1, g, ctx->stream, sp + (long)(b * step) * stride,
+                              sp + (long)((b + 1) * step) * stride, op.ci, op.vals.p, n, D, w, Z, b > 0);
             }
         } else {
-            spmm_f32_pass(g, ctx->stream, op.rp, op.rp + 1, op.ci, op.vals.p, n, D, w, Z, 0);
+            spmm_f32_pass(prof, true, true, g, ctx->stream, op.rp, op.rp + 1, op.ci, op.vals.p, n, D, w, Z, 0);
         }
         profile_end(ctx, prof, w);
         IPT_HIPCHK(hipGetLastError());
