@@ -15,6 +15,34 @@ struct BlockCluster {
     int pad;
 };
 
+// Update grid of a cluster of width 2..32 (ipt_block_width.h).  Widths of
+// at least kSeriesWidth: near blocks first (the rows close to the
+// cluster's spectrum, P = next_pow2(m) lanes per row, 256 / P rows per
+// block and pass, kNearRowSlots rows per pass), then one far block per
+// kFarRows rows (kFarLanes lanes per row).  Narrower widths: one block per
+// 256 / P rows, every row eliminated.  Each block owns one partial slot of
+// the cluster.
+constexpr int kSeriesWidth = 9;
+constexpr int kNearRowSlots = 512;
+constexpr int kFarLanes = 4;
+constexpr int kFarRows = 256 / kFarLanes;
+
+__host__ __device__ constexpr int width_lanes(int m)
+{
+    return m <= 2 ? 2 : m <= 4 ? 4 : m <= 8 ? 8 : m <= 16 ? 16 : 32;
+}
+
+__host__ __device__ constexpr int near_blocks_for(int m)
+{
+    return kNearRowSlots / (256 / width_lanes(m));
+}
+
+__host__ __device__ constexpr int update_blocks_for(int m, int n)
+{
+    return m >= kSeriesWidth ? near_blocks_for(m) + (n + kFarRows - 1) / kFarRows
+                             : (n + 256 / width_lanes(m) - 1) / (256 / width_lanes(m));
+}
+
 // Device-resident description of all clusters, grouped by lane class
 // P = next_pow2(m) in {1,...,64}.
 struct BlockPlan {

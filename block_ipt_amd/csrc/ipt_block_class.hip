@@ -23,7 +23,8 @@ __device__ __forceinline__ void class_rows(int m, int cid, int lblk, const Block
     }
 }
 
-// blockIdx.y = cluster list[blockIdx.y] (widths LO..HI), blockIdx.x = row block.
+// blockIdx.y = cluster list[blockIdx.y] (widths LO..HI), blockIdx.x = near
+// or far block (the widths of a class have the same near-block count).
 template <int LO, int HI>
 __global__ __launch_bounds__(256) void k_block_update_class(BlockUpdateArgs g, const int *__restrict__ list)
 {
@@ -34,10 +35,10 @@ __global__ __launch_bounds__(256) void k_block_update_class(BlockUpdateArgs g, c
 template <int LO, int HI>
 int launch_class(const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream)
 {
-    constexpr int P = HI <= 2 ? 2 : HI <= 4 ? 4 : HI <= 8 ? 8 : 16;
     static_assert(LO > HI / 2, "one lane class per launch");
-    constexpr int rows = 256 / P;
-    const dim3 grid((unsigned)((g.n + rows - 1) / rows), (unsigned)count);
+    static_assert((LO >= kSeriesWidth) == (HI >= kSeriesWidth) && near_blocks_for(LO) == near_blocks_for(HI),
+                  "one lane class and one row split per launch");
+    const dim3 grid((unsigned)update_blocks_for(HI, g.n), (unsigned)count);
     hipLaunchKernelGGL((k_block_update_class<LO, HI>), grid, dim3(256), 0, stream, g, list);
     return hipGetLastError() == hipSuccess ? IPT_CUDA_SUCCESS : IPT_CUDA_CUDA_ERROR;
 }

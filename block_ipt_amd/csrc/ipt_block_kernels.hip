@@ -640,9 +640,10 @@ int block_plan_create(Context *ctx, const std::vector<std::pair<int, int>> &rang
                     cl.m, kMaxWideCluster);
             return IPT_CUDA_INVALID_ARGUMENT;
         }
-        // width 1: partials per 64-row wavefront; wider: per row block
+        // width 1: partials per 64-row wavefront; 2..32: per near and far
+        // block; wider: per row block
         const int rows = cl.m == 1 ? 64 : cl.m > 64 ? block_wide_rows() : 256 >> cls;
-        cl.nblocks = (n + rows - 1) / rows;
+        cl.nblocks = cl.m >= 2 && cl.m <= 32 ? update_blocks_for(cl.m, n) : (n + rows - 1) / rows;
         cl.pad = 0;
         plan->partial_stride = std::max(plan->partial_stride, cl.nblocks);
         off += cl.m;

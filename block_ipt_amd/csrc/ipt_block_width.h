@@ -99,161 +99,396 @@ __device__ __forceinline__ void dominant_steps(double (&a)[M + 1], int t, bool &
     }
 }
 
+// Elimination of one row (the P lanes of group `grp` of the calling wave;
+// lane t holds equation t of (d_r I - h^T) x = d_r B_r - (HB)_r).  Adds the
+// row's norm and operand-maximum contributions to the lane's accumulators.
 template <int M>
-__device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const BlockUpdateArgs &g)
+__device__ __forceinline__ void eliminate_row(long row, int t, const BlockCluster &cl, const BlockUpdateArgs &g,
+                                              const double (*hs)[M + 1], const double *hsum, double escale,
+                                              double &sdn, double &sbn, double &mstep, double &manc)
 {
-    constexpr int P = M <= 2 ? 2 : M <= 4 ? 4 : M <= 8 ? 8 : M <= 16 ? 16 : 32;
+    constexpr int P = width_lanes(M);
+    const long ld = g.ld;
+    const bool emit = g.em.zhi != nullptr;
+    const double *Bc = g.B + (long)cl.off * ld;
+    const double *Yc = g.Y + (long)cl.off * ld;
+    double *Bnc = g.Bn + (long)cl.off * ld;
+    if (row >= cl.first && row <= cl.last) {
+        if (t < M) {
+            const double nv = (row == cl.first + t) ? 1.0 : 0.0;
+            const double ov = Bc[row + (long)t * ld];
+            Bnc[row + (long)t * ld] = nv;
+            sdn += (nv - ov) * (nv - ov);
+            sbn += ov * ov;
+            if (emit) {
+                const double av = g.em.anc != nullptr ? g.em.anc[row + (long)(cl.off + t) * ld] : ov;
+                emit_store(g.em, g.em.mode, row, cl.off + t, nv, av, escale);
+                mstep = fmax(mstep, fabs(nv - ov));
+                manc = fmax(manc, fabs(nv - av));
+            }
+        }
+        return;
+    }
+    const double dr = g.d[row];
+    // Lanes t >= M (padding up to P) load the equation of lane M-1: they
+    // are never the pivot lane, nothing reads their values, and unpredicated
+    // loads keep the register count of a power-of-two width.
+    const int te = t < M ? t : M - 1;
+    double a[M + 1];
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        // h(u, t) = Y[first+u, off+t]
+        a[u] = (u == t ? dr : 0.0) - hs[te][u];
+    }
+    a[M] = dr * Bc[row + (long)te * ld] - Yc[row + (long)te * ld];
+    int mypc = -1;
+    double mypiv = 1.0;
+    bool bad = false;
+    const bool mine = t >= M || fabs(dr - hs[te][te]) > 2.0 * hsum[t];
+    const unsigned long long ball = __ballot(mine);
+    const int base = (threadIdx.x & 63) & ~(P - 1);
+    const unsigned long long gmask = ((1ULL << P) - 1ULL) << base;
+    const bool dominant = (ball & gmask) == gmask;
+    if (dominant) {
+        dominant_steps<M, P, 0>(a, t, bad, mypc, mypiv);
+    } else {
+        bool used = false;
+#pragma unroll
+        for (int pc = 0; pc < M; ++pc) {
+            if (!bad) {
+                double cand = (t < M && !used) ? fabs(a[pc]) : -1.0;
+                int cidx = t;
+#pragma unroll
+                for (int o = 1; o < P; o <<= 1) {
+                    const double oc = __shfl_xor(cand, o, P);
+                    const int oi = __shfl_xor(cidx, o, P);
+                    if (oc > cand || (oc == cand && oi < cidx)) {
+                        cand = oc;
+                        cidx = oi;
+                    }
+                }
+                if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
+                    bad = true;
+                } else {
+                    const double piv = __shfl(a[pc], cidx, P);
+                    const bool me = t == cidx;
+                    const double alpha = me ? 0.0 : -(a[pc] / piv);
+#pragma unroll
+                    for (int c = pc + 1; c <= M; ++c) {
+                        a[c] = fma(alpha, __shfl(a[c], cidx, P), a[c]);
+                    }
+                    if (me) {
+                        used = true;
+                        mypc = pc;
+                        mypiv = piv;
+                    }
+                }
+            }
+        }
+    }
+    if (bad) {
+        if (t == 0) {
+            atomicOr(&g.flags[0], 1);
+        }
+    } else if (mypc >= 0) {
+        double v = a[M] / mypiv;
+        const double ov = Bc[row + (long)mypc * ld];
+        if (g.damping < 1.0) {
+            v = (1.0 - g.damping) * ov + g.damping * v;
+        }
+        if (!isfinite(v)) {
+            atomicOr(&g.flags[0], 1);
+            v = 0.0;
+        }
+        Bnc[row + (long)mypc * ld] = v;
+        sdn += (v - ov) * (v - ov);
+        sbn += ov * ov;
+        if (emit) {
+            const double av = g.em.anc != nullptr ? g.em.anc[row + (long)(cl.off + mypc) * ld] : ov;
+            emit_store(g.em, g.em.mode, row, cl.off + mypc, v, av, escale);
+            mstep = fmax(mstep, fabs(v - ov));
+            manc = fmax(manc, fabs(v - av));
+        }
+    }
+}
+
+// Far row (kFarLanes = 4 lanes): with c0 the mean of diag(h) and
+// nrm = ||h - c0 I||_1 (largest column sum), a row whose diagonal lies
+// farther than 8 nrm from c0 has s = 1 / (d_r - c0) and rho = |s| nrm <= 1/8,
+// so
+//     x^T = b^T (d_r I - h)^-1 = s b^T sum_k (s (h - c0 I))^k,
+// a series whose k-th term is at most rho^k of the first.  Horner's rule
+// z <- b^T + s z (h - c0 I), K - 1 times, then x^T = s z, with K the wave's
+// term count (rho^K <= 2^-54 for its largest rho: the truncation is below
+// the rounding of the elimination).  Lane q of the row's four owns columns
+// [qS, qS + S), S = ceil(M / 4); each step takes the other lanes' z by DPP
+// quad broadcasts and reads h(i, j) = ht[i][j] for its columns from LDS.
+// acc[t] += sum over the columns i of lane SRC of z_i h(i, j0 + t), z_i
+// taken from lane SRC of the quad by DPP (quad_perm broadcast)
+template <int M, int SRC>
+__device__ __forceinline__ void series_terms(const double (&z)[(M + 3) / 4], const double (*ht)[4 * ((M + 3) / 4)],
+                                             int j0, double (&acc)[(M + 3) / 4])
+{
+    constexpr int S = (M + 3) / 4;
+    constexpr int U = M - SRC * S < S ? M - SRC * S : S;  // columns i of lane SRC
+    if constexpr (U > 0) {
+        // all of this lane's h values first (one wait for the LDS), then the
+        // broadcasts and FMAs; columns past M hold zeros, so no lane-
+        // dependent guard is needed
+        double hv[U][S];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int t = 0; t < S; ++t) {
+                hv[u][t] = ht[SRC * S + u][j0 + t];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double zi = __builtin_amdgcn_update_dpp(z[u], z[u], SRC | (SRC << 2) | (SRC << 4) | (SRC << 6),
+                                                          0xf, 0xf, false);
+#pragma unroll
+            for (int t = 0; t < S; ++t) {
+                acc[t] = fma(zi, hv[u][t], acc[t]);
+            }
+        }
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void series_row(long row, int q, const BlockCluster &cl, const BlockUpdateArgs &g,
+                                           const double (*ht)[4 * ((M + 3) / 4)], double c0, double s, int K,
+                                           double escale,
+                                           double &sdn, double &sbn, double &mstep, double &manc)
+{
+    constexpr int S = (M + 3) / 4;
+    const long ld = g.ld;
+    const bool emit = g.em.zhi != nullptr;
+    const double *Bc = g.B + (long)cl.off * ld;
+    const double *Yc = g.Y + (long)cl.off * ld;
+    double *Bnc = g.Bn + (long)cl.off * ld;
+    const double dr = g.d[row];
+    const int j0 = q * S;
+    double b[S], z[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int j = j0 + t;
+        b[t] = j < M ? dr * Bc[row + (long)j * ld] - Yc[row + (long)j * ld] : 0.0;
+        z[t] = b[t];
+    }
+#pragma unroll 1
+    for (int k = 1; k < K; ++k) {
+        // h is read from LDS on every pass: hoisted out of the loop its
+        // values would not fit in registers
+        __asm__ volatile("" ::: "memory");
+        double acc[S];
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            acc[t] = -c0 * z[t];
+        }
+        series_terms<M, 0>(z, ht, j0, acc);
+        series_terms<M, 1>(z, ht, j0, acc);
+        series_terms<M, 2>(z, ht, j0, acc);
+        series_terms<M, 3>(z, ht, j0, acc);
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+            z[t] = fma(s, acc[t], b[t]);
+        }
+    }
+    bool bad = false;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        const int j = j0 + t;
+        if (j < M) {
+            const double ov = Bc[row + (long)j * ld];
+            double v = s * z[t];
+            if (g.damping < 1.0) {
+                v = (1.0 - g.damping) * ov + g.damping * v;
+            }
+            if (!isfinite(v)) {
+                bad = true;
+                v = 0.0;
+            }
+            Bnc[row + (long)j * ld] = v;
+            sdn += (v - ov) * (v - ov);
+            sbn += ov * ov;
+            if (emit) {
+                const double av = g.em.anc != nullptr ? g.em.anc[row + (long)(cl.off + j) * ld] : ov;
+                emit_store(g.em, g.em.mode, row, cl.off + j, v, av, escale);
+                mstep = fmax(mstep, fabs(v - ov));
+                manc = fmax(manc, fabs(v - av));
+            }
+        }
+    }
+    if (bad) {
+        atomicOr(&g.flags[0], 1);
+    }
+}
+
+// One block of a width-M cluster's update: bx < near_blocks_for(M) is a
+// near block, the rest are far blocks of kFarRows rows.  A row r is near
+// when |d_r - c0| <= 8 nrm or it is a row of the cluster (not a number:
+// far); near and far blocks apply the same test to the same values, so
+// every row is updated exactly once.
+template <int M>
+__device__ __forceinline__ void block_update_rows_m(int cid, int bx, const BlockUpdateArgs &g)
+{
+    constexpr int P = width_lanes(M);
     constexpr int GROUPS = 256 / P;
+    constexpr int NB = near_blocks_for(M);
     const BlockCluster cl = g.clusters[cid];
     const long ld = g.ld;
     const int t = threadIdx.x % P;
     const int grp = threadIdx.x / P;
-    const long row = (long)lblk * GROUPS + grp;
     const bool live = g.active[cid] != 0;
     double sdn = 0.0, sbn = 0.0;
     const bool emit = g.em.zhi != nullptr;
     const double escale = emit ? emit_scale(g.em, cid) : 1.0;
     double mstep = 0.0, manc = 0.0;
-    if (emit && live && lblk == 0 && grp == 0 && t < M) {
-        g.em.colscale[cl.off + t] = escale;
+    if (emit && live && bx == 0 && threadIdx.x < M) {
+        g.em.colscale[cl.off + threadIdx.x] = escale;
     }
     // h staged in LDS once per block (coalesced: column u of h is M
-    // consecutive rows of Y), hs[t][u] = h(u, t), so that the equations of
-    // every row of the block read it from there
+    // consecutive rows of Y), hs[t][u] = h(u, t)
     __shared__ double hs[M][M + 1];
+    __shared__ double ht[M][4 * ((M + 3) / 4)];  // ht[u][t] = h(u, t) (zero past M), for the far rows
     __shared__ double hsum[P];
+    __shared__ double cinfo[2];
     if (live) {
         const double *Yc = g.Y + (long)cl.off * ld;
         for (int e = threadIdx.x; e < M * M; e += 256) {
             const int u = e % M, tc = e / M;
-            hs[tc][u] = Yc[cl.first + u + (long)tc * ld];
+            const double v = Yc[cl.first + u + (long)tc * ld];
+            hs[tc][u] = v;
+            ht[u][tc] = v;
+        }
+        for (int e = threadIdx.x; e < M * (4 * ((M + 3) / 4) - M); e += 256) {
+            const int w = 4 * ((M + 3) / 4) - M;
+            ht[e / w][M + e % w] = 0.0;
         }
     }
     __syncthreads();
-    // off-diagonal absolute row sums of h (see block_update_rows)
-    if (threadIdx.x < P) {
-        double acc = 0.0;
-        if (threadIdx.x < M && live) {
+    if (live && threadIdx.x < 64) {
+        // off-diagonal absolute row sums of h (the elimination's dominance
+        // test), the centre c0 and the bound 8 ||h - c0 I||_1 of the near
+        // rows; lane sums in a fixed order, so every block gets the same
+        const int u0 = threadIdx.x;
+        double acc = 0.0, dg = 0.0;
+        if (u0 < M) {
+            dg = hs[u0][u0];
 #pragma unroll
             for (int u = 0; u < M; ++u) {
-                if (u != (int)threadIdx.x) {
-                    acc += fabs(hs[u][threadIdx.x]);
+                if (u != u0) {
+                    acc += fabs(hs[u][u0]);
                 }
             }
         }
-        hsum[threadIdx.x] = acc;
+        if (u0 < P) {
+            hsum[u0] = acc;
+        }
+        double sdg = dg;
+        for (int o = 1; o < 64; o <<= 1) {
+            sdg += __shfl_xor(sdg, o, 64);
+        }
+        const double c0 = sdg / (double)M;
+        double col = 0.0;
+        if (u0 < M) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                col += fabs(hs[u0][i] - (i == u0 ? c0 : 0.0));
+            }
+        }
+        for (int o = 1; o < 64; o <<= 1) {
+            col = fmax(col, __shfl_xor(col, o, 64));
+        }
+        if (u0 == 0) {
+            cinfo[0] = c0;
+            cinfo[1] = 8.0 * col;
+        }
     }
     __syncthreads();
-
-    if (row < g.n) {
-        const double *Bc = g.B + (long)cl.off * ld;
-        const double *Yc = g.Y + (long)cl.off * ld;
-        double *Bnc = g.Bn + (long)cl.off * ld;
-        if (!live) {
-            if (t < M) {
-                Bnc[row + (long)t * ld] = Bc[row + (long)t * ld];
+    if constexpr (M < kSeriesWidth) {
+        // narrow: every row eliminated, one row per lane group
+        const long row = (long)bx * GROUPS + grp;
+        if (row < g.n) {
+            if (live) {
+                eliminate_row<M>(row, t, cl, g, hs, hsum, escale, sdn, sbn, mstep, manc);
+            } else if (t < M) {
+                g.Bn[row + (long)(cl.off + t) * ld] = g.B[row + (long)(cl.off + t) * ld];
             }
-        } else if (row >= cl.first && row <= cl.last) {
-            if (t < M) {
-                const double nv = (row == cl.first + t) ? 1.0 : 0.0;
-                const double ov = Bc[row + (long)t * ld];
-                Bnc[row + (long)t * ld] = nv;
-                sdn += (nv - ov) * (nv - ov);
-                sbn += ov * ov;
-                if (emit) {
-                    const double av = g.em.anc != nullptr ? g.em.anc[row + (long)(cl.off + t) * ld] : ov;
-                    emit_store(g.em, g.em.mode, row, cl.off + t, nv, av, escale);
-                    mstep = fmax(mstep, fabs(nv - ov));
-                    manc = fmax(manc, fabs(nv - av));
+        }
+    }
+    const double c0 = live ? cinfo[0] : 0.0, thr = live ? cinfo[1] : 0.0;
+    // not a number (c0 or thr): no row but the cluster's is near, and the
+    // far path flags the non-finite results
+    auto is_near = [&](long r) { return (r >= cl.first && r <= cl.last) || fabs(g.d[r] - c0) <= thr; };
+    if constexpr (M < kSeriesWidth) {
+    } else if (bx < NB) {
+        if (live) {
+            // hull [lo, hi) of the near rows: the cluster's rows [first,
+            // last] widened by the rows next to them with d in [c0 - thr,
+            // c0 + thr], counted 256 at a time on each side (d ascending, so
+            // those rows are a run ending at first - 1 and a run starting at
+            // last + 1); no number: no rows counted
+            long lo = cl.first, hi = cl.last + 1;
+            for (;;) {
+                const long r = lo - 1 - threadIdx.x;
+                const int c = __syncthreads_count(r >= 0 && g.d[r] >= c0 - thr);
+                lo -= c;
+                if (c < 256) {
+                    break;
+                }
+            }
+            for (;;) {
+                const long r = hi + threadIdx.x;
+                const int c = __syncthreads_count(r < g.n && g.d[r] <= c0 + thr);
+                hi += c;
+                if (c < 256) {
+                    break;
+                }
+            }
+#pragma unroll 1
+            for (long base = lo + (long)bx * GROUPS; base < hi; base += (long)NB * GROUPS) {
+                const long row = base + grp;
+                if (row < hi && row < g.n && is_near(row)) {
+                    eliminate_row<M>(row, t, cl, g, hs, hsum, escale, sdn, sbn, mstep, manc);
+                }
+            }
+        }
+    } else {
+        const long row = (long)(bx - NB) * kFarRows + threadIdx.x / kFarLanes;
+        const int q = threadIdx.x % kFarLanes;
+        const bool in = row < g.n;
+        if (!live) {
+            if (in) {
+                const double *Bc = g.B + (long)cl.off * ld;
+                double *Bnc = g.Bn + (long)cl.off * ld;
+                for (int j = q; j < M; j += kFarLanes) {
+                    Bnc[row + (long)j * ld] = Bc[row + (long)j * ld];
                 }
             }
         } else {
-            const double dr = g.d[row];
-            // Lanes t >= M (padding up to P) load the equation of lane M-1:
-            // they are never the pivot lane, nothing reads their values, and
-            // unpredicated loads keep the register count of a power-of-two
-            // width.
-            const int te = t < M ? t : M - 1;
-            double a[M + 1];
-#pragma unroll
-            for (int u = 0; u < M; ++u) {
-                // h(u, t) = Y[first+u, off+t]
-                a[u] = (u == t ? dr : 0.0) - hs[te][u];
+            const bool far = in && !is_near(row);
+            const double s = far ? 1.0 / (g.d[row] - c0) : 0.0;
+            // the wave's term count from its largest ratio rho = |s| nrm
+            double rho = far ? fabs(s) * 0.125 * thr : 0.0;
+            for (int o = 1; o < 64; o <<= 1) {
+                rho = fmax(rho, __shfl_xor(rho, o, 64));
             }
-            a[M] = dr * Bc[row + (long)te * ld] - Yc[row + (long)te * ld];
-            int mypc = -1;
-            double mypiv = 1.0;
-            bool bad = false;
-            const bool mine = t >= M || fabs(dr - hs[te][te]) > 2.0 * hsum[t];
-            const unsigned long long ball = __ballot(mine);
-            const int base = (threadIdx.x & 63) & ~(P - 1);
-            const unsigned long long gmask = ((1ULL << P) - 1ULL) << base;
-            const bool dominant = (ball & gmask) == gmask;
-            if (dominant) {
-                dominant_steps<M, P, 0>(a, t, bad, mypc, mypiv);
-            } else {
-                bool used = false;
-#pragma unroll
-                for (int pc = 0; pc < M; ++pc) {
-                    if (!bad) {
-                        double cand = (t < M && !used) ? fabs(a[pc]) : -1.0;
-                        int cidx = t;
-#pragma unroll
-                        for (int o = 1; o < P; o <<= 1) {
-                            const double oc = __shfl_xor(cand, o, P);
-                            const int oi = __shfl_xor(cidx, o, P);
-                            if (oc > cand || (oc == cand && oi < cidx)) {
-                                cand = oc;
-                                cidx = oi;
-                            }
-                        }
-                        if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
-                            bad = true;
-                        } else {
-                            const double piv = __shfl(a[pc], cidx, P);
-                            const bool me = t == cidx;
-                            const double alpha = me ? 0.0 : -(a[pc] / piv);
-#pragma unroll
-                            for (int c = pc + 1; c <= M; ++c) {
-                                a[c] = fma(alpha, __shfl(a[c], cidx, P), a[c]);
-                            }
-                            if (me) {
-                                used = true;
-                                mypc = pc;
-                                mypiv = piv;
-                            }
-                        }
-                    }
-                }
+            int K = 1;
+            if (rho > 0.0) {
+                K = (int)ceil(54.0 / -log2(rho));
+                K = K < 1 ? 1 : K > 20 ? 20 : K;
             }
-            if (bad) {
-                if (t == 0) {
-                    atomicOr(&g.flags[0], 1);
-                }
-            } else if (mypc >= 0) {
-                double v = a[M] / mypiv;
-                const double ov = Bc[row + (long)mypc * ld];
-                if (g.damping < 1.0) {
-                    v = (1.0 - g.damping) * ov + g.damping * v;
-                }
-                if (!isfinite(v)) {
-                    atomicOr(&g.flags[0], 1);
-                    v = 0.0;
-                }
-                Bnc[row + (long)mypc * ld] = v;
-                sdn += (v - ov) * (v - ov);
-                sbn += ov * ov;
-                if (emit) {
-                    const double av = g.em.anc != nullptr ? g.em.anc[row + (long)(cl.off + mypc) * ld] : ov;
-                    emit_store(g.em, g.em.mode, row, cl.off + mypc, v, av, escale);
-                    mstep = fmax(mstep, fabs(v - ov));
-                    manc = fmax(manc, fabs(v - av));
-                }
+            // the quad's lanes exchange values by DPP: all four take the
+            // row's path together (far is the same for the four)
+            if (far) {
+                series_row<M>(row, q, cl, g, ht, c0, s, K, escale, sdn, sbn, mstep, manc);
             }
         }
     }
     if (emit) {
-        emit_block_max(g.em, (long)cid * g.pstride + lblk, mstep, manc);
+        emit_block_max(g.em, (long)cid * g.pstride + bx, mstep, manc);
     }
     __shared__ double sa[4], sb[4];
     for (int o = 32; o > 0; o >>= 1) {
@@ -266,13 +501,14 @@ __device__ __forceinline__ void block_update_rows_m(int cid, int lblk, const Blo
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const long slot = (long)cid * g.pstride + lblk;
+        const long slot = (long)cid * g.pstride + bx;
         g.partials[2 * slot] = sa[0] + sa[1] + sa[2] + sa[3];
         g.partials[2 * slot + 1] = sb[0] + sb[1] + sb[2] + sb[3];
     }
 }
 
-// blockIdx.y = cluster list[blockIdx.y] (all of width M), blockIdx.x = row block.
+// blockIdx.y = cluster list[blockIdx.y] (all of width M), blockIdx.x = near
+// or far block.
 template <int M>
 __global__ __launch_bounds__(256) void k_block_update_m(BlockUpdateArgs g, const int *__restrict__ list)
 {
@@ -283,9 +519,7 @@ __global__ __launch_bounds__(256) void k_block_update_m(BlockUpdateArgs g, const
 template <int M>
 int launch_block_width(const int *list, int count, const BlockUpdateArgs &g, hipStream_t stream)
 {
-    constexpr int P = M <= 2 ? 2 : M <= 4 ? 4 : M <= 8 ? 8 : M <= 16 ? 16 : 32;
-    constexpr int rows = 256 / P;
-    const dim3 grid((unsigned)((g.n + rows - 1) / rows), (unsigned)count);
+    const dim3 grid((unsigned)update_blocks_for(M, g.n), (unsigned)count);
     hipLaunchKernelGGL(k_block_update_m<M>, grid, dim3(256), 0, stream, g, list);
     return hipGetLastError() == hipSuccess ? IPT_CUDA_SUCCESS : IPT_CUDA_CUDA_ERROR;
 }
