@@ -890,6 +890,121 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
         }
     }
     rebuild_runs();
+    // Status of the exact steps: the finalisation's status block (status
+    // bits, active flags, fixed-point residuals).  Lagged (IPT_BLOCK_LAGGED_
+    // STATUS=1; the default when every step is exact, the mixed path's
+    // exact steps being few): it is queued to pinned memory and read after
+    // the next step has been enqueued, so the GPU does not wait for the
+    // host's round trip.  The host's view (products, frozen launches, the
+    // next exact step, the stop) is then one step behind: a cluster found
+    // inactive has its columns in both iterate buffers after that next step
+    // (its update copies them), and the loop may run one step past the last
+    // cluster's convergence, which changes no iterate.
+    const size_t nstat = (size_t)2 + 2 * plan.nclusters;
+    const bool lagged = nstat <= 4096 && ctx->host_status != nullptr &&
+                        env_int("IPT_BLOCK_LAGGED_STATUS", mixed ? 0 : 1) != 0;
+    struct EventHolder {
+        hipEvent_t e = nullptr;
+        ~EventHolder()
+        {
+            if (e != nullptr) {
+                (void)hipEventDestroy(e);
+            }
+        }
+    } stat_ev;
+    if (lagged) {
+        IPT_HIPCHK(hipEventCreateWithFlags(&stat_ev.e, hipEventDisableTiming));
+    }
+    int pending = -1;  // exact step whose status is queued and not yet read
+    bool done = false;
+    // Host side of an exact step's status (sit: its step): frozen clusters,
+    // the products' runs, the next exact step, errors and the stop.
+    // copy_frozen: copy newly frozen clusters to the other iterate buffer
+    // (not needed when the following step has already been enqueued).
+    auto process_status = [&](int sit, const double *stv, bool copy_frozen) -> int {
+        int flags[3];
+        flags[0] = 0;
+        flags[1] = (int)stv[0];
+        flags[2] = (int)stv[1];
+        for (int c = 0; c < plan.nclusters; ++c) {
+            act[(size_t)c] = stv[(size_t)2 + c] != 0.0 ? 1 : 0;
+            fp_now[(size_t)c] = stv[(size_t)2 + plan.nclusters + c];
+        }
+        // After an exact step an inactive cluster is frozen for good: its
+        // columns are in both iterate buffers, and launches whose clusters
+        // are all frozen are skipped from here on.
+        for (int c = 0; c < plan.nclusters; ++c) {
+            if (!act[(size_t)c] && !plan.frozen[(size_t)c]) {
+                if (copy_frozen) {
+                    const size_t o = (size_t)plan.host[(size_t)c].off * n;
+                    IPT_HIPCHK(hipMemcpyAsync(nxt + o, cur + o, (size_t)n * plan.host[(size_t)c].m * sizeof(double),
+                                              hipMemcpyDeviceToDevice, ctx->stream));
+                }
+                plan.frozen[(size_t)c] = 1;
+            }
+        }
+        rebuild_runs();
+        if (mixed) {
+            int steps = anchor_every + 1;
+            if (t_prev >= 0) {
+                // Dense input: the low-precision product costs the same
+                // whatever the active width, so the next exact step waits
+                // for the slowest cluster.  CSR input: products cost in
+                // proportion to the active width, so the next exact step
+                // comes when the first cluster is predicted to converge,
+                // which freezes it and narrows the following products.
+                double need = 0.0, need_first = INFINITY;
+                bool known = true;
+                for (int c = 0; c < plan.nclusters; ++c) {
+                    if (!act[(size_t)c]) {
+                        continue;
+                    }
+                    const double f1 = fp_now[(size_t)c], f0 = fp_prev[(size_t)c];
+                    if (!(f1 > 0.0) || !(f0 > f1) || !std::isfinite(f0)) {
+                        known = false;
+                        continue;
+                    }
+                    const double rate = std::log(f1 / f0) / (double)(sit - t_prev);  // < 0
+                    const double nc = std::log(p.tol / f1) / rate;
+                    need = std::max(need, nc);
+                    need_first = std::min(need_first, nc);
+                }
+                // CSR: 10% margin, a late exact step costing less than an
+                // extra one.  Dense: none; by the time one cluster is left
+                // the exact product covers only its columns and costs about
+                // two low-precision steps, so an early exact step costs no
+                // more than a late one.
+                if (sparse_products && std::isfinite(need_first)) {
+                    steps = std::min(steps, std::max(2, (int)std::ceil(1.1 * need_first) + 1));
+                } else if (!sparse_products && known) {
+                    steps = std::min(steps, std::max(2, (int)std::ceil(need)));
+                }
+            }
+            fp_prev = fp_now;
+            t_prev = sit;
+            next_exact = sit + steps;
+            if (p.verbose) {
+                double worst = 0.0;
+                int worst_c = -1;
+                for (int c = 0; c < plan.nclusters; ++c) {
+                    if (act[(size_t)c] && !(fp_now[(size_t)c] <= worst)) {
+                        worst = fp_now[(size_t)c];
+                        worst_c = c;
+                    }
+                }
+                fprintf(stderr, "IPT block-cluster mixed: exact step %d, %d clusters active, largest fp %.3e "
+                        "(cluster %d), next exact step %d\n", sit, flags[2], worst, worst_c, next_exact);
+            }
+        }
+        if (flags[1] != 0) {
+            fprintf(stderr,
+                    "IPT block-cluster: block solve failed (%s) at iteration %d; identity fallback is disabled\n",
+                    (flags[1] & 2) ? "singular" : "non-finite norm", sit + 1);
+            return IPT_CUDA_INVALID_ARGUMENT;
+        }
+        done = flags[2] == 0;
+        return IPT_CUDA_SUCCESS;
+    };
     for (int it = 0; it < p.maxiter && !runs.empty(); ++it) {
         const bool exact = !mixed || it == next_exact || it == p.maxiter - 1;
         const size_t ho = (size_t)hull_lo * n;
@@ -947,96 +1062,43 @@ int block_pipeline(Context *ctx, const ApplyFn &apply, const ApplyFn &identity_p
                                       ctx->stream));
         }
         std::swap(cur, nxt);
+        if (pending >= 0) {
+            // the status of the previous exact step, read while this step
+            // runs on the GPU
+            IPT_HIPCHK(hipEventSynchronize(stat_ev.e));
+            trace_mark("exact step status on the host");
+            const int sit = pending;
+            pending = -1;
+            IPT_STCHK(process_status(sit, ctx->host_status, false));
+            if (done) {
+                break;
+            }
+        }
         if (!exact) {
             continue;
         }
-        int flags[3];
+        if (lagged) {
+            IPT_HIPCHK(hipMemcpyAsync(ctx->host_status, plan.status.p, nstat * sizeof(double), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+            IPT_HIPCHK(hipEventRecord(stat_ev.e, ctx->stream));
+            pending = it;
+            continue;
+        }
         {
             // one read-back per exact step (the finalisation's status block)
-            std::vector<double> stv((size_t)2 + 2 * plan.nclusters);
+            std::vector<double> stv(nstat);
             IPT_STCHK(read_host(ctx, stv.data(), plan.status.p, stv.size() * sizeof(double)));
             trace_mark("exact step status on the host");
-            flags[0] = 0;
-            flags[1] = (int)stv[0];
-            flags[2] = (int)stv[1];
-            for (int c = 0; c < plan.nclusters; ++c) {
-                act[(size_t)c] = stv[(size_t)2 + c] != 0.0 ? 1 : 0;
-                fp_now[(size_t)c] = stv[(size_t)2 + plan.nclusters + c];
-            }
+            IPT_STCHK(process_status(it, stv.data(), true));
         }
-        // After an exact step an inactive cluster is frozen for good: its
-        // columns go to the other iterate buffer once, and launches whose
-        // clusters are all frozen are skipped from here on.
-        for (int c = 0; c < plan.nclusters; ++c) {
-            if (!act[(size_t)c] && !plan.frozen[(size_t)c]) {
-                const size_t o = (size_t)plan.host[(size_t)c].off * n;
-                IPT_HIPCHK(hipMemcpyAsync(nxt + o, cur + o, (size_t)n * plan.host[(size_t)c].m * sizeof(double),
-                                          hipMemcpyDeviceToDevice, ctx->stream));
-                plan.frozen[(size_t)c] = 1;
-            }
-        }
-        rebuild_runs();
-        if (mixed) {
-            int steps = anchor_every + 1;
-            if (t_prev >= 0) {
-                // Dense input: the low-precision product costs the same
-                // whatever the active width, so the next exact step waits
-                // for the slowest cluster.  CSR input: products cost in
-                // proportion to the active width, so the next exact step
-                // comes when the first cluster is predicted to converge,
-                // which freezes it and narrows the following products.
-                double need = 0.0, need_first = INFINITY;
-                bool known = true;
-                for (int c = 0; c < plan.nclusters; ++c) {
-                    if (!act[(size_t)c]) {
-                        continue;
-                    }
-                    const double f1 = fp_now[(size_t)c], f0 = fp_prev[(size_t)c];
-                    if (!(f1 > 0.0) || !(f0 > f1) || !std::isfinite(f0)) {
-                        known = false;
-                        continue;
-                    }
-                    const double rate = std::log(f1 / f0) / (double)(it - t_prev);  // < 0
-                    const double nc = std::log(p.tol / f1) / rate;
-                    need = std::max(need, nc);
-                    need_first = std::min(need_first, nc);
-                }
-                // CSR: 10% margin, a late exact step costing less than an
-                // extra one.  Dense: none; by the time one cluster is left
-                // the exact product covers only its columns and costs about
-                // two low-precision steps, so an early exact step costs no
-                // more than a late one.
-                if (sparse_products && std::isfinite(need_first)) {
-                    steps = std::min(steps, std::max(2, (int)std::ceil(1.1 * need_first) + 1));
-                } else if (!sparse_products && known) {
-                    steps = std::min(steps, std::max(2, (int)std::ceil(need)));
-                }
-            }
-            fp_prev = fp_now;
-            t_prev = it;
-            next_exact = it + steps;
-            if (p.verbose) {
-                double worst = 0.0;
-                int worst_c = -1;
-                for (int c = 0; c < plan.nclusters; ++c) {
-                    if (act[(size_t)c] && !(fp_now[(size_t)c] <= worst)) {
-                        worst = fp_now[(size_t)c];
-                        worst_c = c;
-                    }
-                }
-                fprintf(stderr, "IPT block-cluster mixed: exact step %d, %d clusters active, largest fp %.3e "
-                        "(cluster %d), next exact step %d\n", it, flags[2], worst, worst_c, next_exact);
-            }
-        }
-        if (flags[1] != 0) {
-            fprintf(stderr,
-                    "IPT block-cluster: block solve failed (%s) at iteration %d; identity fallback is disabled\n",
-                    (flags[1] & 2) ? "singular" : "non-finite norm", it + 1);
-            return IPT_CUDA_INVALID_ARGUMENT;
-        }
-        if (flags[2] == 0) {
+        if (done) {
             break;
         }
+    }
+    if (pending >= 0) {
+        // the last step's status (the loop ended at the step limit)
+        IPT_HIPCHK(hipEventSynchronize(stat_ev.e));
+        IPT_STCHK(process_status(pending, ctx->host_status, false));
     }
     // per-cluster iterations and residuals; stalled clusters start the
     // Rayleigh-Ritz basis again from their identity blocks

@@ -185,6 +185,7 @@ int context_init(Context *ctx, hipStream_t stream)
     }
     IPT_HIPCHK(hipHostMalloc((void **)&ctx->host_scalars, 4096 * sizeof(double), hipHostMallocDefault));
     IPT_HIPCHK(hipHostMalloc((void **)&ctx->host_flags, 4096 * sizeof(int), hipHostMallocDefault));
+    IPT_HIPCHK(hipHostMalloc((void **)&ctx->host_status, 4096 * sizeof(double), hipHostMallocDefault));
     return IPT_CUDA_SUCCESS;
 }
 
@@ -204,6 +205,9 @@ void context_destroy(Context *ctx)
     }
     if (ctx->host_scalars != nullptr) {
         (void)hipHostFree(ctx->host_scalars);
+    }
+    if (ctx->host_status != nullptr) {
+        (void)hipHostFree(ctx->host_status);
     }
     if (ctx->host_flags != nullptr) {
         (void)hipHostFree(ctx->host_flags);

@@ -42,6 +42,7 @@ struct Context {
     bool own_stream = false;
     double *host_scalars = nullptr;  // pinned, 4096 doubles
     int *host_flags = nullptr;       // pinned, 4096 ints
+    double *host_status = nullptr;   // pinned, 4096 doubles: the block iteration's queued status read
     void *host_pinned = nullptr;     // pinned staging for device-to-host reads, grows
     size_t host_pinned_bytes = 0;
     void *slots[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
