@@ -16,7 +16,7 @@ from block_ipt_amd.abi import IptError  # noqa: E402
 def block_env(monkeypatch):
     monkeypatch.setenv("IPT_BLOCK_CLUSTER", "1")
     for v in ("IPT_DAVIDSON_ENRICH", "IPT_DAVIDSON_STEPS", "IPT_BLOCK_CLUSTER_MAXITER", "IPT_BLOCK_CLUSTER_TOL",
-              "IPT_COUPLING_GAP_RATIO_TOL", "IPT_MAX_LIFT_BLOCK_SIZE"):
+              "IPT_COUPLING_GAP_RATIO_TOL", "IPT_MAX_LIFT_BLOCK_SIZE", "IPT_BLOCK_LAGGED_STATUS"):
         monkeypatch.delenv(v, raising=False)
     return monkeypatch
 
@@ -55,6 +55,35 @@ def test_dense_block_matches_oracle_and_lapack(block_env, n, k, seed, hi, vs):
     ov, ovec, oinfo = orc.block_pipeline(k, dense=H, maxiter=400, tol=1e-12)
     assert np.abs(gl - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
     assert info["basis_cols"] == oinfo["basis_cols"]
+    assert info["iterations"] == oinfo["max_iterations"]
+
+
+def _far_rows_case(n=4000, k=16, seed=5, vs=0.055):
+    """k targets packed in [0, 0.01] (one cluster), the other rows dense in
+    [0.05, 0.4]: about 700 rows lie within 8 ||h - c0 I||_1 of the cluster's
+    centre, more than one pass of the near-row blocks (512 rows)."""
+    rng = np.random.default_rng(seed)
+    d = np.concatenate([np.sort(rng.uniform(0.0, 0.01, k)), np.sort(rng.uniform(0.05, 0.4, n - k))])
+    V = rng.standard_normal((n, n)) * vs / np.sqrt(n)
+    H = np.triu(V, 1)
+    H = H + H.T
+    H[np.diag_indices(n)] = d
+    return H
+
+
+@pytest.mark.parametrize("lagged", ["0", "1"])
+def test_near_and_far_rows_of_a_wide_cluster(block_env, lagged):
+    # width 16: near rows by elimination over two passes, far rows by the
+    # series; the same pairs as LAPACK and the oracle's iteration count,
+    # with the status read at once or one step late
+    block_env.setenv("IPT_BLOCK_LAGGED_STATUS", lagged)
+    H = _far_rows_case()
+    rg, _ = orc.target_clusters(16, dense=H)
+    assert rg == [(0, 15)]
+    gv, gl, info = api.ipt_cuda_tol(H, 16, 1e-12, 400)
+    _check_eigs(H, gv, gl, 16)
+    ov, ovec, oinfo = orc.block_pipeline(16, dense=H, maxiter=400, tol=1e-12)
+    assert np.abs(gl - ov).max() <= 1e-10 * max(1.0, np.abs(ov).max())
     assert info["iterations"] == oinfo["max_iterations"]
 
 

@@ -68,19 +68,22 @@ def _scaled_residuals(H, X, lam):
 def mixed_env(monkeypatch):
     monkeypatch.setenv("IPT_BLOCK_CLUSTER", "1")
     for v in ("IPT_DAVIDSON_ENRICH", "IPT_BLOCK_CLUSTER_MAXITER", "IPT_BLOCK_CLUSTER_TOL", "IPT_PRECISION",
-              "IPT_MIXED_ANCHOR_EVERY"):
+              "IPT_MIXED_ANCHOR_EVERY", "IPT_BLOCK_LAGGED_STATUS"):
         monkeypatch.delenv(v, raising=False)
     return monkeypatch
 
 
-@pytest.mark.parametrize("mode", ["fp16", "bf16", "bf16x3", "fp32"])
-def test_mixed_block_solve_matches_fp64(mixed_env, mode):
-    """C1 recipe (N=2048, k=16, d ~ U[0,10], V scale 0.05/sqrt(N))."""
+@pytest.mark.parametrize("mode,lagged", [("fp16", "0"), ("bf16", "0"), ("bf16x3", "0"), ("fp32", "0"),
+                                         ("fp16", "1")])
+def test_mixed_block_solve_matches_fp64(mixed_env, mode, lagged):
+    """C1 recipe (N=2048, k=16, d ~ U[0,10], V scale 0.05/sqrt(N)); the
+    exact steps' status read at once or one step late."""
     n, k = 2048, 16
     H = orc.rand_dense(n, 0, 0.0, 10.0, 0.05)
     v64, l64, i64 = api.ipt_cuda_tol(H, k, 1e-12, 400)
     mixed_env.setenv("IPT_PRECISION", mode)
     mixed_env.setenv("IPT_MIXED_ANCHOR_EVERY", "10")
+    mixed_env.setenv("IPT_BLOCK_LAGGED_STATUS", lagged)
     vm, lm, im = api.ipt_cuda_tol(H, k, 1e-12, 400)
     assert np.abs(lm - l64).max() <= 1e-10 * max(1.0, np.abs(l64).max())
     w = sl.eigh(H, eigvals_only=True, subset_by_index=[0, k - 1])
