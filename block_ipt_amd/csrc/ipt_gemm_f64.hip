@@ -10,6 +10,8 @@
 // CUs x 4); split-K partials are summed in a fixed order by a second
 // kernel, so results are deterministic.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "ipt_internal.h"
 
@@ -151,19 +153,74 @@ static void launch_gemm(const ProfileToken &prof, dim3 grid, hipStream_t st, con
                  ncols, kchunk);
 }
 
+// Workgroups of `kernel` (256 threads, static LDS only) resident at once on
+// the current device: compute units x workgroups per unit at the kernel's
+// register and LDS footprint (cached per kernel).
+static int resident_workgroups(const void *kernel)
+{
+    static std::mutex mu;
+    static std::map<const void *, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = cache.find(kernel);
+    if (it != cache.end()) {
+        return it->second;
+    }
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1024;
+    }
+    const int v = std::max(1, cus * per);
+    cache[kernel] = v;
+    return v;
+}
+
+// k splits of a product with gx row tiles: about 1024 workgroups, rounded to
+// whole rounds of the resident ones (a last round that fills a third of
+// the machine would run at a third of its rate), each k chunk at least 256.
+static int product_splits(int gx, int resident, int n)
+{
+    const int rounds = std::max(1, (1024 + resident / 2) / resident);
+    int s = (rounds * resident + gx / 2) / gx;
+    s = std::min(s, std::max(1, (n + 255) / 256));
+    return std::max(1, s);
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_gemm_f64_packed(const double2 *__restrict__ PA, int nkp,
+                                                         const double *__restrict__ Z, long ldz,
+                                                         double *__restrict__ Y, long ldy, long split_stride, int n,
+                                                         int ncols, long kchunk);
+
+template <int NB>
+static const void *gemm_kernel(bool packed)
+{
+    return packed ? reinterpret_cast<const void *>(k_gemm_f64_packed<NB>)
+                  : reinterpret_cast<const void *>(k_gemm_f64<NB>);
+}
+
+static const void *gemm_kernel_nb(int NB, bool packed)
+{
+    switch (NB) {
+    case 1: return gemm_kernel<1>(packed);
+    case 2: return gemm_kernel<2>(packed);
+    case 3: return gemm_kernel<3>(packed);
+    case 4: return gemm_kernel<4>(packed);
+    case 5: return gemm_kernel<5>(packed);
+    case 6: return gemm_kernel<6>(packed);
+    case 7: return gemm_kernel<7>(packed);
+    default: return gemm_kernel<8>(packed);
+    }
+}
+
 static int gemm_f64_pass(Context *ctx, const double *H, long ldh, const double *Z, long ldz,
                          double *Y, long ldy, int n, int ncols)
 {
     const int NB = (ncols + 15) / 16;  // 1..8
     const int gx = (n + 63) / 64;
-    int splits = (1024 + gx - 1) / gx;
-    const int max_splits = (n + 255) / 256;
-    if (splits > max_splits) {
-        splits = max_splits;
-    }
-    if (splits < 1) {
-        splits = 1;
-    }
+    int splits = product_splits(gx, resident_workgroups(gemm_kernel_nb(NB, false)), n);
     long kchunk = ((long)n + splits - 1) / splits;
     kchunk = (kchunk + 15) / 16 * 16;
     splits = (int)(((long)n + kchunk - 1) / kchunk);
@@ -396,8 +453,7 @@ static int gemm_packed_pass(Context *ctx, const F64Packed &op, const double *Z, 
     const int n = op.n;
     const int NB = (ncols + 15) / 16;
     const int gx = (n + 127) / 128;
-    // about 1024 workgroups, k chunks of at least 256 (multiples of 16)
-    int splits = std::max(1, std::min((1024 + gx - 1) / gx, (n + 255) / 256));
+    int splits = product_splits(gx, resident_workgroups(gemm_kernel_nb(NB, true)), n);
     long kchunk = ((long)n + splits - 1) / splits;
     kchunk = (kchunk + 15) / 16 * 16;
     splits = (int)(((long)n + kchunk - 1) / kchunk);
