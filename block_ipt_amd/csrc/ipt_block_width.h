@@ -3,13 +3,14 @@
 //
 // The elimination is that of block_update_rows<P> in ipt_block_kernels.hip
 // (row r owned by P = next_pow2(M) lanes, lane t holding equation t of
-// (d_r I - h^T) x = d_r B_r - (HB)_r, implicit partial pivoting by a
-// shuffle arg-max, the column-dominance shortcut); the loops run to M
-// instead of P, the pivot rows are not scaled (each unknown is divided by
-// its pivot once at the end, one FMA per eliminated entry), and when every
-// column of a row's system is dominant the pivot broadcasts are DPP moves
-// instead of LDS shuffles: the same pivots, results within a few units of
-// rounding.  A width's kernel then holds only the pivot steps and
+// (d_r I - h^T) x = d_r B_r - (HB)_r, implicit partial pivoting, the
+// column-dominance shortcut); the loops run to M instead of P, the pivot
+// rows are not scaled (each unknown is divided by its pivot once at the
+// end, one FMA per eliminated entry), the pivot search is a DPP butterfly
+// instead of a shuffle reduction, and when every column of a row's system
+// is dominant the pivot broadcasts are DPP moves instead of LDS shuffles:
+// the same pivots, results within a few units of rounding.  A width's
+// kernel then holds only the pivot steps and
 // columns it uses: one kernel for every width up to 32 with the loops run
 // to P needs ~120 KB of instructions (twice the 64 KB instruction cache a
 // CU pair shares) and the registers of its widest path in every block.
@@ -68,6 +69,68 @@ __device__ __forceinline__ double group_bcast(double v)
         const unsigned olo = L < 16 ? rl[0] : rl[1];
         const unsigned ohi = L < 16 ? rh[0] : rh[1];
         return __longlong_as_double((long long)(((unsigned long long)ohi << 32) | olo));
+    }
+}
+
+// The value of the lane selected by the DPP control CTRL (quad_perm,
+// row_half_mirror, row_mirror), per 32-bit half for doubles.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+
+// The value of lane i ^ 16 (the other 16-lane row of a 32-lane group), by
+// gfx950's permlane16_swap: of the swap of v with itself, a lane of an even
+// row takes the second result, a lane of an odd row the first.
+__device__ __forceinline__ unsigned other_row_u32(unsigned v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (threadIdx.x & 16) ? r[0] : r[1];
+}
+
+__device__ __forceinline__ double other_row_f64(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = other_row_u32((unsigned)u), hi = other_row_u32((unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Arg-max over the P lanes of a group (largest cand, ties to the lowest
+// index), every lane of the group left with the result: a butterfly of DPP
+// exchanges (lane ^ 1, lane ^ 2 by quad_perm; the mirrored half row and the
+// mirrored row pair up the quads and the halves; permlane16_swap the two
+// rows of a 32-lane group).  The order is total, so the result is that of
+// any pairing: the same pivot as a shuffle reduction, without LDS traffic.
+template <int P>
+__device__ __forceinline__ void group_argmax(double &cand, int &cidx)
+{
+    auto take = [&](double oc, int oi) {
+        if (oc > cand || (oc == cand && oi < cidx)) {
+            cand = oc;
+            cidx = oi;
+        }
+    };
+    take(dpp_f64<0xB1>(cand), dpp_i32<0xB1>(cidx));  // quad_perm [1,0,3,2]
+    if constexpr (P >= 4) {
+        take(dpp_f64<0x4E>(cand), dpp_i32<0x4E>(cidx));  // quad_perm [2,3,0,1]
+    }
+    if constexpr (P >= 8) {
+        take(dpp_f64<0x141>(cand), dpp_i32<0x141>(cidx));  // row_half_mirror
+    }
+    if constexpr (P >= 16) {
+        take(dpp_f64<0x140>(cand), dpp_i32<0x140>(cidx));  // row_mirror
+    }
+    if constexpr (P >= 32) {
+        const double oc = other_row_f64(cand);
+        const int oi = (int)other_row_u32((unsigned)cidx);
+        take(oc, oi);
     }
 }
 
@@ -158,15 +221,7 @@ __device__ __forceinline__ void eliminate_row(long row, int t, const BlockCluste
             if (!bad) {
                 double cand = (t < M && !used) ? fabs(a[pc]) : -1.0;
                 int cidx = t;
-#pragma unroll
-                for (int o = 1; o < P; o <<= 1) {
-                    const double oc = __shfl_xor(cand, o, P);
-                    const int oi = __shfl_xor(cidx, o, P);
-                    if (oc > cand || (oc == cand && oi < cidx)) {
-                        cand = oc;
-                        cidx = oi;
-                    }
-                }
+                group_argmax<P>(cand, cidx);
                 if (!(cand > 64.0 * DBL_EPSILON) || !isfinite(cand)) {
                     bad = true;
                 } else {
