@@ -223,7 +223,7 @@ def sparse_workload(a):
         dvec, dval, _, _ = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, nnz, 1e-12, 200)
         api.free_device_result(dvec, dval)
     api.profile_reset()
-    api.profile_enable(True, only=["spmm", "spmm_f32"])
+    api.profile_enable(True, only=["spmm", "spmm_f32", "spmm_tiles"])
     hip.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -236,6 +236,8 @@ def sparse_workload(a):
     api.profile_enable(False)
     sp_ms, sp_launches = api.profile_get("spmm")
     lp_ms, lp_launches = api.profile_get("spmm_f32")
+    tl_ms, tl_products = api.profile_get("spmm_tiles")
+    tl_cols = api.profile_columns("spmm_tiles")
     stats = api.last_solve_stats()
     # independent residuals: H X with the library's CSR product, norms on the host
     X = hip.to_host(dvec, (n, k))
@@ -272,6 +274,10 @@ def sparse_workload(a):
         "config": {"workload": "sparse orbital-sum model N=%d, nnz=%d, k=%d lowest, seed %d" % (n, nnz, k, a.seed),
                    "precision": a.sparse_precision},
         "converged_pairs": converged, "max_residual": float(res.max()), "solve_stats": stats,
+        # products of at most 4 columns through the LDS-tiled layout
+        # (k_spmm_tiles, one matrix pass per two columns), per timed solve
+        "narrow_tiled_spmm": {"ms_per_solve": tl_ms / max(1, a.steps), "products": tl_products,
+                              "avg_columns": tl_cols / max(1, tl_products)},
         "roofline": {"bound": "hbm", "achieved": alg / (avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "k_spmm_rowmajor",
                      "avg_ms": avg, "launches": sp_launches, "algorithmic_bytes": alg,
@@ -457,18 +463,24 @@ def spmm_workload(a):
         comp = 12.0 * nnz + 4.0 * (n + 1) + 16.0 * n * k
         # one pass over the whole matrix, then column-sliced passes (the
         # automatic count of the block path, and fixed counts for k <= 8)
-        counts = [None, 0] + ([2, 4, 8, 16] if k <= 8 else [])
+        # and, for k <= 4, the LDS-tiled product (its layout built on each
+        # call, outside the timed launches)
+        counts = [None, 0] + ([2, 4, 8, 16] if k <= 8 else []) + (["tiled"] if k <= 4 else [])
         for sl in counts:
+            fam = "spmm_tiles" if sl == "tiled" else "spmm"
             api.profile_reset()
-            api.profile_enable(True, only=["spmm"])
+            api.profile_enable(True, only=[fam])
             for _ in range(max(1, a.steps)):
                 if sl is None:
                     api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
+                elif sl == "tiled":
+                    api.csr_spmm_tiled(drp, dci, dv, n, dX, k, dY, k, k)
                 else:
                     api.csr_spmm_sliced(drp, dci, dv, n, dX, k, dY, k, k, sl)
             api.profile_enable(False)
-            ms, cnt = api.profile_get("spmm")
-            key = "k%d" % k if sl is None else "k%d_slices%s" % (k, "auto" if sl == 0 else sl)
+            ms, cnt = api.profile_get(fam)
+            key = ("k%d" % k if sl is None else "k%d_tiled" % k if sl == "tiled"
+                   else "k%d_slices%s" % (k, "auto" if sl == 0 else sl))
             out[key] = {"avg_ms": ms / cnt, "launches": cnt, "compulsory_bytes": comp,
                         "compulsory_GB/s": comp / (ms / cnt) / 1e6}
     for p in (drp, dci, dv, dX, dY):

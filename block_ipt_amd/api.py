@@ -402,6 +402,15 @@ def csr_spmm_sliced(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, sl
         raise IptError(st, "ipt_hip_csr_spmm_sliced")
 
 
+def csr_spmm_tiled(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, handle=None):
+    """The CSR product through the LDS-tiled layout (built on each call;
+    columns ascending per row)."""
+    st = lib().ipt_hip_csr_spmm_tiled(_vpp(d_row_ptr), _vpp(d_col_ind), _vpp(d_values), int(n), _vpp(d_X),
+                                      int(ldx), _vpp(d_Y), int(ldy), int(k), handle)
+    if st != 0:
+        raise IptError(st, "ipt_hip_csr_spmm_tiled")
+
+
 def cholqr_mixed(d_Q, n, k, handle=None):
     st = lib().ipt_hip_cholqr_mixed(_vpp(d_Q), int(n), int(k), handle)
     if st != 0:

@@ -34,6 +34,7 @@ SOURCES = [
     "ipt_gen_sparse.hip",
     "ipt_sparse.hip",
     "ipt_sparse_lp.hip",
+    "ipt_csr_tiles.hip",
     "ipt_acx.hip",
     "ipt_mlp.hip",
     "ipt_profile.cpp",

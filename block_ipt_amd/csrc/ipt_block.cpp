@@ -1689,6 +1689,36 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
     return IPT_CUDA_SUCCESS;
 }
 
+// The LDS-tiled layout for the narrow products of a large CSR
+// (ipt_csr_tiles.hip): IPT_CSR_TILES = 0 never, 1 (default) when one column
+// of X (8n bytes) exceeds an XCD's 4 MiB L2, 2 always.  tl->groups stays 0
+// when not taken, also when a row's columns do not ascend (the layout's own
+// check); *maxw = widest product taken through it (IPT_CSR_TILES_WIDTH).
+static int narrow_tiles(Context *ctx, const int *rp, const int *ci, const double *v, int n, CsrTiles *tl, int *maxw)
+{
+    const int mode = env_int("IPT_CSR_TILES", 1);
+    *maxw = std::max(1, env_int("IPT_CSR_TILES_WIDTH", 4));
+    if (mode == 0 || (mode == 1 && n < (1 << 19))) {
+        return IPT_CUDA_SUCCESS;
+    }
+    int nnz = 0;
+    IPT_HIPCHK(hipMemcpyAsync(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    IPT_HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int st = csr_tiles_create(ctx, rp, ci, v, n, nnz, tl);
+    if (st == IPT_CUDA_INVALID_ARGUMENT) {
+        tl->groups = 0;
+        tl->keys.release();
+        tl->vals.release();
+        tl->start.release();
+        tl->ltile.release();
+        tl->lstart.release();
+        tl->glen.release();
+        return IPT_CUDA_SUCCESS;
+    }
+    trace_mark("narrow-product tiles built");
+    return st;
+}
+
 // Sparse block-cluster solve with the matrix resident on the device (CSC of
 // a symmetric matrix = its CSR).  Runs when every row's indices ascend and
 // the diagonal is already sorted (no permutation needed); otherwise sets
@@ -1772,7 +1802,14 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
     if (env_int("IPT_CSR_SLICES", 1) != 0) {
         IPT_STCHK(csr_slices_create(ctx, d_rp, d_ci, n, &slices));
     }
+    CsrTiles tiles;
+    int tiles_w = 0;
+    IPT_STCHK(narrow_tiles(ctx, d_rp, d_ci, d_v, n, &tiles, &tiles_w));
     ApplyFn apply = [&](const double *X, double *Y, int nc) -> int {
+        if (tiles.groups > 0 && nc <= tiles_w) {
+            // column-major in and out: no transposition
+            return spmm_csr_tiles(ctx, tiles, X, 1, n, Y, 1, n, nc);
+        }
         for (int c0 = 0; c0 < nc; c0 += chunk) {
             const int w = std::min(chunk, nc - c0);
             IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));
@@ -1959,7 +1996,13 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
         if (env_int("IPT_CSR_SLICES", 1) != 0) {
             IPT_STCHK(csr_slices_create(ctx, rp.p, ci.p, n, &slices));
         }
+        CsrTiles tiles;
+        int tiles_w = 0;
+        IPT_STCHK(narrow_tiles(ctx, rp.p, ci.p, vv.p, n, &tiles, &tiles_w));
         ApplyFn apply = [&](const double *X, double *Y, int nc) -> int {
+            if (tiles.groups > 0 && nc <= tiles_w) {
+                return spmm_csr_tiles(ctx, tiles, X, 1, n, Y, 1, n, nc);
+            }
             for (int c0 = 0; c0 < nc; c0 += 128) {
                 const int w = std::min(128, nc - c0);
                 IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));

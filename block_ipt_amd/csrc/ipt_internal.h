@@ -141,7 +141,15 @@ struct DevBuf {
 
 // ------------------------------------------------------------------ profiling
 
-enum { PROF_GEMM_F64 = 0, PROF_LOWP_GEMM = 1, PROF_BLOCK_UPDATE = 2, PROF_SPMM = 3, PROF_SPMM_F32 = 4, PROF_COUNT = 5 };
+enum {
+    PROF_GEMM_F64 = 0,
+    PROF_LOWP_GEMM = 1,
+    PROF_BLOCK_UPDATE = 2,
+    PROF_SPMM = 3,
+    PROF_SPMM_F32 = 4,
+    PROF_SPMM_TILES = 5,
+    PROF_COUNT = 6
+};
 struct ProfileToken {
     int id = -1;
     hipEvent_t start = nullptr;
@@ -257,6 +265,31 @@ struct CsrSlices {
 int csr_slices_create(Context *ctx, const int *row_ptr, const int *col_ind, int n, CsrSlices *sl);
 // passes the product of width k takes (1: the whole matrix at once)
 int csr_slices_for(const CsrSlices &sl, int k);
+// The entries of a CSR (columns ascending within each row) regrouped into
+// buckets of (4096-row group, 4096-column tile) for the narrow products
+// with x in LDS (ipt_csr_tiles.hip): per entry a 32-bit key (row in the
+// group << 12 | column in the tile) and the value; start has
+// groups * tiles + 1 offsets.
+struct CsrTiles {
+    int n = 0;
+    int groups = 0;
+    int tiles = 0;
+    DevBuf<long> start;
+    DevBuf<unsigned> keys;
+    DevBuf<double> vals;
+    // per group (stride tiles, tiles + 1): its nonempty buckets' tiles and
+    // starts (then the group's end), and their count
+    DevBuf<int> ltile;
+    DevBuf<long> lstart;
+    DevBuf<int> glen;
+};
+int csr_tiles_create(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals, int n, long long nnz,
+                     CsrTiles *out);
+// Y = A X through the tiles, two columns per launch: element (r, c) of X at
+// X[r * xrs + c * xcs], of Y at Y[r * yrs + c * ycs] (row-major: xrs = ld,
+// xcs = 1; column-major: xrs = 1, xcs = ld)
+int spmm_csr_tiles(Context *ctx, const CsrTiles &tl, const double *X, long xrs, long xcs, double *Y, long yrs,
+                   long ycs, int k);
 // Y = A X, X and Y row-major (ld >= k); with slices, narrow products run
 // as one pass per column slice (gathers confined to a slice of X).
 int spmm_csr_rowmajor(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals,

@@ -11,7 +11,7 @@
 namespace ipt {
 
 namespace {
-const char *kNames[PROF_COUNT] = {"gemm_f64", "lowp_gemm", "block_update", "spmm", "spmm_f32"};
+const char *kNames[PROF_COUNT] = {"gemm_f64", "lowp_gemm", "block_update", "spmm", "spmm_f32", "spmm_tiles"};
 std::mutex g_mu;
 unsigned g_mask = 0;  // bit i: family i is timed
 struct Timed {

@@ -65,6 +65,17 @@ int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, const double *d
 int ipt_hip_csr_spmm_sliced(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n,
                             const double *d_X, int ldx, double *d_Y, int ldy, int k, int slices, void *handle);
 
+/* The same product through the LDS-tiled layout (addition, no reference
+ * counterpart; the block path takes it for its narrow products on large
+ * matrices): the entries regrouped once into buckets of 4096 rows x 4096
+ * columns, one workgroup per row group staging 64 KiB of X (8192 rows of
+ * one column or 4096 rows of two) in LDS, so every gather is an LDS read
+ * and the matrix is streamed once per two columns.  Builds
+ * the layout on each call (the block path builds it once per solve); the
+ * sums agree with the row-major product to rounding (LDS atomic adds). */
+int ipt_hip_csr_spmm_tiled(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n,
+                           const double *d_X, int ldx, double *d_Y, int ldy, int k, void *handle);
+
 /* In place: Q (n x k, column-major, leading dimension n, k <= n) <- the
  * orthonormal factor of its QR factorisation (positive diagonal R), by
  * shifted Cholesky QR 3 in FP64 on the device: columns normalised, one

@@ -18,7 +18,8 @@ from block_ipt_amd import api  # noqa: E402
 def block_env(monkeypatch):
     monkeypatch.setenv("IPT_BLOCK_CLUSTER", "1")
     for v in ("IPT_DAVIDSON_ENRICH", "IPT_DAVIDSON_STEPS", "IPT_BLOCK_CLUSTER_MAXITER", "IPT_BLOCK_CLUSTER_TOL",
-              "IPT_COUPLING_GAP_RATIO_TOL", "IPT_MAX_LIFT_BLOCK_SIZE", "IPT_PRECISION", "IPT_DAVIDSON_PRECOND"):
+              "IPT_COUPLING_GAP_RATIO_TOL", "IPT_MAX_LIFT_BLOCK_SIZE", "IPT_PRECISION", "IPT_DAVIDSON_PRECOND",
+              "IPT_CSR_TILES", "IPT_CSR_TILES_WIDTH"):
         monkeypatch.delenv(v, raising=False)
     return monkeypatch
 
@@ -119,6 +120,39 @@ def test_spmm_sliced_rejects_bad_counts(dev):
             api.csr_spmm_sliced(drp, dci, dv, n, dX, 1, dX, 1, 1, bad)
 
 
+@pytest.mark.parametrize("n,k", [(300000, 1), (150000, 2), (20001, 3), (4095, 1), (8193, 2)])
+def test_spmm_tiled(dev, n, k):
+    """The LDS-tiled product (4096-row groups, 8192-column tiles): rows and
+    columns past a group or tile edge, one partial group, X with ld = k
+    (column c at stride k), against scipy.  The LDS sums take the products
+    in the order the atomic adds land, so the bar is rounding, not bits."""
+    rp, ci, v = _orbital_cached(n, 24, 7)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    X = np.random.default_rng(n + k).standard_normal((n, k))
+    drp, dci, dv, dX = dev.put(rp), dev.put(ci), dev.put(v), dev.put(X)
+    dY = dev.hip.malloc(n * k * 8)
+    dev.ptrs.append(dY)
+    api.csr_spmm_tiled(drp, dci, dv, n, dX, k, dY, k, k)
+    Y = dev.hip.to_host(dY, (n, k), order="C")
+    ref = A @ X
+    assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_spmm_tiled_rejects_unsorted_rows(dev):
+    """A row whose columns descend across a tile edge cannot be bucketed:
+    the layout's check refuses it instead of misplacing the entry."""
+    n = 20000
+    rp = np.array([0, 2] + [2] * (n - 1), dtype=np.int32)
+    ci = np.array([9000, 100], dtype=np.int32)
+    v = np.array([1.0, 2.0])
+    drp, dci, dv = dev.put(rp), dev.put(ci), dev.put(v)
+    dX = dev.hip.malloc(n * 8)
+    dY = dev.hip.malloc(n * 8)
+    dev.ptrs += [dX, dY]
+    with pytest.raises(api.IptError):
+        api.csr_spmm_tiled(drp, dci, dv, n, dX, 1, dY, 1, 1)
+
+
 def _solve_device(dev, rp, ci, v, n, k, tol=1e-12, maxiter=300):
     drp, dci, dv = dev.put(rp), dev.put(ci), dev.put(v)
     dvec, dval, it, fp = api.ipt_cuda_sparse_csc_device_tol(drp, dci, dv, n, k, ci.size, tol, maxiter)
@@ -172,6 +206,33 @@ def test_device_sparse_block_nonsymmetric_rejected(block_env, dev):
     v[off[3]] += 0.1
     with pytest.raises(api.IptError):
         _solve_device(dev, rp, ci, v, n, 4)
+
+
+def test_device_sparse_block_tiled_narrow_products(block_env, dev):
+    """The products of the sparse solve through the LDS-tiled layout (forced
+    at n = 300000, below the automatic size, and for every width up to 64,
+    so that each product of the solve takes it): the same pairs as with the
+    row-major product to rounding, residuals <= 1e-10, and the tiled kernel
+    ran."""
+    n, m, k = 300000, 16, 8
+    rp, ci, v = orc.orbital_csr(n, m, 3)
+    block_env.setenv("IPT_CSR_TILES", "0")
+    vec0, val0, _, _ = _solve_device(dev, rp, ci, v, n, k)
+    block_env.setenv("IPT_CSR_TILES", "2")
+    block_env.setenv("IPT_CSR_TILES_WIDTH", "64")
+    api.profile_reset()
+    api.profile_enable(True, only=["spmm_tiles"])
+    try:
+        vec, val, it, fp = _solve_device(dev, rp, ci, v, n, k)
+    finally:
+        api.profile_enable(False)
+    _, products = api.profile_get("spmm_tiles")
+    assert products > 0
+    assert np.abs(val - val0).max() <= 1e-10 * max(1.0, np.abs(val0).max())
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    normH = np.abs(A).sum(axis=1).max()
+    res = np.linalg.norm(A @ vec - vec * val, axis=0) / (max(1.0, normH) * np.maximum(1.0, np.linalg.norm(vec, axis=0)))
+    assert res.max() <= 1e-10
 
 
 @pytest.mark.parametrize("n,m,k,seed", [(3000, 20, 8, 1), (20000, 40, 32, 2), (300000, 16, 8, 3)])
