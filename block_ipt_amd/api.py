@@ -404,7 +404,7 @@ def csr_spmm_sliced(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, sl
 
 def csr_spmm_tiled(d_row_ptr, d_col_ind, d_values, n, d_X, ldx, d_Y, ldy, k, handle=None):
     """The CSR product through the LDS-tiled layout (built on each call;
-    columns ascending per row)."""
+    any column order within a row, n <= 2^25)."""
     st = lib().ipt_hip_csr_spmm_tiled(_vpp(d_row_ptr), _vpp(d_col_ind), _vpp(d_values), int(n), _vpp(d_X),
                                       int(ldx), _vpp(d_Y), int(ldy), int(k), handle)
     if st != 0:

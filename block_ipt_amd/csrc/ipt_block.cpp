@@ -1692,8 +1692,8 @@ int block_dense_device(Context *ctx, const double *dH, int n, int k, int maxiter
 // The LDS-tiled layout for the narrow products of a large CSR
 // (ipt_csr_tiles.hip): IPT_CSR_TILES = 0 never, 1 (default) when one column
 // of X (8n bytes) exceeds an XCD's 4 MiB L2, 2 always.  tl->groups stays 0
-// when not taken, also when a row's columns do not ascend (the layout's own
-// check); *maxw = widest product taken through it (IPT_CSR_TILES_WIDTH).
+// when not taken, also for n > 2^25 or a column index outside [0, n) (the
+// layout's own checks); *maxw = widest product taken through it (IPT_CSR_TILES_WIDTH).
 static int narrow_tiles(Context *ctx, const int *rp, const int *ci, const double *v, int n, CsrTiles *tl, int *maxw)
 {
     const int mode = env_int("IPT_CSR_TILES", 1);

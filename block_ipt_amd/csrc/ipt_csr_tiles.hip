@@ -26,7 +26,7 @@ constexpr int kTileColsLog = 12;
 constexpr int kGroupRows = 1 << kGroupRowsLog;  // 4096 rows: 32 KiB of sums
 constexpr int kTileCols = 1 << kTileColsLog;    // 4096 columns: 32 KiB per column of x
 constexpr int kThreads = 1024;
-constexpr int kRowsPerThread = kGroupRows / kThreads;
+constexpr int kMaxTiles = 8192;  // column tiles (n <= 2^25): 64 KiB of LDS counters
 
 // Exclusive prefix over the block of one value per thread (kThreads
 // threads, row order = thread order); *total gets the block's sum.
@@ -59,12 +59,14 @@ __device__ __forceinline__ long block_exclusive_scan(long v, long *wsum, long *t
     return r;
 }
 
-// Entries of each row of group blockIdx.x in each column tile (columns
-// ascend within a row, so a row's entries of a tile are one run; thread t
-// walks rows 4t..4t+3 of the group): count of bucket (group, tile) (pass 0),
-// or, given the buckets' start offsets, the entries copied into their
-// buckets in row order (pass 1).  Pass 0 sets *bad for a row whose columns
-// do not ascend or leave [0, n).
+// Bucket counts (pass 0) or the scatter into the buckets (pass 1) of row
+// group blockIdx.x: a wavefront per row reads the row's entries 64 at a
+// time (coalesced), the bucket of an entry is its column tile, and an LDS
+// counter per tile (pass 0: the count; pass 1: the next free position,
+// starting at the bucket's offset) takes one atomic add per entry.  Within
+// a bucket the entries land in the order of those adds; the product adds
+// them in LDS in arbitrary order as well.  Pass 0 sets *bad for a column
+// outside [0, n).  ntiles <= kMaxTiles (LDS).
 template <int PASS>
 __global__ __launch_bounds__(kThreads) void k_tiles_build(const int *__restrict__ rp, const int *__restrict__ ci,
                                                           const double *__restrict__ v, int n, int ntiles,
@@ -72,66 +74,40 @@ __global__ __launch_bounds__(kThreads) void k_tiles_build(const int *__restrict_
                                                           unsigned *__restrict__ keys, double *__restrict__ vals,
                                                           int *__restrict__ bad)
 {
+    __shared__ unsigned long long ctr[kMaxTiles];
     const int g = blockIdx.x;
-    const int tid = threadIdx.x;
-    __shared__ long wsum[kThreads / 64];
-    __shared__ long total;
-    int cur[kRowsPerThread], end[kRowsPerThread];
-    bool wrong = false;
-#pragma unroll
-    for (int j = 0; j < kRowsPerThread; ++j) {
-        const long r = (long)g * kGroupRows + kRowsPerThread * tid + j;
-        cur[j] = r < n ? rp[r] : 0;
-        end[j] = r < n ? rp[r + 1] : 0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = tid; t < ntiles; t += kThreads) {
+        ctr[t] = PASS == 0 ? 0ull : (unsigned long long)start[(long)g * ntiles + t];
     }
-    for (int t = 0; t < ntiles; ++t) {
-        const int c0 = t * kTileCols, cend = c0 + kTileCols;
-        int cnt[kRowsPerThread];
-        long mine = 0;
-#pragma unroll
-        for (int j = 0; j < kRowsPerThread; ++j) {
-            int e = cur[j];
-            while (e < end[j]) {
-                const int c = ci[e];
-                if (c >= cend) {
-                    break;
-                }
-                wrong |= c < c0 || c >= n;
-                ++e;
+    __syncthreads();
+    bool wrong = false;
+    const long r_end = min((long)n, ((long)g + 1) * kGroupRows);
+    for (long r = (long)g * kGroupRows + wave; r < r_end; r += kThreads / 64) {
+        const unsigned rl = (unsigned)(r - (long)g * kGroupRows) << kTileColsLog;
+        const int pe = rp[r + 1];
+        for (int e = rp[r] + lane; e < pe; e += 64) {
+            const int c = ci[e];
+            if (c < 0 || c >= n) {
+                wrong = true;
+                continue;
             }
-            cnt[j] = e - cur[j];
-            mine += cnt[j];
-        }
-        const long before = block_exclusive_scan(mine, wsum, &total);
-        if (PASS == 0) {
-            if (tid == 0) {
-                count[(long)g * ntiles + t] = total;
+            if (PASS == 0) {
+                atomicAdd(&ctr[c >> kTileColsLog], 1ull);
+            } else {
+                const unsigned long long dst = atomicAdd(&ctr[c >> kTileColsLog], 1ull);
+                keys[dst] = rl | ((unsigned)c & (kTileCols - 1));
+                vals[dst] = v[e];
             }
-        } else {
-            long dst = start[(long)g * ntiles + t] + before;
-#pragma unroll
-            for (int j = 0; j < kRowsPerThread; ++j) {
-                const unsigned row_key = (unsigned)(kRowsPerThread * tid + j) << kTileColsLog;
-                for (int q = 0; q < cnt[j]; ++q) {
-                    const int e = cur[j] + q;
-                    keys[dst + q] = row_key | ((unsigned)(ci[e] - c0) & (kTileCols - 1));
-                    vals[dst + q] = v[e];
-                }
-                dst += cnt[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kRowsPerThread; ++j) {
-            cur[j] += cnt[j];
         }
     }
     if (PASS == 0) {
-#pragma unroll
-        for (int j = 0; j < kRowsPerThread; ++j) {
-            wrong |= cur[j] != end[j];
-        }
         if (wrong) {
             atomicOr(bad, 1);
+        }
+        __syncthreads();
+        for (int t = tid; t < ntiles; t += kThreads) {
+            count[(long)g * ntiles + t] = (long)ctr[t];
         }
     }
 }
@@ -312,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_tiles(const int *__restrict__
 int csr_tiles_create(Context *ctx, const int *row_ptr, const int *col_ind, const double *vals, int n, long long nnz,
                      CsrTiles *out)
 {
-    if (n <= 0 || nnz < 0) {
+    if (n <= 0 || nnz < 0 || (n + kTileCols - 1) / kTileCols > kMaxTiles) {
         return IPT_CUDA_INVALID_ARGUMENT;
     }
     out->n = n;
@@ -331,8 +307,8 @@ int csr_tiles_create(Context *ctx, const int *row_ptr, const int *col_ind, const
                        n, out->tiles, count.p, (const long *)nullptr, (unsigned *)nullptr, (double *)nullptr, bad.p);
     hipLaunchKernelGGL(k_tiles_offsets, dim3(1), dim3(kThreads), 0, ctx->stream, count.p, nb, out->start.p);
     IPT_HIPCHK(hipGetLastError());
-    // the layout holds every entry only for rows with ascending columns in
-    // range, and the scatter trusts the counts: check before pass 1
+    // the scatter trusts the counts: columns in range, and the counts add
+    // up to the entries (row pointers from 0), checked before pass 1
     int hb[1] = {0};
     long total = 0;
     IPT_HIPCHK(hipMemcpyAsync(hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

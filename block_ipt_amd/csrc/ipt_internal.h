@@ -265,7 +265,7 @@ struct CsrSlices {
 int csr_slices_create(Context *ctx, const int *row_ptr, const int *col_ind, int n, CsrSlices *sl);
 // passes the product of width k takes (1: the whole matrix at once)
 int csr_slices_for(const CsrSlices &sl, int k);
-// The entries of a CSR (columns ascending within each row) regrouped into
+// The entries of a CSR (any column order within a row) regrouped into
 // buckets of (4096-row group, 4096-column tile) for the narrow products
 // with x in LDS (ipt_csr_tiles.hip): per entry a 32-bit key (row in the
 // group << 12 | column in the tile) and the value; start has

@@ -70,9 +70,11 @@ int ipt_hip_csr_spmm_sliced(const int *d_row_ptr, const int *d_col_ind, const do
  * matrices): the entries regrouped once into buckets of 4096 rows x 4096
  * columns, one workgroup per row group staging 64 KiB of X (8192 rows of
  * one column or 4096 rows of two) in LDS, so every gather is an LDS read
- * and the matrix is streamed once per two columns.  Builds
- * the layout on each call (the block path builds it once per solve); the
- * sums agree with the row-major product to rounding (LDS atomic adds). */
+ * and the matrix is streamed once per two columns.  Builds the layout on
+ * each call (the block path builds it once per solve); n <= 2^25, any
+ * column order within a row, IPT_CUDA_INVALID_ARGUMENT for a column index
+ * outside [0, n); the sums agree with the row-major product to rounding
+ * (LDS atomic adds). */
 int ipt_hip_csr_spmm_tiled(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n,
                            const double *d_X, int ldx, double *d_Y, int ldy, int k, void *handle);
 

@@ -138,19 +138,30 @@ def test_spmm_tiled(dev, n, k):
     assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-def test_spmm_tiled_rejects_unsorted_rows(dev):
-    """A row whose columns descend across a tile edge cannot be bucketed:
-    the layout's check refuses it instead of misplacing the entry."""
+def test_spmm_tiled_unsorted_rows_and_bad_columns(dev):
+    """The bucketing takes each entry by its own column: rows whose columns
+    descend (across tile edges and within one tile) give scipy's product;
+    a column index outside [0, n) is refused before anything is written."""
     n = 20000
-    rp = np.array([0, 2] + [2] * (n - 1), dtype=np.int32)
-    ci = np.array([9000, 100], dtype=np.int32)
-    v = np.array([1.0, 2.0])
-    drp, dci, dv = dev.put(rp), dev.put(ci), dev.put(v)
-    dX = dev.hip.malloc(n * 8)
-    dY = dev.hip.malloc(n * 8)
-    dev.ptrs += [dX, dY]
+    rng = np.random.default_rng(5)
+    rp, ci, v = _orbital_cached(n, 24, 7)
+    ci = ci.copy()
+    for r in range(0, n, 7):  # reverse every seventh row
+        ci[rp[r]:rp[r + 1]] = ci[rp[r]:rp[r + 1]][::-1]
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    X = rng.standard_normal((n, 2))
+    drp, dci, dv, dX = dev.put(rp), dev.put(ci), dev.put(v), dev.put(X)
+    dY = dev.hip.malloc(n * 2 * 8)
+    dev.ptrs.append(dY)
+    api.csr_spmm_tiled(drp, dci, dv, n, dX, 2, dY, 2, 2)
+    Y = dev.hip.to_host(dY, (n, 2), order="C")
+    ref = A @ X
+    assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
+    bad = ci.copy()
+    bad[5] = n
+    dbad = dev.put(bad)
     with pytest.raises(api.IptError):
-        api.csr_spmm_tiled(drp, dci, dv, n, dX, 1, dY, 1, 1)
+        api.csr_spmm_tiled(drp, dbad, dv, n, dX, 2, dY, 2, 2)
 
 
 def _solve_device(dev, rp, ci, v, n, k, tol=1e-12, maxiter=300):
