@@ -436,6 +436,27 @@ def sparse_summary(api, hip, seed):
     return out
 
 
+def row_stride(w, elem=8):
+    """The block path's row stride for a row-major X of w columns: the
+    smallest of w, next_pow2(w) and w rounded up to a 128-byte line with
+    the fewest lines per row (as the library's csr_row_stride)."""
+    import math
+
+    def lines(ld):
+        stride, nbytes = ld * elem, w * elem
+        period = 128 // math.gcd(stride % 128 or 128, 128)
+        return sum(((r * stride) % 128 + nbytes - 1) // 128 + 1 for r in range(period)) / period
+
+    if w <= 1:
+        return 1
+    per = 128 // elem
+    best = w
+    for c in (w, 1 << (w - 1).bit_length(), (w + per - 1) // per * per):
+        if lines(c) < lines(best) - 1e-9 or (abs(lines(c) - lines(best)) < 1e-9 and c < best):
+            best = c
+    return best
+
+
 def spmm_workload(a):
     """CSR products of the N=1e6 orbital-sum matrix (200 draws per row,
     seed 0) at the widths of --widths, after a 1 GiB device copy (the
@@ -455,7 +476,7 @@ def spmm_workload(a):
     drp, dci, dv = hip.malloc((n + 1) * 4), hip.malloc(cap * 4), hip.malloc(cap * 8)
     nnz = api.generate_orbital_csr(n, m, a.seed, drp, dci, dv, cap)
     widths = [int(w) for w in a.widths.split(",")]
-    kmax = max(widths)
+    kmax = max(1 << (w - 1).bit_length() for w in widths)
     dX = hip.to_device(np.random.default_rng(0).standard_normal(n * kmax))
     dY = hip.malloc(n * kmax * 8)
     out = {"workload": "CSR product, orbital-sum model N=%d, nnz=%d, seed %d" % (n, nnz, a.seed)}
@@ -465,7 +486,8 @@ def spmm_workload(a):
         # automatic count of the block path, and fixed counts for k <= 8)
         # and, for k <= 4, the LDS-tiled product (its layout built on each
         # call, outside the timed launches)
-        counts = [None, 0] + ([2, 4, 8, 16] if k <= 8 else []) + (["tiled"] if k <= 4 else [])
+        counts = ([None, 0] + ([2, 4, 8, 16] if k <= 8 else [4, 8] if k <= 32 else []) + (["tiled"] if k <= 4 else [])
+                  + (["padded", "padded_auto"] if k & (k - 1) and k > 4 else []))
         for sl in counts:
             fam = "spmm_tiles" if sl == "tiled" else "spmm"
             api.profile_reset()
@@ -475,11 +497,16 @@ def spmm_workload(a):
                     api.csr_spmm(drp, dci, dv, n, dX, k, dY, k, k)
                 elif sl == "tiled":
                     api.csr_spmm_tiled(drp, dci, dv, n, dX, k, dY, k, k)
+                elif sl == "padded":
+                    # rows of X at the block path's stride (fewest 128-byte lines)
+                    api.csr_spmm(drp, dci, dv, n, dX, row_stride(k), dY, k, k)
+                elif sl == "padded_auto":
+                    api.csr_spmm_sliced(drp, dci, dv, n, dX, row_stride(k), dY, k, k, 0)
                 else:
                     api.csr_spmm_sliced(drp, dci, dv, n, dX, k, dY, k, k, sl)
             api.profile_enable(False)
             ms, cnt = api.profile_get(fam)
-            key = ("k%d" % k if sl is None else "k%d_tiled" % k if sl == "tiled"
+            key = ("k%d" % k if sl is None else "k%d_%s" % (k, sl) if sl in ("tiled", "padded", "padded_auto")
                    else "k%d_slices%s" % (k, "auto" if sl == 0 else sl))
             out[key] = {"avg_ms": ms / cnt, "launches": cnt, "compulsory_bytes": comp,
                         "compulsory_GB/s": comp / (ms / cnt) / 1e6}

@@ -1795,7 +1795,8 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
     maxcols = std::max(maxcols, std::min(n, k + std::max(16, (2 * k + 2) / 3)) + k + 1);
     const int chunk = std::min(maxcols, 128);
     DevBuf<double> xr, yr;
-    IPT_STCHK(xr.alloc((size_t)n * chunk));
+    // rows of X at csr_row_stride(w) <= round_up(w, 16) doubles
+    IPT_STCHK(xr.alloc((size_t)n * ((chunk + 15) / 16 * 16)));
     IPT_STCHK(yr.alloc((size_t)n * chunk));
     // column slices for the narrow products (rows ascend on this path)
     CsrSlices slices;
@@ -1812,8 +1813,9 @@ int block_sparse_device(Context *ctx, const int *d_rp, const int *d_ci, const do
         }
         for (int c0 = 0; c0 < nc; c0 += chunk) {
             const int w = std::min(chunk, nc - c0);
-            IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));
-            IPT_STCHK(spmm_csr_rowmajor(ctx, d_rp, d_ci, d_v, n, xr.p, w, yr.p, w, w,
+            const long ld = csr_row_stride(w);
+            IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, ld, n, w));
+            IPT_STCHK(spmm_csr_rowmajor(ctx, d_rp, d_ci, d_v, n, xr.p, ld, yr.p, w, w,
                                         slices.nslices > 0 ? &slices : nullptr));
             IPT_STCHK(rowmajor_to_colmajor(ctx, yr.p, w, Y + (size_t)c0 * n, n, n, w));
         }
@@ -1990,7 +1992,7 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
             maxcols += r.second - r.first + 1;
         }
         maxcols = std::max(maxcols, std::min(n, k + std::max(16, (2 * k + 2) / 3)) + k + 1);
-        IPT_STCHK(xr.alloc((size_t)n * std::min(maxcols, 128)));
+        IPT_STCHK(xr.alloc((size_t)n * ((std::min(maxcols, 128) + 15) / 16 * 16)));
         IPT_STCHK(yr.alloc((size_t)n * std::min(maxcols, 128)));
         CsrSlices slices;
         if (env_int("IPT_CSR_SLICES", 1) != 0) {
@@ -2005,8 +2007,9 @@ int block_sparse_host_entry(const int *col_ptr, const int *row_ind, const double
             }
             for (int c0 = 0; c0 < nc; c0 += 128) {
                 const int w = std::min(128, nc - c0);
-                IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, w, n, w));
-                IPT_STCHK(spmm_csr_rowmajor(ctx, rp.p, ci.p, vv.p, n, xr.p, w, yr.p, w, w,
+                const long ld = csr_row_stride(w);
+                IPT_STCHK(colmajor_to_rowmajor(ctx, X + (size_t)c0 * n, n, xr.p, ld, n, w));
+                IPT_STCHK(spmm_csr_rowmajor(ctx, rp.p, ci.p, vv.p, n, xr.p, ld, yr.p, w, w,
                                             slices.nslices > 0 ? &slices : nullptr));
                 IPT_STCHK(rowmajor_to_colmajor(ctx, yr.p, w, Y + (size_t)c0 * n, n, n, w));
             }

@@ -265,6 +265,10 @@ struct CsrSlices {
 int csr_slices_create(Context *ctx, const int *row_ptr, const int *col_ind, int n, CsrSlices *sl);
 // passes the product of width k takes (1: the whole matrix at once)
 int csr_slices_for(const CsrSlices &sl, int k);
+// Leading dimension for a row-major X of w columns of elem bytes whose rows
+// the CSR product gathers: the smallest of w, next_pow2(w) and w rounded up
+// to a 128-byte line with the fewest lines per row (at most that rounding)
+long csr_row_stride(int w, int elem = 8);
 // The entries of a CSR (any column order within a row) regrouped into
 // buckets of (4096-row group, 4096-column tile) for the narrow products
 // with x in LDS (ipt_csr_tiles.hip): per entry a 32-bit key (row in the

@@ -438,15 +438,64 @@ int csr_slices_create(Context *ctx, const int *row_ptr, const int *col_ind, int 
 int csr_slices_for(const CsrSlices &sl, int k)
 {
     // k <= 8 and X beyond 2 MiB: 4 or 8 passes, at most about 4 MiB of X
-    // per pass (an XCD's L2) where 8 passes allow it.  Measured on the
-    // N = 1e6 orbital model (200 draws per row): the best counts were 4 at
-    // k = 1 and 2, 8 at k = 4 and 8 (16 passes were slower at every width:
-    // each pass re-reads Y and the row bounds)
+    // per pass (an XCD's L2) where 8 passes allow it; 9 <= k <= 32 and X
+    // beyond 16 MiB: 8 passes.  Measured on the N = 1e6 orbital model (200
+    // draws per row): the best counts were 4 at k = 1 and 2, 8 at k = 4 to
+    // 32 (k = 16: 6.49 ms against 7.58 in one pass, k = 32: 13.2 against
+    // 14.0); 16 passes were slower at every width (each pass re-reads Y and
+    // the row bounds)
     const long xbytes = (long)sl.n * k * 8;
-    if (k > 8 || xbytes <= (2L << 20) || sl.nslices < 8) {
+    if (k > 32 || xbytes <= (2L << 20) || sl.nslices < 8) {
         return 1;
     }
+    if (k > 8) {
+        return xbytes > (16L << 20) ? 8 : 1;
+    }
     return xbytes > (16L << 20) ? 8 : 4;
+}
+
+long csr_row_stride(int w, int elem)
+{
+    // 128-byte lines a row of w elements of elem bytes touches, averaged
+    // over the row offsets that stride ld produces (they repeat every 128 /
+    // gcd(elem ld, 128) rows); the smallest ld with the fewest lines.  w = 12: ld 16,
+    // one line instead of 1.5 (10.5 -> 7.6 ms at N = 1e6); w = 27: ld 32,
+    // 2 lines instead of 2.6 (18.2 -> 14.0 ms)
+    if (w <= 1) {
+        return 1;
+    }
+    auto lines = [w, elem](long ld) {
+        const long stride = ld * elem, bytes = (long)w * elem;
+        long g = 128, a = stride % 128;
+        while (a != 0) {
+            const long t = g % a;
+            g = a;
+            a = t;
+        }
+        const long period = 128 / g;
+        long total = 0;
+        for (long r = 0; r < period; ++r) {
+            const long off = (r * stride) % 128;
+            total += (off + bytes - 1) / 128 + 1;
+        }
+        return (double)total / (double)period;
+    };
+    long p2 = 1;
+    while (p2 < w) {
+        p2 <<= 1;
+    }
+    const long per_line = 128 / elem;
+    const long cands[3] = {(long)w, p2, ((long)w + per_line - 1) / per_line * per_line};
+    long best = w;
+    double best_lines = lines(w);
+    for (long c : cands) {
+        const double l = lines(c);
+        if (l < best_lines - 1e-9 || (l < best_lines + 1e-9 && c < best)) {
+            best = c;
+            best_lines = l;
+        }
+    }
+    return best;
 }
 
 // ------------------------------------------------------------------ layouts

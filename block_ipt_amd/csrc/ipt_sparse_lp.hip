@@ -25,10 +25,11 @@ __global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, lo
 constexpr int kTr = 32;   // rows per tile
 constexpr int kTc = 128;  // columns per tile
 
-// D (row-major FP32, ld w) = B - B_a (column-major FP64, ld ldb) for columns
-// [0, w), through an LDS tile so both sides are read / written contiguously.
+// D (row-major FP32, ld ldd >= w) = B - B_a (column-major FP64, ld ldb) for
+// columns [0, w), through an LDS tile so both sides are read / written
+// contiguously.
 __global__ __launch_bounds__(256) void k_delta_rows_f32(const double *__restrict__ B, const double *__restrict__ Ba,
-                                                        long ldb, int n, int w, float *__restrict__ D)
+                                                        long ldb, int n, int w, float *__restrict__ D, long ldd)
 {
     __shared__ float t[kTr][kTc + 1];
     const int r0 = blockIdx.x * kTr;
@@ -45,12 +46,13 @@ __global__ __launch_bounds__(256) void k_delta_rows_f32(const double *__restrict
     for (int e = threadIdx.x; e < kTr * kTc; e += 256) {
         const int c = e % kTc, r = e / kTc;
         if (r0 + r < n && c < w) {
-            D[(long)(r0 + r) * w + c] = t[r][c];
+            D[(long)(r0 + r) * ldd + c] = t[r][c];
         }
     }
 }
 
-// Z (row-major FP32, ld w) = E D: a wavefront per row, S lanes per column
+// Z (row-major FP32, ld w) = E D (D row-major, ld ldx): a wavefront per
+// row, S lanes per column
 // group, the row's (column, value) pairs read 64 at a time and handed out
 // by shuffles (as k_spmm_rowmajor); the diagonal entry is skipped.  Row r's
 // entries are [rs[r], re[r]) (the whole row, or one column slice; with
@@ -59,8 +61,8 @@ template <int S>
 __global__ __launch_bounds__(256) void k_spmm_offdiag_f32(const int *__restrict__ rs, const int *__restrict__ re,
                                                           const int *__restrict__ ci,
                                                           const float *__restrict__ vals, int n,
-                                                          const float *__restrict__ X, int w, float *__restrict__ Z,
-                                                          int accumulate)
+                                                          const float *__restrict__ X, long ldx, int w,
+                                                          float *__restrict__ Z, int accumulate)
 {
     constexpr int G = 64 / S;
     constexpr int STEPS = 64 / G;
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(256) void k_spmm_offdiag_f32(const int *__restrict_
                     }
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
-                        x[u] = xc[(long)c[u] * w];
+                        x[u] = xc[(long)c[u] * ldx];
                     }
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
@@ -145,32 +147,32 @@ __global__ __launch_bounds__(256) void k_delta_epilogue(const float *__restrict_
 
 template <int S>
 void launch_spmm_f32(const ProfileToken &prof, bool first, bool last, dim3 g, hipStream_t st, const int *rs,
-                     const int *re, const int *ci, const float *v, int n, const float *X, int w, float *Z,
-                     int accumulate)
+                     const int *re, const int *ci, const float *v, int n, const float *X, long ldx, int w,
+                     float *Z, int accumulate)
 {
-    launch_timed(prof, first, last, k_spmm_offdiag_f32<S>, g, dim3(256), 0, st, rs, re, ci, v, n, X, w, Z,
+    launch_timed(prof, first, last, k_spmm_offdiag_f32<S>, g, dim3(256), 0, st, rs, re, ci, v, n, X, ldx, w, Z,
                  accumulate);
 }
 
 // one pass of the product; first / last: the pass opens / closes the timed span
 void spmm_f32_pass(const ProfileToken &prof, bool first, bool last, dim3 g, hipStream_t st, const int *rs,
-                   const int *re, const int *ci, const float *v, int n, const float *X, int w, float *Z,
-                   int accumulate)
+                   const int *re, const int *ci, const float *v, int n, const float *X, long ldx, int w,
+                   float *Z, int accumulate)
 {
     if (w <= 1) {
-        launch_spmm_f32<1>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<1>(prof, first, last, g, st, rs, re, ci, v, n, X, ldx, w, Z, accumulate);
     } else if (w <= 2) {
-        launch_spmm_f32<2>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<2>(prof, first, last, g, st, rs, re, ci, v, n, X, ldx, w, Z, accumulate);
     } else if (w <= 4) {
-        launch_spmm_f32<4>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<4>(prof, first, last, g, st, rs, re, ci, v, n, X, ldx, w, Z, accumulate);
     } else if (w <= 8) {
-        launch_spmm_f32<8>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<8>(prof, first, last, g, st, rs, re, ci, v, n, X, ldx, w, Z, accumulate);
     } else if (w <= 16) {
-        launch_spmm_f32<16>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<16>(prof, first, last, g, st, rs, re, ci, v, n, X, ldx, w, Z, accumulate);
     } else if (w <= 32) {
-        launch_spmm_f32<32>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<32>(prof, first, last, g, st, rs, re, ci, v, n, X, ldx, w, Z, accumulate);
     } else {
-        launch_spmm_f32<64>(prof, first, last, g, st, rs, re, ci, v, n, X, w, Z, accumulate);
+        launch_spmm_f32<64>(prof, first, last, g, st, rs, re, ci, v, n, X, ldx, w, Z, accumulate);
     }
 }
 
@@ -195,30 +197,34 @@ int sparse_lowp_apply_delta(Context *ctx, const SparseLowp &op, const double *B,
 {
     const int n = op.n;
     const int chunk = std::min(nc, kTc);
+    // D rows at csr_row_stride(w, 4) <= round_up(w, 32) floats
+    const size_t dcols = (size_t)(chunk + 31) / 32 * 32;
     float *D = nullptr;
-    IPT_STCHK(context_scratch(ctx, SLOT_LOWP, (size_t)2 * n * chunk * sizeof(float), (void **)&D));
-    float *Z = D + (size_t)n * chunk;
+    IPT_STCHK(context_scratch(ctx, SLOT_LOWP, (dcols + chunk) * n * sizeof(float), (void **)&D));
+    float *Z = D + (size_t)n * dcols;
     const dim3 tiles((unsigned)((n + kTr - 1) / kTr));
     const long wblocks = std::min(((long)n * 64 + 255) / 256, 65536L);
     const dim3 g((unsigned)wblocks);
     for (int c0 = 0; c0 < nc; c0 += chunk) {
         const int w = std::min(chunk, nc - c0);
         const size_t o = (size_t)c0 * ld;
-        hipLaunchKernelGGL(k_delta_rows_f32, tiles, dim3(256), 0, ctx->stream, B + o, Ba + o, ld, n, w, D);
+        const long ldd = csr_row_stride(w, 4);
+        hipLaunchKernelGGL(k_delta_rows_f32, tiles, dim3(256), 0, ctx->stream, B + o, Ba + o, ld, n, w, D, ldd);
         IPT_HIPCHK(hipGetLastError());
         const ProfileToken prof = profile_begin_launches(PROF_SPMM_F32);
-        // FP32 rows of X: the pass count of an FP64 product half as wide
-        const int ns = op.slices != nullptr ? csr_slices_for(*op.slices, (w + 1) / 2) : 1;
+        // FP32 rows of X: the pass count of an FP64 product half as wide, for
+        // widths up to 16 (measured); wider FP32 products in one pass
+        const int ns = op.slices != nullptr && w <= 16 ? csr_slices_for(*op.slices, (w + 1) / 2) : 1;
         if (ns > 1) {
             const int *sp = op.slices->split.p;
             const long stride = (long)op.slices->n;
             const int step = op.slices->nslices / ns;
             for (int b = 0; b < ns; ++b) {
                 spmm_f32_pass(prof, b == 0, b == ns - 1, g, ctx->stream, sp + (long)(b * step) * stride,
-                              sp + (long)((b + 1) * step) * stride, op.ci, op.vals.p, n, D, w, Z, b > 0);
+                              sp + (long)((b + 1) * step) * stride, op.ci, op.vals.p, n, D, ldd, w, Z, b > 0);
             }
         } else {
-            spmm_f32_pass(prof, true, true, g, ctx->stream, op.rp, op.rp + 1, op.ci, op.vals.p, n, D, w, Z, 0);
+            spmm_f32_pass(prof, true, true, g, ctx->stream, op.rp, op.rp + 1, op.ci, op.vals.p, n, D, ldd, w, Z, 0);
         }
         profile_end(ctx, prof, w);
         IPT_HIPCHK(hipGetLastError());
