@@ -212,9 +212,8 @@ int sparse_lowp_apply_delta(Context *ctx, const SparseLowp &op, const double *B,
         hipLaunchKernelGGL(k_delta_rows_f32, tiles, dim3(256), 0, ctx->stream, B + o, Ba + o, ld, n, w, D, ldd);
         IPT_HIPCHK(hipGetLastError());
         const ProfileToken prof = profile_begin_launches(PROF_SPMM_F32);
-        // FP32 rows of X: the pass count of an FP64 product half as wide, for
-        // widths up to 16 (measured); wider FP32 products in one pass
-        const int ns = op.slices != nullptr && w <= 16 ? csr_slices_for(*op.slices, (w + 1) / 2) : 1;
+        // FP32 rows of X: the pass count of an FP64 product half as wide
+        const int ns = op.slices != nullptr ? csr_slices_for(*op.slices, (w + 1) / 2) : 1;
         if (ns > 1) {
             const int *sp = op.slices->split.p;
             const long stride = (long)op.slices->n;
