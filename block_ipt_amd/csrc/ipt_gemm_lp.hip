@@ -449,11 +449,14 @@ __global__ __launch_bounds__(256) void k_lowp_epilogue(const f4_t *__restrict__ 
 }
 
 // k blocks per wavefront: the B fragments it holds in registers take
-// KW x NB x 4 VGPRs per plane; chosen so no variant spills (256 VGPRs at two
-// wavefronts per SIMD)
+// KW x NB x 4 VGPRs per plane (FP8: half, two k blocks per 16-byte entry);
+// chosen so no variant spills (256 VGPRs at two wavefronts per SIMD; FP8 at
+// NB = 8 and KW = 8: 230).  Fewer k blocks per wavefront mean more splits,
+// so more FP32 partials per product: FP8 at 128 columns with KW = 4 wrote
+// as many partial bytes as it read of E
 constexpr int lowp_kw(int mode, int nb)
 {
-    return mode == LOWP_FP8 ? (nb <= 5 ? 8 : 4)
+    return mode == LOWP_FP8 ? 8
                             : lowp_two_planes(mode) ? (nb == 1 ? 8 : nb <= 5 ? 4 : 2) : (nb <= 5 ? 8 : 4);
 }
 
