@@ -487,7 +487,8 @@ def spmm_workload(a):
         # and, for k <= 4, the LDS-tiled product (its layout built on each
         # call, outside the timed launches)
         counts = ([None, 0] + ([2, 4, 8, 16] if k <= 8 else [4, 8] if k <= 32 else []) + (["tiled"] if k <= 4 else [])
-                  + (["padded", "padded_auto"] if k & (k - 1) and k > 4 else []))
+                  + (["padded", "padded_auto"] if k & (k - 1) and k > 4 else [])
+                  + (["padded_s16"] if 8 < k <= 32 else []))
         for sl in counts:
             fam = "spmm_tiles" if sl == "tiled" else "spmm"
             api.profile_reset()
@@ -502,11 +503,13 @@ def spmm_workload(a):
                     api.csr_spmm(drp, dci, dv, n, dX, row_stride(k), dY, k, k)
                 elif sl == "padded_auto":
                     api.csr_spmm_sliced(drp, dci, dv, n, dX, row_stride(k), dY, k, k, 0)
+                elif sl == "padded_s16":
+                    api.csr_spmm_sliced(drp, dci, dv, n, dX, row_stride(k), dY, k, k, 16)
                 else:
                     api.csr_spmm_sliced(drp, dci, dv, n, dX, k, dY, k, k, sl)
             api.profile_enable(False)
             ms, cnt = api.profile_get(fam)
-            key = ("k%d" % k if sl is None else "k%d_%s" % (k, sl) if sl in ("tiled", "padded", "padded_auto")
+            key = ("k%d" % k if sl is None else "k%d_%s" % (k, sl) if sl in ("tiled", "padded", "padded_auto", "padded_s16")
                    else "k%d_slices%s" % (k, "auto" if sl == 0 else sl))
             out[key] = {"avg_ms": ms / cnt, "launches": cnt, "compulsory_bytes": comp,
                         "compulsory_GB/s": comp / (ms / cnt) / 1e6}

@@ -439,17 +439,18 @@ int csr_slices_for(const CsrSlices &sl, int k)
 {
     // k <= 8 and X beyond 2 MiB: 4 or 8 passes, at most about 4 MiB of X
     // per pass (an XCD's L2) where 8 passes allow it; 9 <= k <= 32 and X
-    // beyond 16 MiB: 8 passes.  Measured on the N = 1e6 orbital model (200
-    // draws per row): the best counts were 4 at k = 1 and 2, 8 at k = 4 to
-    // 32 (k = 16: 6.49 ms against 7.58 in one pass, k = 32: 13.2 against
-    // 14.0); 16 passes were slower at every width (each pass re-reads Y and
-    // the row bounds)
+    // beyond 16 MiB: 8 passes, 16 from 17 columns.  Measured on the N = 1e6
+    // orbital model (200 draws per row, X rows at csr_row_stride): the best
+    // counts were 4 at k = 1 and 2, 8 at k = 4 to 16 (k = 16: 6.49 ms
+    // against 7.58 in one pass and 7.53 in 16), 16 at k = 24 to 32 (k = 27:
+    // 12.86 ms against 13.17 in 8; k = 32: 13.02 against 13.21); more passes
+    // re-read Y and the row bounds more often
     const long xbytes = (long)sl.n * k * 8;
     if (k > 32 || xbytes <= (2L << 20) || sl.nslices < 8) {
         return 1;
     }
     if (k > 8) {
-        return xbytes > (16L << 20) ? 8 : 1;
+        return xbytes <= (16L << 20) ? 1 : k > 16 && sl.nslices >= 16 ? 16 : 8;
     }
     return xbytes > (16L << 20) ? 8 : 4;
 }
