@@ -60,8 +60,12 @@ int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, const double *d
  * run as column-sliced passes: pass b multiplies the entries with columns
  * in one slice of [0, n) and adds to Y, so its gathers stay within that
  * slice of X.  slices = 0: the pass count the block path uses for width k
- * (1 for k > 8 or X of at most 2 MiB, else 4 passes up to 16 MiB of X
- * and 8 beyond); 1, 2, 4, 8, 16: that many passes. */
+ * (1 for k > 32 or X of at most 2 MiB; k <= 8: 4 passes up to 16 MiB of X
+ * and 8 beyond; 9 <= k <= 32: 1 pass up to 16 MiB of X, beyond it 8
+ * passes, 16 from k = 17); 1, 2, 4, 8, 16: that many passes.  The block
+ * path also lays X out at the row stride with the fewest 128-byte lines
+ * (ldx from next_pow2(k) or k rounded up to 16 where that touches fewer
+ * lines). */
 int ipt_hip_csr_spmm_sliced(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n,
                             const double *d_X, int ldx, double *d_Y, int ldy, int k, int slices, void *handle);
 
