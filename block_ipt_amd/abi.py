@@ -249,6 +249,7 @@ SIGNATURES = {
     "ipt_hip_csr_spmm": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _vp]),
     "ipt_hip_csr_spmm_sliced": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _c.c_int, _vp]),
     "ipt_hip_csr_spmm_tiled": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _vp, _c.c_int, _vp, _c.c_int, _c.c_int, _vp]),
+    "ipt_hip_csr_row_stride": (_c.c_long, [_c.c_int, _c.c_int]),
     "ipt_hip_cholqr_mixed": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp]),
     "ipt_hip_last_davidson_stats": (_c.c_int, [_vp, _ip, _ip, _ip, _ip, _c.POINTER(_c.c_longlong), _dp]),
     "ipt_hip_last_cluster_ranges": (_c.c_int, [_vp, _ip, _c.c_int, _ip]),

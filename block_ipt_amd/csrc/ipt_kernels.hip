@@ -582,3 +582,13 @@ int rowmajor_to_colmajor(Context *ctx, const double *A, long lda, double *B, lon
 }
 
 }  // namespace ipt
+
+using namespace ipt;
+
+extern "C" long ipt_hip_csr_row_stride(int w, int elem)
+{
+    if (w <= 0 || (elem != 4 && elem != 8)) {
+        return -1;
+    }
+    return csr_row_stride(w, elem);
+}

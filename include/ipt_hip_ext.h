@@ -69,6 +69,14 @@ int ipt_hip_csr_spmm(const int *d_row_ptr, const int *d_col_ind, const double *d
 int ipt_hip_csr_spmm_sliced(const int *d_row_ptr, const int *d_col_ind, const double *d_values, int n,
                             const double *d_X, int ldx, double *d_Y, int ldy, int k, int slices, void *handle);
 
+/* The row stride (in elements of elem = 8 or 4 bytes) the block path gives
+ * a row-major X of w columns before a CSR product: among w, next_pow2(w)
+ * and w rounded up to a 128-byte line, the smallest with the fewest
+ * 128-byte lines per row on average over the row offsets.  Host only (no
+ * reference counterpart; exported for the layout's tests); -1 for w <= 0
+ * or another elem. */
+long ipt_hip_csr_row_stride(int w, int elem);
+
 /* The same product through the LDS-tiled layout (addition, no reference
  * counterpart; the block path takes it for its narrow products on large
  * matrices): the entries regrouped once into buckets of 4096 rows x 4096

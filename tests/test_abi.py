@@ -184,3 +184,38 @@ int main(void) {
         assert l.acx_cuda_device_tol(*args, None, ctypes.byref(r)) == 1
         assert r.iterations == 0 and not r.d_vectors
     assert l.acx_cuda_device_tol(p, 4, 2, 1e-12, 10, None, None) == 1
+
+
+def _lines_brute(w, ld, elem, rows=4096):
+    """128-byte lines touched by rows 0..rows-1 of a row-major array of w
+    elements per row at stride ld, counted row by row."""
+    total = 0
+    for r in range(rows):
+        first = (r * ld * elem) // 128
+        last = (r * ld * elem + w * elem - 1) // 128
+        total += last - first + 1
+    return total / rows
+
+
+def test_csr_row_stride_layout():
+    """The row stride the block path gives X before a CSR product: never
+    more lines per row than the unpadded stride (brute-force count over
+    4096 rows), aligned widths kept, within the block path's allocation
+    bound (w rounded up to a line), and the measured cases (8-byte rows:
+    12 -> 16, 27 -> 32; 20 and 40 already touch the fewest lines)."""
+    f = abi.lib().ipt_hip_csr_row_stride
+    assert f(0, 8) == -1 and f(5, 2) == -1
+    for elem in (8, 4):
+        per = 128 // elem
+        for w in range(1, 131):
+            ld = f(w, elem)
+            assert w <= ld <= -(-w // per) * per
+            assert ld in (w, 1 << (w - 1).bit_length(), -(-w // per) * per)
+            assert _lines_brute(w, ld, elem) <= _lines_brute(w, w, elem) + 1e-12
+            if (w * elem) % 128 == 0:
+                assert ld == w
+            # no candidate touches fewer lines
+            for c in {w, 1 << (w - 1).bit_length(), -(-w // per) * per}:
+                assert _lines_brute(w, ld, elem) <= _lines_brute(w, c, elem) + 1e-12
+    assert [f(w, 8) for w in (12, 27, 20, 40)] == [16, 32, 20, 40]
+    assert f(27, 4) == 32 and f(1, 8) == 1
