@@ -90,11 +90,12 @@ def _orbital_cached(n, m, seed):
 
 
 @pytest.mark.parametrize("slices", [0, 1, 2, 8, 16])
-@pytest.mark.parametrize("n,k", [(300000, 1), (150000, 4), (20001, 3), (40000, 9)])
+@pytest.mark.parametrize("n,k", [(300000, 1), (150000, 4), (20001, 3), (40000, 9), (150000, 20)])
 def test_spmm_column_slices(dev, n, k, slices):
     """Column-sliced passes of the CSR product (rows of the orbital model,
     ascending columns): every pass count, including the automatic one (X
-    of 2.4, 4.8, 0.5 and 2.9 MB: 4, 4, 1 and 1 passes), against scipy; a
+    of 2.4, 4.8, 0.5, 2.9 and 24 MB: 4, 4, 1, 1 and 16 passes), against
+    scipy; a
     slice bound off by one would drop or double the entries at a slice
     edge."""
     rp, ci, v = _orbital_cached(n, 24, 7)
@@ -106,6 +107,29 @@ def test_spmm_column_slices(dev, n, k, slices):
     api.csr_spmm_sliced(drp, dci, dv, n, dX, k, dY, k, k, slices)
     Y = dev.hip.to_host(dY, (n, k), order="C")
     ref = A @ X
+    assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("k", [12, 27])
+def test_spmm_padded_row_stride(dev, k):
+    """X at the row stride the block path uses (12 -> 16, 27 -> 32 doubles,
+    ipt_hip_csr_row_stride) with the automatic pass count (X of 14.4 and
+    25.9 MB: 1 and 16 passes): the padding columns never enter the sums
+    (they hold NaN here), against scipy."""
+    n = 150000
+    ld = api.lib().ipt_hip_csr_row_stride(k, 8)
+    assert ld == {12: 16, 27: 32}[k]
+    rp, ci, v = _orbital_cached(n, 24, 7)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    X = np.full((n, ld), np.nan)
+    X[:, :k] = np.random.default_rng(k).standard_normal((n, k))
+    drp, dci, dv, dX = dev.put(rp), dev.put(ci), dev.put(v), dev.put(X)
+    dY = dev.hip.malloc(n * k * 8)
+    dev.ptrs.append(dY)
+    api.csr_spmm_sliced(drp, dci, dv, n, dX, ld, dY, k, k, 0)
+    Y = dev.hip.to_host(dY, (n, k), order="C")
+    ref = A @ X[:, :k]
+    assert np.isfinite(Y).all()
     assert np.abs(Y - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
